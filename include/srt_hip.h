@@ -1,0 +1,164 @@
+/*
+ * srt_hip.h - C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * The reference renderer (woAIxuexiSR/SRT) has no C ABI: its hot path is the
+ * C++ `RenderPass` interface (src/passes/renderpass.h:10-30) whose
+ * `PathTracer` pass (src/passes/integrator/path_tracer.cu:5-33) launches an
+ * OptiX megakernel (src/shader/integrator/path_tracer.cu:50-169) over
+ * acceleration structures built by `OptixRayTracer`
+ * (src/optix/optix_ray_tracer.cu:213-463).  Each entry point below names the
+ * reference interface it replaces.  The C++ passes in srt_amd/host call these
+ * functions; so do the Python bindings.
+ *
+ * Conventions
+ *   - Every function returns SRT_OK (0) or a negative SRT_E_* code and never
+ *     exits the process (the reference prints and calls exit(-1)).
+ *     srt_last_error() returns the message of the last failure on a context.
+ *   - Host arrays passed in are copied; the caller keeps ownership.
+ *     Scene handles are owned by the context.
+ *   - A context is used from one host thread at a time, except srt_submit /
+ *     srt_poll / srt_wait, which are thread-safe.
+ *   - The film is RGBA float32, row-major, pixel index = iy * width + ix, row 0
+ *     at the bottom of the image (src/passes/film.h:7-20).
+ */
+#ifndef SRT_HIP_H
+#define SRT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SRT_OK 0
+#define SRT_E_INVALID (-1)   /* bad argument or unknown name */
+#define SRT_E_NO_DEVICE (-2) /* no HIP device with that ordinal */
+#define SRT_E_HIP (-3)       /* HIP runtime error */
+#define SRT_E_OOM (-4)       /* exceeds the context's HBM budget or allocation failed */
+#define SRT_E_NOT_FOUND (-5) /* unknown scene / ticket handle */
+#define SRT_E_LIMIT (-6)     /* exceeds a documented limit (BVH depth, leaf size, sizes) */
+#define SRT_E_STATE (-7)     /* call not valid in the current state */
+
+typedef struct srt_ctx srt_ctx;
+typedef uint64_t srt_scene;
+
+/* BxDF kinds, src/device_include/scene/bxdf.h:62 */
+#define SRT_BXDF_DIFFUSE 0
+#define SRT_BXDF_DIFFUSE_TRANSMISSION 1
+#define SRT_BXDF_DIELECTRIC 2
+#define SRT_BXDF_DISNEY 3
+
+/* Material: BxDF parameters (bxdf.h:64-76) + GMaterial colours (gmaterial.h:9-12).
+ * 80 bytes.  A material is emissive iff intensity > 0. */
+typedef struct srt_material {
+    int32_t type;
+    float ior, metallic, subsurface, roughness, specular, specular_tint, anisotropic, sheen, sheen_tint,
+        clearcoat, clearcoat_gloss, spec_trans;
+    float base_color[3];
+    float emission_color[3];
+    float intensity;
+} srt_material;
+
+/* World-space triangle meshes (one identity instance per mesh).  Replaces the
+ * host Scene -> GScene upload (src/scene/scene.cu:84-288). */
+typedef struct srt_scene_desc {
+    int32_t num_vertices;
+    const float* positions;           /* 3 * num_vertices */
+    const float* normals;             /* 3 * num_vertices, or NULL */
+    const float* tangents;            /* 3 * num_vertices, or NULL */
+    int32_t num_triangles;
+    const uint32_t* indices;          /* 3 * num_triangles */
+    int32_t num_meshes;
+    const int32_t* mesh_first_triangle; /* num_meshes + 1 offsets into the triangles */
+    const int32_t* mesh_material;     /* num_meshes */
+    int32_t num_materials;
+    const srt_material* materials;
+    float env_color[3];               /* constant environment (light.h:27-44) */
+} srt_scene_desc;
+
+/* Pinhole camera, src/renderer/renderer.cu:125-140 (LookAt + vertical fov). */
+typedef struct srt_camera {
+    float position[3], target[3], up[3];
+    float fov_deg;
+} srt_camera;
+
+/* PathTracer pass parameters, src/passes/integrator/path_tracer.h:34-38
+ * (defaults 1, 16, 4, 1, 1).  `seed` replaces the host random frame seed
+ * (path_tracer.cu:17).  mlp_mode: 0 plain tracer, 1 query the radiance MLP
+ * after vertex 2 (needs srt_mlp_*). */
+typedef struct srt_pt_params {
+    int32_t samples_per_pixel, max_depth, rr_depth, use_nee, use_mis;
+    uint32_t seed;
+    int32_t mlp_mode;
+    int32_t reserved;
+} srt_pt_params;
+
+typedef struct srt_bvh_opts {
+    int32_t max_leaf;      /* 1..8, default 4 */
+    int32_t cluster_size;  /* 0: automatic (max(64, n/1024)) */
+} srt_bvh_opts;
+
+/* ---- context (replaces OptixRayTracer's context/stream, optix_ray_tracer.cu:37-56) ---- */
+int srt_create(int device, size_t hbm_budget_bytes, srt_ctx** out);
+void srt_destroy(srt_ctx* ctx);
+const char* srt_last_error(const srt_ctx* ctx);
+/* tunables: "path_pool" (path slots per chunk), "profile" (0/1 per-kernel HIP events) */
+int srt_set_option(srt_ctx* ctx, const char* key, int64_t value);
+size_t srt_hbm_used(const srt_ctx* ctx);
+
+/* ---- scenes (replaces Scene::build_gscene, scene.cu:84-288) ---- */
+int srt_scene_upload(srt_ctx* ctx, const srt_scene_desc* desc, srt_scene* out);
+int srt_scene_release(srt_ctx* ctx, srt_scene scene);
+/* light CDF and per-mesh area as built on upload (scene.cu:240-264, mesh.cpp:3-18);
+ * any pointer may be NULL.  Returns the number of lights or a negative code. */
+int srt_scene_lights(srt_ctx* ctx, srt_scene scene, float* light_cdf, int32_t* light_mesh, float* area_cdf,
+                     float* mesh_area, float* weight_sum);
+
+/* ---- acceleration structure (replaces OptixRayTracer::build_gas/build_ias,
+ *      optix_ray_tracer.cu:213-359, and update_as, :443-463) ---- */
+int srt_scene_build_bvh(srt_ctx* ctx, srt_scene scene, const srt_bvh_opts* opts);
+/* info[0] nodes, [1] root ref, [2] max depth, [3] clusters, [4] top-level nodes,
+ * [5] cluster size, [6] triangles, [7] max leaf */
+int srt_scene_bvh_info(srt_ctx* ctx, srt_scene scene, int32_t info[8]);
+/* host copies: nodes (64 B each), sorted->primitive order, sorted triangles (12 floats each) */
+int srt_scene_bvh_download(srt_ctx* ctx, srt_scene scene, void* nodes, uint32_t* order, float* tris);
+/* linear blend skinning (scene.cu:336-395): 4 (bone id, weight) per vertex; id -1 ends the list */
+int srt_scene_set_skin(srt_ctx* ctx, srt_scene scene, const int32_t* bone_ids, const float* bone_weights);
+/* per frame: skin with bone matrices (3x4 row-major, n_bones of them), then refit the BVH */
+int srt_scene_skin_refit(srt_ctx* ctx, srt_scene scene, const float* bones_3x4, int32_t n_bones);
+int srt_scene_download_vertices(srt_ctx* ctx, srt_scene scene, float* positions, float* normals, float* tangents);
+
+/* ---- rays (replaces optixTrace closest / shadow, path_tracer.cu:75,124) ----
+ * Host arrays.  any = 0: closest hit, prim = -1 on miss; any = 1: prim = 1 if
+ * anything is hit in [tmin, tmax[i]], else 0. */
+int srt_trace_rays(srt_ctx* ctx, srt_scene scene, int32_t n, const float* origins, const float* dirs, float tmin,
+                   const float* tmax, int32_t any, float* t, float* u, float* v, int32_t* prim);
+
+/* ---- render (replaces PathTracer::render + the megakernel, path_tracer.cu:13-33) ----
+ * Writes float4 RGBA (alpha 1) for width*height pixels.  out_on_device = 1:
+ * rgba_out is device memory; 0: host memory; rgba_out == NULL keeps the film in
+ * the context (see srt_film_download). */
+int srt_render(srt_ctx* ctx, srt_scene scene, const srt_camera* camera, const srt_pt_params* params, int32_t width,
+               int32_t height, float* rgba_out, int32_t out_on_device);
+int srt_film_download(srt_ctx* ctx, float* rgba_host, int64_t num_pixels);
+
+/* ---- post-processing (Accumulate::render accumulate.cu:10-29,
+ *      ToneMapping::render tonemapping.cu:5-70) on device or host buffers ----
+ * tonemap type: 0 None, 1 Clamp, 2 Reinhard, 3 Uncharted2, 4 ACES */
+int srt_tonemap(srt_ctx* ctx, float* rgba, int64_t num_pixels, int32_t type, float exposure, int32_t use_gamma,
+                int32_t on_device);
+int srt_accumulate(srt_ctx* ctx, float* acc, float* rgba, int64_t num_pixels, int32_t frame_count, int32_t on_device);
+
+/* ---- statistics (the reference's PROFILE scopes, src/utils/profiler.h:156-219) ----
+ * Per-kernel totals since the last reset, filled in order of kernel name.
+ * Returns the number of kernels; names are written NUL-separated into names. */
+int srt_stats(srt_ctx* ctx, char* names, int32_t names_len, double* ms, int64_t* launches, int64_t* items, int32_t max);
+void srt_stats_reset(srt_ctx* ctx);
+/* segments traced and shadow rays of the last srt_render */
+int srt_last_counts(srt_ctx* ctx, int64_t* segments, int64_t* shadow_rays);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SRT_HIP_H */

@@ -1,0 +1,309 @@
+"""ctypes bindings of the C ABI declared in include/srt_hip.h.
+
+The library is srt_amd/libsrt_hip.so, built in-tree by srt_amd/build.py.
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .scenes import MAT_DTYPE, SceneData
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsrt_hip.so")
+
+SRT_OK = 0
+ERRORS = {-1: "SRT_E_INVALID", -2: "SRT_E_NO_DEVICE", -3: "SRT_E_HIP", -4: "SRT_E_OOM", -5: "SRT_E_NOT_FOUND",
+          -6: "SRT_E_LIMIT", -7: "SRT_E_STATE"}
+
+# every symbol include/srt_hip.h declares (checked by the CPU tests)
+EXPORTED = [
+    "srt_create", "srt_destroy", "srt_last_error", "srt_set_option", "srt_hbm_used",
+    "srt_scene_upload", "srt_scene_release", "srt_scene_lights",
+    "srt_scene_build_bvh", "srt_scene_bvh_info", "srt_scene_bvh_download",
+    "srt_scene_set_skin", "srt_scene_skin_refit", "srt_scene_download_vertices",
+    "srt_trace_rays", "srt_render", "srt_film_download", "srt_tonemap", "srt_accumulate",
+    "srt_stats", "srt_stats_reset", "srt_last_counts",
+]
+
+TONEMAP = {"None": 0, "Clamp": 1, "Reinhard": 2, "Uncharted2": 3, "ACES": 4}
+
+
+class SrtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("num_vertices", C.c_int32), ("positions", C.c_void_p), ("normals", C.c_void_p), ("tangents", C.c_void_p),
+        ("num_triangles", C.c_int32), ("indices", C.c_void_p), ("num_meshes", C.c_int32),
+        ("mesh_first_triangle", C.c_void_p), ("mesh_material", C.c_void_p), ("num_materials", C.c_int32),
+        ("materials", C.c_void_p), ("env_color", C.c_float * 3),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov_deg", C.c_float)]
+
+
+class PtParams(C.Structure):
+    _fields_ = [("samples_per_pixel", C.c_int32), ("max_depth", C.c_int32), ("rr_depth", C.c_int32),
+                ("use_nee", C.c_int32), ("use_mis", C.c_int32), ("seed", C.c_uint32), ("mlp_mode", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class BvhOpts(C.Structure):
+    _fields_ = [("max_leaf", C.c_int32), ("cluster_size", C.c_int32)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} is missing (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    sig = {
+        "srt_create": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(P)]),
+        "srt_destroy": (None, [P]),
+        "srt_last_error": (C.c_char_p, [P]),
+        "srt_set_option": (C.c_int, [P, C.c_char_p, C.c_int64]),
+        "srt_hbm_used": (C.c_size_t, [P]),
+        "srt_scene_upload": (C.c_int, [P, C.POINTER(SceneDesc), C.POINTER(C.c_uint64)]),
+        "srt_scene_release": (C.c_int, [P, C.c_uint64]),
+        "srt_scene_lights": (C.c_int, [P, C.c_uint64, P, P, P, P, P]),
+        "srt_scene_build_bvh": (C.c_int, [P, C.c_uint64, C.POINTER(BvhOpts)]),
+        "srt_scene_bvh_info": (C.c_int, [P, C.c_uint64, P]),
+        "srt_scene_bvh_download": (C.c_int, [P, C.c_uint64, P, P, P]),
+        "srt_scene_set_skin": (C.c_int, [P, C.c_uint64, P, P]),
+        "srt_scene_skin_refit": (C.c_int, [P, C.c_uint64, P, C.c_int32]),
+        "srt_scene_download_vertices": (C.c_int, [P, C.c_uint64, P, P, P]),
+        "srt_trace_rays": (C.c_int, [P, C.c_uint64, C.c_int32, P, P, C.c_float, P, C.c_int32, P, P, P, P]),
+        "srt_render": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32, C.c_int32, P,
+                                 C.c_int32]),
+        "srt_film_download": (C.c_int, [P, P, C.c_int64]),
+        "srt_tonemap": (C.c_int, [P, P, C.c_int64, C.c_int32, C.c_float, C.c_int32, C.c_int32]),
+        "srt_accumulate": (C.c_int, [P, P, P, C.c_int64, C.c_int32, C.c_int32]),
+        "srt_stats": (C.c_int, [P, P, C.c_int32, P, P, P, C.c_int32]),
+        "srt_stats_reset": (None, [P]),
+        "srt_last_counts": (C.c_int, [P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """Owns one srt_ctx (device memory, stream, resident scenes)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        r = self.lib.srt_create(device, hbm_budget, C.byref(h))
+        if r != SRT_OK:
+            raise SrtError(r, "srt_create failed")
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            self.lib.srt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, r):
+        if r < 0:
+            raise SrtError(r, self.lib.srt_last_error(self.h).decode())
+        return r
+
+    def set_option(self, key, value):
+        self.check(self.lib.srt_set_option(self.h, key.encode(), int(value)))
+
+    def hbm_used(self):
+        return int(self.lib.srt_hbm_used(self.h))
+
+    # ---- scenes ----
+    def upload(self, s: SceneData) -> int:
+        arrs = dict(
+            pos=np.ascontiguousarray(s.positions, np.float32),
+            idx=np.ascontiguousarray(s.indices, np.uint32),
+            mf=np.ascontiguousarray(s.mesh_first, np.int32),
+            mm=np.ascontiguousarray(s.mesh_material, np.int32),
+            mats=np.ascontiguousarray(s.materials, MAT_DTYPE),
+            nrm=None if s.normals is None else np.ascontiguousarray(s.normals, np.float32),
+            tan=None if s.tangents is None else np.ascontiguousarray(s.tangents, np.float32),
+        )
+        d = SceneDesc()
+        d.num_vertices = arrs["pos"].shape[0]
+        d.positions = _ptr(arrs["pos"])
+        d.normals = _ptr(arrs["nrm"])
+        d.tangents = _ptr(arrs["tan"])
+        d.num_triangles = arrs["idx"].shape[0]
+        d.indices = _ptr(arrs["idx"])
+        d.num_meshes = arrs["mm"].shape[0]
+        d.mesh_first_triangle = _ptr(arrs["mf"])
+        d.mesh_material = _ptr(arrs["mm"])
+        d.num_materials = arrs["mats"].shape[0]
+        d.materials = _ptr(arrs["mats"])
+        for i in range(3):
+            d.env_color[i] = s.env[i]
+        out = C.c_uint64()
+        self.check(self.lib.srt_scene_upload(self.h, C.byref(d), C.byref(out)))
+        return out.value
+
+    def release(self, scene):
+        self.check(self.lib.srt_scene_release(self.h, scene))
+
+    def lights(self, scene, nf, nmesh):
+        n = self.check(self.lib.srt_scene_lights(self.h, scene, None, None, None, None, None))
+        cdf = np.zeros(max(n, 1), np.float32)
+        mesh = np.zeros(max(n, 1), np.int32)
+        acdf = np.zeros(nf, np.float32)
+        marea = np.zeros(nmesh, np.float32)
+        ws = C.c_float()
+        self.check(self.lib.srt_scene_lights(self.h, scene, _ptr(cdf), _ptr(mesh), _ptr(acdf), _ptr(marea), C.byref(ws)))
+        return cdf[:n], mesh[:n], acdf, marea, ws.value
+
+    def build_bvh(self, scene, max_leaf=4, cluster_size=0):
+        o = BvhOpts(max_leaf, cluster_size)
+        self.check(self.lib.srt_scene_build_bvh(self.h, scene, C.byref(o)))
+
+    def bvh_info(self, scene):
+        info = np.zeros(8, np.int32)
+        self.check(self.lib.srt_scene_bvh_info(self.h, scene, _ptr(info)))
+        return dict(num_nodes=int(info[0]), root=int(info[1]), max_depth=int(info[2]), clusters=int(info[3]),
+                    top_nodes=int(info[4]), cluster_size=int(info[5]), triangles=int(info[6]), max_leaf=int(info[7]))
+
+    def bvh_download(self, scene):
+        info = self.bvh_info(scene)
+        nodes = np.zeros((info["num_nodes"], 16), np.uint32)
+        order = np.zeros(info["triangles"], np.uint32)
+        tris = np.zeros((info["triangles"], 12), np.float32)
+        self.check(self.lib.srt_scene_bvh_download(self.h, scene, _ptr(nodes), _ptr(order), _ptr(tris)))
+        return nodes, order, tris, info
+
+    def set_skin(self, scene, bone_ids, bone_weights):
+        bi = np.ascontiguousarray(bone_ids, np.int32)
+        bw = np.ascontiguousarray(bone_weights, np.float32)
+        self.check(self.lib.srt_scene_set_skin(self.h, scene, _ptr(bi), _ptr(bw)))
+
+    def skin_refit(self, scene, bones_3x4):
+        b = np.ascontiguousarray(bones_3x4, np.float32)
+        self.check(self.lib.srt_scene_skin_refit(self.h, scene, _ptr(b), b.shape[0]))
+
+    def download_vertices(self, scene, nv, normals=False, tangents=False):
+        p = np.zeros((nv, 3), np.float32)
+        n = np.zeros((nv, 3), np.float32) if normals else None
+        t = np.zeros((nv, 3), np.float32) if tangents else None
+        self.check(self.lib.srt_scene_download_vertices(self.h, scene, _ptr(p), _ptr(n), _ptr(t)))
+        return p, n, t
+
+    # ---- rays ----
+    def trace(self, scene, origins, dirs, tmin=1e-3, tmax=None, any_hit=False):
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = o.shape[0]
+        tm = np.full(n, 1e16, np.float32) if tmax is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(tmax, np.float32), (n,)))
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.int32)
+        self.check(self.lib.srt_trace_rays(self.h, scene, n, _ptr(o), _ptr(d), C.c_float(tmin), _ptr(tm),
+                                           1 if any_hit else 0, _ptr(t), _ptr(u), _ptr(v), _ptr(prim)))
+        return t, u, v, prim
+
+    # ---- render ----
+    @staticmethod
+    def camera(cam: dict) -> Camera:
+        c = Camera()
+        for i in range(3):
+            c.position[i] = cam["position"][i]
+            c.target[i] = cam["target"][i]
+            c.up[i] = cam["up"][i]
+        c.fov_deg = cam["fov"]
+        return c
+
+    def render(self, scene, cam: dict, width, height, spp, max_depth=16, rr_depth=4, use_nee=1, use_mis=1, seed=0,
+               out=None, device_ptr=None):
+        """Render and return a (H*W, 4) float32 host array, or write to `device_ptr` (device memory)."""
+        p = PtParams(spp, max_depth, rr_depth, use_nee, use_mis, seed, 0, 0)
+        c = self.camera(cam)
+        if device_ptr is not None:
+            self.check(self.lib.srt_render(self.h, scene, C.byref(c), C.byref(p), width, height,
+                                           C.c_void_p(device_ptr), 1))
+            return None
+        if out is None:
+            out = np.zeros((width * height, 4), np.float32)
+        self.check(self.lib.srt_render(self.h, scene, C.byref(c), C.byref(p), width, height, _ptr(out), 0))
+        return out
+
+    def render_resident(self, scene, cam, width, height, spp, max_depth=16, rr_depth=4, use_nee=1, use_mis=1, seed=0):
+        """Render into the context's own device film (no host copy)."""
+        p = PtParams(spp, max_depth, rr_depth, use_nee, use_mis, seed, 0, 0)
+        c = self.camera(cam)
+        self.check(self.lib.srt_render(self.h, scene, C.byref(c), C.byref(p), width, height, None, 0))
+
+    def film(self, npix):
+        out = np.zeros((npix, 4), np.float32)
+        self.check(self.lib.srt_film_download(self.h, _ptr(out), npix))
+        return out
+
+    def tonemap(self, rgba, type="None", exposure=0.0, use_gamma=True):
+        a = np.ascontiguousarray(rgba, np.float32).copy()
+        t = TONEMAP[type] if isinstance(type, str) else int(type)
+        self.check(self.lib.srt_tonemap(self.h, _ptr(a), a.shape[0], t, C.c_float(exposure), 1 if use_gamma else 0, 0))
+        return a
+
+    def accumulate(self, acc, rgba, frame_count):
+        a = np.ascontiguousarray(acc, np.float32).copy()
+        p = np.ascontiguousarray(rgba, np.float32).copy()
+        self.check(self.lib.srt_accumulate(self.h, _ptr(a), _ptr(p), a.shape[0], frame_count, 0))
+        return a, p
+
+    # ---- statistics ----
+    def stats(self):
+        mx = 64
+        names = C.create_string_buffer(4096)
+        ms = np.zeros(mx, np.float64)
+        la = np.zeros(mx, np.int64)
+        it = np.zeros(mx, np.int64)
+        k = self.check(self.lib.srt_stats(self.h, names, 4096, _ptr(ms), _ptr(la), _ptr(it), mx))
+        raw = names.raw.split(b"\0")
+        return {raw[i].decode(): dict(ms=float(ms[i]), launches=int(la[i]), items=int(it[i])) for i in range(k)}
+
+    def stats_reset(self):
+        self.lib.srt_stats_reset(self.h)
+
+    def last_counts(self):
+        a = C.c_int64()
+        b = C.c_int64()
+        self.check(self.lib.srt_last_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value

@@ -1,0 +1,828 @@
+// GPU BVH construction, refit and linear blend skinning for gfx950.
+//
+// Replaces OptiX's acceleration-structure build and update
+// (src/optix/optix_ray_tracer.cu:213-359 build, :443-463 update) and the
+// skinning lambda of Scene::update_gscene (src/scene/scene.cu:336-395).
+//
+// Build (deterministic: every decision uses integers, an exact total-order
+// min/max on floats, or float formulas evaluated in a fixed order, so the
+// bytes are reproducible by a host execution of the same steps):
+//   1. triangle boxes, box centroids, centroid bounds (two-level reduction)
+//   2. 63-bit Morton codes of centroids, stable radix sort of (code, id)
+//   3. Karras (HPG 2012) binary radix tree, index tie-break for equal codes
+//   4. bottom-up pass with per-node arrival counters: boxes, primitive counts,
+//      SAH costs, collapse-to-leaf decisions (leaves of <= max_leaf triangles)
+//   5. frontier of the highest subtrees with <= T primitives ("clusters")
+//   6. top of the tree rebuilt with a 16-bin SAH sweep over the clusters, in one
+//      workgroup, breadth-first node numbering
+//   7. emit 64-byte nodes (two child boxes per node) and the triangles in
+//      Morton order (48 bytes each, primitive and mesh id in the w lanes)
+// Refit keeps the topology and recomputes child boxes bottom-up.
+#include "srt_internal.h"
+#include <hipcub/hipcub.hpp>
+#include <cstdio>
+#include <algorithm>
+
+namespace srt {
+
+// total order on floats (-0 < +0): key, inverse, min/max
+SRT_D uint32_t fkey(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+SRT_D float fromkey(uint32_t k) { return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
+SRT_D float kmin(float a, float b) { return (fkey(b) < fkey(a)) ? b : a; }
+SRT_D float kmax(float a, float b) { return (fkey(b) > fkey(a)) ? b : a; }
+
+struct DBox { float lo[3], hi[3]; };
+SRT_D DBox empty_box() {
+    DBox b;
+    for (int i = 0; i < 3; i++) { b.lo[i] = __builtin_huge_valf(); b.hi[i] = -__builtin_huge_valf(); }
+    return b;
+}
+SRT_D void grow(DBox& b, const DBox& o) {
+    for (int i = 0; i < 3; i++) { b.lo[i] = kmin(b.lo[i], o.lo[i]); b.hi[i] = kmax(b.hi[i], o.hi[i]); }
+}
+SRT_D float box_area(const DBox& b) {
+    float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    if (!(dx >= 0.0f)) return 0.0f;
+    return 2.0f * ((dx * dy + dy * dz) + dz * dx);
+}
+
+// SoA box storage
+struct BoxArr { float *lox, *loy, *loz, *hix, *hiy, *hiz; };
+SRT_D DBox ldbox(const BoxArr& a, size_t i) {
+    DBox b;
+    b.lo[0] = a.lox[i]; b.lo[1] = a.loy[i]; b.lo[2] = a.loz[i];
+    b.hi[0] = a.hix[i]; b.hi[1] = a.hiy[i]; b.hi[2] = a.hiz[i];
+    return b;
+}
+SRT_D void stbox(const BoxArr& a, size_t i, const DBox& b) {
+    a.lox[i] = b.lo[0]; a.loy[i] = b.lo[1]; a.loz[i] = b.lo[2];
+    a.hix[i] = b.hi[0]; a.hiy[i] = b.hi[1]; a.hiz[i] = b.hi[2];
+}
+
+// ---------------------------------------------------------------------------
+// 1. triangle boxes + centroid bounds
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_tri_boxes(const float* pos, const uint32_t* idx, int n, BoxArr tb, float* cen,
+                                                   uint32_t* partial) {
+    __shared__ uint32_t red[6][256];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    DBox cb = empty_box();
+    if (i < n) {
+        DBox b = empty_box();
+        for (int v = 0; v < 3; v++) {
+            const float* p = pos + 3 * (size_t)idx[3 * (size_t)i + v];
+            for (int a = 0; a < 3; a++) { b.lo[a] = kmin(b.lo[a], p[a]); b.hi[a] = kmax(b.hi[a], p[a]); }
+        }
+        stbox(tb, i, b);
+        for (int a = 0; a < 3; a++) {
+            float c = (b.lo[a] + b.hi[a]) * 0.5f;
+            cen[3 * (size_t)i + a] = c;
+            cb.lo[a] = c; cb.hi[a] = c;
+        }
+    }
+    for (int a = 0; a < 3; a++) { red[a][threadIdx.x] = fkey(cb.lo[a]); red[3 + a][threadIdx.x] = fkey(cb.hi[a]); }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int a = 0; a < 3; a++) {
+                red[a][threadIdx.x] = min(red[a][threadIdx.x], red[a][threadIdx.x + s]);
+                red[3 + a][threadIdx.x] = max(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[6 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_reduce_bounds(const uint32_t* partial, int nblocks, uint32_t* out6) {
+    __shared__ uint32_t red[6][256];
+    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int a = 0; a < 3; a++) { lo[a] = min(lo[a], partial[6 * (size_t)b + a]); hi[a] = max(hi[a], partial[6 * (size_t)b + 3 + a]); }
+    for (int a = 0; a < 3; a++) { red[a][threadIdx.x] = lo[a]; red[3 + a][threadIdx.x] = hi[a]; }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int a = 0; a < 3; a++) {
+                red[a][threadIdx.x] = min(red[a][threadIdx.x], red[a][threadIdx.x + s]);
+                red[3 + a][threadIdx.x] = max(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out6[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// 2. Morton codes (21 bits per axis, x in the highest bit of each triple)
+// ---------------------------------------------------------------------------
+SRT_D uint64_t expand_bits21(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffULL;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+    x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+    x = (x | (x << 2)) & 0x1249249249249249ULL;
+    return x;
+}
+SRT_D uint32_t morton_axis(float c, float lo, float ext) {
+    float t = (ext > 0.0f) ? (c - lo) / ext : 0.0f;
+    float q = t * 2097152.0f;
+    return (q >= 2097151.0f) ? 2097151u : (q > 0.0f ? (uint32_t)q : 0u);
+}
+__global__ void __launch_bounds__(256) k_morton(const float* cen, const uint32_t* cb6, int n, uint64_t* keys, uint32_t* vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t q[3];
+    for (int a = 0; a < 3; a++) {
+        float lo = fromkey(cb6[a]), hi = fromkey(cb6[3 + a]);
+        q[a] = morton_axis(cen[3 * (size_t)i + a], lo, hi - lo);
+    }
+    keys[i] = (expand_bits21(q[0]) << 2) | (expand_bits21(q[1]) << 1) | expand_bits21(q[2]);
+    vals[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// 3. Karras radix tree
+// ---------------------------------------------------------------------------
+SRT_D int kdelta(const uint64_t* c, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = c[i], b = c[j];
+    if (a != b) return __clzll((long long)(a ^ b));
+    return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+}
+// child refs: >= 0 internal node, < 0 leaf (-k-1)
+__global__ void __launch_bounds__(256) k_karras(const uint64_t* sc, int n, int32_t* kl, int32_t* kr, int32_t* kfirst,
+                                                int32_t* par_int, int32_t* par_leaf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (kdelta(sc, n, i, i + 1) - kdelta(sc, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = kdelta(sc, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(sc, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (kdelta(sc, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = kdelta(sc, n, i, j);
+    int sp = 0, t = l;
+    do {
+        t = (t + 1) / 2;
+        if (kdelta(sc, n, i, i + (sp + t) * d) > dnode) sp += t;
+    } while (t > 1);
+    int gamma = i + sp * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    int32_t left = (lo == gamma) ? -(gamma + 1) : gamma;
+    int32_t right = (hi == gamma + 1) ? -(gamma + 2) : gamma + 1;
+    kl[i] = left;
+    kr[i] = right;
+    kfirst[i] = lo;
+    if (left < 0) par_leaf[-left - 1] = i; else par_int[left] = i;
+    if (right < 0) par_leaf[-right - 1] = i; else par_int[right] = i;
+}
+
+// ---------------------------------------------------------------------------
+// 4. bottom-up boxes / counts / SAH costs / collapse
+// ---------------------------------------------------------------------------
+struct KArr {
+    BoxArr box;
+    int32_t* count;
+    float* cost;
+    int32_t* collapse;
+};
+__global__ void __launch_bounds__(256) k_bottom_up(int n, const uint32_t* order, BoxArr tb, const int32_t* kl, const int32_t* kr,
+                                                   const int32_t* par_int, const int32_t* par_leaf, KArr K, int32_t* flags,
+                                                   int max_leaf, float c_trav, float c_isect) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int p = par_leaf[k];
+    while (p >= 0) {
+        __threadfence();
+        int old = atomicAdd(&flags[p], 1);
+        if (old == 0) return;
+        __threadfence();
+        DBox bl, br;
+        int cl, cr;
+        float costl, costr;
+        int32_t L = kl[p], R = kr[p];
+        if (L < 0) { int q = -L - 1; bl = ldbox(tb, order[q]); cl = 1; costl = c_isect * 1.0f * box_area(bl); }
+        else { bl = ldbox(K.box, L); cl = K.count[L]; costl = K.cost[L]; }
+        if (R < 0) { int q = -R - 1; br = ldbox(tb, order[q]); cr = 1; costr = c_isect * 1.0f * box_area(br); }
+        else { br = ldbox(K.box, R); cr = K.count[R]; costr = K.cost[R]; }
+        DBox b = bl;
+        grow(b, br);
+        int cnt = cl + cr;
+        float sa = box_area(b);
+        float ci = c_trav * sa + costl + costr;
+        float cf = c_isect * (float)cnt * sa;
+        int col = (cnt <= max_leaf) && (cf <= ci);
+        stbox(K.box, p, b);
+        K.count[p] = cnt;
+        K.cost[p] = col ? cf : ci;
+        K.collapse[p] = col;
+        p = par_int[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 5. cluster frontier (marks the first sorted position of each cluster)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_frontier(int n, int T, const int32_t* kfirst, const int32_t* kcount, const int32_t* par_int,
+                                                  const int32_t* par_leaf, int32_t* mark) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    // x < n-1: internal node x; n-1 <= x < 2n-1: leaf x-(n-1)
+    if (x >= 2 * n - 1) return;
+    int32_t ref, first, cnt, parent;
+    if (x < n - 1) { ref = x; first = kfirst[x]; cnt = kcount[x]; parent = par_int[x]; }
+    else { int k = x - (n - 1); ref = -(k + 1); first = k; cnt = 1; parent = par_leaf[k]; }
+    if (cnt > T) return;
+    if (parent >= 0 && kcount[parent] <= T) return;
+    mark[first] = ref;
+}
+
+// cluster items for the top-level SAH
+struct Items {
+    BoxArr box;
+    float *cx, *cy, *cz;
+    int32_t* count;
+    int32_t* ref;
+};
+__global__ void __launch_bounds__(256) k_items(int m, const int32_t* cl, const uint32_t* order, BoxArr tb, KArr K,
+                                               const int32_t* kfirst, int top, Items it) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    int32_t c = cl[q];
+    DBox b;
+    int cnt;
+    int32_t ref;
+    if (c < 0) {
+        int k = -c - 1;
+        b = ldbox(tb, order[k]); cnt = 1; ref = leaf_ref((uint32_t)k, 1u);
+    } else {
+        b = ldbox(K.box, c); cnt = K.count[c];
+        ref = K.collapse[c] ? leaf_ref((uint32_t)kfirst[c], (uint32_t)cnt) : top + c;
+    }
+    stbox(it.box, q, b);
+    it.cx[q] = (b.lo[0] + b.hi[0]) * 0.5f;
+    it.cy[q] = (b.lo[1] + b.hi[1]) * 0.5f;
+    it.cz[q] = (b.lo[2] + b.hi[2]) * 0.5f;
+    it.count[q] = cnt;
+    it.ref[q] = ref;
+}
+
+SRT_D void set_child(BvhNode* nd, int side, const DBox& b, int32_t ref) {
+    if (side == 0) {
+        nd->a.x = b.lo[0]; nd->a.y = b.hi[0]; nd->a.z = b.lo[1]; nd->a.w = b.hi[1];
+        nd->c.x = b.lo[2]; nd->c.y = b.hi[2];
+        nd->d.x = ref;
+    } else {
+        nd->b.x = b.lo[0]; nd->b.y = b.hi[0]; nd->b.z = b.lo[1]; nd->b.w = b.hi[1];
+        nd->c.z = b.lo[2]; nd->c.w = b.hi[2];
+        nd->d.y = ref;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 6. top-level binned SAH, one workgroup, breadth-first
+// ---------------------------------------------------------------------------
+constexpr int kTopThreads = 1024;
+constexpr int kBins = 16;
+
+SRT_D int bin_of(float c, float lo, float scale) {
+    int bi = (int)((c - lo) * scale);
+    if (bi > kBins - 1) bi = kBins - 1;
+    if (bi < 0) bi = 0;
+    return bi;
+}
+SRT_D float item_c(const Items& it, int a, int i) { return a == 0 ? it.cx[i] : (a == 1 ? it.cy[i] : it.cz[i]); }
+SRT_D void copy_item(const Items& s, int i, const Items& d, int j) {
+    d.box.lox[j] = s.box.lox[i]; d.box.loy[j] = s.box.loy[i]; d.box.loz[j] = s.box.loz[i];
+    d.box.hix[j] = s.box.hix[i]; d.box.hiy[j] = s.box.hiy[i]; d.box.hiz[j] = s.box.hiz[i];
+    d.cx[j] = s.cx[i]; d.cy[j] = s.cy[i]; d.cz[j] = s.cz[i];
+    d.count[j] = s.count[i]; d.ref[j] = s.ref[i];
+}
+
+__global__ void __launch_bounds__(kTopThreads) k_top_sah(int m, Items it, Items tmp, int32_t* task_b, int32_t* task_e,
+                                                         int32_t* task_node, BvhNode* nodes) {
+    __shared__ uint32_t s_cb[6];
+    __shared__ uint32_t s_bin[3][kBins][6];
+    __shared__ int32_t s_cnt[3][kBins];
+    __shared__ int32_t s_scan[kTopThreads];
+    __shared__ uint32_t s_box[2][6];
+    __shared__ int32_t s_head, s_tail, s_next, s_axis, s_split, s_mid, s_run;
+    __shared__ float s_lo[3], s_ext[3];
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        task_b[0] = 0; task_e[0] = m; task_node[0] = 0;
+        s_head = 0; s_tail = 1; s_next = 1;
+    }
+    __syncthreads();
+    for (;;) {
+        __syncthreads();
+        if (s_head == s_tail) break;
+        const int b = task_b[s_head], e = task_e[s_head], node = task_node[s_head];
+        if (tid < 6) s_cb[tid] = (tid < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
+        for (int j = tid; j < 3 * kBins * 6; j += kTopThreads)
+            (&s_bin[0][0][0])[j] = ((j % 6) < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
+        for (int j = tid; j < 3 * kBins; j += kTopThreads) (&s_cnt[0][0])[j] = 0;
+        if (tid < 12) (&s_box[0][0])[tid] = ((tid % 6) < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
+        __syncthreads();
+        // centroid bounds
+        for (int i = b + tid; i < e; i += kTopThreads) {
+            for (int a = 0; a < 3; a++) {
+                uint32_t k = fkey(item_c(it, a, i));
+                atomicMin(&s_cb[a], k);
+                atomicMax(&s_cb[3 + a], k);
+            }
+        }
+        __syncthreads();
+        if (tid < 3) {
+            float lo = fromkey(s_cb[tid]), hi = fromkey(s_cb[3 + tid]);
+            s_lo[tid] = lo;
+            s_ext[tid] = hi - lo;
+        }
+        __syncthreads();
+        // bins for the three axes
+        for (int i = b + tid; i < e; i += kTopThreads) {
+            DBox ib = ldbox(it.box, i);
+            int cnt = it.count[i];
+            for (int a = 0; a < 3; a++) {
+                float ex = s_ext[a];
+                if (!(ex > 0.0f)) continue;
+                float scale = (float)kBins / ex;
+                int bi = bin_of(item_c(it, a, i), s_lo[a], scale);
+                for (int q = 0; q < 3; q++) {
+                    atomicMin(&s_bin[a][bi][q], fkey(ib.lo[q]));
+                    atomicMax(&s_bin[a][bi][3 + q], fkey(ib.hi[q]));
+                }
+                atomicAdd(&s_cnt[a][bi], cnt);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float best = __builtin_huge_valf();
+            int best_axis = -1, best_split = -1;
+            for (int a = 0; a < 3; a++) {
+                if (!(s_ext[a] > 0.0f)) continue;
+                float la[kBins], ra[kBins];
+                int lc[kBins], rc[kBins];
+                DBox acc = empty_box();
+                int cnt = 0;
+                for (int k = 0; k < kBins; k++) {
+                    DBox bb;
+                    for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[a][k][q]); bb.hi[q] = fromkey(s_bin[a][k][3 + q]); }
+                    grow(acc, bb);
+                    cnt += s_cnt[a][k];
+                    la[k] = box_area(acc);
+                    lc[k] = cnt;
+                }
+                acc = empty_box();
+                cnt = 0;
+                for (int k = kBins - 1; k >= 0; k--) {
+                    DBox bb;
+                    for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[a][k][q]); bb.hi[q] = fromkey(s_bin[a][k][3 + q]); }
+                    grow(acc, bb);
+                    cnt += s_cnt[a][k];
+                    ra[k] = box_area(acc);
+                    rc[k] = cnt;
+                }
+                for (int k = 0; k < kBins - 1; k++) {
+                    if (lc[k] == 0 || rc[k + 1] == 0) continue;
+                    float cost = la[k] * (float)lc[k] + ra[k + 1] * (float)rc[k + 1];
+                    if (cost < best) { best = cost; best_axis = a; best_split = k; }
+                }
+            }
+            s_axis = best_axis;
+            s_split = best_split;
+            s_run = 0;
+        }
+        __syncthreads();
+        const int axis = s_axis, split = s_split;
+        int mid;
+        if (axis < 0) {
+            mid = (b + e) / 2;
+        } else {
+            const float scale = (float)kBins / s_ext[axis];
+            const float lo = s_lo[axis];
+            for (int pass = 0; pass < 2; pass++) {
+                const int base = (pass == 0) ? b : b + s_run;   // s_run holds nleft after pass 0
+                __syncthreads();
+                int run = 0;
+                for (int start = b; start < e; start += kTopThreads) {
+                    int i = start + tid;
+                    int flag = 0;
+                    if (i < e) {
+                        bool left = bin_of(item_c(it, axis, i), lo, scale) <= split;
+                        flag = (pass == 0) ? (left ? 1 : 0) : (left ? 0 : 1);
+                    }
+                    s_scan[tid] = flag;
+                    __syncthreads();
+                    for (int off = 1; off < kTopThreads; off <<= 1) {
+                        int v = (tid >= off) ? s_scan[tid - off] : 0;
+                        __syncthreads();
+                        s_scan[tid] += v;
+                        __syncthreads();
+                    }
+                    if (flag) copy_item(it, i, tmp, base + run + s_scan[tid] - 1);
+                    run += s_scan[kTopThreads - 1];
+                    __syncthreads();
+                }
+                if (pass == 0 && tid == 0) s_run = run;
+                __syncthreads();
+            }
+            mid = b + s_run;
+            __syncthreads();
+            for (int i = b + tid; i < e; i += kTopThreads) copy_item(tmp, i, it, i);
+            __syncthreads();
+        }
+        // child boxes
+        for (int i = b + tid; i < e; i += kTopThreads) {
+            int side = (i < mid) ? 0 : 1;
+            DBox ib = ldbox(it.box, i);
+            for (int q = 0; q < 3; q++) {
+                atomicMin(&s_box[side][q], fkey(ib.lo[q]));
+                atomicMax(&s_box[side][3 + q], fkey(ib.hi[q]));
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            BvhNode* nd = nodes + node;
+            int rb[2] = {b, mid}, re[2] = {mid, e};
+            for (int side = 0; side < 2; side++) {
+                DBox bx;
+                for (int q = 0; q < 3; q++) { bx.lo[q] = fromkey(s_box[side][q]); bx.hi[q] = fromkey(s_box[side][3 + q]); }
+                int32_t ref;
+                if (re[side] - rb[side] == 1) ref = it.ref[rb[side]];
+                else {
+                    ref = s_next++;
+                    task_b[s_tail] = rb[side]; task_e[s_tail] = re[side]; task_node[s_tail] = ref;
+                    s_tail++;
+                }
+                set_child(nd, side, bx, ref);
+            }
+            s_head++;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 7. emit the Karras nodes that remain, triangles, parents, depth
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_emit(int n, int T, int top, int max_leaf, const int32_t* kl, const int32_t* kr,
+                                              const int32_t* kfirst, const int32_t* par_int, const uint32_t* order, BoxArr tb,
+                                              KArr K, BvhNode* nodes, int32_t* valid) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    if (K.count[i] > T || K.collapse[i]) return;
+    for (int p = par_int[i]; p >= 0 && K.count[p] <= max_leaf; p = par_int[p])
+        if (K.collapse[p]) return;
+    BvhNode* nd = nodes + top + i;
+    int32_t ch[2] = {kl[i], kr[i]};
+    for (int side = 0; side < 2; side++) {
+        int32_t c = ch[side];
+        DBox b;
+        int32_t ref;
+        if (c < 0) { int k = -c - 1; b = ldbox(tb, order[k]); ref = leaf_ref((uint32_t)k, 1u); }
+        else {
+            b = ldbox(K.box, c);
+            ref = K.collapse[c] ? leaf_ref((uint32_t)kfirst[c], (uint32_t)K.count[c]) : top + c;
+        }
+        set_child(nd, side, b, ref);
+    }
+    valid[top + i] = 1;
+}
+
+__global__ void __launch_bounds__(256) k_tris(int n, const uint32_t* order, const float* pos, const uint32_t* idx,
+                                              const int32_t* tri_mesh, float4* tris) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t p = order[k];
+    const uint32_t* ix = idx + 3 * (size_t)p;
+    const float* a = pos + 3 * (size_t)ix[0];
+    const float* b = pos + 3 * (size_t)ix[1];
+    const float* c = pos + 3 * (size_t)ix[2];
+    tris[3 * (size_t)k + 0] = make_float4(a[0], a[1], a[2], __uint_as_float(p));
+    tris[3 * (size_t)k + 1] = make_float4(b[0], b[1], b[2], __int_as_float(tri_mesh[p]));
+    tris[3 * (size_t)k + 2] = make_float4(c[0], c[1], c[2], 0.0f);
+}
+
+// parents in the final tree: parent index * 2 + side, -1 for the root
+__global__ void __launch_bounds__(256) k_final_parents(int num_nodes, const BvhNode* nodes, const int32_t* valid, int32_t* parent) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_nodes || !valid[i]) return;
+    int4 d = nodes[i].d;
+    if (d.x >= 0) parent[d.x] = 2 * i;
+    if (d.y >= 0) parent[d.y] = 2 * i + 1;
+}
+
+__global__ void __launch_bounds__(256) k_depth(int num_nodes, const BvhNode* nodes, const int32_t* valid, const int32_t* parent,
+                                               int32_t* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_nodes || !valid[i]) return;
+    int4 d = nodes[i].d;
+    if (d.x >= 0 && d.y >= 0) return;      // only nodes holding a leaf end a path
+    int depth = 1;
+    for (int p = parent[i]; p >= 0; p = parent[p >> 1]) depth++;
+    atomicMax(out, depth);
+}
+
+// ---------------------------------------------------------------------------
+// refit: leaf boxes from the current triangles, then boxes bottom-up
+// ---------------------------------------------------------------------------
+SRT_D DBox leaf_box(const float4* tris, int32_t ref) {
+    DBox b = empty_box();
+    uint32_t f = leaf_first(ref), c = leaf_count(ref);
+    for (uint32_t k = f; k < f + c; k++)
+        for (int v = 0; v < 3; v++) {
+            float4 p = tris[3 * (size_t)k + v];
+            b.lo[0] = kmin(b.lo[0], p.x); b.hi[0] = kmax(b.hi[0], p.x);
+            b.lo[1] = kmin(b.lo[1], p.y); b.hi[1] = kmax(b.hi[1], p.y);
+            b.lo[2] = kmin(b.lo[2], p.z); b.hi[2] = kmax(b.hi[2], p.z);
+        }
+    return b;
+}
+SRT_D DBox node_union(const BvhNode* nd) {
+    DBox l, r;
+    float4 a = nd->a, b = nd->b, c = nd->c;
+    l.lo[0] = a.x; l.hi[0] = a.y; l.lo[1] = a.z; l.hi[1] = a.w; l.lo[2] = c.x; l.hi[2] = c.y;
+    r.lo[0] = b.x; r.hi[0] = b.y; r.lo[1] = b.z; r.hi[1] = b.w; r.lo[2] = c.z; r.hi[2] = c.w;
+    grow(l, r);
+    return l;
+}
+__global__ void __launch_bounds__(256) k_refit(int num_nodes, BvhNode* nodes, const int32_t* valid, const int32_t* parent,
+                                               const float4* tris, int32_t* flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_nodes || !valid[i]) return;
+    BvhNode* nd = nodes + i;
+    int4 d = nd->d;
+    int arrivals = 0;
+    if (d.x < 0) { set_child(nd, 0, leaf_box(tris, d.x), d.x); arrivals++; }
+    if (d.y < 0) { set_child(nd, 1, leaf_box(tris, d.y), d.y); arrivals++; }
+    if (arrivals == 0) return;
+    __threadfence();
+    int old = atomicAdd(&flags[i], arrivals);
+    if (old + arrivals < 2) return;
+    int x = i;
+    for (;;) {
+        __threadfence();
+        DBox u = node_union(nodes + x);
+        int p = parent[x];
+        if (p < 0) return;
+        int pn = p >> 1, side = p & 1;
+        BvhNode* pd = nodes + pn;
+        set_child(pd, side, u, side ? pd->d.y : pd->d.x);
+        __threadfence();
+        old = atomicAdd(&flags[pn], 1);
+        if (old + 1 < 2) return;
+        x = pn;
+    }
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// scene.cu:362-392: skinned position / normal / tangent from up to 4 bones
+__global__ void __launch_bounds__(256) k_skin(int nv, const float* op, const float* on, const float* ot, const int32_t* bid,
+                                              const float* bw, int nb, const float* bones, float* pos, float* nrm, float* tan) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    f3 v = mk(0.0f), nn = mk(0.0f), tt = mk(0.0f);
+    f3 ov = ld3(op + 3 * (size_t)i);
+    f3 onn = on ? ld3(on + 3 * (size_t)i) : mk(0.0f);
+    f3 ott = ot ? ld3(ot + 3 * (size_t)i) : mk(0.0f);
+    for (int j = 0; j < 4; j++) {
+        int id = bid[4 * (size_t)i + j];
+        float w = bw[4 * (size_t)i + j];
+        if (id == -1) {
+            if (j == 0) { v = ov; nn = onn; tt = ott; }
+            break;
+        }
+        if (id >= nb) continue;
+        const float* m = bones + 12 * (size_t)id;
+        f3 pp = mk(m[0] * ov.x + m[1] * ov.y + m[2] * ov.z + m[3], m[4] * ov.x + m[5] * ov.y + m[6] * ov.z + m[7],
+                   m[8] * ov.x + m[9] * ov.y + m[10] * ov.z + m[11]);
+        v = v + pp * w;
+        f3 pn = mk(m[0] * onn.x + m[1] * onn.y + m[2] * onn.z, m[4] * onn.x + m[5] * onn.y + m[6] * onn.z,
+                   m[8] * onn.x + m[9] * onn.y + m[10] * onn.z);
+        nn = nn + pn * w;
+        f3 pt = mk(m[0] * ott.x + m[1] * ott.y + m[2] * ott.z, m[4] * ott.x + m[5] * ott.y + m[6] * ott.z,
+                   m[8] * ott.x + m[9] * ott.y + m[10] * ott.z);
+        tt = tt + pt * w;
+    }
+    pos[3 * (size_t)i] = v.x; pos[3 * (size_t)i + 1] = v.y; pos[3 * (size_t)i + 2] = v.z;
+    if (nrm) { nrm[3 * (size_t)i] = nn.x; nrm[3 * (size_t)i + 1] = nn.y; nrm[3 * (size_t)i + 2] = nn.z; }
+    if (tan) { tan[3 * (size_t)i] = tt.x; tan[3 * (size_t)i + 1] = tt.y; tan[3 * (size_t)i + 2] = tt.z; }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+struct Tmp {
+    srt_ctx* c;
+    std::vector<DevBuf> bufs;
+    explicit Tmp(srt_ctx* cc) : c(cc) {}
+    ~Tmp() { for (auto& b : bufs) dev_free(c, b); }
+    template <class T> T* get(size_t count) {
+        bufs.emplace_back();
+        if (dev_alloc(c, bufs.back(), std::max<size_t>(count, 1) * sizeof(T))) return nullptr;
+        return (T*)bufs.back().p;
+    }
+};
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
+}  // namespace
+
+int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
+    if (max_leaf < 1 || max_leaf > kMaxLeaf) return set_err(c, SRT_E_LIMIT, "max_leaf must be in [1, %d]", kMaxLeaf);
+    const int n = s->nf;
+    if (n < 1) return set_err(c, SRT_E_INVALID, "scene has no triangles");
+    if ((int64_t)n >= (1ll << 28)) return set_err(c, SRT_E_LIMIT, "too many triangles for the leaf encoding");
+    hipStream_t st = c->stream;
+    const int T = cluster_size > 0 ? cluster_size : std::max(64, n / 1024);
+    const float c_trav = 1.0f, c_isect = 1.0f;
+    Tmp tmp(c);
+    // boxes & centroids
+    BoxArr tb;
+    tb.lox = tmp.get<float>(n); tb.loy = tmp.get<float>(n); tb.loz = tmp.get<float>(n);
+    tb.hix = tmp.get<float>(n); tb.hiy = tmp.get<float>(n); tb.hiz = tmp.get<float>(n);
+    float* cen = tmp.get<float>(3 * (size_t)n);
+    unsigned nb = nblk(n);
+    uint32_t* partial = tmp.get<uint32_t>(6 * (size_t)nb);
+    uint32_t* cb6 = tmp.get<uint32_t>(6);
+    uint64_t* keys = tmp.get<uint64_t>(n);
+    uint64_t* keys2 = tmp.get<uint64_t>(n);
+    uint32_t* vals = tmp.get<uint32_t>(n);
+    int32_t* kl = tmp.get<int32_t>(n);
+    int32_t* kr = tmp.get<int32_t>(n);
+    int32_t* kfirst = tmp.get<int32_t>(n);
+    int32_t* par_int = tmp.get<int32_t>(n);
+    int32_t* par_leaf = tmp.get<int32_t>(n);
+    int32_t* flags = tmp.get<int32_t>(n);
+    KArr K;
+    K.box.lox = tmp.get<float>(n); K.box.loy = tmp.get<float>(n); K.box.loz = tmp.get<float>(n);
+    K.box.hix = tmp.get<float>(n); K.box.hiy = tmp.get<float>(n); K.box.hiz = tmp.get<float>(n);
+    K.count = tmp.get<int32_t>(n); K.cost = tmp.get<float>(n); K.collapse = tmp.get<int32_t>(n);
+    int32_t* tri_mesh = (int32_t*)s->tri_mesh.p;
+    if (!cen || !K.collapse) return set_err(c, SRT_E_OOM, "BVH build scratch exceeds the HBM budget");
+
+    // final outputs
+    dev_free(c, s->order);
+    dev_free(c, s->tris);
+    if (dev_alloc(c, s->order, (size_t)n * 4) || dev_alloc(c, s->tris, (size_t)n * 48)) return SRT_E_OOM;
+    uint32_t* order = (uint32_t*)s->order.p;
+
+    k_tri_boxes<<<nb, 256, 0, st>>>((const float*)s->pos.p, (const uint32_t*)s->idx.p, n, tb, cen, partial);
+    k_reduce_bounds<<<1, 256, 0, st>>>(partial, (int)nb, cb6);
+    k_morton<<<nb, 256, 0, st>>>(cen, cb6, n, keys, vals);
+    {
+        size_t tbytes = 0;
+        hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, keys, keys2, vals, order, n, 0, 63, st);
+        void* tstore = tmp.get<char>(tbytes);
+        if (!tstore) return set_err(c, SRT_E_OOM, "sort scratch");
+        if (int r = hip_check(c, hipcub::DeviceRadixSort::SortPairs(tstore, tbytes, keys, keys2, vals, order, n, 0, 63, st), "radix sort"))
+            return r;
+    }
+    k_fill_i32<<<nblk(n), 256, 0, st>>>(par_int, n, -1);
+    k_fill_i32<<<nblk(n), 256, 0, st>>>(flags, n, 0);
+    if (n > 1) {
+        k_karras<<<nblk(n - 1), 256, 0, st>>>(keys2, n, kl, kr, kfirst, par_int, par_leaf);
+        k_bottom_up<<<nblk(n), 256, 0, st>>>(n, order, tb, kl, kr, par_int, par_leaf, K, flags, max_leaf, c_trav, c_isect);
+    }
+    // host needs a few scalars: root count/collapse
+    int32_t root_count = 1, root_collapse = 0, root_first = 0;
+    if (n > 1) {
+        hip_check(c, hipMemcpyAsync(&root_count, K.count, 4, hipMemcpyDeviceToHost, st), "copy");
+        hip_check(c, hipMemcpyAsync(&root_collapse, K.collapse, 4, hipMemcpyDeviceToHost, st), "copy");
+        hip_check(c, hipMemcpyAsync(&root_first, kfirst, 4, hipMemcpyDeviceToHost, st), "copy");
+    }
+    if (int r = hip_check(c, hipStreamSynchronize(st), "bvh build (sort/tree)")) return r;
+
+    // clusters
+    int m = 1;
+    int32_t* clusters = nullptr;
+    if (n > 1 && n > T) {
+        int32_t* mark = tmp.get<int32_t>(n);
+        clusters = tmp.get<int32_t>(n);
+        int32_t* nsel = tmp.get<int32_t>(1);
+        if (!nsel) return set_err(c, SRT_E_OOM, "cluster scratch");
+        k_fill_i32<<<nblk(n), 256, 0, st>>>(mark, n, 0x7fffffff);
+        k_frontier<<<nblk(2 * (int64_t)n - 1), 256, 0, st>>>(n, T, kfirst, K.count, par_int, par_leaf, mark);
+        struct NotSentinel {
+            __host__ __device__ bool operator()(const int32_t& x) const { return x != 0x7fffffff; }
+        };
+        size_t tbytes = 0;
+        hipcub::DeviceSelect::If(nullptr, tbytes, mark, clusters, nsel, n, NotSentinel(), st);
+        void* tstore = tmp.get<char>(tbytes);
+        if (!tstore) return set_err(c, SRT_E_OOM, "select scratch");
+        if (int r = hip_check(c, hipcub::DeviceSelect::If(tstore, tbytes, mark, clusters, nsel, n, NotSentinel(), st), "select"))
+            return r;
+        hip_check(c, hipMemcpyAsync(&m, nsel, 4, hipMemcpyDeviceToHost, st), "copy");
+        if (int r = hip_check(c, hipStreamSynchronize(st), "cluster select")) return r;
+    }
+    const int top = m - 1;
+    const int num_nodes = std::max(top + std::max(n - 1, 0), 1);
+    dev_free(c, s->nodes);
+    dev_free(c, s->node_parent);
+    dev_free(c, s->refit_counter);
+    dev_free(c, s->node_valid);
+    if (dev_alloc(c, s->nodes, (size_t)num_nodes * sizeof(BvhNode)) || dev_alloc(c, s->node_parent, (size_t)num_nodes * 4) ||
+        dev_alloc(c, s->refit_counter, (size_t)num_nodes * 4) || dev_alloc(c, s->node_valid, (size_t)num_nodes * 4))
+        return SRT_E_OOM;
+    BvhNode* nodes = (BvhNode*)s->nodes.p;
+    int32_t* valid = (int32_t*)s->node_valid.p;
+    k_fill_i32<<<nblk((int64_t)num_nodes * 16), 256, 0, st>>>((int32_t*)nodes, (int64_t)num_nodes * 16, 0);
+    k_fill_i32<<<nblk(num_nodes), 256, 0, st>>>(valid, num_nodes, 0);
+
+    int32_t root;
+    if (m > 1) {
+        Items it, it2;
+        auto mk_items = [&](Items& x) {
+            x.box.lox = tmp.get<float>(m); x.box.loy = tmp.get<float>(m); x.box.loz = tmp.get<float>(m);
+            x.box.hix = tmp.get<float>(m); x.box.hiy = tmp.get<float>(m); x.box.hiz = tmp.get<float>(m);
+            x.cx = tmp.get<float>(m); x.cy = tmp.get<float>(m); x.cz = tmp.get<float>(m);
+            x.count = tmp.get<int32_t>(m); x.ref = tmp.get<int32_t>(m);
+        };
+        mk_items(it);
+        mk_items(it2);
+        int32_t* tb_ = tmp.get<int32_t>(m);
+        int32_t* te_ = tmp.get<int32_t>(m);
+        int32_t* tn_ = tmp.get<int32_t>(m);
+        if (!tn_) return set_err(c, SRT_E_OOM, "top-level scratch");
+        k_items<<<nblk(m), 256, 0, st>>>(m, clusters, order, tb, K, kfirst, top, it);
+        k_top_sah<<<1, kTopThreads, 0, st>>>(m, it, it2, tb_, te_, tn_, nodes);
+        k_fill_i32<<<nblk(top), 256, 0, st>>>(valid, top, 1);
+        root = 0;
+    } else if (n == 1) {
+        root = leaf_ref(0u, 1u);
+    } else {
+        root = root_collapse ? leaf_ref((uint32_t)root_first, (uint32_t)root_count) : top + 0;
+    }
+    if (n > 1)
+        k_emit<<<nblk(n - 1), 256, 0, st>>>(n, T, top, max_leaf, kl, kr, kfirst, par_int, order, tb, K, nodes, valid);
+    k_tris<<<nblk(n), 256, 0, st>>>(n, order, (const float*)s->pos.p, (const uint32_t*)s->idx.p, tri_mesh, (float4*)s->tris.p);
+    int32_t* parent = (int32_t*)s->node_parent.p;
+    k_fill_i32<<<nblk(num_nodes), 256, 0, st>>>(parent, num_nodes, -1);
+    k_final_parents<<<nblk(num_nodes), 256, 0, st>>>(num_nodes, nodes, valid, parent);
+    int32_t* dmax = tmp.get<int32_t>(1);
+    k_fill_i32<<<1, 64, 0, st>>>(dmax, 1, 0);
+    k_depth<<<nblk(num_nodes), 256, 0, st>>>(num_nodes, nodes, valid, parent, dmax);
+    int32_t depth = 0;
+    hip_check(c, hipMemcpyAsync(&depth, dmax, 4, hipMemcpyDeviceToHost, st), "copy");
+    if (int r = hip_check(c, hipGetLastError(), "bvh launch")) return r;
+    if (int r = hip_check(c, hipStreamSynchronize(st), "bvh build")) return r;
+    if (root < 0) depth = 0;
+    if (depth > kMaxDepth) return set_err(c, SRT_E_LIMIT, "BVH depth %d exceeds the traversal limit %d", depth, kMaxDepth);
+    s->root = root;
+    s->num_nodes = num_nodes;
+    s->max_depth = depth;
+    s->num_clusters = m;
+    s->num_top = top;
+    s->cluster_size = T;
+    s->max_leaf = max_leaf;
+    s->has_bvh = true;
+    return SRT_OK;
+}
+
+int bvh_refit(srt_ctx* c, SceneRec* s) {
+    if (!s->has_bvh) return set_err(c, SRT_E_STATE, "refit before build");
+    hipStream_t st = c->stream;
+    const int n = s->nf;
+    k_tris<<<nblk(n), 256, 0, st>>>(n, (const uint32_t*)s->order.p, (const float*)s->pos.p, (const uint32_t*)s->idx.p,
+                                    (const int32_t*)s->tri_mesh.p, (float4*)s->tris.p);
+    if (s->root >= 0) {
+        k_fill_i32<<<nblk(s->num_nodes), 256, 0, st>>>((int32_t*)s->refit_counter.p, s->num_nodes, 0);
+        k_refit<<<nblk(s->num_nodes), 256, 0, st>>>(s->num_nodes, (BvhNode*)s->nodes.p, (const int32_t*)s->node_valid.p,
+                                                    (const int32_t*)s->node_parent.p, (const float4*)s->tris.p,
+                                                    (int32_t*)s->refit_counter.p);
+    }
+    if (int r = hip_check(c, hipGetLastError(), "refit launch")) return r;
+    return hip_check(c, hipStreamSynchronize(st), "refit");
+}
+
+int skin_vertices(srt_ctx* c, SceneRec* s, const float* bones_3x4, int nb) {
+    if (!s->bone_ids.p) return set_err(c, SRT_E_STATE, "no skin weights set on this scene");
+    if (nb < 0 || nb > 4096) return set_err(c, SRT_E_LIMIT, "bone count %d outside [0, 4096]", nb);
+    hipStream_t st = c->stream;
+    if (s->nbones < nb || !s->bones.p) {
+        dev_free(c, s->bones);
+        if (dev_alloc(c, s->bones, (size_t)std::max(nb, 1) * 48)) return SRT_E_OOM;
+        s->nbones = nb;
+    }
+    if (nb > 0)
+        if (int r = hip_check(c, hipMemcpyAsync(s->bones.p, bones_3x4, (size_t)nb * 48, hipMemcpyHostToDevice, st), "bones upload"))
+            return r;
+    hipEvent_t ev;
+    prof_begin(c, "skin", &ev);
+    k_skin<<<nblk(s->nv), 256, 0, st>>>(s->nv, (const float*)s->orig_pos.p, (const float*)s->orig_nrm.p,
+                                        (const float*)s->orig_tan.p, (const int32_t*)s->bone_ids.p, (const float*)s->bone_w.p,
+                                        nb, (const float*)s->bones.p, (float*)s->pos.p, (float*)s->nrm.p, (float*)s->tan.p);
+    prof_end(c, "skin", ev, s->nv);
+    return hip_check(c, hipGetLastError(), "skin launch");
+}
+
+}  // namespace srt

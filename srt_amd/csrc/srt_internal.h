@@ -1,0 +1,190 @@
+// Internal data layout of the SRT HIP library (device buffers in HBM and the
+// host-side bookkeeping around them).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <map>
+#include <mutex>
+#include "dev_math.h"
+#include "../../include/srt_hip.h"
+
+namespace srt {
+
+// ---------------------------------------------------------------------------
+// BVH node: 64 bytes, both children's boxes stored in the parent so one node
+// fetch (four 16-byte loads) tests two boxes.
+//   a = (L.lo.x, L.hi.x, L.lo.y, L.hi.y)
+//   b = (R.lo.x, R.hi.x, R.lo.y, R.hi.y)
+//   c = (L.lo.z, L.hi.z, R.lo.z, R.hi.z)
+//   d = (left ref, right ref, 0, 0)
+// A ref >= 0 is a node index; a ref < 0 is a leaf:
+//   bit 31 set | first sorted triangle << 3 | (count - 1), count <= 8.
+// ---------------------------------------------------------------------------
+struct alignas(16) BvhNode {
+    float4 a, b, c;
+    int4 d;
+};
+static_assert(sizeof(BvhNode) == 64, "node is 64 bytes");
+
+__host__ __device__ inline int32_t leaf_ref(uint32_t first, uint32_t count) {
+    return (int32_t)(0x80000000u | (first << 3) | (count - 1u));
+}
+__host__ __device__ inline uint32_t leaf_first(int32_t r) { return ((uint32_t)r & 0x7fffffffu) >> 3; }
+__host__ __device__ inline uint32_t leaf_count(int32_t r) { return ((uint32_t)r & 7u) + 1u; }
+
+constexpr int kMaxLeaf = 8;
+constexpr int kMaxDepth = 96;      // checked after every build
+constexpr int kStackLds = 32;      // traversal stack entries per lane kept in LDS
+constexpr int kTraceBlock = 128;   // traversal workgroup size (2 waves)
+
+// ---------------------------------------------------------------------------
+// device view of one resident scene
+// ---------------------------------------------------------------------------
+struct DevScene {
+    const float4* tris;        // 3 float4 per sorted triangle: (v0, prim) (v1, mesh) (v2, 0)
+    const BvhNode* nodes;
+    int32_t root;
+    int32_t num_tris;
+    const uint32_t* idx;       // 3 per triangle (original order)
+    const float* pos;          // 3 per vertex
+    const float* nrm;          // 3 per vertex or null
+    const float* tan;          // 3 per vertex or null
+    const int32_t* mesh_mat;   // per mesh
+    const int32_t* mesh_first; // num_meshes + 1
+    const float* area_cdf;     // per triangle, per-mesh segments
+    const MatRec* mats;
+    const float* light_cdf;    // per light
+    const int32_t* light_mesh; // per light
+    int32_t num_lights;
+    float light_wsum;
+    float env[3];
+};
+
+// camera in the form the kernels use (computed on the host in float)
+struct DevCamera {
+    float pos[3], X[3], Y[3], Z[3];
+    float nw, nh;
+};
+
+struct DevParams {
+    int32_t spp, max_depth, rr_depth, use_nee, use_mis;
+    uint32_t seed;
+    int32_t width, height;
+};
+
+// ---------------------------------------------------------------------------
+// Structure-of-arrays path queue.  Entry i of a queue is one live path
+// segment: its ray and the path state that travels with it.
+// ---------------------------------------------------------------------------
+struct PathQueue {
+    float *ox, *oy, *oz, *dx, *dy, *dz;   // ray
+    float *br, *bg, *bb;                  // throughput beta
+    float* pdf;                           // scatter pdf of the previous bounce
+    uint32_t* flags;                      // bit0: previous bounce specular
+    uint64_t* rng;                        // PCG32 state (stream implied by the sample)
+    uint32_t* slot;                       // path slot in the current chunk
+};
+struct HitBuf {
+    float *t, *u, *v;
+    int32_t* tri;                         // sorted triangle position or -1
+};
+struct ShadowQueue {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+    float *cr, *cg, *cb;                  // contribution if unoccluded
+    uint32_t* slot;
+};
+
+// counters that live in device memory between kernels of one chunk
+enum {
+    CNT_Q0 = 0, CNT_Q1 = 1, CNT_SHADOW = 2, CNT_WORK = 3, CNT_WORK2 = 4,
+    CNT_MAX = 16
+};
+
+// ---------------------------------------------------------------------------
+// host-side records
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct SceneRec {
+    int32_t nv = 0, nf = 0, nmesh = 0, nmat = 0, nlights = 0;
+    bool has_nrm = false, has_tan = false;
+    float env[3] = {0, 0, 0};
+    float light_wsum = 0.0f;
+    // device arrays
+    DevBuf pos, nrm, tan, idx, mesh_mat, mesh_first, area_cdf, mats, light_cdf, light_mesh, tri_mesh;
+    DevBuf orig_pos, orig_nrm, orig_tan, bone_ids, bone_w, bones;    // skinning
+    int32_t nbones = 0;
+    // BVH
+    bool has_bvh = false;
+    DevBuf nodes, tris, order, node_parent, refit_counter, node_valid;
+    int32_t root = 0, num_nodes = 0, max_depth = 0, num_clusters = 0, num_top = 0, cluster_size = 0;
+    int32_t max_leaf = 4;
+    // host copies needed for light CDF construction
+    std::vector<float> h_mesh_area;
+};
+
+struct KernelStat {
+    double ms = 0.0;
+    int64_t launches = 0;
+    int64_t items = 0;
+};
+
+}  // namespace srt
+
+struct srt_ctx {
+    int device = 0;
+    size_t budget = 0;
+    size_t used = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::mutex mu;
+    uint64_t next_scene = 1;
+    std::map<uint64_t, srt::SceneRec*> scenes;
+    // render pool
+    int64_t pool_cap = 0;                 // path slots allocated
+    int64_t pool_request = 16 * 1024 * 1024;
+    srt::DevBuf q[2][14], hits[4], shq[11], Lbuf[3], counters, film_acc, film_out, gstack;
+    int64_t film_pixels = 0;
+    int64_t gstack_threads = 0;
+    // profiling
+    bool profile = false;
+    std::map<std::string, srt::KernelStat> stats;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    // last render counters
+    int64_t last_segments = 0, last_shadow_rays = 0;
+    void* serve = nullptr;                // serving loop state
+    void* mlp = nullptr;                  // radiance MLP state
+    void (*serve_free)(srt_ctx*) = nullptr;
+    void (*mlp_free)(srt_ctx*) = nullptr;
+};
+
+namespace srt {
+// allocation helpers (budget-checked)
+int dev_alloc(srt_ctx* c, DevBuf& b, size_t bytes);
+void dev_free(srt_ctx* c, DevBuf& b);
+int set_err(srt_ctx* c, int code, const char* fmt, ...);
+int hip_check(srt_ctx* c, hipError_t e, const char* what);
+// kernel timing
+void prof_begin(srt_ctx* c, const char* name, hipEvent_t* ev);
+void prof_end(srt_ctx* c, const char* name, hipEvent_t ev0, int64_t items);
+void prof_collect(srt_ctx* c);
+
+// BVH (bvh.hip)
+int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size);
+int bvh_refit(srt_ctx* c, SceneRec* s);
+int skin_vertices(srt_ctx* c, SceneRec* s, const float* bones_3x4, int nb);
+// wavefront integrator (wavefront.hip)
+int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevParams& p, float* film_dev);
+int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin, const float* tmax_dev,
+               int any, float* t, float* u, float* v, int32_t* prim);
+int tonemap_dev(srt_ctx* c, float* rgba, int n, int type, float exposure, int use_gamma);
+int accumulate_dev(srt_ctx* c, float* acc, float* rgba, int n, int frame_count);
+DevScene dev_scene(const SceneRec* s);
+DevCamera make_camera(const float pos[3], const float target[3], const float up[3], float fov, int W, int H);
+}  // namespace srt

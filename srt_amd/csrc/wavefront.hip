@@ -1,0 +1,755 @@
+// Wavefront path tracer for gfx950: raygen -> traverse -> shade (Disney BSDF
+// sample/eval + next-event estimation) -> shadow traverse -> finalize, plus the
+// tone-mapping and accumulation post-passes.
+//
+// Per-path semantics follow the reference megakernel
+// (src/shader/integrator/path_tracer.cu:50-169) rule for rule; the stage split
+// follows the reference wavefront pass (src/passes/wavefront/wavefront.cu:40-300)
+// but every stage runs on compacted structure-of-arrays queues:
+//   - one path slot per (pixel, sample); its PCG32 stream is keyed by
+//     (pixel, seed, sample), so results do not depend on queue order;
+//   - surviving paths and shadow rays are appended with one atomic per wave
+//     (64-lane ballot + popcount prefix);
+//   - radiance is accumulated per slot (no float atomics) and reduced per pixel
+//     in sample order, reproducing the megakernel's summation order.
+#include "srt_internal.h"
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cmath>
+
+namespace srt {
+
+// ---------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------
+struct RayPre {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+SRT_D float sel(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+SRT_D RayPre ray_pre(f3 d) {
+    RayPre r;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    float dkz = sel(d, kz);
+    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = sel(d, kx) / dkz;
+    r.Sy = sel(d, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    return r;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013).  Returns
+// t with barycentrics u (weight of v1) and v (weight of v2).
+SRT_D bool intersect_tri(const RayPre& r, f3 o, f3 v0, f3 v1, f3 v2, float tmin, float tmax, float* t_out, float* u_out,
+                         float* v_out) {
+    f3 A = v0 - o, B = v1 - o, C = v2 - o;
+    float Akz = sel(A, r.kz), Bkz = sel(B, r.kz), Ckz = sel(C, r.kz);
+    float Ax = sel(A, r.kx) - r.Sx * Akz;
+    float Ay = sel(A, r.ky) - r.Sy * Akz;
+    float Bx = sel(B, r.kx) - r.Sx * Bkz;
+    float By = sel(B, r.ky) - r.Sy * Bkz;
+    float Cx = sel(C, r.kx) - r.Sx * Ckz;
+    float Cy = sel(C, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.Sz * Akz;
+    float Bz = r.Sz * Bkz;
+    float Cz = r.Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= tmin && t <= tmax)) return false;
+    *t_out = t;
+    *u_out = V * rcp;
+    *v_out = W * rcp;
+    return true;
+}
+
+constexpr float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013): conservative slab test
+
+struct TraceOut { float t, u, v; int32_t pos; };
+
+// Stack: the first kStackLds entries of each lane live in LDS (lane-interleaved,
+// entry e of lane l at lds[e * kTraceBlock + l]); deeper entries spill to a
+// per-thread global area.  The build guarantees depth <= kMaxDepth.
+template <bool ANY>
+SRT_D bool traverse(const DevScene& S, f3 o, f3 d, float tmin, float tmax, int32_t* lds, int32_t* gst, size_t gstride,
+                    TraceOut* out) {
+    RayPre rp = ray_pre(d);
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float best_t = tmax;
+    int32_t best_prim = 0x7fffffff, best_pos = -1;
+    float bu = 0.0f, bv = 0.0f;
+    int32_t cur = S.root;
+    int sp = 0;
+    for (;;) {
+        if (cur >= 0) {
+            const BvhNode* np = S.nodes + cur;
+            float4 na = np->a, nb = np->b, nc = np->c;
+            int4 nd = np->d;
+            float lx0 = (na.x - o.x) * inv.x, lx1 = (na.y - o.x) * inv.x;
+            float ly0 = (na.z - o.y) * inv.y, ly1 = (na.w - o.y) * inv.y;
+            float lz0 = (nc.x - o.z) * inv.z, lz1 = (nc.y - o.z) * inv.z;
+            float rx0 = (nb.x - o.x) * inv.x, rx1 = (nb.y - o.x) * inv.x;
+            float ry0 = (nb.z - o.y) * inv.y, ry1 = (nb.w - o.y) * inv.y;
+            float rz0 = (nc.z - o.z) * inv.z, rz1 = (nc.w - o.z) * inv.z;
+            float tln = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), tmin));
+            float tlf = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), best_t)) * kRobust;
+            float trn = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), tmin));
+            float trf = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), best_t)) * kRobust;
+            bool hl = tln <= tlf, hr = trn <= trf;
+            if (hl && hr) {
+                int32_t nearc = nd.x, farc = nd.y;
+                if (trn < tln) { nearc = nd.y; farc = nd.x; }
+                if (sp < kStackLds) lds[sp * kTraceBlock] = farc;
+                else gst[(size_t)(sp - kStackLds) * gstride] = farc;
+                sp++;
+                cur = nearc;
+                continue;
+            }
+            if (hl) { cur = nd.x; continue; }
+            if (hr) { cur = nd.y; continue; }
+        } else {
+            uint32_t f = leaf_first(cur), c = leaf_count(cur);
+            for (uint32_t k = f; k < f + c; k++) {
+                const float4* tp = S.tris + 3 * (size_t)k;
+                float4 A = tp[0], B = tp[1], C = tp[2];
+                float t, u, v;
+                if (!intersect_tri(rp, o, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), tmin, best_t, &t, &u, &v))
+                    continue;
+                if (ANY) return true;
+                int32_t prim = __float_as_int(A.w);
+                if (t < best_t || (t == best_t && prim < best_prim)) {
+                    best_t = t; best_prim = prim; best_pos = (int32_t)k; bu = u; bv = v;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        cur = (sp < kStackLds) ? lds[sp * kTraceBlock] : gst[(size_t)(sp - kStackLds) * gstride];
+    }
+    if (ANY) return false;
+    out->t = best_t; out->u = bu; out->v = bv; out->pos = best_pos;
+    return best_pos >= 0;
+}
+
+// ---------------------------------------------------------------------------
+// wave-level queue append: one atomic per wave
+// ---------------------------------------------------------------------------
+SRT_D int wave_append(bool pred, uint32_t* counter) {
+    unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return -1;
+    int lane = __lane_id();
+    unsigned long long below = (lane == 0) ? 0ull : (mask & ((1ull << lane) - 1ull));
+    uint32_t prefix = (uint32_t)__popcll(below);
+    uint32_t total = (uint32_t)__popcll(mask);
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+    return pred ? (int)(base + prefix) : -1;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+struct ChunkInfo {
+    int64_t path_begin;   // first path id of the chunk (path id = sample * WH + pixel)
+    int32_t n;            // paths in the chunk
+    int32_t wh;           // pixels
+};
+
+__global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* Lg, float* Lb, DevCamera cam, DevParams p,
+                                                ChunkInfo ch, uint32_t* counts) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) counts[CNT_Q0] = (uint32_t)ch.n;
+    if (i >= ch.n) return;
+    int64_t pid = ch.path_begin + i;
+    uint32_t sample = (uint32_t)(pid / ch.wh);
+    uint32_t pixel = (uint32_t)(pid - (int64_t)sample * ch.wh);
+    int ix = (int)(pixel % (uint32_t)p.width), iy = (int)(pixel / (uint32_t)p.width);
+    Pcg32 rng = path_rng(pixel, p.seed, sample);
+    float r0 = rng.next_float();
+    float xx = ((float)ix + r0) / (float)p.width;
+    float r1 = rng.next_float();
+    float yy = ((float)iy + r1) / (float)p.height;
+    float px = (xx - 0.5f) * cam.nw, py = (yy - 0.5f) * cam.nh;
+    f3 dir = normalize(mk(px, py, 1.0f));
+    f3 X = ld3(cam.X), Y = ld3(cam.Y), Z = ld3(cam.Z), CP = ld3(cam.pos);
+    f3 zero = mk(0.0f);
+    f3 ro = zero.x * X + zero.y * Y + zero.z * Z + CP;
+    f3 rd = dir.x * X + dir.y * Y + dir.z * Z;
+    q.ox[i] = ro.x; q.oy[i] = ro.y; q.oz[i] = ro.z;
+    q.dx[i] = rd.x; q.dy[i] = rd.y; q.dz[i] = rd.z;
+    q.br[i] = 1.0f; q.bg[i] = 1.0f; q.bb[i] = 1.0f;
+    q.pdf[i] = 1.0f;
+    q.flags[i] = 1u;
+    q.rng[i] = rng.state;
+    q.slot[i] = (uint32_t)i;
+    Lr[i] = 0.0f; Lg[i] = 0.0f; Lb[i] = 0.0f;
+}
+
+// closest hit for every entry of queue `qi`; also clears the counters the
+// following shade stage appends to
+__global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
+                                                      int32_t* gstack, unsigned long long* totals) {
+    __shared__ int32_t lds[kStackLds * kTraceBlock];
+    const int tid = threadIdx.x;
+    const size_t gtid = (size_t)blockIdx.x * blockDim.x + tid;
+    const size_t gthreads = (size_t)gridDim.x * blockDim.x;
+    const uint32_t count = counts[qi];
+    if (gtid == 0) {
+        counts[qi ^ 1] = 0u;
+        counts[CNT_SHADOW] = 0u;
+        totals[0] += count;
+    }
+    for (size_t i = gtid; i < count; i += gthreads) {
+        f3 o = mk(q.ox[i], q.oy[i], q.oz[i]);
+        f3 d = mk(q.dx[i], q.dy[i], q.dz[i]);
+        TraceOut to;
+        bool hit = traverse<false>(S, o, d, 1e-3f, 1e16f, lds + tid, gstack + gtid, gthreads, &to);
+        h.t[i] = hit ? to.t : 0.0f;
+        h.u[i] = hit ? to.u : 0.0f;
+        h.v[i] = hit ? to.v : 0.0f;
+        h.tri[i] = hit ? to.pos : -1;
+    }
+}
+
+struct LightSample { f3 pos, normal, emission; float pdf; };
+
+// light.h:61-76 and gmesh.h:45-80, with the float lower CDF bound (D2)
+SRT_D LightSample light_sample(const DevScene& S, f2 rnd) {
+    int idx = binary_search(S.light_cdf, S.num_lights, rnd.x);
+    float lower = (idx == 0) ? 0.0f : S.light_cdf[idx - 1];
+    rnd.x = (rnd.x - lower) / (S.light_cdf[idx] - lower);
+    int mesh = S.light_mesh[idx];
+    int f0 = S.mesh_first[mesh], fn = S.mesh_first[mesh + 1] - f0;
+    const float* ac = S.area_cdf + f0;
+    int f = binary_search(ac, fn, rnd.x);
+    float lo2 = (f == 0) ? 0.0f : ac[f - 1];
+    rnd.x = (rnd.x - lo2) / (ac[f] - lo2);
+    f2 uv = uniform_sample_triangle(rnd);
+    int prim = f0 + f;
+    const uint32_t* ix = S.idx + 3 * (size_t)prim;
+    f3 v0 = ld3(S.pos + 3 * (size_t)ix[0]), v1 = ld3(S.pos + 3 * (size_t)ix[1]), v2 = ld3(S.pos + 3 * (size_t)ix[2]);
+    float w = 1.0f - uv.x - uv.y;
+    LightSample ls;
+    ls.pos = v0 * w + v1 * uv.x + v2 * uv.y;
+    f3 normal = normalize(cross(v1 - v0, v2 - v0));
+    if (S.nrm) {
+        f3 n0 = ld3(S.nrm + 3 * (size_t)ix[0]), n1 = ld3(S.nrm + 3 * (size_t)ix[1]), n2 = ld3(S.nrm + 3 * (size_t)ix[2]);
+        f3 sn = n0 * w + n1 * uv.x + n2 * uv.y;
+        if (dot(normal, sn) < 0.0f) normal = -normal;
+    }
+    ls.normal = normal;
+    const MatRec& m = S.mats[S.mesh_mat[mesh]];
+    ls.emission = mk(m.emission_color[0], m.emission_color[1], m.emission_color[2]) * m.intensity;
+    ls.pdf = m.intensity / S.light_wsum;
+    return ls;
+}
+
+SRT_D bool finite3(f3 v) {
+    return !(__builtin_isnan(v.x) || __builtin_isinf(v.x) || __builtin_isnan(v.y) || __builtin_isinf(v.y) ||
+             __builtin_isnan(v.z) || __builtin_isinf(v.z));
+}
+
+// One bounce for every entry of queue `qi` (path_tracer.cu:80-161).
+__global__ void __launch_bounds__(256) k_shade(DevScene S, DevParams P, ChunkInfo ch, int depth, PathQueue qin, HitBuf h,
+                                               PathQueue qout, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
+                                               uint32_t* counts, int qi) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    const uint32_t count = counts[qi];
+    const bool last = (depth + 1 >= P.max_depth);
+    for (uint32_t base = (uint32_t)wave * 64u; base < count; base += (uint32_t)nwaves * 64u) {
+        const uint32_t i = base + (uint32_t)lane;
+        bool emit_next = false, emit_shadow = false;
+        // outputs of this lane
+        f3 n_o = mk(0.0f), n_d = mk(0.0f), n_beta = mk(0.0f);
+        float n_pdf = 0.0f;
+        uint32_t n_flags = 0;
+        uint64_t n_rng = 0;
+        f3 s_d = mk(0.0f), s_c = mk(0.0f);
+        float s_tmax = 0.0f;
+        uint32_t slot = 0;
+        if (i < count) {
+            slot = qin.slot[i];
+            f3 ro = mk(qin.ox[i], qin.oy[i], qin.oz[i]);
+            f3 rd = mk(qin.dx[i], qin.dy[i], qin.dz[i]);
+            f3 beta = mk(qin.br[i], qin.bg[i], qin.bb[i]);
+            float scatter_pdf = qin.pdf[i];
+            bool specular = (qin.flags[i] & 1u) != 0u;
+            int32_t tri = h.tri[i];
+            const int64_t pid = ch.path_begin + slot;
+            const uint32_t sample = (uint32_t)(pid / ch.wh);
+            Pcg32 rng = rng_resume(qin.rng[i], sample);
+            if (tri < 0) {
+                // miss (path_tracer.cu:81-85)
+                f3 env = mk(S.env[0], S.env[1], S.env[2]);
+                f3 add = beta * env;
+                Lr[slot] = Lr[slot] + add.x; Lg[slot] = Lg[slot] + add.y; Lb[slot] = Lb[slot] + add.z;
+            } else {
+                // hit reconstruction (hit_info.h:27-65)
+                const float4* tp = S.tris + 3 * (size_t)tri;
+                float4 A = tp[0], B = tp[1], C = tp[2];
+                const int32_t prim = __float_as_int(A.w);
+                const int32_t mesh = __float_as_int(B.w);
+                f3 v0 = mk(A.x, A.y, A.z), v1 = mk(B.x, B.y, B.z), v2 = mk(C.x, C.y, C.z);
+                float u = h.u[i], v = h.v[i];
+                float w = 1.0f - u - v;
+                f3 pos = v0 * w + v1 * u + v2 * v;
+                f3 normal = normalize(cross(v1 - v0, v2 - v0));
+                f3 sn = normal;
+                const uint32_t* ix = S.idx + 3 * (size_t)prim;
+                if (S.nrm) {
+                    f3 n0 = ld3(S.nrm + 3 * (size_t)ix[0]), n1 = ld3(S.nrm + 3 * (size_t)ix[1]), n2 = ld3(S.nrm + 3 * (size_t)ix[2]);
+                    sn = normalize(n0 * w + n1 * u + n2 * v);
+                }
+                if (dot(normal, sn) < 0.0f) normal = -normal;
+                f3 tg;
+                if (S.tan) {
+                    f3 t0 = ld3(S.tan + 3 * (size_t)ix[0]), t1 = ld3(S.tan + 3 * (size_t)ix[1]), t2 = ld3(S.tan + 3 * (size_t)ix[2]);
+                    tg = t0 * w + t1 * u + t2 * v;
+                } else {
+                    tg = (fabsf(sn.x) > fabsf(sn.y)) ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+                }
+                const MatRec& m = S.mats[S.mesh_mat[mesh]];
+                f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
+                if (mat_is_emissive(m)) {
+                    // emitter hit with MIS (path_tracer.cu:88-106)
+                    float cos_i = dot(normal, -rd);
+                    if (cos_i > 1e-4f) {
+                        float mis = 1.0f;
+                        if (P.use_nee && !specular) {
+                            mis = 0.0f;
+                            if (P.use_mis) {
+                                float t2 = dot(pos - ro, pos - ro);
+                                float lpdf = (m.intensity / S.light_wsum) * t2 / cos_i;
+                                mis = scatter_pdf / (lpdf + scatter_pdf);
+                            }
+                        }
+                        f3 Le = mk(m.emission_color[0], m.emission_color[1], m.emission_color[2]) * m.intensity;
+                        f3 add = beta * Le * mis;
+                        Lr[slot] = Lr[slot] + add.x; Lg[slot] = Lg[slot] + add.y; Lb[slot] = Lb[slot] + add.z;
+                    }
+                } else {
+                    Onb onb = make_onb(sn, tg);
+                    float cos_i = dot(normal, -rd);
+                    if (cos_i < 0.0f) normal = -normal;
+                    // next-event estimation (path_tracer.cu:113-142); the shadow ray
+                    // and its would-be contribution go to the shadow queue
+                    if (P.use_nee && !mat_is_specular(m)) {
+                        f2 lr;
+                        lr.x = rng.next_float();
+                        lr.y = rng.next_float();
+                        LightSample ls = light_sample(S, lr);
+                        f3 sdir = normalize(ls.pos - pos);
+                        float cos_o = dot(normal, sdir);
+                        float cos_light = dot(ls.normal, -sdir);
+                        float t = length(ls.pos - pos);
+                        if ((cos_light > 1e-4f) && (ls.pdf > 1e-4f) && (t > 1e-3f) && (cos_o > 0.0f || mat_is_transmissive(m))) {
+                            float mis = 1.0f;
+                            float lpdf = ls.pdf * t * t / cos_light;
+                            if (P.use_mis) {
+                                float mpdf = mat_pdf(m, sdir, -rd, onb, color);
+                                mis = lpdf / (lpdf + mpdf);
+                            }
+                            f3 f = mat_eval(m, sdir, -rd, onb, color);
+                            s_c = beta * f * fabsf(cos_o) * ls.emission * mis / lpdf;
+                            s_d = sdir;
+                            s_tmax = t - 1e-3f;
+                            emit_shadow = true;
+                        }
+                    }
+                    if (!last) {
+                        // BSDF sampling, throughput update, Russian roulette (path_tracer.cu:145-161)
+                        f2 br;
+                        br.x = rng.next_float();
+                        br.y = rng.next_float();
+                        BxSample ms = mat_sample(m, -rd, br, onb, color);
+                        bool alive = !(ms.pdf <= 1e-4f);
+                        if (alive && dot(ms.wi, normal) < 0.0f && !mat_is_transmissive(m)) alive = false;
+                        if (alive) {
+                            beta = beta * (ms.f / ms.pdf);
+                            if (!ms.delta) beta = beta * fabsf(dot(ms.wi, normal));
+                            if (depth >= P.rr_depth) {
+                                float q = fmaxf(fmaxf(beta.x, fmaxf(beta.y, beta.z)), 0.05f);
+                                if (rng.next_float() > q) alive = false;
+                                else beta = beta / q;
+                            }
+                        }
+                        if (alive) {
+                            emit_next = true;
+                            n_o = pos; n_d = ms.wi; n_beta = beta; n_pdf = ms.pdf;
+                            n_flags = mat_is_specular(m) ? 1u : 0u;
+                            n_rng = rng.state;
+                        }
+                    }
+                    if (emit_shadow) n_o = pos;
+                }
+            }
+        }
+        // compaction (all 64 lanes execute the ballots)
+        int js = wave_append(emit_shadow, counts + CNT_SHADOW);
+        if (js >= 0) {
+            sq.ox[js] = n_o.x; sq.oy[js] = n_o.y; sq.oz[js] = n_o.z;
+            sq.dx[js] = s_d.x; sq.dy[js] = s_d.y; sq.dz[js] = s_d.z;
+            sq.tmax[js] = s_tmax;
+            sq.cr[js] = s_c.x; sq.cg[js] = s_c.y; sq.cb[js] = s_c.z;
+            sq.slot[js] = slot;
+        }
+        int jn = wave_append(emit_next, counts + (qi ^ 1));
+        if (jn >= 0) {
+            qout.ox[jn] = n_o.x; qout.oy[jn] = n_o.y; qout.oz[jn] = n_o.z;
+            qout.dx[jn] = n_d.x; qout.dy[jn] = n_d.y; qout.dz[jn] = n_d.z;
+            qout.br[jn] = n_beta.x; qout.bg[jn] = n_beta.y; qout.bb[jn] = n_beta.z;
+            qout.pdf[jn] = n_pdf;
+            qout.flags[jn] = n_flags;
+            qout.rng[jn] = n_rng;
+            qout.slot[jn] = slot;
+        }
+    }
+}
+
+// Shadow rays: any hit in [1e-3, tmax]; unoccluded rays add their contribution.
+__global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
+                                                       uint32_t* counts, int32_t* gstack, unsigned long long* totals) {
+    __shared__ int32_t lds[kStackLds * kTraceBlock];
+    const int tid = threadIdx.x;
+    const size_t gtid = (size_t)blockIdx.x * blockDim.x + tid;
+    const size_t gthreads = (size_t)gridDim.x * blockDim.x;
+    const uint32_t count = counts[CNT_SHADOW];
+    if (gtid == 0) totals[1] += count;
+    for (size_t i = gtid; i < count; i += gthreads) {
+        f3 o = mk(sq.ox[i], sq.oy[i], sq.oz[i]);
+        f3 d = mk(sq.dx[i], sq.dy[i], sq.dz[i]);
+        TraceOut to;
+        bool occluded = traverse<true>(S, o, d, 1e-3f, sq.tmax[i], lds + tid, gstack + gtid, gthreads, &to);
+        if (!occluded) {
+            uint32_t slot = sq.slot[i];
+            Lr[slot] = Lr[slot] + sq.cr[i];
+            Lg[slot] = Lg[slot] + sq.cg[i];
+            Lb[slot] = Lb[slot] + sq.cb[i];
+        }
+    }
+}
+
+// Per pixel: add this chunk's samples in sample order (path_tracer.cu:164-165),
+// dropping non-finite samples but keeping the 1/spp weight.
+__global__ void __launch_bounds__(256) k_finalize(const float* Lr, const float* Lg, const float* Lb, float* acc, ChunkInfo ch,
+                                                  int spp, int last_chunk, float4* film) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ch.wh) return;
+    const int64_t end = ch.path_begin + ch.n;
+    int64_t s0 = (ch.path_begin - q + ch.wh - 1) / ch.wh;
+    if (s0 < 0) s0 = 0;
+    float ax = acc[3 * (size_t)q], ay = acc[3 * (size_t)q + 1], az = acc[3 * (size_t)q + 2];
+    const float fspp = (float)spp;
+    for (int64_t s = s0;; s++) {
+        int64_t pid = s * ch.wh + q;
+        if (pid >= end) break;
+        size_t slot = (size_t)(pid - ch.path_begin);
+        f3 L = mk(Lr[slot], Lg[slot], Lb[slot]);
+        if (finite3(L)) {
+            ax = ax + L.x / fspp;
+            ay = ay + L.y / fspp;
+            az = az + L.z / fspp;
+        }
+    }
+    acc[3 * (size_t)q] = ax; acc[3 * (size_t)q + 1] = ay; acc[3 * (size_t)q + 2] = az;
+    if (last_chunk) film[q] = make_float4(ax, ay, az, 1.0f);
+}
+
+__global__ void k_zero(float* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// standalone ray queries (first-hit parity, shadow parity)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, int n, const float* o3, const float* d3, float tmin,
+                                                           const float* tmax, int any, float* t, float* u, float* v,
+                                                           int32_t* prim, int32_t* gstack) {
+    __shared__ int32_t lds[kStackLds * kTraceBlock];
+    const int tid = threadIdx.x;
+    const size_t gtid = (size_t)blockIdx.x * blockDim.x + tid;
+    const size_t gthreads = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = gtid; i < (size_t)n; i += gthreads) {
+        f3 o = ld3(o3 + 3 * i), d = ld3(d3 + 3 * i);
+        TraceOut to;
+        if (any) {
+            prim[i] = traverse<true>(S, o, d, tmin, tmax[i], lds + tid, gstack + gtid, gthreads, &to) ? 1 : 0;
+        } else {
+            bool hit = traverse<false>(S, o, d, tmin, tmax[i], lds + tid, gstack + gtid, gthreads, &to);
+            t[i] = hit ? to.t : 0.0f;
+            u[i] = hit ? to.u : 0.0f;
+            v[i] = hit ? to.v : 0.0f;
+            prim[i] = hit ? __float_as_int(S.tris[3 * (size_t)to.pos].w) : -1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// post-processing (tonemapping.cu:14-68, accumulate.cu:22-26)
+// ---------------------------------------------------------------------------
+__global__ void k_tonemap(float4* px, int64_t n, int type, float ex, int use_gamma) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = px[i];
+    f3 c = mk(p.x, p.y, p.z) * ex;
+    switch (type) {
+    case 1: c = mk(clampf(c.x, 0.0f, 1.0f), clampf(c.y, 0.0f, 1.0f), clampf(c.z, 0.0f, 1.0f)); break;
+    case 2: { float L = luminance(c); c = c / (1.0f + L); break; }
+    case 3: {
+        const float A = 0.15f, B = 0.50f, Cc = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f;
+        float cx = ((c.x * (A * c.x + Cc * B) + D * E) / (c.x * (A * c.x + B) + D * F)) - E / F;
+        float cy = ((c.y * (A * c.y + Cc * B) + D * E) / (c.y * (A * c.y + B) + D * F)) - E / F;
+        float cz = ((c.z * (A * c.z + Cc * B) + D * E) / (c.z * (A * c.z + B) + D * F)) - E / F;
+        const float Wt = 11.2f;
+        float white = ((Wt * (A * Wt + Cc * B) + D * E) / (Wt * (A * Wt + B) + D * F)) - E / F;
+        c = mk(cx, cy, cz) / white;
+        break;
+    }
+    case 4: {
+        c = c * 0.6f;
+        const float a = 2.51f, b = 0.03f, cc = 2.43f, d = 0.59f, e = 0.14f;
+        float cx = (c.x * (a * c.x + b)) / (c.x * (cc * c.x + d) + e);
+        float cy = (c.y * (a * c.y + b)) / (c.y * (cc * c.y + d) + e);
+        float cz = (c.z * (a * c.z + b)) / (c.z * (cc * c.z + d) + e);
+        c = mk(clampf(cx, 0.0f, 1.0f), clampf(cy, 0.0f, 1.0f), clampf(cz, 0.0f, 1.0f));
+        break;
+    }
+    default: break;
+    }
+    if (use_gamma) {
+        const float g = 1.0f / 2.2f;
+        c = mk(pt_powf(c.x, g), pt_powf(c.y, g), pt_powf(c.z, g));
+    }
+    px[i] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+__global__ void k_accumulate(float4* acc, float4* px, int64_t n, int frame_count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float w = 1.0f / (float)(frame_count + 1);
+    float4 a = acc[i], p = px[i];
+    float4 r = make_float4(a.x * (1.0f - w) + p.x * w, a.y * (1.0f - w) + p.y * w, a.z * (1.0f - w) + p.z * w,
+                           a.w * (1.0f - w) + p.w * w);
+    acc[i] = r;
+    px[i] = r;
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+static int num_cus(srt_ctx* c) {
+    static int cached = -1;
+    if (cached < 0) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) return 256;
+        cached = prop.multiProcessorCount;
+    }
+    return cached;
+}
+
+static int trace_grid(srt_ctx* c, int64_t n) {
+    int64_t want = (n + kTraceBlock - 1) / kTraceBlock;
+    int64_t cap = (int64_t)num_cus(c) * 10;
+    if (want > cap) want = cap;
+    if (want < 1) want = 1;
+    return (int)want;
+}
+
+static int ensure_gstack(srt_ctx* c, int64_t threads) {
+    if (c->gstack_threads >= threads) return SRT_OK;
+    dev_free(c, c->gstack);
+    int r = dev_alloc(c, c->gstack, (size_t)threads * (kMaxDepth - kStackLds) * sizeof(int32_t));
+    if (r) return r;
+    c->gstack_threads = threads;
+    return SRT_OK;
+}
+
+static int ensure_pool(srt_ctx* c, int64_t slots) {
+    if (c->pool_cap >= slots) return SRT_OK;
+    for (int k = 0; k < 2; k++)
+        for (auto& b : c->q[k]) dev_free(c, b);
+    for (auto& b : c->hits) dev_free(c, b);
+    for (auto& b : c->shq) dev_free(c, b);
+    for (auto& b : c->Lbuf) dev_free(c, b);
+    c->pool_cap = 0;
+    const size_t n = (size_t)slots;
+    for (int k = 0; k < 2; k++) {
+        for (int j = 0; j < 14; j++) {
+            size_t bytes = (j == 11) ? 8 * n : 4 * n;   // 11: rng (u64)
+            if (j == 13) bytes = 0;                       // unused spare
+            if (bytes && dev_alloc(c, c->q[k][j], bytes)) return SRT_E_OOM;
+        }
+    }
+    for (int j = 0; j < 4; j++) if (dev_alloc(c, c->hits[j], 4 * n)) return SRT_E_OOM;
+    for (int j = 0; j < 11; j++) if (dev_alloc(c, c->shq[j], 4 * n)) return SRT_E_OOM;
+    for (int j = 0; j < 3; j++) if (dev_alloc(c, c->Lbuf[j], 4 * n)) return SRT_E_OOM;
+    c->pool_cap = slots;
+    return SRT_OK;
+}
+
+static PathQueue make_queue(srt_ctx* c, int k) {
+    PathQueue q;
+    auto f = [&](int j) { return (float*)c->q[k][j].p; };
+    q.ox = f(0); q.oy = f(1); q.oz = f(2); q.dx = f(3); q.dy = f(4); q.dz = f(5);
+    q.br = f(6); q.bg = f(7); q.bb = f(8); q.pdf = f(9);
+    q.flags = (uint32_t*)c->q[k][10].p;
+    q.rng = (uint64_t*)c->q[k][11].p;
+    q.slot = (uint32_t*)c->q[k][12].p;
+    return q;
+}
+
+DevCamera make_camera(const float pos[3], const float target[3], const float up[3], float fov, int W, int H) {
+    // camera.h:41-53, 165-170; transform.h:136-145 (host float arithmetic)
+    auto sub = [](const float* a, const float* b, float* o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; };
+    auto norm = [](float* v) {
+        float inv = 1.0f / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        v[0] = v[0] * inv; v[1] = v[1] * inv; v[2] = v[2] * inv;
+    };
+    auto crs = [](const float* a, const float* b, float* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    float z[3], x[3], y[3];
+    sub(target, pos, z); norm(z);
+    crs(up, z, x); norm(x);
+    crs(z, x, y);
+    norm(x); norm(y); norm(z);
+    DevCamera cam;
+    for (int i = 0; i < 3; i++) { cam.pos[i] = pos[i]; cam.X[i] = x[i]; cam.Y[i] = y[i]; cam.Z[i] = z[i]; }
+    float aspect = (float)W / (float)H;
+    float rad = (fov * 0.5f) * 3.14159265358979323846f / 180.0f;
+    cam.nh = 2.0f * std::tan(rad);
+    cam.nw = cam.nh * aspect;
+    return cam;
+}
+
+int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevParams& p, float* film_dev) {
+    const int64_t wh = (int64_t)p.width * p.height;
+    const int64_t total = wh * p.spp;
+    int64_t pool = c->pool_request;
+    if (pool > total) pool = total;
+    if (pool > 0x7fffffff) pool = 0x7fffffff;
+    if (int r = ensure_pool(c, pool)) return r;
+    const int tgrid = trace_grid(c, pool);
+    if (int r = ensure_gstack(c, (int64_t)tgrid * kTraceBlock)) return r;
+    if (c->film_pixels < wh) {
+        dev_free(c, c->film_acc);
+        if (dev_alloc(c, c->film_acc, (size_t)wh * 3 * sizeof(float))) return SRT_E_OOM;
+        c->film_pixels = wh;
+    }
+    DevScene S = dev_scene(s);
+    uint32_t* counts = (uint32_t*)c->counters.p;
+    unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + 64);
+    hipStream_t st = c->stream;
+    hipEvent_t ev;
+    {
+        size_t n3 = (size_t)wh * 3;
+        k_zero<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>((float*)c->film_acc.p, n3);
+        k_zero<<<1, 64, 0, st>>>((float*)totals, 4);
+    }
+    PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
+    HitBuf H;
+    H.t = (float*)c->hits[0].p; H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
+    ShadowQueue SQ;
+    {
+        auto f = [&](int j) { return (float*)c->shq[j].p; };
+        SQ.ox = f(0); SQ.oy = f(1); SQ.oz = f(2); SQ.dx = f(3); SQ.dy = f(4); SQ.dz = f(5); SQ.tmax = f(6);
+        SQ.cr = f(7); SQ.cg = f(8); SQ.cb = f(9); SQ.slot = (uint32_t*)c->shq[10].p;
+    }
+    float* Lr = (float*)c->Lbuf[0].p; float* Lg = (float*)c->Lbuf[1].p; float* Lb = (float*)c->Lbuf[2].p;
+    int32_t* gst = (int32_t*)c->gstack.p;
+    const int shade_cap = num_cus(c) * 8;
+    for (int64_t begin = 0; begin < total; begin += pool) {
+        ChunkInfo ch;
+        ch.path_begin = begin;
+        ch.n = (int32_t)((total - begin < pool) ? (total - begin) : pool);
+        ch.wh = (int32_t)wh;
+        prof_begin(c, "raygen", &ev);
+        k_raygen<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>(Q[0], Lr, Lg, Lb, cam, p, ch, counts);
+        prof_end(c, "raygen", ev, ch.n);
+        int shade_grid = (int)std::min<int64_t>((ch.n + 255) / 256, shade_cap);
+        int tg = trace_grid(c, ch.n);
+        for (int depth = 0; depth < p.max_depth; depth++) {
+            int qi = depth & 1;
+            prof_begin(c, "trace", &ev);
+            k_trace<<<tg, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
+            prof_end(c, "trace", ev, 0);
+            prof_begin(c, "shade", &ev);
+            k_shade<<<shade_grid, 256, 0, st>>>(S, p, ch, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi);
+            prof_end(c, "shade", ev, 0);
+            if (p.use_nee) {
+                prof_begin(c, "shadow", &ev);
+                k_shadow<<<tg, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
+                prof_end(c, "shadow", ev, 0);
+            }
+        }
+        bool last = begin + pool >= total;
+        prof_begin(c, "finalize", &ev);
+        k_finalize<<<(unsigned)((wh + 255) / 256), 256, 0, st>>>(Lr, Lg, Lb, (float*)c->film_acc.p, ch, p.spp, last ? 1 : 0,
+                                                                (float4*)film_dev);
+        prof_end(c, "finalize", ev, wh);
+    }
+    if (int r = hip_check(c, hipGetLastError(), "render launch")) return r;
+    unsigned long long tot[2] = {0, 0};
+    if (int r = hip_check(c, hipMemcpyAsync(tot, totals, sizeof(tot), hipMemcpyDeviceToHost, st), "counters")) return r;
+    if (int r = hip_check(c, hipStreamSynchronize(st), "render")) return r;
+    c->last_segments = (int64_t)tot[0];
+    c->last_shadow_rays = (int64_t)tot[1];
+    prof_collect(c);
+    return SRT_OK;
+}
+
+int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin, const float* tmax_dev,
+               int any, float* t, float* u, float* v, int32_t* prim) {
+    DevScene S = dev_scene(s);
+    int grid = trace_grid(c, n);
+    if (int r = ensure_gstack(c, (int64_t)grid * kTraceBlock)) return r;
+    k_trace_rays<<<grid, kTraceBlock, 0, c->stream>>>(S, n, o_dev, d_dev, tmin, tmax_dev, any, t, u, v, prim,
+                                                       (int32_t*)c->gstack.p);
+    if (int r = hip_check(c, hipGetLastError(), "trace launch")) return r;
+    return hip_check(c, hipStreamSynchronize(c->stream), "trace");
+}
+
+int tonemap_dev(srt_ctx* c, float* rgba, int n, int type, float exposure, int use_gamma) {
+    float ex = std::pow(2.0f, exposure);   // host, as tonemapping.cu:11
+    hipEvent_t ev;
+    prof_begin(c, "tonemap", &ev);
+    k_tonemap<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>((float4*)rgba, n, type, ex, use_gamma);
+    prof_end(c, "tonemap", ev, n);
+    if (int r = hip_check(c, hipGetLastError(), "tonemap launch")) return r;
+    return hip_check(c, hipStreamSynchronize(c->stream), "tonemap");
+}
+
+int accumulate_dev(srt_ctx* c, float* acc, float* rgba, int n, int frame_count) {
+    hipEvent_t ev;
+    prof_begin(c, "accumulate", &ev);
+    k_accumulate<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>((float4*)acc, (float4*)rgba, n, frame_count);
+    prof_end(c, "accumulate", ev, n);
+    if (int r = hip_check(c, hipGetLastError(), "accumulate launch")) return r;
+    return hip_check(c, hipStreamSynchronize(c->stream), "accumulate");
+}
+
+}  // namespace srt
