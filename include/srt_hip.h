@@ -103,7 +103,8 @@ typedef struct srt_bvh_opts {
 int srt_create(int device, size_t hbm_budget_bytes, srt_ctx** out);
 void srt_destroy(srt_ctx* ctx);
 const char* srt_last_error(const srt_ctx* ctx);
-/* tunables: "path_pool" (path slots per chunk), "profile" (0/1 per-kernel HIP events) */
+/* tunables: "path_pool" (path slots per chunk), "profile" (0/1 per-kernel HIP events),
+ * "count_visits" (0/1 count BVH node visits and triangle tests) */
 int srt_set_option(srt_ctx* ctx, const char* key, int64_t value);
 size_t srt_hbm_used(const srt_ctx* ctx);
 
@@ -157,6 +158,10 @@ int srt_stats(srt_ctx* ctx, char* names, int32_t names_len, double* ms, int64_t*
 void srt_stats_reset(srt_ctx* ctx);
 /* segments traced and shadow rays of the last srt_render */
 int srt_last_counts(srt_ctx* ctx, int64_t* segments, int64_t* shadow_rays);
+/* node visits and triangle tests of the last srt_render made with the option
+ * "count_visits" = 1: out[0] closest-hit node visits, [1] closest-hit triangle
+ * tests, [2] shadow node visits, [3] shadow triangle tests (0 otherwise) */
+int srt_last_visits(srt_ctx* ctx, int64_t out[4]);
 
 #ifdef __cplusplus
 }

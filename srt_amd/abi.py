@@ -26,7 +26,7 @@ EXPORTED = [
     "srt_scene_build_bvh", "srt_scene_bvh_info", "srt_scene_bvh_download",
     "srt_scene_set_skin", "srt_scene_skin_refit", "srt_scene_download_vertices",
     "srt_trace_rays", "srt_render", "srt_film_download", "srt_tonemap", "srt_accumulate",
-    "srt_stats", "srt_stats_reset", "srt_last_counts",
+    "srt_stats", "srt_stats_reset", "srt_last_counts", "srt_last_visits",
 ]
 
 TONEMAP = {"None": 0, "Clamp": 1, "Reinhard": 2, "Uncharted2": 3, "ACES": 4}
@@ -96,6 +96,7 @@ def load_library(path: str = LIB_PATH):
         "srt_stats": (C.c_int, [P, P, C.c_int32, P, P, P, C.c_int32]),
         "srt_stats_reset": (None, [P]),
         "srt_last_counts": (C.c_int, [P, P, P]),
+        "srt_last_visits": (C.c_int, [P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -301,6 +302,12 @@ class Context:
 
     def stats_reset(self):
         self.lib.srt_stats_reset(self.h)
+
+    def last_visits(self):
+        out = np.zeros(4, np.int64)
+        self.check(self.lib.srt_last_visits(self.h, _ptr(out)))
+        return dict(closest_nodes=int(out[0]), closest_tris=int(out[1]), shadow_nodes=int(out[2]),
+                    shadow_tris=int(out[3]))
 
     def last_counts(self):
         a = C.c_int64()

@@ -177,6 +177,7 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
         return SRT_OK;
     }
     if (!strcmp(key, "profile")) { c->profile = value != 0; return SRT_OK; }
+    if (!strcmp(key, "count_visits")) { c->count_visits = value != 0; return SRT_OK; }
     return set_err(c, SRT_E_INVALID, "unknown option '%s'", key);
 }
 
@@ -508,6 +509,12 @@ int srt_last_counts(srt_ctx* c, int64_t* segments, int64_t* shadow) {
     if (!c) return SRT_E_INVALID;
     if (segments) *segments = c->last_segments;
     if (shadow) *shadow = c->last_shadow_rays;
+    return SRT_OK;
+}
+
+int srt_last_visits(srt_ctx* c, int64_t out[4]) {
+    if (!c || !out) return SRT_E_INVALID;
+    for (int k = 0; k < 4; k++) out[k] = c->last_visits[k];
     return SRT_OK;
 }
 

@@ -158,6 +158,8 @@ struct srt_ctx {
     std::vector<hipEvent_t> event_pool;
     // last render counters
     int64_t last_segments = 0, last_shadow_rays = 0;
+    int64_t last_visits[4] = {0, 0, 0, 0};   // node visits / triangle tests: closest, shadow
+    bool count_visits = false;
     void* serve = nullptr;                // serving loop state
     void* mlp = nullptr;                  // radiance MLP state
     void (*serve_free)(srt_ctx*) = nullptr;

@@ -89,9 +89,9 @@ struct TraceOut { float t, u, v; int32_t pos; };
 // Stack: the first kStackLds entries of each lane live in LDS (lane-interleaved,
 // entry e of lane l at lds[e * kTraceBlock + l]); deeper entries spill to a
 // per-thread global area.  The build guarantees depth <= kMaxDepth.
-template <bool ANY>
+template <bool ANY, bool COUNT = false>
 SRT_D bool traverse(const DevScene& S, f3 o, f3 d, float tmin, float tmax, int32_t* lds, int32_t* gst, size_t gstride,
-                    TraceOut* out) {
+                    TraceOut* out, uint32_t* nvisit = nullptr, uint32_t* ntest = nullptr) {
     RayPre rp = ray_pre(d);
     f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float best_t = tmax;
@@ -101,6 +101,7 @@ SRT_D bool traverse(const DevScene& S, f3 o, f3 d, float tmin, float tmax, int32
     int sp = 0;
     for (;;) {
         if (cur >= 0) {
+            if (COUNT) (*nvisit)++;
             const BvhNode* np = S.nodes + cur;
             float4 na = np->a, nb = np->b, nc = np->c;
             int4 nd = np->d;
@@ -131,6 +132,7 @@ SRT_D bool traverse(const DevScene& S, f3 o, f3 d, float tmin, float tmax, int32
             for (uint32_t k = f; k < f + c; k++) {
                 const float4* tp = S.tris + 3 * (size_t)k;
                 float4 A = tp[0], B = tp[1], C = tp[2];
+                if (COUNT) (*ntest)++;
                 float t, u, v;
                 if (!intersect_tri(rp, o, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), tmin, best_t, &t, &u, &v))
                     continue;
@@ -207,7 +209,9 @@ __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* L
 }
 
 // closest hit for every entry of queue `qi`; also clears the counters the
-// following shade stage appends to
+// following shade stage appends to.  COUNT: also total node visits and
+// triangle tests (an untimed pass used for the algorithmic byte count).
+template <bool COUNT>
 __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
                                                       int32_t* gstack, unsigned long long* totals) {
     __shared__ int32_t lds[kStackLds * kTraceBlock];
@@ -220,15 +224,20 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, 
         counts[CNT_SHADOW] = 0u;
         totals[0] += count;
     }
+    uint32_t nv = 0, nt = 0;
     for (size_t i = gtid; i < count; i += gthreads) {
         f3 o = mk(q.ox[i], q.oy[i], q.oz[i]);
         f3 d = mk(q.dx[i], q.dy[i], q.dz[i]);
         TraceOut to;
-        bool hit = traverse<false>(S, o, d, 1e-3f, 1e16f, lds + tid, gstack + gtid, gthreads, &to);
+        bool hit = traverse<false, COUNT>(S, o, d, 1e-3f, 1e16f, lds + tid, gstack + gtid, gthreads, &to, &nv, &nt);
         h.t[i] = hit ? to.t : 0.0f;
         h.u[i] = hit ? to.u : 0.0f;
         h.v[i] = hit ? to.v : 0.0f;
         h.tri[i] = hit ? to.pos : -1;
+    }
+    if (COUNT) {
+        atomicAdd(&totals[2], (unsigned long long)nv);
+        atomicAdd(&totals[3], (unsigned long long)nt);
     }
 }
 
@@ -430,6 +439,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, DevParams P, ChunkInf
 }
 
 // Shadow rays: any hit in [1e-3, tmax]; unoccluded rays add their contribution.
+template <bool COUNT>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
                                                        uint32_t* counts, int32_t* gstack, unsigned long long* totals) {
     __shared__ int32_t lds[kStackLds * kTraceBlock];
@@ -438,17 +448,22 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue 
     const size_t gthreads = (size_t)gridDim.x * blockDim.x;
     const uint32_t count = counts[CNT_SHADOW];
     if (gtid == 0) totals[1] += count;
+    uint32_t nv = 0, nt = 0;
     for (size_t i = gtid; i < count; i += gthreads) {
         f3 o = mk(sq.ox[i], sq.oy[i], sq.oz[i]);
         f3 d = mk(sq.dx[i], sq.dy[i], sq.dz[i]);
         TraceOut to;
-        bool occluded = traverse<true>(S, o, d, 1e-3f, sq.tmax[i], lds + tid, gstack + gtid, gthreads, &to);
+        bool occluded = traverse<true, COUNT>(S, o, d, 1e-3f, sq.tmax[i], lds + tid, gstack + gtid, gthreads, &to, &nv, &nt);
         if (!occluded) {
             uint32_t slot = sq.slot[i];
             Lr[slot] = Lr[slot] + sq.cr[i];
             Lg[slot] = Lg[slot] + sq.cg[i];
             Lb[slot] = Lb[slot] + sq.cb[i];
         }
+    }
+    if (COUNT) {
+        atomicAdd(&totals[4], (unsigned long long)nv);
+        atomicAdd(&totals[5], (unsigned long long)nt);
     }
 }
 
@@ -668,7 +683,7 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
     {
         size_t n3 = (size_t)wh * 3;
         k_zero<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>((float*)c->film_acc.p, n3);
-        k_zero<<<1, 64, 0, st>>>((float*)totals, 4);
+        k_zero<<<1, 64, 0, st>>>((float*)totals, 16);
     }
     PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
     HitBuf H;
@@ -695,14 +710,16 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
         for (int depth = 0; depth < p.max_depth; depth++) {
             int qi = depth & 1;
             prof_begin(c, "trace", &ev);
-            k_trace<<<tg, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
+            if (c->count_visits) k_trace<true><<<tg, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
+            else k_trace<false><<<tg, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
             prof_end(c, "trace", ev, 0);
             prof_begin(c, "shade", &ev);
             k_shade<<<shade_grid, 256, 0, st>>>(S, p, ch, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi);
             prof_end(c, "shade", ev, 0);
             if (p.use_nee) {
                 prof_begin(c, "shadow", &ev);
-                k_shadow<<<tg, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
+                if (c->count_visits) k_shadow<true><<<tg, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
+                else k_shadow<false><<<tg, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
                 prof_end(c, "shadow", ev, 0);
             }
         }
@@ -713,11 +730,12 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
         prof_end(c, "finalize", ev, wh);
     }
     if (int r = hip_check(c, hipGetLastError(), "render launch")) return r;
-    unsigned long long tot[2] = {0, 0};
+    unsigned long long tot[6] = {0, 0, 0, 0, 0, 0};
     if (int r = hip_check(c, hipMemcpyAsync(tot, totals, sizeof(tot), hipMemcpyDeviceToHost, st), "counters")) return r;
     if (int r = hip_check(c, hipStreamSynchronize(st), "render")) return r;
     c->last_segments = (int64_t)tot[0];
     c->last_shadow_rays = (int64_t)tot[1];
+    for (int k = 0; k < 4; k++) c->last_visits[k] = (int64_t)tot[2 + k];
     prof_collect(c);
     return SRT_OK;
 }
