@@ -347,12 +347,18 @@ void bvh_refit(const Scene& s, Bvh& bvh) {
 namespace {
 const float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013, "Robust BVH Ray Traversal")
 
+// Slab test with the near/far plane of each axis chosen by the sign of the
+// inverse direction (Williams et al. 2005).  A ray lying exactly in a slab
+// plane with a zero direction component gives 0 * inf = NaN, which fmax/fmin
+// ignore, so that axis adds no constraint (conservative).
 inline bool slab(const float lo[3], const float hi[3], const float o[3], const float inv[3], float tmin, float tmax, float* tnear) {
-    float t0x = (lo[0] - o[0]) * inv[0], t1x = (hi[0] - o[0]) * inv[0];
-    float t0y = (lo[1] - o[1]) * inv[1], t1y = (hi[1] - o[1]) * inv[1];
-    float t0z = (lo[2] - o[2]) * inv[2], t1z = (hi[2] - o[2]) * inv[2];
-    float tn = std::fmax(std::fmax(std::fmin(t0x, t1x), std::fmin(t0y, t1y)), std::fmax(std::fmin(t0z, t1z), tmin));
-    float tf = std::fmin(std::fmin(std::fmax(t0x, t1x), std::fmax(t0y, t1y)), std::fmin(std::fmax(t0z, t1z), tmax));
+    float tn = tmin, tf = tmax;
+    for (int a = 0; a < 3; a++) {
+        float nb = (inv[a] < 0.0f) ? hi[a] : lo[a];
+        float fb = (inv[a] < 0.0f) ? lo[a] : hi[a];
+        tn = std::fmax(tn, (nb - o[a]) * inv[a]);
+        tf = std::fmin(tf, (fb - o[a]) * inv[a]);
+    }
     tf = tf * kRobust;
     *tnear = tn;
     return tn <= tf;
@@ -545,6 +551,9 @@ static V3<R> render_pixel(const Scene& s, const Tracer& tr, const Camera& cam, c
     V3<R> result(R(0));
     V3<R> X = cvt<R>(cam.X), Y = cvt<R>(cam.Y), Z = cvt<R>(cam.Z), CP = cvt<R>(cam.pos);
     V3<R> env((R)s.env[0], (R)s.env[1], (R)s.env[2]);
+    // a scene without area lights has no light CDF to sample: NEE is off
+    // (the reference would index an empty CDF in light.h:63)
+    const bool use_nee = p.use_nee && s.nlights() > 0;
     for (int smp = sample_begin; smp < sample_end; smp++) {
         Pcg32 rng;
         path_rng(rng, (uint32_t)pixel, p.seed, (uint32_t)smp);
@@ -575,7 +584,7 @@ static V3<R> render_pixel(const Scene& s, const Tracer& tr, const Camera& cam, c
                 R cos_i = dot(info.normal, -rd);
                 if (cos_i <= (R)1e-4f) break;
                 R mis = R(1);
-                if (p.use_nee && !specular) {
+                if (use_nee && !specular) {
                     mis = R(0);
                     if (p.use_mis) {
                         R t2 = dot(info.pos - ro, info.pos - ro);
@@ -589,7 +598,7 @@ static V3<R> render_pixel(const Scene& s, const Tracer& tr, const Camera& cam, c
             }
             R cos_i = dot(info.normal, -rd);
             if (cos_i < R(0)) info.normal = -info.normal;
-            if (p.use_nee && !mat_is_specular(m)) {
+            if (use_nee && !mat_is_specular(m)) {
                 V2<R> lr;
                 lr.x = (R)rng.next_float();
                 lr.y = (R)rng.next_float();

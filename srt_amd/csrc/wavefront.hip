@@ -94,6 +94,9 @@ SRT_D bool traverse(const DevScene& S, f3 o, f3 d, float tmin, float tmax, int32
                     TraceOut* out, uint32_t* nvisit = nullptr, uint32_t* ntest = nullptr) {
     RayPre rp = ray_pre(d);
     f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    // near/far slab planes chosen by direction sign; a ray lying in a slab plane
+    // with a zero direction component yields NaN, which fmaxf/fminf ignore
+    const bool sx = inv.x < 0.0f, sy = inv.y < 0.0f, sz = inv.z < 0.0f;
     float best_t = tmax;
     int32_t best_prim = 0x7fffffff, best_pos = -1;
     float bu = 0.0f, bv = 0.0f;
@@ -105,16 +108,16 @@ SRT_D bool traverse(const DevScene& S, f3 o, f3 d, float tmin, float tmax, int32
             const BvhNode* np = S.nodes + cur;
             float4 na = np->a, nb = np->b, nc = np->c;
             int4 nd = np->d;
-            float lx0 = (na.x - o.x) * inv.x, lx1 = (na.y - o.x) * inv.x;
-            float ly0 = (na.z - o.y) * inv.y, ly1 = (na.w - o.y) * inv.y;
-            float lz0 = (nc.x - o.z) * inv.z, lz1 = (nc.y - o.z) * inv.z;
-            float rx0 = (nb.x - o.x) * inv.x, rx1 = (nb.y - o.x) * inv.x;
-            float ry0 = (nb.z - o.y) * inv.y, ry1 = (nb.w - o.y) * inv.y;
-            float rz0 = (nc.z - o.z) * inv.z, rz1 = (nc.w - o.z) * inv.z;
-            float tln = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), tmin));
-            float tlf = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), best_t)) * kRobust;
-            float trn = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), tmin));
-            float trf = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), best_t)) * kRobust;
+            float lnx = ((sx ? na.y : na.x) - o.x) * inv.x, lfx = ((sx ? na.x : na.y) - o.x) * inv.x;
+            float lny = ((sy ? na.w : na.z) - o.y) * inv.y, lfy = ((sy ? na.z : na.w) - o.y) * inv.y;
+            float lnz = ((sz ? nc.y : nc.x) - o.z) * inv.z, lfz = ((sz ? nc.x : nc.y) - o.z) * inv.z;
+            float rnx = ((sx ? nb.y : nb.x) - o.x) * inv.x, rfx = ((sx ? nb.x : nb.y) - o.x) * inv.x;
+            float rny = ((sy ? nb.w : nb.z) - o.y) * inv.y, rfy = ((sy ? nb.z : nb.w) - o.y) * inv.y;
+            float rnz = ((sz ? nc.w : nc.z) - o.z) * inv.z, rfz = ((sz ? nc.z : nc.w) - o.z) * inv.z;
+            float tln = fmaxf(fmaxf(fmaxf(tmin, lnx), lny), lnz);
+            float tlf = fminf(fminf(fminf(best_t, lfx), lfy), lfz) * kRobust;
+            float trn = fmaxf(fmaxf(fmaxf(tmin, rnx), rny), rnz);
+            float trf = fminf(fminf(fminf(best_t, rfx), rfy), rfz) * kRobust;
             bool hl = tln <= tlf, hr = trn <= trf;
             if (hl && hr) {
                 int32_t nearc = nd.x, farc = nd.y;

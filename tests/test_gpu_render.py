@@ -101,3 +101,17 @@ def test_accumulate_matches_oracle(ctx):
         acc, out = ctx.accumulate(acc, px, f)
         acc2, out2 = O.accumulate(acc2, px, f)
         assert np.array_equal(acc, acc2) and np.array_equal(out, out2)
+
+
+def test_converges_to_fp64_golden(ctx):
+    """GPU float render at 256 spp against the committed FP64 4096-spp image
+    (MC noise bound: the oracle's own float render measures RMSE 0.017)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cornell_64x64_fp64_4096spp.npy"))
+    s = scenes.cornell()
+    h = ctx.upload(s)
+    img = ctx.render(h, s.camera, 64, 64, 256, 5, 4, 1, 1, 1000)
+    ctx.release(h)
+    rmse = np.sqrt(np.mean((img[:, :3].astype(np.float64) - g) ** 2))
+    assert rmse < 0.03, rmse
+    assert abs(img[:, :3].mean() / g.mean() - 1) < 0.01

@@ -62,3 +62,24 @@ def test_random_rays_closest_and_any(ctx):
     np.testing.assert_array_equal(vis, vis2)
     assert 0.05 < vis.mean() < 0.95
     ctx.release(h)
+
+
+def test_watertight_and_axis_rays_match_brute_force(ctx):
+    """Rays through vertices / edges of a closed mesh and rays lying in box
+    planes with zero direction components (the slab NaN case)."""
+    uv, ut = scenes.icosphere_f4()
+    v = uv.astype(np.float32)
+    s = scenes.SceneData(positions=v, indices=ut.astype(np.uint32), mesh_first=np.array([0, 320], np.int32),
+                         mesh_material=np.array([0], np.int32),
+                         materials=np.asarray([scenes.material()], scenes.MAT_DTYPE))
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    targets = np.concatenate([v, 0.5 * (v[ut[:, 0]] + v[ut[:, 1]])]).astype(np.float32)
+    o = (targets * 3.0).astype(np.float32)
+    d = targets - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    t, u, vv, p = ctx.trace(h, o, d)
+    assert np.all(p >= 0)
+    osc = O.Scene(s)
+    np.testing.assert_array_equal(p, O.trace(osc, o, d)[3])
+    ctx.release(h)

@@ -1,0 +1,171 @@
+"""Oracle pins: BxDF and light sampling.
+
+- Lambert (Diffuse BxDF) reduces to the textbook albedo/pi and cos/pi.
+- Disney reflection lobes are reciprocal: f(wi, wo) = f(wo, wi).
+- pdf(sample) consistency: the pdf returned by sample() equals sample_pdf()
+  of the sampled direction (bxdf.h:289-310 vs :365-380).
+- Unbiasedness of the sampler + pdf: E[max(cos,0)/pdf] over sampled directions
+  equals the cosine integral pi over the hemisphere (the estimator the
+  integrator relies on); a wrong pdf or sampler term shifts it.
+- Light sampling: the MC irradiance estimate at a point from quad lights,
+  built from the oracle's light samples, matches the analytic polygon
+  form factor (Lambert's formula); the light CDF is proportional to
+  intensity * area (scene.cu:240-264, mesh.cpp:3-18).
+"""
+import numpy as np
+import pytest
+
+from srt_amd import scenes
+from oracle import pyoracle as O
+
+RNG = np.random.default_rng(12345)
+
+
+def _hemi(n, rng):
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    v[:, 2] = np.abs(v[:, 2]) + 1e-3
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return v
+
+
+DISNEY_REFLECT = [
+    dict(roughness=1.0),
+    dict(roughness=0.5, metallic=0.0, specular=0.5),
+    dict(roughness=0.3, metallic=1.0),
+    dict(roughness=0.2, clearcoat=1.0, clearcoat_gloss=0.3),
+    dict(roughness=0.7, sheen=1.0, sheen_tint=0.5, subsurface=0.6),
+    dict(roughness=0.4, anisotropic=0.8, specular_tint=0.7),
+    dict(roughness=0.05, metallic=0.0, clearcoat=1.0),
+]
+
+
+def test_lambert_closed_form():
+    m = scenes.material(type=scenes.BXDF_DIFFUSE)
+    col = (0.2, 0.5, 0.9)
+    wo = _hemi(100, RNG)
+    wi = _hemi(100, RNG)
+    f = O.bxdf(m, "eval", wi=wi, wo=wo, color=col)
+    assert np.allclose(f, np.asarray(col) / np.pi, rtol=1e-12)
+    p = O.bxdf(m, "pdf", wi=wi, wo=wo, color=col)
+    assert np.allclose(p, wi[:, 2] / np.pi, rtol=1e-12)
+    below = wi * np.array([1, 1, -1])
+    assert np.all(O.bxdf(m, "eval", wi=below, wo=wo, color=col) == 0)
+
+
+@pytest.mark.parametrize("params", DISNEY_REFLECT)
+def test_disney_reciprocity(params):
+    m = scenes.material(**params)
+    col = (0.8, 0.3, 0.1)
+    wo = _hemi(2000, RNG)
+    wi = _hemi(2000, RNG)
+    a = O.bxdf(m, "eval", wi=wi, wo=wo, color=col)
+    b = O.bxdf(m, "eval", wi=wo, wo=wi, color=col)
+    assert np.allclose(a, b, rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("params", DISNEY_REFLECT + [dict(roughness=0.5, spec_trans=0.8)])
+def test_disney_pdf_of_sample(params):
+    m = scenes.material(**params)
+    col = (0.6, 0.6, 0.2)
+    n = 5000
+    wo = _hemi(n, RNG)
+    rnd = RNG.uniform(size=(n, 2))
+    s = O.bxdf(m, "sample", wo=wo, rnd=rnd, color=col)
+    ok = s["pdf"] > 0
+    assert ok.mean() > 0.5
+    p = O.bxdf(m, "pdf", wi=s["wi"][ok], wo=wo[ok], color=col)
+    assert np.allclose(p, s["pdf"][ok], rtol=1e-9)
+    f = O.bxdf(m, "eval", wi=s["wi"][ok], wo=wo[ok], color=col)
+    assert np.allclose(f, s["f"][ok], rtol=1e-12)
+
+
+@pytest.mark.parametrize("params", [dict(type=scenes.BXDF_DIFFUSE)] + DISNEY_REFLECT[:6])
+def test_sampler_unbiased_cosine_integral(params):
+    params = dict(params)
+    t = params.pop("type", scenes.BXDF_DISNEY)
+    m = scenes.material(type=t, **params)
+    col = (0.5, 0.5, 0.5)
+    n = 200000
+    rng = np.random.default_rng(7)
+    for wo in ([0.0, 0.0, 1.0], [0.6, 0.0, 0.8], [0.0, 0.95, 0.31225]):
+        wo = np.tile(np.asarray(wo) / np.linalg.norm(wo), (n, 1))
+        s = O.bxdf(m, "sample", wo=wo, rnd=rng.uniform(size=(n, 2)), color=col)
+        g = np.where((s["pdf"] > 0) & (s["wi"][:, 2] > 0), s["wi"][:, 2] / np.maximum(s["pdf"], 1e-300), 0.0)
+        est, err = g.mean(), g.std() / np.sqrt(n)
+        assert abs(est - np.pi) < max(5 * err, 2e-3), (params, est, err)
+
+
+def test_float_path_tracks_fp64():
+    for params in DISNEY_REFLECT:
+        m = scenes.material(**params)
+        wo = _hemi(500, RNG)
+        wi = _hemi(500, RNG)
+        a = O.bxdf(m, "eval", wi=wi, wo=wo, color=(0.7, 0.4, 0.2), fp64=True)
+        b = O.bxdf(m, "eval", wi=wi, wo=wo, color=(0.7, 0.4, 0.2), fp64=False)
+        big = a > 1e-3
+        assert np.allclose(b[big], a[big], rtol=2e-4), params
+
+
+def _polygon_irradiance(x, n, verts):
+    """Lambert's formula: E/L = 1/2 sum_i Theta_i (Gamma_i . n), vertices counter-clockwise as seen from x."""
+    u = verts - x
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    tot = 0.0
+    for i in range(len(u)):
+        a, b = u[i], u[(i + 1) % len(u)]
+        th = np.arccos(np.clip(np.dot(a, b), -1, 1))
+        g = np.cross(a, b)
+        g /= np.linalg.norm(g)
+        tot += th * np.dot(g, n)
+    return abs(tot) / 2.0
+
+
+def _light_scene(lights):
+    b = scenes._Builder()
+    mats = [scenes.material()]
+    b.quad((-5, -1, -5), (5, -1, -5), (5, -1, 5), (-5, -1, 5), 0, (0, 1, 0))   # a receiver plane (not used)
+    for k, (p, ke) in enumerate(lights):
+        mats.append(scenes.material(emission=ke))
+        b.quad(*p, k + 1, (0, -1, 0))
+    return b.finish(mats, with_normals=False)
+
+
+def test_light_cdf_intensity_times_area():
+    quads = [((-0.5, 2, -0.5), (0.5, 2, -0.5), (0.5, 2, 0.5), (-0.5, 2, 0.5)),
+             ((1, 3, 0), (3, 3, 0), (3, 3, 1), (1, 3, 1))]
+    s = _light_scene([(quads[0], (4.0, 4.0, 4.0)), (quads[1], (1.0, 2.0, 2.0))])
+    cdf, mesh, acdf, marea, ws = O.Scene(s).lights()
+    assert list(mesh) == [1, 2]
+    w = np.array([np.sqrt(48.0) * 1.0, 3.0 * 2.0])
+    assert np.allclose(marea[1:], [1.0, 2.0], rtol=1e-6)
+    assert np.allclose(cdf, np.cumsum(w) / w.sum(), rtol=1e-6)
+    assert np.isclose(ws, w.sum(), rtol=1e-6)
+    assert acdf[s.mesh_first[1] + 1] == 1.0 and acdf[s.mesh_first[2] + 1] == 1.0
+
+
+@pytest.mark.parametrize("x", [(0.0, 0.0, 0.0), (0.7, 0.2, -0.4), (2.5, 0.5, 0.5)])
+def test_light_sampling_matches_polygon_irradiance(x):
+    q1 = np.array([(-0.5, 2, -0.5), (0.5, 2, -0.5), (0.5, 2, 0.5), (-0.5, 2, 0.5)], float)
+    q2 = np.array([(1, 3, 0), (3, 3, 0), (3, 3, 1), (1, 3, 1)], float)
+    ke1, ke2 = (3.0, 3.0, 3.0), (0.0, 2.0, 0.0)
+    s = _light_scene([(tuple(map(tuple, q1)), ke1), (tuple(map(tuple, q2)), ke2)])
+    osc = O.Scene(s)
+    g = (np.arange(512) + 0.5) / 512
+    rnd = np.stack(np.meshgrid(g, g), -1).reshape(-1, 2)
+    ls = osc.light_sample(rnd).astype(np.float64)
+    x = np.asarray(x, float)
+    n = np.array([0.0, 1.0, 0.0])
+    d = ls[:, :3] - x
+    t = np.linalg.norm(d, axis=1)
+    w = d / t[:, None]
+    cos_o = np.maximum(w @ n, 0)
+    cos_l = np.maximum(-(w * ls[:, 3:6]).sum(1), 0)
+    # emitted radiance of the sampled light, identified by position
+    on2 = ls[:, 1] > 2.5
+    Le = np.where(on2, np.linalg.norm(ke2), np.linalg.norm(ke1))
+    est = np.mean(Le * cos_o * cos_l / (t * t * ls[:, 6]))
+    exact = np.linalg.norm(ke1) * _polygon_irradiance(x, n, q1) + np.linalg.norm(ke2) * _polygon_irradiance(x, n, q2)
+    assert abs(est - exact) / exact < 2e-3, (est, exact)
+    # samples lie on the lights with downward normals
+    assert np.allclose(ls[:, 4], -1.0)
