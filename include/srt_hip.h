@@ -151,6 +151,13 @@ int srt_tonemap(srt_ctx* ctx, float* rgba, int64_t num_pixels, int32_t type, flo
                 int32_t on_device);
 int srt_accumulate(srt_ctx* ctx, float* acc, float* rgba, int64_t num_pixels, int32_t frame_count, int32_t on_device);
 
+/* ---- device buffers owned by the context (the reference's GPUMemory<T> for the
+ *      Film, src/passes/film.h:7-20); budget-checked ----
+ * kind: 0 host->device, 1 device->host, 2 device->device */
+int srt_device_alloc(srt_ctx* ctx, size_t bytes, void** out);
+int srt_device_free(srt_ctx* ctx, void* ptr);
+int srt_memcpy(srt_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t kind);
+
 /* ---- statistics (the reference's PROFILE scopes, src/utils/profiler.h:156-219) ----
  * Per-kernel totals since the last reset, filled in order of kernel name.
  * Returns the number of kernels; names are written NUL-separated into names. */

@@ -27,6 +27,7 @@ EXPORTED = [
     "srt_scene_set_skin", "srt_scene_skin_refit", "srt_scene_download_vertices",
     "srt_trace_rays", "srt_render", "srt_film_download", "srt_tonemap", "srt_accumulate",
     "srt_stats", "srt_stats_reset", "srt_last_counts", "srt_last_visits",
+    "srt_device_alloc", "srt_device_free", "srt_memcpy",
 ]
 
 TONEMAP = {"None": 0, "Clamp": 1, "Reinhard": 2, "Uncharted2": 3, "ACES": 4}
@@ -97,6 +98,9 @@ def load_library(path: str = LIB_PATH):
         "srt_stats_reset": (None, [P]),
         "srt_last_counts": (C.c_int, [P, P, P]),
         "srt_last_visits": (C.c_int, [P, P]),
+        "srt_device_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
+        "srt_device_free": (C.c_int, [P, P]),
+        "srt_memcpy": (C.c_int, [P, P, P, C.c_size_t, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -20,7 +20,7 @@ HOSTLIB = os.path.join(HERE, "libsrt_host.so")
 SRTRACER = os.path.join(HERE, "srtracer")
 
 HIP_SOURCES = ["srt_api.hip", "wavefront.hip", "bvh.hip", "mlp.hip", "serve.hip"]
-HOST_SOURCES = ["json.cpp", "passes.cpp", "renderer.cpp", "scene_io.cpp"]
+HOST_SOURCES = ["json.cpp", "scene_io.cpp", "passes.cpp", "renderer.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SRT_OFFLOAD_ARCH", "gfx950")

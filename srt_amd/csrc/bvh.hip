@@ -681,7 +681,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     k_morton<<<nb, 256, 0, st>>>(cen, cb6, n, keys, vals);
     {
         size_t tbytes = 0;
-        hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, keys, keys2, vals, order, n, 0, 63, st);
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, keys, keys2, vals, order, n, 0, 63, st);
         void* tstore = tmp.get<char>(tbytes);
         if (!tstore) return set_err(c, SRT_E_OOM, "sort scratch");
         if (int r = hip_check(c, hipcub::DeviceRadixSort::SortPairs(tstore, tbytes, keys, keys2, vals, order, n, 0, 63, st), "radix sort"))
@@ -716,7 +716,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
             __host__ __device__ bool operator()(const int32_t& x) const { return x != 0x7fffffff; }
         };
         size_t tbytes = 0;
-        hipcub::DeviceSelect::If(nullptr, tbytes, mark, clusters, nsel, n, NotSentinel(), st);
+        (void)hipcub::DeviceSelect::If(nullptr, tbytes, mark, clusters, nsel, n, NotSentinel(), st);
         void* tstore = tmp.get<char>(tbytes);
         if (!tstore) return set_err(c, SRT_E_OOM, "select scratch");
         if (int r = hip_check(c, hipcub::DeviceSelect::If(tstore, tbytes, mark, clusters, nsel, n, NotSentinel(), st), "select"))

@@ -56,7 +56,7 @@ void prof_begin(srt_ctx* c, const char* name, hipEvent_t* ev) {
     hipEvent_t e;
     if (!c->event_pool.empty()) { e = c->event_pool.back(); c->event_pool.pop_back(); }
     else if (hipEventCreate(&e) != hipSuccess) return;
-    hipEventRecord(e, c->stream);
+    (void)hipEventRecord(e, c->stream);
     *ev = e;
 }
 
@@ -68,7 +68,7 @@ void prof_end(srt_ctx* c, const char* name, hipEvent_t ev0, int64_t items) {
     hipEvent_t e;
     if (!c->event_pool.empty()) { e = c->event_pool.back(); c->event_pool.pop_back(); }
     else if (hipEventCreate(&e) != hipSuccess) return;
-    hipEventRecord(e, c->stream);
+    (void)hipEventRecord(e, c->stream);
     c->pending.push_back({name, {ev0, e}});
 }
 
@@ -143,7 +143,7 @@ int srt_create(int device, size_t hbm_budget_bytes, srt_ctx** out) {
     (void)hipMemGetInfo(&free_b, &total_b);
     c->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(free_b * 0.9);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SRT_E_HIP; }
-    if (dev_alloc(c, c->counters, 256)) { hipStreamDestroy(c->stream); delete c; return SRT_E_OOM; }
+    if (dev_alloc(c, c->counters, 256)) { (void)hipStreamDestroy(c->stream); delete c; return SRT_E_OOM; }
     (void)hipMemset(c->counters.p, 0, 256);
     *out = c;
     return SRT_OK;
@@ -162,8 +162,10 @@ void srt_destroy(srt_ctx* c) {
     for (auto& b : c->shq) dev_free(c, b);
     for (auto& b : c->Lbuf) dev_free(c, b);
     dev_free(c, c->counters); dev_free(c, c->film_acc); dev_free(c, c->film_out); dev_free(c, c->gstack);
-    for (auto e : c->event_pool) hipEventDestroy(e);
-    hipStreamDestroy(c->stream);
+    for (auto& kv : c->user_bufs) { DevBuf b; b.p = kv.first; b.bytes = kv.second; dev_free(c, b); }
+    c->user_bufs.clear();
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -475,6 +477,36 @@ int srt_accumulate(srt_ctx* c, float* acc, float* rgba, int64_t n, int32_t frame
     dev_free(c, ta);
     dev_free(c, tp);
     return r;
+}
+
+int srt_device_alloc(srt_ctx* c, size_t bytes, void** out) {
+    if (!c || !out) return SRT_E_INVALID;
+    DevBuf b;
+    if (int r = dev_alloc(c, b, bytes)) return r;
+    c->user_bufs[b.p] = b.bytes;
+    *out = b.p;
+    return SRT_OK;
+}
+
+int srt_device_free(srt_ctx* c, void* p) {
+    if (!c) return SRT_E_INVALID;
+    auto it = c->user_bufs.find(p);
+    if (it == c->user_bufs.end()) return set_err(c, SRT_E_NOT_FOUND, "pointer not allocated by srt_device_alloc");
+    (void)hipStreamSynchronize(c->stream);
+    DevBuf b;
+    b.p = p;
+    b.bytes = it->second;
+    dev_free(c, b);
+    c->user_bufs.erase(it);
+    return SRT_OK;
+}
+
+int srt_memcpy(srt_ctx* c, void* dst, const void* src, size_t bytes, int32_t kind) {
+    if (!c || (!dst && bytes) || (!src && bytes) || kind < 0 || kind > 2) return SRT_E_INVALID;
+    if (bytes == 0) return SRT_OK;
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    if (int r = hip_check(c, hipStreamSynchronize(c->stream), "sync")) return r;
+    return hip_check(c, hipMemcpy(dst, src, bytes, k), "srt_memcpy");
 }
 
 int srt_stats(srt_ctx* c, char* names, int32_t names_len, double* ms, int64_t* launches, int64_t* items, int32_t max) {

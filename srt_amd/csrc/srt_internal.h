@@ -160,6 +160,7 @@ struct srt_ctx {
     int64_t last_segments = 0, last_shadow_rays = 0;
     int64_t last_visits[4] = {0, 0, 0, 0};   // node visits / triangle tests: closest, shadow
     bool count_visits = false;
+    std::map<void*, size_t> user_bufs;    // srt_device_alloc allocations
     void* serve = nullptr;                // serving loop state
     void* mlp = nullptr;                  // radiance MLP state
     void (*serve_free)(srt_ctx*) = nullptr;

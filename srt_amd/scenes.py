@@ -29,6 +29,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 
 import numpy as np
 
@@ -329,3 +330,55 @@ def tube(rings=1001, segments=100, radius=0.5, length=10.0, num_bones=64, seed=1
 
 
 WORKLOADS = {"cornell": cornell, "spheres": spheres, "tube": tube}
+
+
+def save_srtscene(s: SceneData, path: str, frames: int | None = None):
+    """Write the binary scene read by the C++ passes (srt_amd/host/scene_io.cpp).
+
+    With ``frames`` and a skinned scene, the bone matrices of frames 0..frames-1
+    are stored so the renderer can animate without Python.
+    """
+    nv = s.positions.shape[0]
+    with open(path, "wb") as f:
+        f.write(b"SRTSCN1\0")
+        f.write(np.asarray([nv, s.num_triangles, len(s.mesh_material), len(s.materials),
+                            int(s.normals is not None), int(s.tangents is not None)], "<i4").tobytes())
+        f.write(np.asarray(s.env, "<f4").tobytes())
+        f.write(np.ascontiguousarray(s.positions, "<f4").tobytes())
+        if s.normals is not None:
+            f.write(np.ascontiguousarray(s.normals, "<f4").tobytes())
+        if s.tangents is not None:
+            f.write(np.ascontiguousarray(s.tangents, "<f4").tobytes())
+        f.write(np.ascontiguousarray(s.indices, "<u4").tobytes())
+        f.write(np.ascontiguousarray(s.mesh_first, "<i4").tobytes())
+        f.write(np.ascontiguousarray(s.mesh_material, "<i4").tobytes())
+        f.write(np.ascontiguousarray(s.materials, MAT_DTYPE).tobytes())
+        if s.skin is not None and frames:
+            nb = s.skin["num_bones"]
+            f.write(np.asarray([nb, frames], "<i4").tobytes())
+            f.write(np.ascontiguousarray(s.skin["bone_ids"], "<i4").tobytes())
+            f.write(np.ascontiguousarray(s.skin["bone_weights"], "<f4").tobytes())
+            for fr in range(frames):
+                f.write(np.ascontiguousarray(s.skin["matrices"](fr), "<f4").tobytes())
+        else:
+            f.write(np.asarray([0, 0], "<i4").tobytes())
+
+
+def save_obj(s: SceneData, path: str):
+    """Write the scene as OBJ + MTL (one group and material per mesh; Ke carries emission)."""
+    base = os.path.splitext(path)[0]
+    mtl = base + ".mtl"
+    with open(mtl, "w") as f:
+        for i, m in enumerate(s.materials):
+            ke = np.asarray(m["emission_color"], np.float64) * float(m["intensity"])
+            kd = m["base_color"]
+            f.write(f"newmtl m{i}\n  Kd {kd[0]:.9g} {kd[1]:.9g} {kd[2]:.9g}\n  Ks 0 0 0\n"
+                    f"  Ke {ke[0]:.9g} {ke[1]:.9g} {ke[2]:.9g}\n  Ni {float(m['ior']):.9g}\n\n")
+    with open(path, "w") as f:
+        f.write(f"mtllib {os.path.basename(mtl)}\n")
+        for v in s.positions:
+            f.write(f"v {v[0]:.9g} {v[1]:.9g} {v[2]:.9g}\n")
+        for k in range(len(s.mesh_material)):
+            f.write(f"g mesh{k}\nusemtl m{int(s.mesh_material[k])}\n")
+            for t in s.indices[s.mesh_first[k]:s.mesh_first[k + 1]]:
+                f.write(f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")
