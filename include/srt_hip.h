@@ -151,6 +151,29 @@ int srt_tonemap(srt_ctx* ctx, float* rgba, int64_t num_pixels, int32_t type, flo
                 int32_t on_device);
 int srt_accumulate(srt_ctx* ctx, float* acc, float* rgba, int64_t num_pixels, int32_t frame_count, int32_t on_device);
 
+/* ---- radiance MLP (replaces the NeuralRadiosity network, src/passes/npr/
+ *      neural_radiosity.cu:6-52, whose training is a stub at :165-169) ----
+ * 9 raw inputs in [0,1] (position normalised to the scene box, (n+1)/2,
+ * (w_o+1)/2) -> frequency encoding (3 octaves, width 63) -> 4 x 64 ReLU ->
+ * 3 outputs; BF16 MFMA with FP32 accumulation; relative-L2 loss; Adam.
+ * Parameters in logical order: W0[64][63] b0[64] W1..W3[64][64]+b W4[3][64] b4[3]. */
+int srt_mlp_num_params(void);
+int srt_mlp_create(srt_ctx* ctx, uint32_t seed, float lr);
+int srt_mlp_set_params(srt_ctx* ctx, const float* params, int32_t n);
+int srt_mlp_get_params(srt_ctx* ctx, float* params, int32_t n);
+int srt_mlp_set_hparams(srt_ctx* ctx, float lr, float beta1, float beta2, float eps);
+int srt_mlp_set_bounds(srt_ctx* ctx, const float lo[3], const float hi[3]);
+/* host arrays: in9 [n][9], out3 [n][3] */
+int srt_mlp_infer(srt_ctx* ctx, int32_t n, const float* in9, float* out3);
+/* one Adam step on (in9, target3) (apply = 1) or gradients only (apply = 0);
+ * loss: mean relative L2 before the step; grads: logical order (may be NULL) */
+int srt_mlp_train_step(srt_ctx* ctx, int32_t n, const float* in9, const float* target3, int32_t apply, float* loss,
+                       float* grads);
+/* checkpoint: weights + Adam state + step + bounds (the reference saves weights
+ * only, neural_radiosity.cu:54-72) */
+int srt_mlp_save(srt_ctx* ctx, const char* path);
+int srt_mlp_load(srt_ctx* ctx, const char* path);
+
 /* ---- device buffers owned by the context (the reference's GPUMemory<T> for the
  *      Film, src/passes/film.h:7-20); budget-checked ----
  * kind: 0 host->device, 1 device->host, 2 device->device */

@@ -28,6 +28,8 @@ EXPORTED = [
     "srt_trace_rays", "srt_render", "srt_film_download", "srt_tonemap", "srt_accumulate",
     "srt_stats", "srt_stats_reset", "srt_last_counts", "srt_last_visits",
     "srt_device_alloc", "srt_device_free", "srt_memcpy",
+    "srt_mlp_num_params", "srt_mlp_create", "srt_mlp_set_params", "srt_mlp_get_params", "srt_mlp_set_hparams",
+    "srt_mlp_set_bounds", "srt_mlp_infer", "srt_mlp_train_step", "srt_mlp_save", "srt_mlp_load",
 ]
 
 TONEMAP = {"None": 0, "Clamp": 1, "Reinhard": 2, "Uncharted2": 3, "ACES": 4}
@@ -101,6 +103,16 @@ def load_library(path: str = LIB_PATH):
         "srt_device_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
         "srt_device_free": (C.c_int, [P, P]),
         "srt_memcpy": (C.c_int, [P, P, P, C.c_size_t, C.c_int32]),
+        "srt_mlp_num_params": (C.c_int, []),
+        "srt_mlp_create": (C.c_int, [P, C.c_uint32, C.c_float]),
+        "srt_mlp_set_params": (C.c_int, [P, P, C.c_int32]),
+        "srt_mlp_get_params": (C.c_int, [P, P, C.c_int32]),
+        "srt_mlp_set_hparams": (C.c_int, [P, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "srt_mlp_set_bounds": (C.c_int, [P, P, P]),
+        "srt_mlp_infer": (C.c_int, [P, C.c_int32, P, P]),
+        "srt_mlp_train_step": (C.c_int, [P, C.c_int32, P, P, C.c_int32, P, P]),
+        "srt_mlp_save": (C.c_int, [P, C.c_char_p]),
+        "srt_mlp_load": (C.c_int, [P, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -292,6 +304,52 @@ class Context:
         p = np.ascontiguousarray(rgba, np.float32).copy()
         self.check(self.lib.srt_accumulate(self.h, _ptr(a), _ptr(p), a.shape[0], frame_count, 0))
         return a, p
+
+    # ---- radiance MLP ----
+    def mlp_create(self, seed=0, lr=1e-3):
+        self.check(self.lib.srt_mlp_create(self.h, seed, C.c_float(lr)))
+
+    def mlp_num_params(self):
+        return int(self.lib.srt_mlp_num_params())
+
+    def mlp_params(self):
+        out = np.zeros(self.mlp_num_params(), np.float32)
+        self.check(self.lib.srt_mlp_get_params(self.h, _ptr(out), out.size))
+        return out
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self.check(self.lib.srt_mlp_set_params(self.h, _ptr(p), p.size))
+
+    def mlp_set_hparams(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.check(self.lib.srt_mlp_set_hparams(self.h, C.c_float(lr), C.c_float(beta1), C.c_float(beta2),
+                                                C.c_float(eps)))
+
+    def mlp_set_bounds(self, lo, hi):
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        self.check(self.lib.srt_mlp_set_bounds(self.h, _ptr(lo), _ptr(hi)))
+
+    def mlp_infer(self, in9):
+        a = np.ascontiguousarray(in9, np.float32).reshape(-1, 9)
+        out = np.zeros((a.shape[0], 3), np.float32)
+        self.check(self.lib.srt_mlp_infer(self.h, a.shape[0], _ptr(a), _ptr(out)))
+        return out
+
+    def mlp_train_step(self, in9, target, apply=True, want_grads=False):
+        a = np.ascontiguousarray(in9, np.float32).reshape(-1, 9)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        loss = C.c_float()
+        g = np.zeros(self.mlp_num_params(), np.float32) if want_grads else None
+        self.check(self.lib.srt_mlp_train_step(self.h, a.shape[0], _ptr(a), _ptr(t), 1 if apply else 0,
+                                               C.byref(loss), _ptr(g)))
+        return (loss.value, g) if want_grads else loss.value
+
+    def mlp_save(self, path):
+        self.check(self.lib.srt_mlp_save(self.h, path.encode()))
+
+    def mlp_load(self, path):
+        self.check(self.lib.srt_mlp_load(self.h, path.encode()))
 
     # ---- statistics ----
     def stats(self):

@@ -1,0 +1,94 @@
+"""GPU radiance MLP (BF16 MFMA, FP32 accumulation) against the float64
+reference of tests/mlp_ref.py with BF16 rounding emulated where the kernels
+round: forward outputs, gradients of every parameter, one Adam step in closed
+form (Kingma & Ba 2015), determinism, checkpoint round trip, and that online
+training reduces the loss."""
+import os
+
+import numpy as np
+import pytest
+
+import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def mctx(ctx):
+    ctx.mlp_create(seed=3)
+    return ctx
+
+
+def data(n, seed):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(size=(n, 9)).astype(np.float32)
+    t = (0.3 + 0.25 * np.sin(2 * np.pi * u[:, :3]) * np.cos(np.pi * u[:, 3:6]) + 0.2 * u[:, 6:9]).astype(np.float32)
+    return u, t
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 1000, 4099])
+def test_forward_matches_reference(mctx, n):
+    p = M.init_params(5)
+    mctx.mlp_set_params(p)
+    assert np.array_equal(mctx.mlp_params(), p.astype(np.float32))
+    u, _ = data(n, n)
+    y = mctx.mlp_infer(u)
+    ref, _ = M.forward(p, u, emulate=True)
+    scale = np.abs(ref).max() + 1e-6
+    # BF16 activations: a value at a rounding boundary can round the other way
+    assert np.abs(y - ref).max() <= 2e-2 * scale
+    assert np.abs(y - ref).mean() <= 1e-3 * scale
+
+
+def test_gradients_match_reference(mctx):
+    p = M.init_params(7)
+    mctx.mlp_set_params(p)
+    u, t = data(8192, 1)
+    loss, g = mctx.mlp_train_step(u, t, apply=False, want_grads=True)
+    gr = M.backward(p, u, t, emulate=True)
+    lr_ = M.loss(p, u, t, emulate=True)
+    assert abs(loss - lr_) / lr_ < 1e-3
+    assert np.array_equal(mctx.mlp_params(), p.astype(np.float32))      # apply=0 leaves weights
+    o = 0
+    for (r, c) in M.SHAPES:
+        for size in (r * c, r):
+            a, b = g[o:o + size], gr[o:o + size]
+            assert np.linalg.norm(a - b) <= 2e-2 * np.linalg.norm(b) + 1e-9, (r, c, size)
+            o += size
+
+
+def test_adam_first_step_closed_form(mctx):
+    p = M.init_params(9).astype(np.float32)
+    mctx.mlp_set_params(p)
+    mctx.mlp_set_hparams(lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8)
+    u, t = data(4096, 2)
+    _, g = mctx.mlp_train_step(u, t, apply=False, want_grads=True)
+    mctx.mlp_train_step(u, t, apply=True)
+    p1 = mctx.mlp_params()
+    # t = 1: m_hat = g, v_hat = g^2 -> step lr * g / (|g| + eps)
+    g64 = g.astype(np.float64)
+    expect = p.astype(np.float64) - 1e-3 * g64 / (np.abs(g64) + 1e-8)
+    assert np.allclose(p1, expect, rtol=0, atol=2e-6)
+
+
+def test_training_is_deterministic_and_learns(ctx, tmp_path):
+    u, t = data(1 << 14, 3)
+    runs = []
+    for rep in range(2):
+        ctx.mlp_create(seed=11, lr=3e-3)
+        losses = [ctx.mlp_train_step(u, t) for _ in range(150)]
+        runs.append((losses, ctx.mlp_params()))
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][0] == runs[1][0]
+    losses = runs[0][0]
+    assert losses[-1] < 0.2 * losses[0], (losses[0], losses[-1])
+    # checkpoint round trip: same weights and the same next step
+    path = str(tmp_path / "mlp.bin")
+    ctx.mlp_save(path)
+    l_a = ctx.mlp_train_step(u, t)
+    p_a = ctx.mlp_params()
+    ctx.mlp_create(seed=99)
+    ctx.mlp_load(path)
+    assert np.array_equal(ctx.mlp_params(), runs[0][1])
+    l_b = ctx.mlp_train_step(u, t)
+    assert l_a == l_b and np.array_equal(ctx.mlp_params(), p_a)
