@@ -169,6 +169,16 @@ int srt_mlp_infer(srt_ctx* ctx, int32_t n, const float* in9, float* out3);
  * loss: mean relative L2 before the step; grads: logical order (may be NULL) */
 int srt_mlp_train_step(srt_ctx* ctx, int32_t n, const float* in9, const float* target3, int32_t apply, float* loss,
                        float* grads);
+/* Online training from the path tracer itself (config: radiance samples per
+ * frame): renders width x height x spp training paths with the plain tracer,
+ * records at vertex 2 (the second surface hit) the features and throughput,
+ * takes (radiance gathered after vertex 2's NEE) / throughput as the target,
+ * keeps the first max_samples samples in path order, and runs `steps` Adam
+ * steps on equal consecutive mini-batches.  Render with mlp_mode = 1 to stop
+ * paths at vertex 2 and add throughput * max(MLP, 0) after its NEE. */
+int srt_mlp_train_frame(srt_ctx* ctx, srt_scene scene, const srt_camera* camera, const srt_pt_params* params,
+                        int32_t width, int32_t height, int64_t max_samples, int32_t steps, float* loss,
+                        int64_t* samples);
 /* checkpoint: weights + Adam state + step + bounds (the reference saves weights
  * only, neural_radiosity.cu:54-72) */
 int srt_mlp_save(srt_ctx* ctx, const char* path);

@@ -30,6 +30,7 @@ EXPORTED = [
     "srt_device_alloc", "srt_device_free", "srt_memcpy",
     "srt_mlp_num_params", "srt_mlp_create", "srt_mlp_set_params", "srt_mlp_get_params", "srt_mlp_set_hparams",
     "srt_mlp_set_bounds", "srt_mlp_infer", "srt_mlp_train_step", "srt_mlp_save", "srt_mlp_load",
+    "srt_mlp_train_frame",
 ]
 
 TONEMAP = {"None": 0, "Clamp": 1, "Reinhard": 2, "Uncharted2": 3, "ACES": 4}
@@ -113,6 +114,8 @@ def load_library(path: str = LIB_PATH):
         "srt_mlp_train_step": (C.c_int, [P, C.c_int32, P, P, C.c_int32, P, P]),
         "srt_mlp_save": (C.c_int, [P, C.c_char_p]),
         "srt_mlp_load": (C.c_int, [P, C.c_char_p]),
+        "srt_mlp_train_frame": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32,
+                                          C.c_int32, C.c_int64, C.c_int32, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -269,9 +272,9 @@ class Context:
         return c
 
     def render(self, scene, cam: dict, width, height, spp, max_depth=16, rr_depth=4, use_nee=1, use_mis=1, seed=0,
-               out=None, device_ptr=None):
+               out=None, device_ptr=None, mlp_mode=0):
         """Render and return a (H*W, 4) float32 host array, or write to `device_ptr` (device memory)."""
-        p = PtParams(spp, max_depth, rr_depth, use_nee, use_mis, seed, 0, 0)
+        p = PtParams(spp, max_depth, rr_depth, use_nee, use_mis, seed, mlp_mode, 0)
         c = self.camera(cam)
         if device_ptr is not None:
             self.check(self.lib.srt_render(self.h, scene, C.byref(c), C.byref(p), width, height,
@@ -282,9 +285,10 @@ class Context:
         self.check(self.lib.srt_render(self.h, scene, C.byref(c), C.byref(p), width, height, _ptr(out), 0))
         return out
 
-    def render_resident(self, scene, cam, width, height, spp, max_depth=16, rr_depth=4, use_nee=1, use_mis=1, seed=0):
+    def render_resident(self, scene, cam, width, height, spp, max_depth=16, rr_depth=4, use_nee=1, use_mis=1, seed=0,
+                        mlp_mode=0):
         """Render into the context's own device film (no host copy)."""
-        p = PtParams(spp, max_depth, rr_depth, use_nee, use_mis, seed, 0, 0)
+        p = PtParams(spp, max_depth, rr_depth, use_nee, use_mis, seed, mlp_mode, 0)
         c = self.camera(cam)
         self.check(self.lib.srt_render(self.h, scene, C.byref(c), C.byref(p), width, height, None, 0))
 
@@ -344,6 +348,16 @@ class Context:
         self.check(self.lib.srt_mlp_train_step(self.h, a.shape[0], _ptr(a), _ptr(t), 1 if apply else 0,
                                                C.byref(loss), _ptr(g)))
         return (loss.value, g) if want_grads else loss.value
+
+    def mlp_train_frame(self, scene, cam, width, height, spp, max_depth=5, rr_depth=4, seed=0, max_samples=1 << 18,
+                        steps=16):
+        p = PtParams(spp, max_depth, rr_depth, 1, 1, seed, 0, 0)
+        c = self.camera(cam)
+        loss = C.c_float()
+        n = C.c_int64()
+        self.check(self.lib.srt_mlp_train_frame(self.h, scene, C.byref(c), C.byref(p), width, height, max_samples,
+                                                steps, C.byref(loss), C.byref(n)))
+        return loss.value, n.value
 
     def mlp_save(self, path):
         self.check(self.lib.srt_mlp_save(self.h, path.encode()))

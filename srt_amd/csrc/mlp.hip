@@ -108,7 +108,9 @@ SRT_D void layer_mfma(const uint16_t* act, const uint16_t* wl, f32x4 acc[ROWS / 
 // inputs H_l transposed ([l][64][n]) for the weight gradients.
 // ---------------------------------------------------------------------------
 template <bool TRAIN>
-__global__ void __launch_bounds__(MLP_BLOCK) k_mlp_fwd(const float* in9, int n, MlpDev M, float* y, uint16_t* Ht) {
+__global__ void __launch_bounds__(MLP_BLOCK) k_mlp_fwd(const float* in9, int n_arg, const uint32_t* n_dev, MlpDev M, float* y,
+                                                       uint16_t* Ht) {
+    const int n = n_dev ? (int)*n_dev : n_arg;     // device-side count (integrator queries)
     __shared__ __attribute__((aligned(16))) uint16_t sw[W_TOTAL];
     __shared__ float sb[B_TOTAL];
     __shared__ __attribute__((aligned(16))) uint16_t sact[MLP_BLOCK / 64][2][TILE * 64];
@@ -462,7 +464,7 @@ static int train_step_dev(srt_ctx* c, MlpState* m, const float* in9, const float
     if (!m->grad.p) if (int r = dev_alloc(c, m->grad, (size_t)(W_TOTAL + B_TOTAL) * 4)) return r;
     hipEvent_t ev;
     prof_begin(c, "mlp_fwd", &ev);
-    k_mlp_fwd<true><<<grid, MLP_BLOCK, 0, st>>>(in9, n, d, (float*)m->y.p, (uint16_t*)m->Ht.p);
+    k_mlp_fwd<true><<<grid, MLP_BLOCK, 0, st>>>(in9, n, nullptr, d, (float*)m->y.p, (uint16_t*)m->Ht.p);
     prof_end(c, "mlp_fwd", ev, n);
     prof_begin(c, "mlp_bwd", &ev);
     k_mlp_bwd<<<grid, MLP_BLOCK, 0, st>>>((const float*)m->y.p, target, n, 1.0f / (3.0f * (float)n), d,
@@ -505,10 +507,23 @@ int mlp_infer_dev(srt_ctx* c, const float* in9, int n, float* out3) {
     if (n == 0) return SRT_OK;
     hipEvent_t ev;
     prof_begin(c, "mlp_infer", &ev);
-    k_mlp_fwd<false><<<fwd_grid(c, n), MLP_BLOCK, 0, c->stream>>>(in9, n, mlp_dev(m), out3, nullptr);
+    k_mlp_fwd<false><<<fwd_grid(c, n), MLP_BLOCK, 0, c->stream>>>(in9, n, nullptr, mlp_dev(m), out3, nullptr);
     prof_end(c, "mlp_infer", ev, n);
     return hip_check(c, hipGetLastError(), "mlp infer launch");
 }
+
+// forward over a queue whose length lives in device memory (at most max_n)
+int mlp_query_dev(srt_ctx* c, const float* in9, const uint32_t* count_dev, int max_n, float* out3) {
+    MlpState* m = mlp_of(c);
+    if (!m) return set_err(c, SRT_E_STATE, "no MLP: call srt_mlp_create first");
+    hipEvent_t ev;
+    prof_begin(c, "mlp_query", &ev);
+    k_mlp_fwd<false><<<fwd_grid(c, max_n), MLP_BLOCK, 0, c->stream>>>(in9, 0, count_dev, mlp_dev(m), out3, nullptr);
+    prof_end(c, "mlp_query", ev, 0);
+    return hip_check(c, hipGetLastError(), "mlp query launch");
+}
+
+bool mlp_ready(srt_ctx* c) { return mlp_of(c) != nullptr; }
 
 int mlp_train_dev(srt_ctx* c, const float* in9, const float* target, int n, float* loss_host) {
     MlpState* m = mlp_of(c);

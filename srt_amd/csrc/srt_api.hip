@@ -164,6 +164,8 @@ void srt_destroy(srt_ctx* c) {
     dev_free(c, c->counters); dev_free(c, c->film_acc); dev_free(c, c->film_out); dev_free(c, c->gstack);
     for (auto& kv : c->user_bufs) { DevBuf b; b.p = kv.first; b.bytes = kv.second; dev_free(c, b); }
     c->user_bufs.clear();
+    for (auto& b : c->mlpbuf) dev_free(c, b);
+    dev_free(c, c->scan_tmp); dev_free(c, c->train_in9); dev_free(c, c->train_t);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -240,6 +242,13 @@ int srt_scene_upload(srt_ctx* c, const srt_scene_desc* d, srt_scene* out) {
     }
     for (auto& x : light_cdf) x /= wsum;
     SceneRec* s = new SceneRec();
+    for (int a = 0; a < 3; a++) { s->bounds_lo[a] = d->positions[a]; s->bounds_hi[a] = d->positions[a]; }
+    for (int64_t i = 0; i < (int64_t)nv; i++)
+        for (int a = 0; a < 3; a++) {
+            float x = d->positions[3 * i + a];
+            s->bounds_lo[a] = std::min(s->bounds_lo[a], x);
+            s->bounds_hi[a] = std::max(s->bounds_hi[a], x);
+        }
     s->nv = nv; s->nf = nf; s->nmesh = nm; s->nmat = d->num_materials; s->nlights = (int32_t)light_mesh.size();
     s->has_nrm = d->normals != nullptr;
     s->has_tan = d->tangents != nullptr;
@@ -411,7 +420,8 @@ int srt_render(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_para
     if ((int64_t)W * H > (1ll << 28)) return set_err(c, SRT_E_LIMIT, "resolution too large");
     if ((int64_t)W * H * p->samples_per_pixel > (1ll << 40)) return set_err(c, SRT_E_LIMIT, "too many paths");
     if (!(cam->fov_deg > 0.0f && cam->fov_deg < 180.0f)) return set_err(c, SRT_E_INVALID, "fov must be in (0, 180)");
-    if (p->mlp_mode != 0) return set_err(c, SRT_E_STATE, "mlp_mode needs srt_render_mlp");
+    if (p->mlp_mode < 0 || p->mlp_mode > 1) return set_err(c, SRT_E_INVALID, "mlp_mode must be 0 or 1");
+    if (p->mlp_mode == 1 && !mlp_ready(c)) return set_err(c, SRT_E_STATE, "mlp_mode 1 needs srt_mlp_create");
     if (!s->has_bvh) {
         if (int r = bvh_build(c, s, 4, 0)) return r;
     }
@@ -424,6 +434,7 @@ int srt_render(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_para
     dp.spp = p->samples_per_pixel; dp.max_depth = p->max_depth; dp.rr_depth = p->rr_depth;
     dp.use_nee = (p->use_nee && s->nlights > 0) ? 1 : 0; dp.use_mis = p->use_mis ? 1 : 0; dp.seed = p->seed;
     dp.width = W; dp.height = H;
+    dp.mlp_mode = p->mlp_mode;
     const int64_t wh = (int64_t)W * H;
     float* film = nullptr;
     if (rgba_out && out_on_device) film = rgba_out;
@@ -437,6 +448,55 @@ int srt_render(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_para
     if (int r = render_wavefront(c, s, dc, dp, film)) return r;
     if (rgba_out && !out_on_device)
         return hip_check(c, hipMemcpy(rgba_out, film, (size_t)wh * 16, hipMemcpyDeviceToHost), "film download");
+    return SRT_OK;
+}
+
+int srt_mlp_train_frame(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_params* p, int32_t W, int32_t H,
+                        int64_t max_samples, int32_t steps, float* loss_out, int64_t* samples_out) {
+    if (!c || !cam || !p || W <= 0 || H <= 0 || max_samples <= 0 || steps <= 0) return SRT_E_INVALID;
+    if (!mlp_ready(c)) return set_err(c, SRT_E_STATE, "no MLP: call srt_mlp_create first");
+    SceneRec* s = find_scene(c, h);
+    if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (p->samples_per_pixel <= 0 || p->max_depth < 3) return set_err(c, SRT_E_INVALID, "training paths need max_depth >= 3");
+    if (!s->has_bvh) if (int r = bvh_build(c, s, 4, 0)) return r;
+    if (c->train_cap < max_samples) {
+        dev_free(c, c->train_in9);
+        dev_free(c, c->train_t);
+        if (int r = dev_alloc(c, c->train_in9, (size_t)max_samples * 36)) return r;
+        if (int r = dev_alloc(c, c->train_t, (size_t)max_samples * 12)) return r;
+        c->train_cap = max_samples;
+    }
+    const int64_t wh = (int64_t)W * H;
+    if (!c->film_out.p || c->film_out.bytes < (size_t)wh * 16) {
+        dev_free(c, c->film_out);
+        if (int r = dev_alloc(c, c->film_out, (size_t)wh * 16)) return r;
+    }
+    DevCamera dc = make_camera(cam->position, cam->target, cam->up, cam->fov_deg, W, H);
+    DevParams dp;
+    dp.spp = p->samples_per_pixel; dp.max_depth = p->max_depth; dp.rr_depth = p->rr_depth;
+    dp.use_nee = (p->use_nee && s->nlights > 0) ? 1 : 0; dp.use_mis = p->use_mis ? 1 : 0; dp.seed = p->seed;
+    dp.width = W; dp.height = H; dp.mlp_mode = 2;
+    TrainOut tr;
+    tr.in9 = (float*)c->train_in9.p;
+    tr.target = (float*)c->train_t.p;
+    tr.cap = max_samples;
+    tr.count = 0;
+    if (int r = render_wavefront(c, s, dc, dp, (float*)c->film_out.p, &tr)) return r;
+    if (samples_out) *samples_out = tr.count;
+    const int64_t batch = tr.count / steps;
+    double lsum = 0.0;
+    int done = 0;
+    for (int k = 0; k < steps && batch > 0; k++) {
+        float l = 0.0f;
+        if (int r = mlp_train_dev(c, tr.in9 + (size_t)k * batch * 9, tr.target + (size_t)k * batch * 3, (int)batch,
+                                  loss_out ? &l : nullptr))
+            return r;
+        lsum += l;
+        done++;
+    }
+    if (loss_out) *loss_out = done ? (float)(lsum / done) : 0.0f;
+    if (int r = hip_check(c, hipStreamSynchronize(c->stream), "train frame")) return r;
+    prof_collect(c);
     return SRT_OK;
 }
 

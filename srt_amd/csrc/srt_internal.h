@@ -72,6 +72,24 @@ struct DevParams {
     int32_t spp, max_depth, rr_depth, use_nee, use_mis;
     uint32_t seed;
     int32_t width, height;
+    int32_t mlp_mode;      // 0 plain, 1 query the radiance MLP at vertex 2, 2 record training samples
+};
+
+// Radiance-MLP hooks of the integrator.  Vertex 2 is the second surface hit
+// (the hit of the trace at depth 1).  Mode 1: after NEE at vertex 2 the path
+// stops and queues (features, throughput); the MLP's prediction times the
+// throughput is added once the bounce's shadow rays are done.  Mode 2: the path
+// continues as in the plain tracer and records (features, throughput) at vertex
+// 2; the radiance gathered after vertex 2's NEE, divided by that throughput,
+// is the training target.
+struct MlpIO {
+    float* q_in9;          // mode 1 queue: features
+    float* q_beta;         // mode 1 queue: throughput (3)
+    uint32_t* q_slot;      // mode 1 queue: path slot
+    float* rec_in9;        // mode 2, per slot
+    float* rec_beta;       // mode 2, per slot
+    uint32_t* rec_valid;   // mode 2, per slot
+    float lo[3], inv_ext[3];
 };
 
 // ---------------------------------------------------------------------------
@@ -98,7 +116,7 @@ struct ShadowQueue {
 
 // counters that live in device memory between kernels of one chunk
 enum {
-    CNT_Q0 = 0, CNT_Q1 = 1, CNT_SHADOW = 2, CNT_WORK = 3, CNT_WORK2 = 4,
+    CNT_Q0 = 0, CNT_Q1 = 1, CNT_SHADOW = 2, CNT_MLPQ = 3, CNT_WORK2 = 4,
     CNT_MAX = 16
 };
 
@@ -120,6 +138,7 @@ struct SceneRec {
     DevBuf orig_pos, orig_nrm, orig_tan, bone_ids, bone_w, bones;    // skinning
     int32_t nbones = 0;
     // BVH
+    float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};   // scene box (MLP input normalisation)
     bool has_bvh = false;
     DevBuf nodes, tris, order, node_parent, refit_counter, node_valid;
     int32_t root = 0, num_nodes = 0, max_depth = 0, num_clusters = 0, num_top = 0, cluster_size = 0;
@@ -161,6 +180,8 @@ struct srt_ctx {
     int64_t last_visits[4] = {0, 0, 0, 0};   // node visits / triangle tests: closest, shadow
     bool count_visits = false;
     std::map<void*, size_t> user_bufs;    // srt_device_alloc allocations
+    srt::DevBuf mlpbuf[10], scan_tmp, train_in9, train_t;   // MLP hooks of the integrator
+    int64_t mlp_pool_cap = 0, train_cap = 0;
     void* serve = nullptr;                // serving loop state
     void* mlp = nullptr;                  // radiance MLP state
     void (*serve_free)(srt_ctx*) = nullptr;
@@ -183,7 +204,20 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size);
 int bvh_refit(srt_ctx* c, SceneRec* s);
 int skin_vertices(srt_ctx* c, SceneRec* s, const float* bones_3x4, int nb);
 // wavefront integrator (wavefront.hip)
-int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevParams& p, float* film_dev);
+// training samples produced by a mode-2 render
+struct TrainOut {
+    float* in9;        // device [cap][9]
+    float* target;     // device [cap][3]
+    int64_t cap;
+    int64_t count;     // samples written
+};
+int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevParams& p, float* film_dev,
+                     TrainOut* train = nullptr);
+// MLP (mlp.hip)
+int mlp_infer_dev(srt_ctx* c, const float* in9, int n, float* out3);
+int mlp_query_dev(srt_ctx* c, const float* in9, const uint32_t* count_dev, int max_n, float* out3);
+int mlp_train_dev(srt_ctx* c, const float* in9, const float* target, int n, float* loss_host);
+bool mlp_ready(srt_ctx* c);
 int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin, const float* tmax_dev,
                int any, float* t, float* u, float* v, int32_t* prim);
 int tonemap_dev(srt_ctx* c, float* rgba, int n, int type, float exposure, int use_gamma);

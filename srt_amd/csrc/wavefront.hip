@@ -14,6 +14,7 @@
 //     in sample order, reproducing the megakernel's summation order.
 #include "srt_internal.h"
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 #include <cstdio>
 #include <cmath>
 
@@ -225,6 +226,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, 
     if (gtid == 0) {
         counts[qi ^ 1] = 0u;
         counts[CNT_SHADOW] = 0u;
+        counts[CNT_MLPQ] = 0u;
         totals[0] += count;
     }
     uint32_t nv = 0, nt = 0;
@@ -285,7 +287,7 @@ SRT_D bool finite3(f3 v) {
 // One bounce for every entry of queue `qi` (path_tracer.cu:80-161).
 __global__ void __launch_bounds__(256) k_shade(DevScene S, DevParams P, ChunkInfo ch, int depth, PathQueue qin, HitBuf h,
                                                PathQueue qout, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
-                                               uint32_t* counts, int qi) {
+                                               uint32_t* counts, int qi, MlpIO mio) {
     const int lane = threadIdx.x & 63;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
@@ -302,6 +304,9 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, DevParams P, ChunkInf
         f3 s_d = mk(0.0f), s_c = mk(0.0f);
         float s_tmax = 0.0f;
         uint32_t slot = 0;
+        bool emit_query = false;
+        float q9[9];
+        f3 q_beta = mk(0.0f);
         if (i < count) {
             slot = qin.slot[i];
             f3 ro = mk(qin.ox[i], qin.oy[i], qin.oz[i]);
@@ -391,7 +396,26 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, DevParams P, ChunkInf
                             emit_shadow = true;
                         }
                     }
-                    if (!last) {
+                    bool mlp_here = (P.mlp_mode != 0) && depth == 1 && !last;
+                    if (mlp_here) {
+                        // vertex 2 features: position in the scene box, (n+1)/2, (w_o+1)/2
+                        f3 wo = -rd;
+                        q9[0] = (pos.x - mio.lo[0]) * mio.inv_ext[0];
+                        q9[1] = (pos.y - mio.lo[1]) * mio.inv_ext[1];
+                        q9[2] = (pos.z - mio.lo[2]) * mio.inv_ext[2];
+                        q9[3] = (normal.x + 1.0f) * 0.5f; q9[4] = (normal.y + 1.0f) * 0.5f; q9[5] = (normal.z + 1.0f) * 0.5f;
+                        q9[6] = (wo.x + 1.0f) * 0.5f; q9[7] = (wo.y + 1.0f) * 0.5f; q9[8] = (wo.z + 1.0f) * 0.5f;
+                        q_beta = beta;
+                        if (P.mlp_mode == 1) emit_query = true;
+                        else {
+                            for (int k = 0; k < 9; k++) mio.rec_in9[(size_t)slot * 9 + k] = q9[k];
+                            mio.rec_beta[(size_t)slot * 3] = beta.x;
+                            mio.rec_beta[(size_t)slot * 3 + 1] = beta.y;
+                            mio.rec_beta[(size_t)slot * 3 + 2] = beta.z;
+                            mio.rec_valid[slot] = 1u;
+                        }
+                    }
+                    if (!last && !emit_query) {
                         // BSDF sampling, throughput update, Russian roulette (path_tracer.cu:145-161)
                         f2 br;
                         br.x = rng.next_float();
@@ -427,6 +451,14 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, DevParams P, ChunkInf
             sq.tmax[js] = s_tmax;
             sq.cr[js] = s_c.x; sq.cg[js] = s_c.y; sq.cb[js] = s_c.z;
             sq.slot[js] = slot;
+        }
+        int jq = wave_append(emit_query, counts + CNT_MLPQ);
+        if (jq >= 0) {
+            for (int k = 0; k < 9; k++) mio.q_in9[(size_t)jq * 9 + k] = q9[k];
+            mio.q_beta[(size_t)jq * 3] = q_beta.x;
+            mio.q_beta[(size_t)jq * 3 + 1] = q_beta.y;
+            mio.q_beta[(size_t)jq * 3 + 2] = q_beta.z;
+            mio.q_slot[jq] = slot;
         }
         int jn = wave_append(emit_next, counts + (qi ^ 1));
         if (jn >= 0) {
@@ -468,6 +500,42 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue 
         atomicAdd(&totals[4], (unsigned long long)nv);
         atomicAdd(&totals[5], (unsigned long long)nt);
     }
+}
+
+// MLP mode 1: L += beta * max(prediction, 0) for the queued vertex-2 paths
+__global__ void __launch_bounds__(256) k_mlp_apply(const uint32_t* counts, const float* pred, const float* beta,
+                                                   const uint32_t* slot, float* Lr, float* Lg, float* Lb) {
+    const uint32_t n = counts[CNT_MLPQ];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t sl = slot[i];
+        Lr[sl] = Lr[sl] + beta[3 * (size_t)i] * fmaxf(pred[3 * (size_t)i], 0.0f);
+        Lg[sl] = Lg[sl] + beta[3 * (size_t)i + 1] * fmaxf(pred[3 * (size_t)i + 1], 0.0f);
+        Lb[sl] = Lb[sl] + beta[3 * (size_t)i + 2] * fmaxf(pred[3 * (size_t)i + 2], 0.0f);
+    }
+}
+
+// MLP mode 2: radiance gathered so far (after vertex 2's NEE) per slot
+__global__ void __launch_bounds__(256) k_snapshot(int n, const float* Lr, const float* Lg, const float* Lb, float* snap) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    snap[3 * (size_t)i] = Lr[i]; snap[3 * (size_t)i + 1] = Lg[i]; snap[3 * (size_t)i + 2] = Lb[i];
+}
+
+// MLP mode 2: per-slot target (L - L_snapshot) / beta, flag valid finite samples
+__global__ void __launch_bounds__(256) k_targets(int n, const float* Lr, const float* Lg, const float* Lb, const float* snap,
+                                                 const float* rec_beta, const uint32_t* rec_valid, float* target,
+                                                 uint32_t* flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float L[3] = {Lr[i], Lg[i], Lb[i]};
+    bool ok = rec_valid[i] != 0u;
+    for (int k = 0; k < 3; k++) {
+        float b = rec_beta[3 * (size_t)i + k];
+        float t = (b > 0.0f) ? (L[k] - snap[3 * (size_t)i + k]) / b : 0.0f;
+        if (__builtin_isnan(t) || __builtin_isinf(t)) ok = false;
+        target[3 * (size_t)i + k] = t;
+    }
+    flag[i] = ok ? 1u : 0u;
 }
 
 // Per pixel: add this chunk's samples in sample order (path_tracer.cu:164-165),
@@ -664,7 +732,31 @@ DevCamera make_camera(const float pos[3], const float target[3], const float up[
     return cam;
 }
 
-int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevParams& p, float* film_dev) {
+__global__ void __launch_bounds__(256) k_scatter_train(int n, const uint32_t* flag, const uint32_t* pos, int64_t base,
+                                                       int64_t cap, const float* rec_in9, const float* tgt, float* out_in9,
+                                                       float* out_t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    int64_t j = base + pos[i];
+    if (j >= cap) return;
+    for (int k = 0; k < 9; k++) out_in9[j * 9 + k] = rec_in9[(size_t)i * 9 + k];
+    for (int k = 0; k < 3; k++) out_t[j * 3 + k] = tgt[(size_t)i * 3 + k];
+}
+
+// per-slot buffers of the MLP hooks (allocated on first use, sized to the pool)
+static int ensure_mlp_bufs(srt_ctx* c, int64_t slots) {
+    if (c->mlp_pool_cap >= slots) return SRT_OK;
+    for (auto& b : c->mlpbuf) dev_free(c, b);
+    const size_t n = (size_t)slots;
+    const size_t sizes[10] = {36 * n, 12 * n, 4 * n, 12 * n, 36 * n, 12 * n, 4 * n, 12 * n, 4 * n, 4 * n};
+    // 0 q_in9, 1 q_beta, 2 q_slot, 3 pred, 4 rec_in9, 5 rec_beta, 6 rec_valid, 7 snap/target, 8 flag, 9 scan
+    for (int j = 0; j < 10; j++)
+        if (int r = dev_alloc(c, c->mlpbuf[j], sizes[j])) return r;
+    c->mlp_pool_cap = slots;
+    return SRT_OK;
+}
+
+int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevParams& p, float* film_dev, TrainOut* train) {
     const int64_t wh = (int64_t)p.width * p.height;
     const int64_t total = wh * p.spp;
     int64_t pool = c->pool_request;
@@ -677,6 +769,21 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
         dev_free(c, c->film_acc);
         if (dev_alloc(c, c->film_acc, (size_t)wh * 3 * sizeof(float))) return SRT_E_OOM;
         c->film_pixels = wh;
+    }
+    MlpIO mio{};
+    if (p.mlp_mode != 0) {
+        if (int r = ensure_mlp_bufs(c, pool)) return r;
+        mio.q_in9 = (float*)c->mlpbuf[0].p;
+        mio.q_beta = (float*)c->mlpbuf[1].p;
+        mio.q_slot = (uint32_t*)c->mlpbuf[2].p;
+        mio.rec_in9 = (float*)c->mlpbuf[4].p;
+        mio.rec_beta = (float*)c->mlpbuf[5].p;
+        mio.rec_valid = (uint32_t*)c->mlpbuf[6].p;
+        for (int i = 0; i < 3; i++) {
+            float e = s->bounds_hi[i] - s->bounds_lo[i];
+            mio.lo[i] = s->bounds_lo[i];
+            mio.inv_ext[i] = e > 0.0f ? 1.0f / e : 1.0f;
+        }
     }
     DevScene S = dev_scene(s);
     uint32_t* counts = (uint32_t*)c->counters.p;
@@ -700,6 +807,7 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
     float* Lr = (float*)c->Lbuf[0].p; float* Lg = (float*)c->Lbuf[1].p; float* Lb = (float*)c->Lbuf[2].p;
     int32_t* gst = (int32_t*)c->gstack.p;
     const int shade_cap = num_cus(c) * 8;
+    if (train) train->count = 0;
     for (int64_t begin = 0; begin < total; begin += pool) {
         ChunkInfo ch;
         ch.path_begin = begin;
@@ -708,6 +816,7 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
         prof_begin(c, "raygen", &ev);
         k_raygen<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>(Q[0], Lr, Lg, Lb, cam, p, ch, counts);
         prof_end(c, "raygen", ev, ch.n);
+        if (p.mlp_mode == 2) k_zero<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>((float*)mio.rec_valid, (size_t)ch.n);
         int shade_grid = (int)std::min<int64_t>((ch.n + 255) / 256, shade_cap);
         int tg = trace_grid(c, ch.n);
         for (int depth = 0; depth < p.max_depth; depth++) {
@@ -717,7 +826,7 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
             else k_trace<false><<<tg, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
             prof_end(c, "trace", ev, 0);
             prof_begin(c, "shade", &ev);
-            k_shade<<<shade_grid, 256, 0, st>>>(S, p, ch, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi);
+            k_shade<<<shade_grid, 256, 0, st>>>(S, p, ch, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, mio);
             prof_end(c, "shade", ev, 0);
             if (p.use_nee) {
                 prof_begin(c, "shadow", &ev);
@@ -725,6 +834,36 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
                 else k_shadow<false><<<tg, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
                 prof_end(c, "shadow", ev, 0);
             }
+            if (depth == 1 && p.mlp_mode == 1 && depth + 1 < p.max_depth) {
+                // queued vertex-2 paths: prediction after the bounce's shadow rays (same L order as NEE first)
+                float* pred = (float*)c->mlpbuf[3].p;
+                if (int r = mlp_query_dev(c, mio.q_in9, counts + CNT_MLPQ, ch.n, pred)) return r;
+                k_mlp_apply<<<shade_grid, 256, 0, st>>>(counts, pred, mio.q_beta, mio.q_slot, Lr, Lg, Lb);
+            }
+            if (depth == 1 && p.mlp_mode == 2)
+                k_snapshot<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>(ch.n, Lr, Lg, Lb, (float*)c->mlpbuf[7].p);
+        }
+        if (p.mlp_mode == 2 && train && train->count < train->cap) {
+            float* snap_t = (float*)c->mlpbuf[7].p;     // snapshot, then overwritten by the targets
+            uint32_t* flag = (uint32_t*)c->mlpbuf[8].p;
+            uint32_t* pos = (uint32_t*)c->mlpbuf[9].p;
+            k_targets<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>(ch.n, Lr, Lg, Lb, snap_t, mio.rec_beta, mio.rec_valid,
+                                                                    snap_t, flag);
+            size_t tbytes = 0;
+            (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, flag, pos, ch.n, st);
+            if (c->scan_tmp.bytes < tbytes) {
+                dev_free(c, c->scan_tmp);
+                if (int r = dev_alloc(c, c->scan_tmp, tbytes)) return r;
+            }
+            if (int r = hip_check(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tbytes, flag, pos, ch.n, st), "scan"))
+                return r;
+            k_scatter_train<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>(ch.n, flag, pos, train->count, train->cap,
+                                                                           mio.rec_in9, snap_t, train->in9, train->target);
+            uint32_t lastpos = 0, lastflag = 0;
+            if (int r = hip_check(c, hipMemcpyAsync(&lastpos, pos + ch.n - 1, 4, hipMemcpyDeviceToHost, st), "count")) return r;
+            if (int r = hip_check(c, hipMemcpyAsync(&lastflag, flag + ch.n - 1, 4, hipMemcpyDeviceToHost, st), "count")) return r;
+            if (int r = hip_check(c, hipStreamSynchronize(st), "train samples")) return r;
+            train->count = std::min<int64_t>(train->cap, train->count + lastpos + lastflag);
         }
         bool last = begin + pool >= total;
         prof_begin(c, "finalize", &ev);

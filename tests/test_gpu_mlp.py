@@ -92,3 +92,38 @@ def test_training_is_deterministic_and_learns(ctx, tmp_path):
     assert np.array_equal(ctx.mlp_params(), runs[0][1])
     l_b = ctx.mlp_train_step(u, t)
     assert l_a == l_b and np.array_equal(ctx.mlp_params(), p_a)
+
+
+def test_integrator_hooks_train_and_query(ctx):
+    """Online training from the tracer's own vertex-2 samples, then paths that
+    stop at vertex 2 and take the MLP's prediction: finite, deterministic, and
+    close to the plain tracer's converged mean on the Cornell box."""
+    from srt_amd import abi, scenes
+    s = scenes.cornell()
+    h = ctx.upload(s)
+    ctx.mlp_create(seed=1, lr=1e-3)
+    with pytest.raises(abi.SrtError):
+        c2 = abi.Context(0)
+        h2 = c2.upload(s)
+        try:
+            c2.render(h2, s.camera, 32, 32, 1, 5, 4, 1, 1, 0, mlp_mode=1)     # no MLP on this context
+        finally:
+            c2.close()
+    losses = []
+    for f in range(40):
+        loss, n = ctx.mlp_train_frame(h, s.camera, 128, 128, 4, 5, 4, seed=1000 + f, max_samples=1 << 15, steps=4)
+        assert 0 < n <= 1 << 15
+        losses.append(loss)
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-5:]) < np.mean(losses[:5])
+    p_a = ctx.mlp_params()
+    img = ctx.render(h, s.camera, 64, 64, 64, 5, 4, 1, 1, 7, mlp_mode=1)
+    assert np.isfinite(img).all()
+    plain = ctx.render(h, s.camera, 64, 64, 256, 5, 4, 1, 1, 8)
+    assert abs(img[:, :3].mean() / plain[:, :3].mean() - 1) < 0.15
+    # same training again from the same seed: identical weights
+    ctx.mlp_create(seed=1, lr=1e-3)
+    for f in range(40):
+        ctx.mlp_train_frame(h, s.camera, 128, 128, 4, 5, 4, seed=1000 + f, max_samples=1 << 15, steps=4)
+    assert np.array_equal(ctx.mlp_params(), p_a)
+    ctx.release(h)
