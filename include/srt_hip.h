@@ -17,7 +17,7 @@
  *   - Host arrays passed in are copied; the caller keeps ownership.
  *     Scene handles are owned by the context.
  *   - A context is used from one host thread at a time, except srt_submit /
- *     srt_poll / srt_wait, which are thread-safe.
+ *     srt_poll / srt_wait / srt_serve_stats, which are thread-safe.
  *   - The film is RGBA float32, row-major, pixel index = iy * width + ix, row 0
  *     at the bottom of the image (src/passes/film.h:7-20).
  */
@@ -104,7 +104,8 @@ int srt_create(int device, size_t hbm_budget_bytes, srt_ctx** out);
 void srt_destroy(srt_ctx* ctx);
 const char* srt_last_error(const srt_ctx* ctx);
 /* tunables: "path_pool" (path slots per chunk), "profile" (0/1 per-kernel HIP events),
- * "count_visits" (0/1 count BVH node visits and triangle tests) */
+ * "count_visits" (0/1 count BVH node visits and triangle tests), "hbm_budget"
+ * (bytes; at least srt_hbm_used) */
 int srt_set_option(srt_ctx* ctx, const char* key, int64_t value);
 size_t srt_hbm_used(const srt_ctx* ctx);
 
@@ -190,6 +191,44 @@ int srt_mlp_load(srt_ctx* ctx, const char* path);
 int srt_device_alloc(srt_ctx* ctx, size_t bytes, void** out);
 int srt_device_free(srt_ctx* ctx, void* ptr);
 int srt_memcpy(srt_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t kind);
+
+/* ---- serving (no counterpart in the reference, whose renderers render one
+ *      frame per loop iteration synchronously, src/renderer/renderer.cu:176-275) ----
+ * srt_serve_start starts a dispatcher thread that owns the context's stream and
+ * path pool until srt_serve_stop.  Requests are admitted first-come first-served
+ * while their device film (28 bytes per pixel) fits in the HBM budget (a request
+ * that can never fit ends with status SRT_E_OOM), and the requests admitted on
+ * one scene share each chunk of `path_pool` paths (0: the "path_pool" option).
+ * A request's image is bit-identical to srt_render with the same arguments.
+ * While serving, only srt_submit, srt_poll, srt_wait, srt_serve_stats,
+ * srt_serve_stop, srt_last_error and srt_hbm_used may be called; every other
+ * entry point returns SRT_E_STATE.  Scenes must have a BVH when submitted. */
+typedef struct srt_request {
+    srt_scene scene;
+    srt_camera camera;
+    srt_pt_params params;      /* mlp_mode 0 or 1 */
+    int32_t width, height;
+    float* rgba_out;           /* host, width * height * 4 floats; written before the status turns 2 */
+} srt_request;
+
+typedef struct srt_result {
+    int32_t status;            /* 0 queued, 1 admitted, 2 done, < 0 error code */
+    int32_t reserved;
+    double submit_ms, admit_ms, done_ms;   /* since srt_serve_start */
+    int64_t paths;             /* width * height * spp */
+    int64_t chunks;            /* chunks the request took part in */
+} srt_result;
+
+int srt_serve_start(srt_ctx* ctx, int64_t path_pool);
+/* waits until every queued and admitted request is done, then stops the thread */
+int srt_serve_stop(srt_ctx* ctx);
+/* thread-safe; copies the request; the ticket stays valid until srt_serve_stop */
+int srt_submit(srt_ctx* ctx, const srt_request* request, uint64_t* ticket);
+int srt_poll(srt_ctx* ctx, uint64_t ticket, srt_result* out);
+/* blocks until the request is done or failed, or timeout_ms passes (< 0: no limit) */
+int srt_wait(srt_ctx* ctx, uint64_t ticket, int32_t timeout_ms, srt_result* out);
+/* paths served and chunks run since start; requests queued and admitted now */
+int srt_serve_stats(srt_ctx* ctx, int64_t* served_paths, int64_t* chunks, int32_t* queued, int32_t* active);
 
 /* ---- statistics (the reference's PROFILE scopes, src/utils/profiler.h:156-219) ----
  * Per-kernel totals since the last reset, filled in order of kernel name.

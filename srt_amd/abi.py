@@ -31,6 +31,7 @@ EXPORTED = [
     "srt_mlp_num_params", "srt_mlp_create", "srt_mlp_set_params", "srt_mlp_get_params", "srt_mlp_set_hparams",
     "srt_mlp_set_bounds", "srt_mlp_infer", "srt_mlp_train_step", "srt_mlp_save", "srt_mlp_load",
     "srt_mlp_train_frame",
+    "srt_serve_start", "srt_serve_stop", "srt_submit", "srt_poll", "srt_wait", "srt_serve_stats",
 ]
 
 TONEMAP = {"None": 0, "Clamp": 1, "Reinhard": 2, "Uncharted2": 3, "ACES": 4}
@@ -63,6 +64,19 @@ class PtParams(C.Structure):
 
 class BvhOpts(C.Structure):
     _fields_ = [("max_leaf", C.c_int32), ("cluster_size", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_uint64), ("camera", Camera), ("params", PtParams), ("width", C.c_int32),
+                ("height", C.c_int32), ("rgba_out", C.c_void_p)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("reserved", C.c_int32), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
+                ("done_ms", C.c_double), ("paths", C.c_int64), ("chunks", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 _lib = None
@@ -116,6 +130,12 @@ def load_library(path: str = LIB_PATH):
         "srt_mlp_load": (C.c_int, [P, C.c_char_p]),
         "srt_mlp_train_frame": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32,
                                           C.c_int32, C.c_int64, C.c_int32, P, P]),
+        "srt_serve_start": (C.c_int, [P, C.c_int64]),
+        "srt_serve_stop": (C.c_int, [P]),
+        "srt_submit": (C.c_int, [P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+        "srt_poll": (C.c_int, [P, C.c_uint64, C.POINTER(Result)]),
+        "srt_wait": (C.c_int, [P, C.c_uint64, C.c_int32, C.POINTER(Result)]),
+        "srt_serve_stats": (C.c_int, [P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -390,3 +410,43 @@ class Context:
         b = C.c_int64()
         self.check(self.lib.srt_last_counts(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    # ---- serving ----
+    def serve_start(self, path_pool=0):
+        self.check(self.lib.srt_serve_start(self.h, path_pool))
+        self._serve_out = {}
+
+    def serve_stop(self):
+        self.check(self.lib.srt_serve_stop(self.h))
+
+    def submit(self, scene, cam: dict, width, height, spp, max_depth=16, rr_depth=4, use_nee=1, use_mis=1, seed=0,
+               mlp_mode=0):
+        """Queue a render request; returns (ticket, output array filled when the request is done)."""
+        out = np.zeros((width * height, 4), np.float32)
+        r = Request()
+        r.scene = scene
+        r.camera = self.camera(cam)
+        r.params = PtParams(spp, max_depth, rr_depth, use_nee, use_mis, seed, mlp_mode, 0)
+        r.width = width
+        r.height = height
+        r.rgba_out = out.ctypes.data
+        t = C.c_uint64()
+        self.check(self.lib.srt_submit(self.h, C.byref(r), C.byref(t)))
+        self._serve_out[t.value] = out      # keeps the buffer alive while the dispatcher writes it
+        return t.value, out
+
+    def poll(self, ticket):
+        res = Result()
+        self.check(self.lib.srt_poll(self.h, ticket, C.byref(res)))
+        return res.as_dict()
+
+    def wait(self, ticket, timeout_ms=-1):
+        res = Result()
+        self.check(self.lib.srt_wait(self.h, ticket, timeout_ms, C.byref(res)))
+        return res.as_dict()
+
+    def serve_stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        q, x = C.c_int32(), C.c_int32()
+        self.check(self.lib.srt_serve_stats(self.h, C.byref(a), C.byref(b), C.byref(q), C.byref(x)))
+        return dict(served_paths=a.value, chunks=b.value, queued=q.value, active=x.value)

@@ -579,6 +579,7 @@ extern "C" {
 int srt_mlp_num_params(void) { return logical_count(); }
 
 int srt_mlp_create(srt_ctx* c, uint32_t seed, float lr) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     if (!(lr > 0.0f)) return set_err(c, SRT_E_INVALID, "learning rate must be positive");
     if (c->mlp) mlp_free_all(c);
@@ -626,6 +627,7 @@ int srt_mlp_create(srt_ctx* c, uint32_t seed, float lr) {
 }
 
 int srt_mlp_set_params(srt_ctx* c, const float* params, int32_t n) {
+    SRT_IDLE(c);
     if (!c || !params) return SRT_E_INVALID;
     MlpState* m = mlp_of(c);
     if (!m) return set_err(c, SRT_E_STATE, "no MLP");
@@ -640,6 +642,7 @@ int srt_mlp_set_params(srt_ctx* c, const float* params, int32_t n) {
 }
 
 int srt_mlp_get_params(srt_ctx* c, float* params, int32_t n) {
+    SRT_IDLE(c);
     if (!c || !params) return SRT_E_INVALID;
     MlpState* m = mlp_of(c);
     if (!m) return set_err(c, SRT_E_STATE, "no MLP");
@@ -653,6 +656,7 @@ int srt_mlp_get_params(srt_ctx* c, float* params, int32_t n) {
 }
 
 int srt_mlp_set_hparams(srt_ctx* c, float lr, float beta1, float beta2, float eps) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     MlpState* m = mlp_of(c);
     if (!m) return set_err(c, SRT_E_STATE, "no MLP");
@@ -662,6 +666,7 @@ int srt_mlp_set_hparams(srt_ctx* c, float lr, float beta1, float beta2, float ep
 }
 
 int srt_mlp_set_bounds(srt_ctx* c, const float lo[3], const float hi[3]) {
+    SRT_IDLE(c);
     if (!c || !lo || !hi) return SRT_E_INVALID;
     if (!mlp_of(c)) return set_err(c, SRT_E_STATE, "no MLP");
     mlp_bounds(c, lo, hi);
@@ -670,6 +675,7 @@ int srt_mlp_set_bounds(srt_ctx* c, const float lo[3], const float hi[3]) {
 
 // host arrays: in9 [n][9] raw features (already normalised to [0,1]), out3 [n][3]
 int srt_mlp_infer(srt_ctx* c, int32_t n, const float* in9, float* out3) {
+    SRT_IDLE(c);
     if (!c || n < 0 || (n && (!in9 || !out3))) return SRT_E_INVALID;
     MlpState* m = mlp_of(c);
     if (!m) return set_err(c, SRT_E_STATE, "no MLP");
@@ -686,6 +692,7 @@ int srt_mlp_infer(srt_ctx* c, int32_t n, const float* in9, float* out3) {
 // the logical parameter order; loss = mean relative L2 before the step
 int srt_mlp_train_step(srt_ctx* c, int32_t n, const float* in9, const float* target3, int32_t apply, float* loss,
                        float* grads) {
+    SRT_IDLE(c);
     if (!c || n <= 0 || !in9 || !target3) return SRT_E_INVALID;
     MlpState* m = mlp_of(c);
     if (!m) return set_err(c, SRT_E_STATE, "no MLP");
@@ -706,6 +713,7 @@ int srt_mlp_train_step(srt_ctx* c, int32_t n, const float* in9, const float* tar
 }
 
 int srt_mlp_save(srt_ctx* c, const char* path) {
+    SRT_IDLE(c);
     if (!c || !path) return SRT_E_INVALID;
     MlpState* m = mlp_of(c);
     if (!m) return set_err(c, SRT_E_STATE, "no MLP");
@@ -729,6 +737,7 @@ int srt_mlp_save(srt_ctx* c, const char* path) {
 }
 
 int srt_mlp_load(srt_ctx* c, const char* path) {
+    SRT_IDLE(c);
     if (!c || !path) return SRT_E_INVALID;
     MlpState* m = mlp_of(c);
     if (!m) return set_err(c, SRT_E_STATE, "no MLP: call srt_mlp_create first");

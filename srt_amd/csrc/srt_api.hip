@@ -17,6 +17,7 @@ int set_err(srt_ctx* c, int code, const char* fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
+    std::lock_guard<std::mutex> lk(c->mu);
     c->err = buf;
     return code;
 }
@@ -30,7 +31,7 @@ int hip_check(srt_ctx* c, hipError_t e, const char* what) {
 int dev_alloc(srt_ctx* c, DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (c->used + bytes > c->budget)
-        return set_err(c, SRT_E_OOM, "allocation of %zu bytes exceeds the HBM budget (%zu of %zu used)", bytes, c->used, c->budget);
+        return set_err(c, SRT_E_OOM, "allocation of %zu bytes exceeds the HBM budget (%zu of %zu used)", bytes, c->used.load(), c->budget);
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) return hip_check(c, e, "hipMalloc");
@@ -165,7 +166,7 @@ void srt_destroy(srt_ctx* c) {
     for (auto& kv : c->user_bufs) { DevBuf b; b.p = kv.first; b.bytes = kv.second; dev_free(c, b); }
     c->user_bufs.clear();
     for (auto& b : c->mlpbuf) dev_free(c, b);
-    dev_free(c, c->scan_tmp); dev_free(c, c->train_in9); dev_free(c, c->train_t);
+    dev_free(c, c->scan_tmp); dev_free(c, c->train_in9); dev_free(c, c->train_t); dev_free(c, c->segbuf);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -174,6 +175,7 @@ void srt_destroy(srt_ctx* c) {
 const char* srt_last_error(const srt_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
+    SRT_IDLE(c);
     if (!c || !key) return SRT_E_INVALID;
     if (!strcmp(key, "path_pool")) {
         if (value < 64 || value > (int64_t)0x7fffffff) return set_err(c, SRT_E_INVALID, "path_pool out of range");
@@ -182,12 +184,19 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
     }
     if (!strcmp(key, "profile")) { c->profile = value != 0; return SRT_OK; }
     if (!strcmp(key, "count_visits")) { c->count_visits = value != 0; return SRT_OK; }
+    if (!strcmp(key, "hbm_budget")) {
+        if (value <= 0 || (size_t)value < c->used.load())
+            return set_err(c, SRT_E_INVALID, "hbm_budget must be positive and cover the %zu bytes in use", c->used.load());
+        c->budget = (size_t)value;
+        return SRT_OK;
+    }
     return set_err(c, SRT_E_INVALID, "unknown option '%s'", key);
 }
 
-size_t srt_hbm_used(const srt_ctx* c) { return c ? c->used : 0; }
+size_t srt_hbm_used(const srt_ctx* c) { return c ? c->used.load() : 0; }
 
 int srt_scene_upload(srt_ctx* c, const srt_scene_desc* d, srt_scene* out) {
+    SRT_IDLE(c);
     if (!c || !d || !out) return SRT_E_INVALID;
     if (d->num_vertices <= 0 || !d->positions) return set_err(c, SRT_E_INVALID, "scene needs vertices");
     if (d->num_triangles <= 0 || !d->indices) return set_err(c, SRT_E_INVALID, "scene needs triangles");
@@ -275,6 +284,7 @@ int srt_scene_upload(srt_ctx* c, const srt_scene_desc* d, srt_scene* out) {
 }
 
 int srt_scene_release(srt_ctx* c, srt_scene h) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene %llu", (unsigned long long)h);
@@ -286,6 +296,7 @@ int srt_scene_release(srt_ctx* c, srt_scene h) {
 
 int srt_scene_lights(srt_ctx* c, srt_scene h, float* light_cdf, int32_t* light_mesh, float* area_cdf, float* mesh_area,
                      float* weight_sum) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
@@ -298,6 +309,7 @@ int srt_scene_lights(srt_ctx* c, srt_scene h, float* light_cdf, int32_t* light_m
 }
 
 int srt_scene_build_bvh(srt_ctx* c, srt_scene h, const srt_bvh_opts* o) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
@@ -313,6 +325,7 @@ int srt_scene_build_bvh(srt_ctx* c, srt_scene h, const srt_bvh_opts* o) {
 }
 
 int srt_scene_bvh_info(srt_ctx* c, srt_scene h, int32_t info[8]) {
+    SRT_IDLE(c);
     if (!c || !info) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
@@ -323,6 +336,7 @@ int srt_scene_bvh_info(srt_ctx* c, srt_scene h, int32_t info[8]) {
 }
 
 int srt_scene_bvh_download(srt_ctx* c, srt_scene h, void* nodes, uint32_t* order, float* tris) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
@@ -335,6 +349,7 @@ int srt_scene_bvh_download(srt_ctx* c, srt_scene h, void* nodes, uint32_t* order
 }
 
 int srt_scene_set_skin(srt_ctx* c, srt_scene h, const int32_t* bone_ids, const float* bone_weights) {
+    SRT_IDLE(c);
     if (!c || !bone_ids || !bone_weights) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
@@ -358,6 +373,7 @@ int srt_scene_set_skin(srt_ctx* c, srt_scene h, const int32_t* bone_ids, const f
 }
 
 int srt_scene_skin_refit(srt_ctx* c, srt_scene h, const float* bones_3x4, int32_t n_bones) {
+    SRT_IDLE(c);
     if (!c || (n_bones > 0 && !bones_3x4)) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
@@ -371,6 +387,7 @@ int srt_scene_skin_refit(srt_ctx* c, srt_scene h, const float* bones_3x4, int32_
 }
 
 int srt_scene_download_vertices(srt_ctx* c, srt_scene h, float* positions, float* normals, float* tangents) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
@@ -383,6 +400,7 @@ int srt_scene_download_vertices(srt_ctx* c, srt_scene h, float* positions, float
 
 int srt_trace_rays(srt_ctx* c, srt_scene h, int32_t n, const float* o, const float* d, float tmin, const float* tmax,
                    int32_t any, float* t, float* u, float* v, int32_t* prim) {
+    SRT_IDLE(c);
     if (!c || n < 0 || !o || !d || !tmax || !prim || (!any && (!t || !u || !v))) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
@@ -411,6 +429,7 @@ int srt_trace_rays(srt_ctx* c, srt_scene h, int32_t n, const float* o, const flo
 
 int srt_render(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_params* p, int32_t W, int32_t H, float* rgba_out,
                int32_t out_on_device) {
+    SRT_IDLE(c);
     if (!c || !cam || !p) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
@@ -453,6 +472,7 @@ int srt_render(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_para
 
 int srt_mlp_train_frame(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_params* p, int32_t W, int32_t H,
                         int64_t max_samples, int32_t steps, float* loss_out, int64_t* samples_out) {
+    SRT_IDLE(c);
     if (!c || !cam || !p || W <= 0 || H <= 0 || max_samples <= 0 || steps <= 0) return SRT_E_INVALID;
     if (!mlp_ready(c)) return set_err(c, SRT_E_STATE, "no MLP: call srt_mlp_create first");
     SceneRec* s = find_scene(c, h);
@@ -501,6 +521,7 @@ int srt_mlp_train_frame(srt_ctx* c, srt_scene h, const srt_camera* cam, const sr
 }
 
 int srt_film_download(srt_ctx* c, float* rgba, int64_t n) {
+    SRT_IDLE(c);
     if (!c || !rgba || n < 0) return SRT_E_INVALID;
     if (!c->film_out.p || (size_t)n * 16 > c->film_out.bytes) return set_err(c, SRT_E_STATE, "no film of that size");
     return hip_check(c, hipMemcpy(rgba, c->film_out.p, (size_t)n * 16, hipMemcpyDeviceToHost), "film download");
@@ -512,6 +533,7 @@ static int post_buffer(srt_ctx* c, float* host, int64_t n, DevBuf& tmp) {
 }
 
 int srt_tonemap(srt_ctx* c, float* rgba, int64_t n, int32_t type, float exposure, int32_t use_gamma, int32_t on_device) {
+    SRT_IDLE(c);
     if (!c || !rgba || n < 0 || n > 0x7fffffff) return SRT_E_INVALID;
     if (type < 0 || type > 4) return set_err(c, SRT_E_INVALID, "unknown tonemap type %d", type);
     if (n == 0) return SRT_OK;
@@ -525,6 +547,7 @@ int srt_tonemap(srt_ctx* c, float* rgba, int64_t n, int32_t type, float exposure
 }
 
 int srt_accumulate(srt_ctx* c, float* acc, float* rgba, int64_t n, int32_t frame_count, int32_t on_device) {
+    SRT_IDLE(c);
     if (!c || !acc || !rgba || n < 0 || n > 0x7fffffff || frame_count < 0) return SRT_E_INVALID;
     if (n == 0) return SRT_OK;
     if (on_device) return accumulate_dev(c, acc, rgba, (int)n, frame_count);
@@ -540,6 +563,7 @@ int srt_accumulate(srt_ctx* c, float* acc, float* rgba, int64_t n, int32_t frame
 }
 
 int srt_device_alloc(srt_ctx* c, size_t bytes, void** out) {
+    SRT_IDLE(c);
     if (!c || !out) return SRT_E_INVALID;
     DevBuf b;
     if (int r = dev_alloc(c, b, bytes)) return r;
@@ -549,6 +573,7 @@ int srt_device_alloc(srt_ctx* c, size_t bytes, void** out) {
 }
 
 int srt_device_free(srt_ctx* c, void* p) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     auto it = c->user_bufs.find(p);
     if (it == c->user_bufs.end()) return set_err(c, SRT_E_NOT_FOUND, "pointer not allocated by srt_device_alloc");
@@ -562,6 +587,7 @@ int srt_device_free(srt_ctx* c, void* p) {
 }
 
 int srt_memcpy(srt_ctx* c, void* dst, const void* src, size_t bytes, int32_t kind) {
+    SRT_IDLE(c);
     if (!c || (!dst && bytes) || (!src && bytes) || kind < 0 || kind > 2) return SRT_E_INVALID;
     if (bytes == 0) return SRT_OK;
     hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
@@ -570,6 +596,7 @@ int srt_memcpy(srt_ctx* c, void* dst, const void* src, size_t bytes, int32_t kin
 }
 
 int srt_stats(srt_ctx* c, char* names, int32_t names_len, double* ms, int64_t* launches, int64_t* items, int32_t max) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     (void)hipStreamSynchronize(c->stream);
     prof_collect(c);
@@ -591,6 +618,7 @@ int srt_stats(srt_ctx* c, char* names, int32_t names_len, double* ms, int64_t* l
 }
 
 void srt_stats_reset(srt_ctx* c) {
+    if (c && c->serve) return;
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
     prof_collect(c);
@@ -598,6 +626,7 @@ void srt_stats_reset(srt_ctx* c) {
 }
 
 int srt_last_counts(srt_ctx* c, int64_t* segments, int64_t* shadow) {
+    SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     if (segments) *segments = c->last_segments;
     if (shadow) *shadow = c->last_shadow_rays;
@@ -605,6 +634,7 @@ int srt_last_counts(srt_ctx* c, int64_t* segments, int64_t* shadow) {
 }
 
 int srt_last_visits(srt_ctx* c, int64_t out[4]) {
+    SRT_IDLE(c);
     if (!c || !out) return SRT_E_INVALID;
     for (int k = 0; k < 4; k++) out[k] = c->last_visits[k];
     return SRT_OK;

@@ -7,6 +7,7 @@
 #include <vector>
 #include <map>
 #include <mutex>
+#include <atomic>
 #include "dev_math.h"
 #include "../../include/srt_hip.h"
 
@@ -74,6 +75,25 @@ struct DevParams {
     int32_t width, height;
     int32_t mlp_mode;      // 0 plain, 1 query the radiance MLP at vertex 2, 2 record training samples
 };
+
+// A segment is one request's contiguous range of paths inside a chunk (path
+// id = sample * W * H + pixel, in increasing order per request).  Segments of
+// one chunk share the path queues; each keeps its own camera, resolution,
+// sample count, depth settings and film.  A single render is one segment per
+// chunk, so a request renders the same bits alone or batched with others.
+struct Segment {
+    int64_t path_begin;
+    int32_t n;               // paths of this segment in the chunk
+    int32_t slot_offset;     // first slot of the segment in the chunk
+    int32_t wh, width, height, spp;
+    uint32_t seed;
+    int32_t max_depth, rr_depth, use_nee, use_mis, mlp_mode;
+    int32_t last;            // the request's final paths: write film_out
+    DevCamera cam;
+    float* film_acc;         // 3 floats per pixel, zeroed before the first chunk
+    float4* film_out;
+};
+constexpr int kMaxSegments = 255;
 
 // Radiance-MLP hooks of the integrator.  Vertex 2 is the second surface hit
 // (the hit of the trace at depth 1).  Mode 1: after NEE at vertex 2 the path
@@ -158,10 +178,10 @@ struct KernelStat {
 struct srt_ctx {
     int device = 0;
     size_t budget = 0;
-    size_t used = 0;
+    std::atomic<size_t> used{0};          // the serving thread allocates films too
     hipStream_t stream = nullptr;
     std::string err;
-    std::mutex mu;
+    std::mutex mu;                        // guards err
     uint64_t next_scene = 1;
     std::map<uint64_t, srt::SceneRec*> scenes;
     // render pool
@@ -181,12 +201,20 @@ struct srt_ctx {
     bool count_visits = false;
     std::map<void*, size_t> user_bufs;    // srt_device_alloc allocations
     srt::DevBuf mlpbuf[10], scan_tmp, train_in9, train_t;   // MLP hooks of the integrator
+    srt::DevBuf segbuf;                   // device copy of the chunk's segment table
     int64_t mlp_pool_cap = 0, train_cap = 0;
-    void* serve = nullptr;                // serving loop state
+    void* serve = nullptr;                // serving loop state; non-null while a dispatcher runs
     void* mlp = nullptr;                  // radiance MLP state
     void (*serve_free)(srt_ctx*) = nullptr;
     void (*mlp_free)(srt_ctx*) = nullptr;
 };
+
+// Entry points other than the serving ones refuse to run while the serving
+// thread owns the context's stream and path pool.
+#define SRT_IDLE(c)                                                                                   \
+    do {                                                                                              \
+        if ((c) && (c)->serve) return srt::set_err((c), SRT_E_STATE, "the context is serving; call srt_serve_stop first"); \
+    } while (0)
 
 namespace srt {
 // allocation helpers (budget-checked)
@@ -213,6 +241,7 @@ struct TrainOut {
 };
 int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevParams& p, float* film_dev,
                      TrainOut* train = nullptr);
+int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, TrainOut* train = nullptr);
 // MLP (mlp.hip)
 int mlp_infer_dev(srt_ctx* c, const float* in9, int n, float* out3);
 int mlp_query_dev(srt_ctx* c, const float* in9, const uint32_t* count_dev, int max_n, float* out3);

@@ -176,26 +176,37 @@ SRT_D int wave_append(bool pred, uint32_t* counter) {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-struct ChunkInfo {
-    int64_t path_begin;   // first path id of the chunk (path id = sample * WH + pixel)
-    int32_t n;            // paths in the chunk
-    int32_t wh;           // pixels
-};
+// Segment lookup: the segment whose slot range holds slot i (segments are in
+// increasing slot_offset order)
+SRT_D int find_segment(const Segment* segs, int nseg, uint32_t i) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if ((uint32_t)segs[mid].slot_offset <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
 
-__global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* Lg, float* Lb, DevCamera cam, DevParams p,
-                                                ChunkInfo ch, uint32_t* counts) {
+// One camera ray per slot (path_tracer.cu:66-68, camera.h:107-117, :27-32).
+// Slot i belongs to segment k: path id pid = path_begin + (i - slot_offset),
+// sample = pid / (W H), pixel = pid mod (W H).
+__global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* Lg, float* Lb, const Segment* segs, int nseg,
+                                                int n, uint32_t* counts) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) counts[CNT_Q0] = (uint32_t)ch.n;
-    if (i >= ch.n) return;
-    int64_t pid = ch.path_begin + i;
-    uint32_t sample = (uint32_t)(pid / ch.wh);
-    uint32_t pixel = (uint32_t)(pid - (int64_t)sample * ch.wh);
-    int ix = (int)(pixel % (uint32_t)p.width), iy = (int)(pixel / (uint32_t)p.width);
-    Pcg32 rng = path_rng(pixel, p.seed, sample);
+    if (i == 0) counts[CNT_Q0] = (uint32_t)n;
+    if (i >= n) return;
+    const int k = find_segment(segs, nseg, (uint32_t)i);
+    const Segment& sg = segs[k];
+    int64_t pid = sg.path_begin + (i - sg.slot_offset);
+    uint32_t sample = (uint32_t)(pid / sg.wh);
+    uint32_t pixel = (uint32_t)(pid - (int64_t)sample * sg.wh);
+    int ix = (int)(pixel % (uint32_t)sg.width), iy = (int)(pixel / (uint32_t)sg.width);
+    const DevCamera& cam = sg.cam;
+    Pcg32 rng = path_rng(pixel, sg.seed, sample);
     float r0 = rng.next_float();
-    float xx = ((float)ix + r0) / (float)p.width;
+    float xx = ((float)ix + r0) / (float)sg.width;
     float r1 = rng.next_float();
-    float yy = ((float)iy + r1) / (float)p.height;
+    float yy = ((float)iy + r1) / (float)sg.height;
     float px = (xx - 0.5f) * cam.nw, py = (yy - 0.5f) * cam.nh;
     f3 dir = normalize(mk(px, py, 1.0f));
     f3 X = ld3(cam.X), Y = ld3(cam.Y), Z = ld3(cam.Z), CP = ld3(cam.pos);
@@ -206,7 +217,7 @@ __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* L
     q.dx[i] = rd.x; q.dy[i] = rd.y; q.dz[i] = rd.z;
     q.br[i] = 1.0f; q.bg[i] = 1.0f; q.bb[i] = 1.0f;
     q.pdf[i] = 1.0f;
-    q.flags[i] = 1u;
+    q.flags[i] = 1u | ((uint32_t)k << 8);        // bit 0: specular, bits 8..: segment
     q.rng[i] = rng.state;
     q.slot[i] = (uint32_t)i;
     Lr[i] = 0.0f; Lg[i] = 0.0f; Lb[i] = 0.0f;
@@ -285,14 +296,13 @@ SRT_D bool finite3(f3 v) {
 }
 
 // One bounce for every entry of queue `qi` (path_tracer.cu:80-161).
-__global__ void __launch_bounds__(256) k_shade(DevScene S, DevParams P, ChunkInfo ch, int depth, PathQueue qin, HitBuf h,
+__global__ void __launch_bounds__(256) k_shade(DevScene S, const Segment* segs, int depth, PathQueue qin, HitBuf h,
                                                PathQueue qout, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
                                                uint32_t* counts, int qi, MlpIO mio) {
     const int lane = threadIdx.x & 63;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
     const uint32_t count = counts[qi];
-    const bool last = (depth + 1 >= P.max_depth);
     for (uint32_t base = (uint32_t)wave * 64u; base < count; base += (uint32_t)nwaves * 64u) {
         const uint32_t i = base + (uint32_t)lane;
         bool emit_next = false, emit_shadow = false;
@@ -313,10 +323,14 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, DevParams P, ChunkInf
             f3 rd = mk(qin.dx[i], qin.dy[i], qin.dz[i]);
             f3 beta = mk(qin.br[i], qin.bg[i], qin.bb[i]);
             float scatter_pdf = qin.pdf[i];
-            bool specular = (qin.flags[i] & 1u) != 0u;
+            const uint32_t flags = qin.flags[i];
+            bool specular = (flags & 1u) != 0u;
+            const uint32_t segk = flags >> 8;
+            const Segment& P = segs[segk];
+            const bool last = (depth + 1 >= P.max_depth);
             int32_t tri = h.tri[i];
-            const int64_t pid = ch.path_begin + slot;
-            const uint32_t sample = (uint32_t)(pid / ch.wh);
+            const int64_t pid = P.path_begin + (int64_t)(slot - (uint32_t)P.slot_offset);
+            const uint32_t sample = (uint32_t)(pid / P.wh);
             Pcg32 rng = rng_resume(qin.rng[i], sample);
             if (tri < 0) {
                 // miss (path_tracer.cu:81-85)
@@ -435,7 +449,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, DevParams P, ChunkInf
                         if (alive) {
                             emit_next = true;
                             n_o = pos; n_d = ms.wi; n_beta = beta; n_pdf = ms.pdf;
-                            n_flags = mat_is_specular(m) ? 1u : 0u;
+                            n_flags = (mat_is_specular(m) ? 1u : 0u) | (segk << 8);
                             n_rng = rng.state;
                         }
                     }
@@ -539,20 +553,22 @@ __global__ void __launch_bounds__(256) k_targets(int n, const float* Lr, const f
 }
 
 // Per pixel: add this chunk's samples in sample order (path_tracer.cu:164-165),
-// dropping non-finite samples but keeping the 1/spp weight.
-__global__ void __launch_bounds__(256) k_finalize(const float* Lr, const float* Lg, const float* Lb, float* acc, ChunkInfo ch,
-                                                  int spp, int last_chunk, float4* film) {
+// dropping non-finite samples but keeping the 1/spp weight.  One grid row per
+// segment; a request's film is written when its last segment completes.
+__global__ void __launch_bounds__(256) k_finalize(const float* Lr, const float* Lg, const float* Lb, const Segment* segs) {
+    const Segment& sg = segs[blockIdx.y];
     int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ch.wh) return;
-    const int64_t end = ch.path_begin + ch.n;
-    int64_t s0 = (ch.path_begin - q + ch.wh - 1) / ch.wh;
+    if (q >= sg.wh) return;
+    const int64_t end = sg.path_begin + sg.n;
+    int64_t s0 = (sg.path_begin - q + sg.wh - 1) / sg.wh;
     if (s0 < 0) s0 = 0;
+    float* acc = sg.film_acc;
     float ax = acc[3 * (size_t)q], ay = acc[3 * (size_t)q + 1], az = acc[3 * (size_t)q + 2];
-    const float fspp = (float)spp;
+    const float fspp = (float)sg.spp;
     for (int64_t s = s0;; s++) {
-        int64_t pid = s * ch.wh + q;
+        int64_t pid = s * sg.wh + q;
         if (pid >= end) break;
-        size_t slot = (size_t)(pid - ch.path_begin);
+        size_t slot = (size_t)(sg.slot_offset + (pid - sg.path_begin));
         f3 L = mk(Lr[slot], Lg[slot], Lb[slot]);
         if (finite3(L)) {
             ax = ax + L.x / fspp;
@@ -561,7 +577,7 @@ __global__ void __launch_bounds__(256) k_finalize(const float* Lr, const float* 
         }
     }
     acc[3 * (size_t)q] = ax; acc[3 * (size_t)q + 1] = ay; acc[3 * (size_t)q + 2] = az;
-    if (last_chunk) film[q] = make_float4(ax, ay, az, 1.0f);
+    if (sg.last && sg.film_out) sg.film_out[q] = make_float4(ax, ay, az, 1.0f);
 }
 
 __global__ void k_zero(float* p, size_t n) {
@@ -756,23 +772,32 @@ static int ensure_mlp_bufs(srt_ctx* c, int64_t slots) {
     return SRT_OK;
 }
 
-int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevParams& p, float* film_dev, TrainOut* train) {
-    const int64_t wh = (int64_t)p.width * p.height;
-    const int64_t total = wh * p.spp;
-    int64_t pool = c->pool_request;
-    if (pool > total) pool = total;
-    if (pool > 0x7fffffff) pool = 0x7fffffff;
-    if (int r = ensure_pool(c, pool)) return r;
-    const int tgrid = trace_grid(c, pool);
-    if (int r = ensure_gstack(c, (int64_t)tgrid * kTraceBlock)) return r;
-    if (c->film_pixels < wh) {
-        dev_free(c, c->film_acc);
-        if (dev_alloc(c, c->film_acc, (size_t)wh * 3 * sizeof(float))) return SRT_E_OOM;
-        c->film_pixels = wh;
+// Runs one chunk: segments (in slot order) of one scene share the path queues.
+int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, TrainOut* train) {
+    if (nseg <= 0 || nseg > kMaxSegments) return set_err(c, SRT_E_LIMIT, "bad segment count %d", nseg);
+    int64_t n = 0;
+    int max_depth = 0, mlp_mode = 0;
+    bool any_nee = false;
+    for (int k = 0; k < nseg; k++) {
+        if (segs_host[k].slot_offset != n) return set_err(c, SRT_E_INVALID, "segments must be contiguous");
+        n += segs_host[k].n;
+        max_depth = std::max(max_depth, (int)segs_host[k].max_depth);
+        any_nee |= segs_host[k].use_nee != 0;
+        mlp_mode = std::max(mlp_mode, (int)segs_host[k].mlp_mode);
     }
+    if (n <= 0 || n > 0x7fffffff) return set_err(c, SRT_E_LIMIT, "chunk of %lld paths", (long long)n);
+    if (int r = ensure_pool(c, n)) return r;
+    const int tgrid = trace_grid(c, n);
+    if (int r = ensure_gstack(c, (int64_t)tgrid * kTraceBlock)) return r;
+    if (!c->segbuf.p) if (int r = dev_alloc(c, c->segbuf, sizeof(Segment) * kMaxSegments)) return r;
+    hipStream_t st = c->stream;
+    // the segment table is small; a synchronous copy keeps the host array free
+    if (int r = hip_check(c, hipMemcpyAsync(c->segbuf.p, segs_host, sizeof(Segment) * nseg, hipMemcpyHostToDevice, st), "segments"))
+        return r;
+    const Segment* segs = (const Segment*)c->segbuf.p;
     MlpIO mio{};
-    if (p.mlp_mode != 0) {
-        if (int r = ensure_mlp_bufs(c, pool)) return r;
+    if (mlp_mode != 0) {
+        if (int r = ensure_mlp_bufs(c, n)) return r;
         mio.q_in9 = (float*)c->mlpbuf[0].p;
         mio.q_beta = (float*)c->mlpbuf[1].p;
         mio.q_slot = (uint32_t*)c->mlpbuf[2].p;
@@ -788,13 +813,6 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
     DevScene S = dev_scene(s);
     uint32_t* counts = (uint32_t*)c->counters.p;
     unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + 64);
-    hipStream_t st = c->stream;
-    hipEvent_t ev;
-    {
-        size_t n3 = (size_t)wh * 3;
-        k_zero<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>((float*)c->film_acc.p, n3);
-        k_zero<<<1, 64, 0, st>>>((float*)totals, 16);
-    }
     PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
     HitBuf H;
     H.t = (float*)c->hits[0].p; H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
@@ -806,70 +824,98 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
     }
     float* Lr = (float*)c->Lbuf[0].p; float* Lg = (float*)c->Lbuf[1].p; float* Lb = (float*)c->Lbuf[2].p;
     int32_t* gst = (int32_t*)c->gstack.p;
-    const int shade_cap = num_cus(c) * 8;
+    const int shade_grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)num_cus(c) * 8);
+    const int nblk = (int)((n + 255) / 256);
+    hipEvent_t ev;
+    prof_begin(c, "raygen", &ev);
+    k_raygen<<<nblk, 256, 0, st>>>(Q[0], Lr, Lg, Lb, segs, nseg, (int)n, counts);
+    prof_end(c, "raygen", ev, n);
+    if (mlp_mode == 2) k_zero<<<nblk, 256, 0, st>>>((float*)mio.rec_valid, (size_t)n);
+    for (int depth = 0; depth < max_depth; depth++) {
+        int qi = depth & 1;
+        prof_begin(c, "trace", &ev);
+        if (c->count_visits) k_trace<true><<<tgrid, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
+        else k_trace<false><<<tgrid, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
+        prof_end(c, "trace", ev, 0);
+        prof_begin(c, "shade", &ev);
+        k_shade<<<shade_grid, 256, 0, st>>>(S, segs, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, mio);
+        prof_end(c, "shade", ev, 0);
+        if (any_nee) {
+            prof_begin(c, "shadow", &ev);
+            if (c->count_visits) k_shadow<true><<<tgrid, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
+            else k_shadow<false><<<tgrid, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
+            prof_end(c, "shadow", ev, 0);
+        }
+        if (depth == 1 && mlp_mode == 1) {
+            // queued vertex-2 paths: prediction after the bounce's shadow rays (NEE first, as in the plain order)
+            float* pred = (float*)c->mlpbuf[3].p;
+            if (int r = mlp_query_dev(c, mio.q_in9, counts + CNT_MLPQ, (int)n, pred)) return r;
+            k_mlp_apply<<<shade_grid, 256, 0, st>>>(counts, pred, mio.q_beta, mio.q_slot, Lr, Lg, Lb);
+        }
+        if (depth == 1 && mlp_mode == 2) k_snapshot<<<nblk, 256, 0, st>>>((int)n, Lr, Lg, Lb, (float*)c->mlpbuf[7].p);
+    }
+    if (mlp_mode == 2 && train && train->count < train->cap) {
+        float* snap_t = (float*)c->mlpbuf[7].p;     // snapshot, then overwritten by the targets
+        uint32_t* flag = (uint32_t*)c->mlpbuf[8].p;
+        uint32_t* pos = (uint32_t*)c->mlpbuf[9].p;
+        k_targets<<<nblk, 256, 0, st>>>((int)n, Lr, Lg, Lb, snap_t, mio.rec_beta, mio.rec_valid, snap_t, flag);
+        size_t tbytes = 0;
+        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, flag, pos, (int)n, st);
+        if (c->scan_tmp.bytes < tbytes) {
+            dev_free(c, c->scan_tmp);
+            if (int r = dev_alloc(c, c->scan_tmp, tbytes)) return r;
+        }
+        if (int r = hip_check(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tbytes, flag, pos, (int)n, st), "scan"))
+            return r;
+        k_scatter_train<<<nblk, 256, 0, st>>>((int)n, flag, pos, train->count, train->cap, mio.rec_in9, snap_t, train->in9,
+                                              train->target);
+        uint32_t lastpos = 0, lastflag = 0;
+        if (int r = hip_check(c, hipMemcpyAsync(&lastpos, pos + n - 1, 4, hipMemcpyDeviceToHost, st), "count")) return r;
+        if (int r = hip_check(c, hipMemcpyAsync(&lastflag, flag + n - 1, 4, hipMemcpyDeviceToHost, st), "count")) return r;
+        if (int r = hip_check(c, hipStreamSynchronize(st), "train samples")) return r;
+        train->count = std::min<int64_t>(train->cap, train->count + lastpos + lastflag);
+    }
+    int maxwh = 0;
+    for (int k = 0; k < nseg; k++) maxwh = std::max(maxwh, (int)segs_host[k].wh);
+    prof_begin(c, "finalize", &ev);
+    k_finalize<<<dim3((unsigned)((maxwh + 255) / 256), (unsigned)nseg), 256, 0, st>>>(Lr, Lg, Lb, segs);
+    prof_end(c, "finalize", ev, maxwh);
+    // the host segment array may be reused by the caller after this returns
+    return hip_check(c, hipStreamSynchronize(st), "chunk");
+}
+
+int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevParams& p, float* film_dev, TrainOut* train) {
+    const int64_t wh = (int64_t)p.width * p.height;
+    const int64_t total = wh * p.spp;
+    int64_t pool = c->pool_request;
+    if (pool > total) pool = total;
+    if (pool > 0x7fffffff) pool = 0x7fffffff;
+    if (c->film_pixels < wh) {
+        dev_free(c, c->film_acc);
+        if (dev_alloc(c, c->film_acc, (size_t)wh * 3 * sizeof(float))) return SRT_E_OOM;
+        c->film_pixels = wh;
+    }
+    hipStream_t st = c->stream;
+    unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + 64);
+    {
+        size_t n3 = (size_t)wh * 3;
+        k_zero<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>((float*)c->film_acc.p, n3);
+        k_zero<<<1, 64, 0, st>>>((float*)totals, 16);
+    }
     if (train) train->count = 0;
+    Segment sg;
+    sg.slot_offset = 0;
+    sg.wh = (int32_t)wh; sg.width = p.width; sg.height = p.height; sg.spp = p.spp;
+    sg.seed = p.seed; sg.max_depth = p.max_depth; sg.rr_depth = p.rr_depth; sg.use_nee = p.use_nee;
+    sg.use_mis = p.use_mis; sg.mlp_mode = p.mlp_mode;
+    sg.cam = cam;
+    sg.film_acc = (float*)c->film_acc.p;
+    sg.film_out = (float4*)film_dev;
     for (int64_t begin = 0; begin < total; begin += pool) {
-        ChunkInfo ch;
-        ch.path_begin = begin;
-        ch.n = (int32_t)((total - begin < pool) ? (total - begin) : pool);
-        ch.wh = (int32_t)wh;
-        prof_begin(c, "raygen", &ev);
-        k_raygen<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>(Q[0], Lr, Lg, Lb, cam, p, ch, counts);
-        prof_end(c, "raygen", ev, ch.n);
-        if (p.mlp_mode == 2) k_zero<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>((float*)mio.rec_valid, (size_t)ch.n);
-        int shade_grid = (int)std::min<int64_t>((ch.n + 255) / 256, shade_cap);
-        int tg = trace_grid(c, ch.n);
-        for (int depth = 0; depth < p.max_depth; depth++) {
-            int qi = depth & 1;
-            prof_begin(c, "trace", &ev);
-            if (c->count_visits) k_trace<true><<<tg, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
-            else k_trace<false><<<tg, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
-            prof_end(c, "trace", ev, 0);
-            prof_begin(c, "shade", &ev);
-            k_shade<<<shade_grid, 256, 0, st>>>(S, p, ch, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, mio);
-            prof_end(c, "shade", ev, 0);
-            if (p.use_nee) {
-                prof_begin(c, "shadow", &ev);
-                if (c->count_visits) k_shadow<true><<<tg, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
-                else k_shadow<false><<<tg, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
-                prof_end(c, "shadow", ev, 0);
-            }
-            if (depth == 1 && p.mlp_mode == 1 && depth + 1 < p.max_depth) {
-                // queued vertex-2 paths: prediction after the bounce's shadow rays (same L order as NEE first)
-                float* pred = (float*)c->mlpbuf[3].p;
-                if (int r = mlp_query_dev(c, mio.q_in9, counts + CNT_MLPQ, ch.n, pred)) return r;
-                k_mlp_apply<<<shade_grid, 256, 0, st>>>(counts, pred, mio.q_beta, mio.q_slot, Lr, Lg, Lb);
-            }
-            if (depth == 1 && p.mlp_mode == 2)
-                k_snapshot<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>(ch.n, Lr, Lg, Lb, (float*)c->mlpbuf[7].p);
-        }
-        if (p.mlp_mode == 2 && train && train->count < train->cap) {
-            float* snap_t = (float*)c->mlpbuf[7].p;     // snapshot, then overwritten by the targets
-            uint32_t* flag = (uint32_t*)c->mlpbuf[8].p;
-            uint32_t* pos = (uint32_t*)c->mlpbuf[9].p;
-            k_targets<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>(ch.n, Lr, Lg, Lb, snap_t, mio.rec_beta, mio.rec_valid,
-                                                                    snap_t, flag);
-            size_t tbytes = 0;
-            (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, flag, pos, ch.n, st);
-            if (c->scan_tmp.bytes < tbytes) {
-                dev_free(c, c->scan_tmp);
-                if (int r = dev_alloc(c, c->scan_tmp, tbytes)) return r;
-            }
-            if (int r = hip_check(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tbytes, flag, pos, ch.n, st), "scan"))
-                return r;
-            k_scatter_train<<<(unsigned)((ch.n + 255) / 256), 256, 0, st>>>(ch.n, flag, pos, train->count, train->cap,
-                                                                           mio.rec_in9, snap_t, train->in9, train->target);
-            uint32_t lastpos = 0, lastflag = 0;
-            if (int r = hip_check(c, hipMemcpyAsync(&lastpos, pos + ch.n - 1, 4, hipMemcpyDeviceToHost, st), "count")) return r;
-            if (int r = hip_check(c, hipMemcpyAsync(&lastflag, flag + ch.n - 1, 4, hipMemcpyDeviceToHost, st), "count")) return r;
-            if (int r = hip_check(c, hipStreamSynchronize(st), "train samples")) return r;
-            train->count = std::min<int64_t>(train->cap, train->count + lastpos + lastflag);
-        }
-        bool last = begin + pool >= total;
-        prof_begin(c, "finalize", &ev);
-        k_finalize<<<(unsigned)((wh + 255) / 256), 256, 0, st>>>(Lr, Lg, Lb, (float*)c->film_acc.p, ch, p.spp, last ? 1 : 0,
-                                                                (float4*)film_dev);
-        prof_end(c, "finalize", ev, wh);
+        sg.path_begin = begin;
+        sg.n = (int32_t)((total - begin < pool) ? (total - begin) : pool);
+        sg.last = (begin + pool >= total) ? 1 : 0;
+        if (int r = run_chunk(c, s, &sg, 1, train)) return r;
     }
     if (int r = hip_check(c, hipGetLastError(), "render launch")) return r;
     unsigned long long tot[6] = {0, 0, 0, 0, 0, 0};

@@ -1,0 +1,106 @@
+"""Serving loop: requests of different scenes, resolutions, sample counts and
+depths batched into shared path chunks render the same bits as each request
+alone through srt_render; admission follows the HBM budget."""
+import numpy as np
+import pytest
+
+from srt_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _cam_variant(cam, dx):
+    c = dict(cam)
+    c["position"] = [cam["position"][0] + dx, cam["position"][1], cam["position"][2]]
+    return c
+
+
+@pytest.fixture(scope="module")
+def served():
+    ctx = abi.Context(0)
+    cb = scenes.cornell()
+    sp = scenes.spheres()
+    hc = ctx.upload(cb)
+    ctx.build_bvh(hc)
+    hs = ctx.upload(sp)
+    ctx.build_bvh(hs)
+    ctx.mlp_create(seed=7)
+    reqs = [
+        (hc, cb.camera, 64, 48, 8, 5, 4, 1, 1, 11, 0),
+        (hc, _cam_variant(cb.camera, 20.0), 32, 32, 16, 3, 2, 1, 0, 12, 0),
+        (hs, sp.camera, 96, 54, 2, 8, 4, 1, 1, 13, 0),
+        (hc, cb.camera, 40, 40, 4, 6, 4, 0, 0, 14, 0),
+        (hs, _cam_variant(sp.camera, -30.0), 48, 27, 3, 4, 4, 1, 1, 15, 0),
+        (hc, cb.camera, 32, 24, 4, 5, 4, 1, 1, 16, 1),      # radiance-MLP query at vertex 2
+    ]
+    alone = [ctx.render(h, cam, W, H, spp, d, rr, nee, mis, seed, mlp_mode=m)
+             for (h, cam, W, H, spp, d, rr, nee, mis, seed, m) in reqs]
+    yield ctx, reqs, alone
+    ctx.close()
+
+
+def _submit_all(ctx, reqs):
+    out = []
+    for (h, cam, W, H, spp, d, rr, nee, mis, seed, m) in reqs:
+        out.append(ctx.submit(h, cam, W, H, spp, d, rr, nee, mis, seed, mlp_mode=m))
+    return out
+
+
+@pytest.mark.parametrize("pool", [3000, 20000, 0])
+def test_mix_equals_alone(served, pool):
+    ctx, reqs, alone = served
+    ctx.serve_start(pool)
+    try:
+        tickets = _submit_all(ctx, reqs)
+        for k, (t, img) in enumerate(tickets):
+            res = ctx.wait(t)
+            assert res["status"] == 2, (k, res)
+            assert np.array_equal(img, alone[k]), f"request {k} differs in the mix (pool {pool})"
+        st = ctx.serve_stats()
+        assert st["served_paths"] == sum(W * H * spp for (_, _, W, H, spp, *_r) in reqs)
+        if pool == 3000:
+            # small chunks: requests on one scene share chunks
+            assert max(ctx.poll(t)["chunks"] for t, _ in tickets) > 1
+    finally:
+        ctx.serve_stop()
+
+
+def test_entry_points_refuse_while_serving(served):
+    ctx, reqs, alone = served
+    ctx.serve_start(0)
+    try:
+        with pytest.raises(abi.SrtError) as e:
+            ctx.render(reqs[0][0], reqs[0][1], 8, 8, 1)
+        assert e.value.code == -7
+        t, img = ctx.submit(*reqs[0][:10])
+        assert ctx.wait(t)["status"] == 2
+        assert np.array_equal(img, alone[0])
+    finally:
+        ctx.serve_stop()
+    # usable again after stop
+    img = ctx.render(*reqs[0][:10])
+    assert np.array_equal(img, alone[0])
+
+
+def test_admission_by_budget(served):
+    ctx, reqs, alone = served
+    h, cam, W, H, spp, d, rr, nee, mis, seed, m = reqs[0]
+    film = W * H * 28
+    budget0 = ctx.lib.srt_hbm_used(ctx.h)
+    ctx.set_option("hbm_budget", budget0 + film + film // 2)   # one request's film at a time
+    ctx.serve_start(0)
+    try:
+        tickets = [ctx.submit(h, cam, W, H, spp, d, rr, nee, mis, seed) for _ in range(4)]
+        big, _ = ctx.submit(h, cam, 1024, 1024, 1, d, rr, nee, mis, seed)   # 28 MiB film never fits
+        res = [ctx.wait(t) for t, _ in tickets]
+        rb = ctx.wait(big)
+    finally:
+        ctx.serve_stop()
+        ctx.set_option("hbm_budget", 1 << 62)
+    for (t, img), r in zip(tickets, res):
+        assert r["status"] == 2
+        assert np.array_equal(img, alone[0])
+    # admitted one after another
+    for a, b in zip(res, res[1:]):
+        assert b["admit_ms"] >= a["done_ms"]
+    assert rb["status"] == -4
