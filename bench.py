@@ -16,7 +16,16 @@ CPU baseline: the oracle (host execution of the reference integrator, float)
 on a fixed subset of rows of the same frame, on all host cores, whose pixels
 are also compared with the GPU frame.
 
+HBM traffic (roofline.traffic) comes from two rocprofv3 counter runs of this
+same command (FETCH_SIZE, then WRITE_SIZE, per dispatch), each scaled by what
+the counter reads for a copy kernel of known size (--mode copy), condensed by
+--pmc-summary into profiles/trace_spheres1m_hbm_traffic.md, which this script
+reads when present.
+
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
+       python bench.py --mode copy          (calibration kernel for the counters)
+       python bench.py --pmc-summary FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
+       python bench.py --kstats-summary DIR  (rocprofv3 --kernel-trace --stats output -> profiles/)
 """
 import argparse
 import json
@@ -44,8 +53,156 @@ TRI_FETCH = 48       # hit triangle record read by shade
 MATERIAL = 80        # material record read by shade
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_spheres1m_hbm_traffic.md")
+COPY_ELEMS = 1 << 28          # calibration copy: 1 GiB read + 1 GiB written (float32)
+# kernels of the frame, as rocprofv3 names them
+KERNEL_KEYS = {"trace": "k_trace<false>", "shadow": "k_shadow<false>", "shade": "k_shade("}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_mode():
+    """Calibration kernel for the HBM byte counters: an elementwise negation of
+    2^28 float32 values (1 GiB read, 1 GiB written), run 3 times."""
+    import torch
+    x = torch.ones(COPY_ELEMS, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": COPY_ELEMS * 4, "copy_bytes_written": COPY_ELEMS * 4}), flush=True)
+
+
+def _pmc_rows(d, counter):
+    """(kernel name, value) per dispatch from a rocprofv3 --pmc output folder
+    (rocpd SQLite database, or counter_collection.csv with -f csv)."""
+    import csv
+    import glob
+    import sqlite3
+    rows = []
+    dbs = glob.glob(os.path.join(d, "**", "*results.db"), recursive=True)
+    for f in dbs:
+        con = sqlite3.connect(f)
+        rows += [(n, float(v)) for n, v in
+                 con.execute("select kernel_name, value from counters_collection where counter_name = ?", (counter,))]
+        con.close()
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] == counter:
+                    rows.append((r["Kernel_Name"], float(r["Counter_Value"])))
+    if not rows:
+        raise SystemExit(f"no {counter} values under {d}")
+    return rows
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
+    """Condense the counter runs into the traffic summary (KiB per dispatch ->
+    calibrated bytes per launch for each frame kernel)."""
+    def per_kernel(d, counter):
+        acc = {}
+        for name, v in _pmc_rows(d, counter):
+            for k, key in KERNEL_KEYS.items():
+                if key in name:
+                    acc.setdefault(k, []).append(v)
+        return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+    def copy_kib(d, counter):
+        vals = [v for name, v in _pmc_rows(d, counter) if "neg" in name.lower()]
+        if not vals:
+            raise SystemExit(f"no negation kernel in {d}")
+        return sum(vals) / len(vals)
+
+    known_kib = COPY_ELEMS * 4 / 1024.0
+    cf, cw = copy_kib(copy_fetch_dir, "FETCH_SIZE"), copy_kib(copy_write_dir, "WRITE_SIZE")
+    ff, fw = known_kib / cf, known_kib / cw
+    fetch, write = per_kernel(fetch_dir, "FETCH_SIZE"), per_kernel(write_dir, "WRITE_SIZE")
+    lines = ["# HBM traffic per launch of the frame kernels (1M-triangle frame, 1920x1080x64spp, depth 8)", "",
+             "Commands (from the repository root, then this summary):", "",
+             "    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_FETCH_SIZE -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu",
+             "    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_WRITE_SIZE -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu",
+             "    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_copy_FETCH_SIZE -o run -- python3 bench.py --mode copy",
+             "    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_copy_WRITE_SIZE -o run -- python3 bench.py --mode copy",
+             "    python3 bench.py --pmc-summary gpurun_out/pmc_FETCH_SIZE gpurun_out/pmc_WRITE_SIZE "
+             "gpurun_out/pmc_copy_FETCH_SIZE gpurun_out/pmc_copy_WRITE_SIZE", "",
+             "Each rocprofv3 run: cd /tmp && export TMPDIR=/tmp, bench.py by its path in the repository.",
+             "The frame runs hold one counting pass (k_trace<true>/k_shadow<true>, not counted here; its k_shade",
+             "dispatches are) and one frame; FETCH_SIZE/WRITE_SIZE are the L2-to-fabric bytes (MALL or HBM).", "",
+             f"Calibration: copy kernel of {COPY_ELEMS * 4} B read + {COPY_ELEMS * 4} B written: "
+             f"FETCH_SIZE {cf:.1f} KiB, WRITE_SIZE {cw:.1f} KiB per dispatch -> factors fetch {ff:.4f}, "
+             f"write {fw:.4f}", "",
+             "| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | calibrated bytes/launch |",
+             "|---|---|---|---|---|"]
+    for k in KERNEL_KEYS:
+        if k in fetch and k in write:
+            b = (fetch[k][0] * ff + write[k][0] * fw) * 1024.0
+            lines.append(f"| {k} | {fetch[k][1]} | {fetch[k][0]:.1f} | {write[k][0]:.1f} | {b:.0f} |")
+    lines.append("")
+    for k in KERNEL_KEYS:
+        if k in fetch and k in write:
+            b = (fetch[k][0] * ff + write[k][0] * fw) * 1024.0
+            lines.append(f"traffic_bytes_per_launch {k} {b:.0f}")
+    text = "\n".join(lines) + "\n"
+    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
+    with open(TRAFFIC_FILE, "w") as fh:
+        fh.write(text)
+    print(text)
+
+
+KSTATS_FILE = os.path.join(ROOT, "profiles", "frame_spheres1m_kernel_stats.md")
+
+
+def kstats_summary(d):
+    """Per-kernel time summary of a `rocprofv3 --kernel-trace --stats` run of
+    this script (rocpd database or kernel_stats.csv)."""
+    import csv
+    import glob
+    import sqlite3
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*results.db"), recursive=True):
+        con = sqlite3.connect(f)
+        # the database view holds microseconds
+        rows += [(n, int(c), float(t) * 1e3, float(a) * 1e3) for n, c, t, a in
+                 con.execute("select name, total_calls, total_duration, average from top_kernels")]
+        con.close()
+    for f in glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                rows.append((r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"])))
+    if not rows:
+        raise SystemExit(f"no kernel statistics under {d}")
+    rows.sort(key=lambda r: -r[2])
+    total = sum(r[2] for r in rows)
+    lines = ["# Kernel times: 1M-triangle frame (1920x1080x64spp, depth 8)", "",
+             "Command (from the repository root; TMPDIR=/tmp):", "",
+             "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_ks -o run -- python3 bench.py --steps 2 --warmup 1 --no-cpu",
+             "    python3 bench.py --kstats-summary gpurun_out/prof_ks", "",
+             "The run renders 4 frames: the untimed counting pass (k_trace<true>, k_shadow<true>), 1 warmup and",
+             "2 timed frames; the BVH build (k_top_sah, k_bottom_up, sorts) runs once.", "",
+             "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
+    for n, c, t, a in rows[:16]:
+        short = n.split("(")[0].replace("void ", "")
+        if len(short) > 70:
+            short = short[:67] + "..."
+        lines.append(f"| `{short}` | {c} | {t / 1e6:.3f} | {a / 1e3:.1f} | {100 * t / total:.1f}% |")
+    text = "\n".join(lines) + "\n"
+    with open(KSTATS_FILE, "w") as fh:
+        fh.write(text)
+    print(text)
+
+
+def measured_traffic(kernel):
+    """Calibrated HBM bytes per launch of `kernel` from the committed counter summary."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        for line in fh:
+            p = line.split()
+            if len(p) == 3 and p[0] == "traffic_bytes_per_launch" and p[1] == kernel:
+                return int(p[2])
+    return None
 
 
 def main():
@@ -56,7 +213,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-row-step", type=int, default=8, help="CPU baseline renders every k-th row")
     ap.add_argument("--path-pool", type=int, default=0)
+    ap.add_argument("--mode", choices=["frame", "copy"], default="frame")
+    ap.add_argument("--pmc-summary", nargs=4, metavar="DIR")
+    ap.add_argument("--kstats-summary", metavar="DIR")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
+    if args.kstats_summary:
+        kstats_summary(args.kstats_summary)
+        return
+    if args.mode == "copy":
+        copy_mode()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the frame would shard by "
               "image tiles or spp slices with one gather/all-reduce, not implemented); run with --gpus 1",
@@ -119,7 +288,7 @@ def main():
     bytes_launch = bytes_step[kern] / launches_per_step
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kern),
                 "bytes_per_launch": int(bytes_launch), "avg_launch_ms": round(avg_ms, 4),
                 "per_kernel_GBs": {k: round(bytes_step[k] / (st[k]["ms"] / args.steps * 1e-3) / 1e9, 1)
                                    for k in bytes_step if k in st and st[k]["ms"] > 0}}

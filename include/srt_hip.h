@@ -103,9 +103,12 @@ typedef struct srt_bvh_opts {
 int srt_create(int device, size_t hbm_budget_bytes, srt_ctx** out);
 void srt_destroy(srt_ctx* ctx);
 const char* srt_last_error(const srt_ctx* ctx);
-/* tunables: "path_pool" (path slots per chunk), "profile" (0/1 per-kernel HIP events),
+/* tunables: "path_pool" (path slots per chunk, default 2^27), "profile" (0/1 per-kernel HIP events),
  * "count_visits" (0/1 count BVH node visits and triangle tests), "hbm_budget"
- * (bytes; at least srt_hbm_used) */
+ * (bytes; at least srt_hbm_used), "trace_stack_lds" (traversal stack entries per
+ * lane in LDS, 4..64, default 16), "trace_refill" (finished lanes that make a
+ * traversal wave fetch new rays, 1..64, default 32), "trace_blocks_per_cu"
+ * (0: as many as fit) */
 int srt_set_option(srt_ctx* ctx, const char* key, int64_t value);
 size_t srt_hbm_used(const srt_ctx* ctx);
 

@@ -184,6 +184,21 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
     }
     if (!strcmp(key, "profile")) { c->profile = value != 0; return SRT_OK; }
     if (!strcmp(key, "count_visits")) { c->count_visits = value != 0; return SRT_OK; }
+    if (!strcmp(key, "trace_stack_lds")) {
+        if (value < 4 || value > 64) return set_err(c, SRT_E_INVALID, "trace_stack_lds must be in [4, 64]");
+        c->trace_lds_depth = (int32_t)value;
+        return SRT_OK;
+    }
+    if (!strcmp(key, "trace_refill")) {
+        if (value < 1 || value > 64) return set_err(c, SRT_E_INVALID, "trace_refill must be in [1, 64]");
+        c->trace_refill = (int32_t)value;
+        return SRT_OK;
+    }
+    if (!strcmp(key, "trace_blocks_per_cu")) {
+        if (value < 0 || value > 64) return set_err(c, SRT_E_INVALID, "trace_blocks_per_cu must be in [0, 64]");
+        c->trace_blocks_per_cu = (int32_t)value;
+        return SRT_OK;
+    }
     if (!strcmp(key, "hbm_budget")) {
         if (value <= 0 || (size_t)value < c->used.load())
             return set_err(c, SRT_E_INVALID, "hbm_budget must be positive and cover the %zu bytes in use", c->used.load());

@@ -37,8 +37,14 @@ __host__ __device__ inline uint32_t leaf_count(int32_t r) { return ((uint32_t)r 
 
 constexpr int kMaxLeaf = 8;
 constexpr int kMaxDepth = 96;      // checked after every build
-constexpr int kStackLds = 32;      // traversal stack entries per lane kept in LDS
 constexpr int kTraceBlock = 128;   // traversal workgroup size (2 waves)
+
+// traversal launch settings (context options "trace_stack_lds", "trace_refill",
+// "trace_blocks_per_cu")
+struct TraceCfg {
+    int32_t lds_depth;   // stack entries per lane kept in LDS, 4..64
+    int32_t refill;      // finished lanes that make a wave fetch new rays, 1..64
+};
 
 // ---------------------------------------------------------------------------
 // device view of one resident scene
@@ -136,7 +142,10 @@ struct ShadowQueue {
 
 // counters that live in device memory between kernels of one chunk
 enum {
-    CNT_Q0 = 0, CNT_Q1 = 1, CNT_SHADOW = 2, CNT_MLPQ = 3, CNT_WORK2 = 4,
+    CNT_Q0 = 0, CNT_Q1 = 1, CNT_SHADOW = 2, CNT_MLPQ = 3,
+    CNT_FETCH_T = 5,     // next queue entry to trace (cleared by raygen and shade)
+    CNT_FETCH_S = 6,     // next shadow entry to trace (cleared by trace)
+    CNT_FETCH_Q = 7,     // next standalone ray query
     CNT_MAX = 16
 };
 
@@ -186,10 +195,11 @@ struct srt_ctx {
     std::map<uint64_t, srt::SceneRec*> scenes;
     // render pool
     int64_t pool_cap = 0;                 // path slots allocated
-    int64_t pool_request = 16 * 1024 * 1024;
+    int64_t pool_request = 1ll << 27;      // ~184 B per slot: 24.7 GB covers a 1080p x 64 spp frame in one chunk
     srt::DevBuf q[2][14], hits[4], shq[11], Lbuf[3], counters, film_acc, film_out, gstack;
     int64_t film_pixels = 0;
-    int64_t gstack_threads = 0;
+    size_t gstack_entries = 0;            // int32 entries allocated in gstack
+    int32_t trace_lds_depth = 16, trace_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
     // profiling
     bool profile = false;
     std::map<std::string, srt::KernelStat> stats;

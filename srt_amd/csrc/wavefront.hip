@@ -85,75 +85,177 @@ SRT_D bool intersect_tri(const RayPre& r, f3 o, f3 v0, f3 v1, f3 v2, float tmin,
 
 constexpr float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013): conservative slab test
 
-struct TraceOut { float t, u, v; int32_t pos; };
+// One ray's traversal state, kept in registers while its lane walks the BVH.
+struct TravState {
+    f3 o, inv;
+    RayPre rp;
+    float tmin, best_t, bu, bv;
+    int32_t best_prim, best_pos;
+    int32_t cur;       // node (>= 0) or leaf (< 0) to visit next
+    int32_t sp;        // stack depth
+    bool sx, sy, sz;   // direction signs select the near/far slab planes
+    bool live;         // false once the stack is empty (or, any-hit, a hit was found)
+};
 
-// Stack: the first kStackLds entries of each lane live in LDS (lane-interleaved,
-// entry e of lane l at lds[e * kTraceBlock + l]); deeper entries spill to a
-// per-thread global area.  The build guarantees depth <= kMaxDepth.
-template <bool ANY, bool COUNT = false>
-SRT_D bool traverse(const DevScene& S, f3 o, f3 d, float tmin, float tmax, int32_t* lds, int32_t* gst, size_t gstride,
-                    TraceOut* out, uint32_t* nvisit = nullptr, uint32_t* ntest = nullptr) {
-    RayPre rp = ray_pre(d);
-    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    // near/far slab planes chosen by direction sign; a ray lying in a slab plane
-    // with a zero direction component yields NaN, which fmaxf/fminf ignore
-    const bool sx = inv.x < 0.0f, sy = inv.y < 0.0f, sz = inv.z < 0.0f;
-    float best_t = tmax;
-    int32_t best_prim = 0x7fffffff, best_pos = -1;
-    float bu = 0.0f, bv = 0.0f;
-    int32_t cur = S.root;
-    int sp = 0;
-    for (;;) {
-        if (cur >= 0) {
-            if (COUNT) (*nvisit)++;
-            const BvhNode* np = S.nodes + cur;
-            float4 na = np->a, nb = np->b, nc = np->c;
-            int4 nd = np->d;
-            float lnx = ((sx ? na.y : na.x) - o.x) * inv.x, lfx = ((sx ? na.x : na.y) - o.x) * inv.x;
-            float lny = ((sy ? na.w : na.z) - o.y) * inv.y, lfy = ((sy ? na.z : na.w) - o.y) * inv.y;
-            float lnz = ((sz ? nc.y : nc.x) - o.z) * inv.z, lfz = ((sz ? nc.x : nc.y) - o.z) * inv.z;
-            float rnx = ((sx ? nb.y : nb.x) - o.x) * inv.x, rfx = ((sx ? nb.x : nb.y) - o.x) * inv.x;
-            float rny = ((sy ? nb.w : nb.z) - o.y) * inv.y, rfy = ((sy ? nb.z : nb.w) - o.y) * inv.y;
-            float rnz = ((sz ? nc.w : nc.z) - o.z) * inv.z, rfz = ((sz ? nc.z : nc.w) - o.z) * inv.z;
-            float tln = fmaxf(fmaxf(fmaxf(tmin, lnx), lny), lnz);
-            float tlf = fminf(fminf(fminf(best_t, lfx), lfy), lfz) * kRobust;
-            float trn = fmaxf(fmaxf(fmaxf(tmin, rnx), rny), rnz);
-            float trf = fminf(fminf(fminf(best_t, rfx), rfy), rfz) * kRobust;
-            bool hl = tln <= tlf, hr = trn <= trf;
-            if (hl && hr) {
-                int32_t nearc = nd.x, farc = nd.y;
-                if (trn < tln) { nearc = nd.y; farc = nd.x; }
-                if (sp < kStackLds) lds[sp * kTraceBlock] = farc;
-                else gst[(size_t)(sp - kStackLds) * gstride] = farc;
-                sp++;
-                cur = nearc;
-                continue;
-            }
-            if (hl) { cur = nd.x; continue; }
-            if (hr) { cur = nd.y; continue; }
+SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root) {
+    st.o = o;
+    st.rp = ray_pre(d);
+    st.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    // a ray lying in a slab plane with a zero direction component yields NaN
+    // slab distances, which fmaxf/fminf ignore
+    st.sx = st.inv.x < 0.0f; st.sy = st.inv.y < 0.0f; st.sz = st.inv.z < 0.0f;
+    st.tmin = tmin;
+    st.best_t = tmax;
+    st.best_prim = 0x7fffffff;
+    st.best_pos = -1;
+    st.bu = 0.0f; st.bv = 0.0f;
+    st.cur = root;
+    st.sp = 0;
+    st.live = true;
+}
+
+// Traversal stack: the first `lds_depth` entries of each lane live in LDS
+// (lane-interleaved: entry e of thread t at lds[e * blockDim + t]); deeper
+// entries go to a per-thread global area (entry e at gst[(e - lds_depth) *
+// gstride]).  The build guarantees a tree depth <= kMaxDepth.
+struct TravStack {
+    int32_t* lds;
+    int32_t* gst;
+    size_t gstride;
+    int lds_depth;
+    int bstride;
+};
+SRT_D void stk_push(const TravStack& k, int32_t sp, int32_t v) {
+    if (sp < k.lds_depth) k.lds[sp * k.bstride] = v;
+    else k.gst[(size_t)(sp - k.lds_depth) * k.gstride] = v;
+}
+SRT_D int32_t stk_get(const TravStack& k, int32_t sp) {
+    return (sp < k.lds_depth) ? k.lds[sp * k.bstride] : k.gst[(size_t)(sp - k.lds_depth) * k.gstride];
+}
+
+// Inner nodes until the lane reaches a leaf or the traversal ends.  Both child
+// boxes are tested with the robust slab test (Ize 2013); when both are hit the
+// nearer child is visited first and the other is pushed.
+template <bool COUNT>
+SRT_D void trav_inner(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nv) {
+    while (st.live && st.cur >= 0) {
+        if (COUNT) nv++;
+        const BvhNode* np = S.nodes + st.cur;
+        const float4 na = np->a, nb = np->b, nc = np->c;
+        const int4 nd = np->d;
+        const f3 o = st.o, inv = st.inv;
+        const bool sx = st.sx, sy = st.sy, sz = st.sz;
+        float lnx = ((sx ? na.y : na.x) - o.x) * inv.x, lfx = ((sx ? na.x : na.y) - o.x) * inv.x;
+        float lny = ((sy ? na.w : na.z) - o.y) * inv.y, lfy = ((sy ? na.z : na.w) - o.y) * inv.y;
+        float lnz = ((sz ? nc.y : nc.x) - o.z) * inv.z, lfz = ((sz ? nc.x : nc.y) - o.z) * inv.z;
+        float rnx = ((sx ? nb.y : nb.x) - o.x) * inv.x, rfx = ((sx ? nb.x : nb.y) - o.x) * inv.x;
+        float rny = ((sy ? nb.w : nb.z) - o.y) * inv.y, rfy = ((sy ? nb.z : nb.w) - o.y) * inv.y;
+        float rnz = ((sz ? nc.w : nc.z) - o.z) * inv.z, rfz = ((sz ? nc.z : nc.w) - o.z) * inv.z;
+        float tln = fmaxf(fmaxf(fmaxf(st.tmin, lnx), lny), lnz);
+        float tlf = fminf(fminf(fminf(st.best_t, lfx), lfy), lfz) * kRobust;
+        float trn = fmaxf(fmaxf(fmaxf(st.tmin, rnx), rny), rnz);
+        float trf = fminf(fminf(fminf(st.best_t, rfx), rfy), rfz) * kRobust;
+        const bool hl = tln <= tlf, hr = trn <= trf;
+        if (hl && hr) {
+            int32_t nearc = nd.x, farc = nd.y;
+            if (trn < tln) { nearc = nd.y; farc = nd.x; }
+            stk_push(k, st.sp, farc);
+            st.sp++;
+            st.cur = nearc;
+        } else if (hl) {
+            st.cur = nd.x;
+        } else if (hr) {
+            st.cur = nd.y;
+        } else if (st.sp == 0) {
+            st.live = false;
         } else {
-            uint32_t f = leaf_first(cur), c = leaf_count(cur);
-            for (uint32_t k = f; k < f + c; k++) {
-                const float4* tp = S.tris + 3 * (size_t)k;
-                float4 A = tp[0], B = tp[1], C = tp[2];
-                if (COUNT) (*ntest)++;
-                float t, u, v;
-                if (!intersect_tri(rp, o, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), tmin, best_t, &t, &u, &v))
-                    continue;
-                if (ANY) return true;
-                int32_t prim = __float_as_int(A.w);
-                if (t < best_t || (t == best_t && prim < best_prim)) {
-                    best_t = t; best_prim = prim; best_pos = (int32_t)k; bu = u; bv = v;
+            st.sp--;
+            st.cur = stk_get(k, st.sp);
+        }
+    }
+}
+
+// The triangles of leaf st.cur, then the next stack entry.  Closest hit:
+// (t, primitive id) lexicographic minimum, so the result does not depend on the
+// visiting order.  Any hit: the first hit ends the traversal; returns true.
+template <bool ANY, bool COUNT>
+SRT_D bool trav_leaf(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nt) {
+    const uint32_t f = leaf_first(st.cur), c = leaf_count(st.cur);
+    for (uint32_t j = f; j < f + c; j++) {
+        const float4* tp = S.tris + 3 * (size_t)j;
+        const float4 A = tp[0], B = tp[1], C = tp[2];
+        if (COUNT) nt++;
+        float t, u, v;
+        if (!intersect_tri(st.rp, st.o, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), st.tmin, st.best_t, &t,
+                           &u, &v))
+            continue;
+        if (ANY) {
+            st.live = false;
+            return true;
+        }
+        const int32_t prim = __float_as_int(A.w);
+        if (t < st.best_t || (t == st.best_t && prim < st.best_prim)) {
+            st.best_t = t; st.best_prim = prim; st.best_pos = (int32_t)j; st.bu = u; st.bv = v;
+        }
+    }
+    if (st.sp == 0) {
+        st.live = false;
+    } else {
+        st.sp--;
+        st.cur = stk_get(k, st.sp);
+    }
+    return false;
+}
+
+// Persistent traversal with dynamic ray fetch.  Each lane holds one ray; when
+// at least `refill` lanes of the wave (or all of them) have finished, the wave
+// takes that many new ray indices from `fetch` with one atomic, so lanes do
+// not idle while the wave's slowest ray finishes.  IO supplies
+//   void load(uint32_t i, TravState&, int32_t root)
+//   void store(uint32_t i, const TravState&, bool occluded)
+// Every wave leaves the loop once `fetch` has passed `count` and its lanes
+// have finished.
+template <bool ANY, bool COUNT, class IO>
+SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, int refill, const TravStack& k, IO& io,
+                            uint32_t& nv, uint32_t& nt) {
+    const int lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    TravState st;
+    st.live = false;
+    uint32_t ri = 0;
+    bool has = false, exhausted = false, occl = false;
+    for (;;) {
+        const bool need = !has && !exhausted;
+        const unsigned long long m = __ballot(need);
+        unsigned long long act = __ballot(has);
+        if (m != 0ull && (act == 0ull || __popcll(m) >= refill)) {
+            const int leader = __ffsll(m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (need) {
+                const uint32_t idx = base + (uint32_t)__popcll(m & below);
+                if (idx < count) {
+                    ri = idx;
+                    has = true;
+                    occl = false;
+                    io.load(idx, st, S.root);
+                } else {
+                    exhausted = true;
                 }
             }
+            act = __ballot(has);
         }
-        if (sp == 0) break;
-        sp--;
-        cur = (sp < kStackLds) ? lds[sp * kTraceBlock] : gst[(size_t)(sp - kStackLds) * gstride];
+        if (act == 0ull) break;
+        if (has) {
+            trav_inner<COUNT>(S, st, k, nv);
+            if (st.live) occl = trav_leaf<ANY, COUNT>(S, st, k, nt);
+            if (!st.live) {
+                io.store(ri, st, occl);
+                has = false;
+            }
+        }
     }
-    if (ANY) return false;
-    out->t = best_t; out->u = bu; out->v = bv; out->pos = best_pos;
-    return best_pos >= 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -193,7 +295,10 @@ SRT_D int find_segment(const Segment* segs, int nseg, uint32_t i) {
 __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* Lg, float* Lb, const Segment* segs, int nseg,
                                                 int n, uint32_t* counts) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) counts[CNT_Q0] = (uint32_t)n;
+    if (i == 0) {
+        counts[CNT_Q0] = (uint32_t)n;
+        counts[CNT_FETCH_T] = 0u;
+    }
     if (i >= n) return;
     const int k = find_segment(segs, nseg, (uint32_t)i);
     const Segment& sg = segs[k];
@@ -223,34 +328,67 @@ __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* L
     Lr[i] = 0.0f; Lg[i] = 0.0f; Lb[i] = 0.0f;
 }
 
-// closest hit for every entry of queue `qi`; also clears the counters the
-// following shade stage appends to.  COUNT: also total node visits and
-// triangle tests (an untimed pass used for the algorithmic byte count).
+// Ray sources/sinks of the persistent traversal loop.
+struct QueueIO {       // closest hit of queue entries -> hit buffer
+    PathQueue q;
+    HitBuf h;
+    SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
+        trav_begin(st, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), 1e-3f, 1e16f, root);
+    }
+    SRT_D void store(uint32_t i, const TravState& st, bool) const {
+        const bool hit = st.best_pos >= 0;
+        h.t[i] = hit ? st.best_t : 0.0f;
+        h.u[i] = hit ? st.bu : 0.0f;
+        h.v[i] = hit ? st.bv : 0.0f;
+        h.tri[i] = hit ? st.best_pos : -1;
+    }
+};
+struct ShadowIO {      // any hit in [1e-3, tmax]; unoccluded rays add their contribution
+    ShadowQueue sq;
+    float *Lr, *Lg, *Lb;
+    SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
+        trav_begin(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root);
+    }
+    SRT_D void store(uint32_t i, const TravState&, bool occluded) const {
+        if (occluded) return;
+        const uint32_t slot = sq.slot[i];     // one shadow ray per slot per bounce: no write conflicts
+        Lr[slot] = Lr[slot] + sq.cr[i];
+        Lg[slot] = Lg[slot] + sq.cg[i];
+        Lb[slot] = Lb[slot] + sq.cb[i];
+    }
+};
+
+SRT_D TravStack make_stack(int32_t* lds, int32_t* gstack, TraceCfg cfg) {
+    const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    TravStack k;
+    k.lds = lds + threadIdx.x;
+    k.gst = gstack + gtid;
+    k.gstride = (size_t)gridDim.x * blockDim.x;
+    k.lds_depth = cfg.lds_depth;
+    k.bstride = (int)blockDim.x;
+    return k;
+}
+
+// Closest hit for every entry of queue `qi`; also clears the counters that the
+// following shade and shadow stages append to or fetch from.  COUNT: also
+// total node visits and triangle tests (an untimed pass used for the
+// algorithmic byte count).
 template <bool COUNT>
 __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
-                                                      int32_t* gstack, unsigned long long* totals) {
-    __shared__ int32_t lds[kStackLds * kTraceBlock];
-    const int tid = threadIdx.x;
-    const size_t gtid = (size_t)blockIdx.x * blockDim.x + tid;
-    const size_t gthreads = (size_t)gridDim.x * blockDim.x;
+                                                      int32_t* gstack, unsigned long long* totals, TraceCfg cfg) {
+    extern __shared__ int32_t lds_stack[];
     const uint32_t count = counts[qi];
-    if (gtid == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         counts[qi ^ 1] = 0u;
         counts[CNT_SHADOW] = 0u;
         counts[CNT_MLPQ] = 0u;
+        counts[CNT_FETCH_S] = 0u;
         totals[0] += count;
     }
+    const TravStack k = make_stack(lds_stack, gstack, cfg);
+    QueueIO io{q, h};
     uint32_t nv = 0, nt = 0;
-    for (size_t i = gtid; i < count; i += gthreads) {
-        f3 o = mk(q.ox[i], q.oy[i], q.oz[i]);
-        f3 d = mk(q.dx[i], q.dy[i], q.dz[i]);
-        TraceOut to;
-        bool hit = traverse<false, COUNT>(S, o, d, 1e-3f, 1e16f, lds + tid, gstack + gtid, gthreads, &to, &nv, &nt);
-        h.t[i] = hit ? to.t : 0.0f;
-        h.u[i] = hit ? to.u : 0.0f;
-        h.v[i] = hit ? to.v : 0.0f;
-        h.tri[i] = hit ? to.pos : -1;
-    }
+    trace_persistent<false, COUNT>(S, count, counts + CNT_FETCH_T, cfg.refill, k, io, nv, nt);
     if (COUNT) {
         atomicAdd(&totals[2], (unsigned long long)nv);
         atomicAdd(&totals[3], (unsigned long long)nt);
@@ -303,6 +441,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, const Segment* segs, 
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
     const uint32_t count = counts[qi];
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_FETCH_T] = 0u;   // the next trace stage fetches from 0
     for (uint32_t base = (uint32_t)wave * 64u; base < count; base += (uint32_t)nwaves * 64u) {
         const uint32_t i = base + (uint32_t)lane;
         bool emit_next = false, emit_shadow = false;
@@ -487,29 +626,18 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, const Segment* segs, 
     }
 }
 
-// Shadow rays: any hit in [1e-3, tmax]; unoccluded rays add their contribution.
+// Shadow rays of the bounce (fetch counter cleared by the trace stage).
 template <bool COUNT>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
-                                                       uint32_t* counts, int32_t* gstack, unsigned long long* totals) {
-    __shared__ int32_t lds[kStackLds * kTraceBlock];
-    const int tid = threadIdx.x;
-    const size_t gtid = (size_t)blockIdx.x * blockDim.x + tid;
-    const size_t gthreads = (size_t)gridDim.x * blockDim.x;
+                                                       uint32_t* counts, int32_t* gstack, unsigned long long* totals,
+                                                       TraceCfg cfg) {
+    extern __shared__ int32_t lds_stack[];
     const uint32_t count = counts[CNT_SHADOW];
-    if (gtid == 0) totals[1] += count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) totals[1] += count;
+    const TravStack k = make_stack(lds_stack, gstack, cfg);
+    ShadowIO io{sq, Lr, Lg, Lb};
     uint32_t nv = 0, nt = 0;
-    for (size_t i = gtid; i < count; i += gthreads) {
-        f3 o = mk(sq.ox[i], sq.oy[i], sq.oz[i]);
-        f3 d = mk(sq.dx[i], sq.dy[i], sq.dz[i]);
-        TraceOut to;
-        bool occluded = traverse<true, COUNT>(S, o, d, 1e-3f, sq.tmax[i], lds + tid, gstack + gtid, gthreads, &to, &nv, &nt);
-        if (!occluded) {
-            uint32_t slot = sq.slot[i];
-            Lr[slot] = Lr[slot] + sq.cr[i];
-            Lg[slot] = Lg[slot] + sq.cg[i];
-            Lb[slot] = Lb[slot] + sq.cb[i];
-        }
-    }
+    trace_persistent<true, COUNT>(S, count, counts + CNT_FETCH_S, cfg.refill, k, io, nv, nt);
     if (COUNT) {
         atomicAdd(&totals[4], (unsigned long long)nv);
         atomicAdd(&totals[5], (unsigned long long)nt);
@@ -588,27 +716,39 @@ __global__ void k_zero(float* p, size_t n) {
 // ---------------------------------------------------------------------------
 // standalone ray queries (first-hit parity, shadow parity)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, int n, const float* o3, const float* d3, float tmin,
-                                                           const float* tmax, int any, float* t, float* u, float* v,
-                                                           int32_t* prim, int32_t* gstack) {
-    __shared__ int32_t lds[kStackLds * kTraceBlock];
-    const int tid = threadIdx.x;
-    const size_t gtid = (size_t)blockIdx.x * blockDim.x + tid;
-    const size_t gthreads = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = gtid; i < (size_t)n; i += gthreads) {
-        f3 o = ld3(o3 + 3 * i), d = ld3(d3 + 3 * i);
-        TraceOut to;
-        if (any) {
-            prim[i] = traverse<true>(S, o, d, tmin, tmax[i], lds + tid, gstack + gtid, gthreads, &to) ? 1 : 0;
-        } else {
-            bool hit = traverse<false>(S, o, d, tmin, tmax[i], lds + tid, gstack + gtid, gthreads, &to);
-            t[i] = hit ? to.t : 0.0f;
-            u[i] = hit ? to.u : 0.0f;
-            v[i] = hit ? to.v : 0.0f;
-            prim[i] = hit ? __float_as_int(S.tris[3 * (size_t)to.pos].w) : -1;
-        }
+struct RaysIO {        // standalone queries on flat arrays
+    const float *o3, *d3, *tmax;
+    float tmin;
+    float *t, *u, *v;
+    int32_t* prim;
+    const float4* tris;
+    int any;
+    SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
+        trav_begin(st, ld3(o3 + 3 * (size_t)i), ld3(d3 + 3 * (size_t)i), tmin, tmax[i], root);
     }
+    SRT_D void store(uint32_t i, const TravState& st, bool occluded) const {
+        if (any) {
+            prim[i] = occluded ? 1 : 0;
+            return;
+        }
+        const bool hit = st.best_pos >= 0;
+        t[i] = hit ? st.best_t : 0.0f;
+        u[i] = hit ? st.bu : 0.0f;
+        v[i] = hit ? st.bv : 0.0f;
+        prim[i] = hit ? __float_as_int(tris[3 * (size_t)st.best_pos].w) : -1;
+    }
+};
+
+__global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, uint32_t n, RaysIO io, uint32_t* fetch,
+                                                           int32_t* gstack, TraceCfg cfg) {
+    extern __shared__ int32_t lds_stack[];
+    const TravStack k = make_stack(lds_stack, gstack, cfg);
+    uint32_t nv = 0, nt = 0;
+    if (io.any) trace_persistent<true, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
+    else trace_persistent<false, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
 }
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 // ---------------------------------------------------------------------------
 // post-processing (tonemapping.cu:14-68, accumulate.cu:22-26)
@@ -673,20 +813,43 @@ static int num_cus(srt_ctx* c) {
     return cached;
 }
 
+static TraceCfg trace_cfg(srt_ctx* c) {
+    TraceCfg cfg;
+    cfg.lds_depth = c->trace_lds_depth;
+    cfg.refill = c->trace_refill;
+    return cfg;
+}
+
+static size_t trace_lds_bytes(const TraceCfg& cfg) { return (size_t)cfg.lds_depth * kTraceBlock * sizeof(int32_t); }
+
+// Persistent traversal grid: as many workgroups as are resident at once
+// (occupancy of the traversal kernel with its LDS stack), capped by the work.
 static int trace_grid(srt_ctx* c, int64_t n) {
+    const TraceCfg cfg = trace_cfg(c);
+    int per_cu = c->trace_blocks_per_cu;
+    if (per_cu <= 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false>, kTraceBlock, trace_lds_bytes(cfg)) !=
+                hipSuccess || nb <= 0)
+            nb = 8;
+        per_cu = nb;
+    }
     int64_t want = (n + kTraceBlock - 1) / kTraceBlock;
-    int64_t cap = (int64_t)num_cus(c) * 10;
+    int64_t cap = (int64_t)num_cus(c) * per_cu;
     if (want > cap) want = cap;
     if (want < 1) want = 1;
     return (int)want;
 }
 
-static int ensure_gstack(srt_ctx* c, int64_t threads) {
-    if (c->gstack_threads >= threads) return SRT_OK;
+// global part of the traversal stacks: (tree depth - LDS entries) per thread
+static int ensure_gstack(srt_ctx* c, const SceneRec* s, int64_t threads) {
+    const int64_t per = std::max<int64_t>(1, (int64_t)s->max_depth + 1 - c->trace_lds_depth);
+    const size_t need = (size_t)(threads * per);
+    if (c->gstack_entries >= need) return SRT_OK;
     dev_free(c, c->gstack);
-    int r = dev_alloc(c, c->gstack, (size_t)threads * (kMaxDepth - kStackLds) * sizeof(int32_t));
-    if (r) return r;
-    c->gstack_threads = threads;
+    c->gstack_entries = 0;
+    if (int r = dev_alloc(c, c->gstack, need * sizeof(int32_t))) return r;
+    c->gstack_entries = need;
     return SRT_OK;
 }
 
@@ -788,7 +951,9 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     if (n <= 0 || n > 0x7fffffff) return set_err(c, SRT_E_LIMIT, "chunk of %lld paths", (long long)n);
     if (int r = ensure_pool(c, n)) return r;
     const int tgrid = trace_grid(c, n);
-    if (int r = ensure_gstack(c, (int64_t)tgrid * kTraceBlock)) return r;
+    if (int r = ensure_gstack(c, s, (int64_t)tgrid * kTraceBlock)) return r;
+    const TraceCfg tcfg = trace_cfg(c);
+    const size_t tlds = trace_lds_bytes(tcfg);
     if (!c->segbuf.p) if (int r = dev_alloc(c, c->segbuf, sizeof(Segment) * kMaxSegments)) return r;
     hipStream_t st = c->stream;
     // the segment table is small; a synchronous copy keeps the host array free
@@ -834,16 +999,17 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     for (int depth = 0; depth < max_depth; depth++) {
         int qi = depth & 1;
         prof_begin(c, "trace", &ev);
-        if (c->count_visits) k_trace<true><<<tgrid, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
-        else k_trace<false><<<tgrid, kTraceBlock, 0, st>>>(S, Q[qi], H, counts, qi, gst, totals);
+        if (c->count_visits) k_trace<true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+        else k_trace<false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
         prof_end(c, "trace", ev, 0);
         prof_begin(c, "shade", &ev);
         k_shade<<<shade_grid, 256, 0, st>>>(S, segs, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, mio);
         prof_end(c, "shade", ev, 0);
         if (any_nee) {
             prof_begin(c, "shadow", &ev);
-            if (c->count_visits) k_shadow<true><<<tgrid, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
-            else k_shadow<false><<<tgrid, kTraceBlock, 0, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals);
+            if (c->count_visits)
+                k_shadow<true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
+            else k_shadow<false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
             prof_end(c, "shadow", ev, 0);
         }
         if (depth == 1 && mlp_mode == 1) {
@@ -932,9 +1098,15 @@ int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* 
                int any, float* t, float* u, float* v, int32_t* prim) {
     DevScene S = dev_scene(s);
     int grid = trace_grid(c, n);
-    if (int r = ensure_gstack(c, (int64_t)grid * kTraceBlock)) return r;
-    k_trace_rays<<<grid, kTraceBlock, 0, c->stream>>>(S, n, o_dev, d_dev, tmin, tmax_dev, any, t, u, v, prim,
-                                                       (int32_t*)c->gstack.p);
+    if (int r = ensure_gstack(c, s, (int64_t)grid * kTraceBlock)) return r;
+    const TraceCfg cfg = trace_cfg(c);
+    RaysIO io;
+    io.o3 = o_dev; io.d3 = d_dev; io.tmax = tmax_dev; io.tmin = tmin;
+    io.t = t; io.u = u; io.v = v; io.prim = prim; io.tris = S.tris; io.any = any;
+    uint32_t* fetch = (uint32_t*)c->counters.p + CNT_FETCH_Q;
+    k_set_u32<<<1, 1, 0, c->stream>>>(fetch, 0u);
+    k_trace_rays<<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch, (int32_t*)c->gstack.p,
+                                                                         cfg);
     if (int r = hip_check(c, hipGetLastError(), "trace launch")) return r;
     return hip_check(c, hipStreamSynchronize(c->stream), "trace");
 }

@@ -23,7 +23,7 @@ def _render_pair(ctx, s, W, H, spp, depth, seed=1, pool=None, threads=0):
         ctx.set_option("path_pool", pool)
     img = ctx.render(h, s.camera, W, H, spp, depth, 4, 1, 1, seed)
     if pool:
-        ctx.set_option("path_pool", 16 * 1024 * 1024)
+        ctx.set_option("path_pool", 1 << 27)
     osc = O.Scene(s)
     ob = O.Bvh(osc)
     ref, _ = O.render(osc, s.camera, W, H, spp, depth, 4, 1, 1, seed, bvh=ob, threads=threads)
@@ -52,7 +52,7 @@ def test_chunking_independent(ctx):
     a = ctx.render(h, s.camera, 64, 48, 8, 5, 4, 1, 1, 3)
     ctx.set_option("path_pool", 1000)   # chunk boundaries inside sample planes
     b = ctx.render(h, s.camera, 64, 48, 8, 5, 4, 1, 1, 3)
-    ctx.set_option("path_pool", 16 * 1024 * 1024)
+    ctx.set_option("path_pool", 1 << 27)
     ctx.release(h)
     assert np.array_equal(a, b)
 
