@@ -23,6 +23,7 @@ the counter reads for a copy kernel of known size (--mode copy), condensed by
 reads when present.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
+       python bench.py --mode skin|mlp|serve (skinned animation; radiance MLP at equal time; serving)
        python bench.py --mode copy          (calibration kernel for the counters)
        python bench.py --pmc-summary FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
        python bench.py --kstats-summary DIR  (rocprofv3 --kernel-trace --stats output -> profiles/)
@@ -193,6 +194,218 @@ def kstats_summary(d):
     print(text)
 
 
+def _stats_ms(st, name, steps):
+    return round(st[name]["ms"] / steps, 3) if name in st else None
+
+
+def skin_mode(args):
+    """Skinned 200k-triangle tube (64-bone chain, 4 bones per vertex): per frame
+    linear blend skinning + BVH refit + a 1280x720, 16 spp, depth 6 render;
+    60 frames (one bone-animation period)."""
+    from srt_amd import abi, scenes
+    s = scenes.tube()
+    R = s.render
+    W, H, spp, depth = R["width"], R["height"], R["spp"], R["max_depth"]
+    frames = s.skin["frames"]
+    mats = [s.skin["matrices"](f) for f in range(frames)]
+    ctx = abi.Context(0)
+    h = ctx.upload(s)
+    t0 = time.perf_counter()
+    ctx.build_bvh(h)
+    bvh_ms = (time.perf_counter() - t0) * 1e3
+    ctx.set_skin(h, s.skin["bone_ids"], s.skin["bone_weights"])
+    for f in range(min(args.warmup, frames)):
+        ctx.skin_refit(h, mats[f])
+        ctx.render_resident(h, s.camera, W, H, spp, depth, R["rr_depth"], 1, 1, R["seed"] + f)
+    ctx.set_option("profile", 1)
+    ctx.stats_reset()
+    skin_ms, render_ms = [], []
+    t_start = time.perf_counter()
+    for f in range(frames):
+        t1 = time.perf_counter()
+        ctx.skin_refit(h, mats[f])                       # synchronizes
+        t2 = time.perf_counter()
+        ctx.render_resident(h, s.camera, W, H, spp, depth, R["rr_depth"], 1, 1, R["seed"] + f)
+        t3 = time.perf_counter()
+        skin_ms.append((t2 - t1) * 1e3)
+        render_ms.append((t3 - t2) * 1e3)
+    elapsed = time.perf_counter() - t_start
+    ctx.set_option("profile", 0)
+    st = ctx.stats()
+    paths = W * H * spp * frames
+    nv = s.positions.shape[0]
+    # k_skin algorithmic bytes: per vertex 4 ids + 4 weights (32 B) read, original
+    # position/normal/tangent read and skinned ones written (12 B each)
+    skin_bytes = nv * (32 + 3 * 12 + 3 * 12)
+    out = {"metric": "Mpaths/s, skinned animation (skin + refit + render per frame)",
+           "value": round(paths / elapsed / 1e6, 2), "unit": "Mpaths/s", "n_gpus": 1, "frames": frames,
+           "ms_per_frame": round(elapsed / frames * 1e3, 2), "higher_is_better": True, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "tube-200k: 1000x100x2 triangles, 64-bone chain, 4 bones/vertex (Dirichlet "
+                                  "weights), floor + light; 1280x720, 16 spp, depth 6, 60 frames",
+                      "triangles": s.num_triangles, "vertices": nv, "bones": s.skin["num_bones"]},
+           "skin_refit_ms": {"mean": round(float(np.mean(skin_ms)), 3), "max": round(float(np.max(skin_ms)), 3)},
+           "render_ms": {"mean": round(float(np.mean(render_ms)), 2), "max": round(float(np.max(render_ms)), 2)},
+           "kernel_ms_per_frame": {k: _stats_ms(st, k, frames) for k in ("skin", "refit", "trace", "shade", "shadow")},
+           "skin_GBs": (round(skin_bytes / (st["skin"]["ms"] / frames * 1e-3) / 1e9, 1)
+                        if "skin" in st and st["skin"]["ms"] > 0 else None),
+           "bvh_build_ms": round(bvh_ms, 1)}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def mlp_mode(args):
+    """Radiance MLP against the plain tracer at equal wall time on the Cornell
+    box (256x256, depth 5): RMSE against the FP64 oracle's 4096-spp image.
+    The MLP is trained online from the tracer's own vertex-2 samples (2^18
+    per frame), then paths stop at vertex 2 and take its prediction."""
+    from srt_amd import abi, scenes
+    gold_path = os.path.join(ROOT, "tests", "golden", "cornell_256x256_fp64_4096spp.npy")
+    gold = np.load(gold_path).astype(np.float64)
+    s = scenes.cornell()
+    W = H = 256
+    depth, rr = 5, 4
+    ctx = abi.Context(0)
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+
+    def rmse(img):
+        return float(np.sqrt(np.mean((img[:, :3].astype(np.float64) - gold) ** 2)))
+
+    def timed_render(spp, seed, mode=0):
+        ctx.render(h, s.camera, W, H, spp, depth, rr, 1, 1, seed, mlp_mode=mode)     # warm
+        t0 = time.perf_counter()
+        img = ctx.render(h, s.camera, W, H, spp, depth, rr, 1, 1, seed + 1, mlp_mode=mode)
+        return img, time.perf_counter() - t0
+
+    spp = 16
+    img_p, t_p = timed_render(spp, 100)
+    # online training: frames of 2^18 samples from 256x256x8 training paths
+    train_frames, steps = args.mlp_frames, args.mlp_steps
+    ctx.mlp_create(seed=1, lr=1e-3)
+    losses = []
+    t0 = time.perf_counter()
+    for f in range(train_frames):
+        loss, n = ctx.mlp_train_frame(h, s.camera, W, H, 8, depth, rr, seed=5000 + f, max_samples=1 << 18,
+                                      steps=steps)
+        losses.append(loss)
+    t_train = time.perf_counter() - t0
+    img_m, t_m = timed_render(spp, 200, mode=1)
+    t_total = t_train + t_m
+    # plain tracer given the same wall time (training + render)
+    spp_eq = max(1, int(round(spp * t_total / t_p)))
+    img_eq, t_eq = timed_render(spp_eq, 300)
+    # plain tracer given only the MLP render's time
+    spp_r = max(1, int(round(spp * t_m / t_p)))
+    img_r, t_r = timed_render(spp_r, 400)
+    out = {"metric": "RMSE vs FP64 4096-spp oracle at equal wall time (radiance MLP vs plain tracer)",
+           "unit": "RMSE", "higher_is_better": False, "dtype": "f32 tracer, bf16 MLP (fp32 accumulation)",
+           "data": "synthetic",
+           "config": {"workload": "cornell-36: 256x256, depth 5, NEE+MIS; MLP 63->4x64->3 ReLU, Adam lr 1e-3, "
+                                  f"{train_frames} training frames x {steps} steps, 2^18 samples/frame",
+                      "golden": os.path.relpath(gold_path, ROOT)},
+           "plain_16spp": {"rmse": round(rmse(img_p), 5), "ms": round(t_p * 1e3, 2),
+                           "Mpaths_s": round(W * H * spp / t_p / 1e6, 1)},
+           "mlp_16spp": {"rmse": round(rmse(img_m), 5), "render_ms": round(t_m * 1e3, 2),
+                         "train_ms": round(t_train * 1e3, 1), "Mpaths_s": round(W * H * spp / t_m / 1e6, 1),
+                         "loss_first": round(float(losses[0]), 4), "loss_last": round(float(losses[-1]), 4)},
+           "plain_equal_total_time": {"spp": spp_eq, "rmse": round(rmse(img_eq), 5), "ms": round(t_eq * 1e3, 2)},
+           "plain_equal_render_time": {"spp": spp_r, "rmse": round(rmse(img_r), 5), "ms": round(t_r * 1e3, 2)}}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def serve_mode(args):
+    """Continuous-batching serving: Poisson arrivals over 8 resident scenes
+    (3 Cornell variants, 3 1M-triangle sphere scenes with different generator
+    seeds, 2 posed skinned tubes); resolution uniform from {256^2, 512^2,
+    1280x720, 1920x1080}, spp uniform from {4, 16, 64}; the radiance MLP
+    (trained on the first Cornell scene) on with probability 1/2 for that
+    scene's requests.  Run 1: the offered rate (default 200/s) for a window,
+    then queued requests are cancelled; reports served Mpaths/s.  Run 2: a
+    rate at about half the measured capacity; reports p50/p99 latency."""
+    from srt_amd import abi, scenes
+    rng = np.random.default_rng(args.serve_seed)
+    ctx = abi.Context(0)
+    sc = []
+    for fov in (40.0, 35.0, 45.0):
+        sc.append(scenes.cornell(fov=fov))
+    for seed in (20261202, 20261203, 20261204):
+        sc.append(scenes.spheres(seed=seed))
+    sc.append(scenes.tube())
+    sc.append(scenes.tube(seed=12))
+    handles = []
+    for k, s in enumerate(sc):
+        h = ctx.upload(s)
+        ctx.build_bvh(h)
+        if getattr(s, "skin", None) is not None:
+            ctx.set_skin(h, s.skin["bone_ids"], s.skin["bone_weights"])
+            ctx.skin_refit(h, s.skin["matrices"](0 if k == 6 else 30))
+        handles.append(h)
+    ctx.mlp_create(seed=1, lr=1e-3)
+    s0 = sc[0]
+    for f in range(16):
+        ctx.mlp_train_frame(handles[0], s0.camera, 256, 256, 8, 5, 4, seed=7000 + f, max_samples=1 << 18, steps=16)
+    res_choices = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    spp_choices = [4, 16, 64]
+
+    def draw():
+        k = int(rng.integers(len(sc)))
+        W, H = res_choices[int(rng.integers(4))]
+        spp = spp_choices[int(rng.integers(3))]
+        m = 1 if (k == 0 and rng.random() < 0.5) else 0
+        return k, W, H, spp, m
+
+    def run(rate, window, keep_images, drain):
+        ctx.serve_start(0)
+        reqs = []
+        t0 = time.perf_counter()
+        t_next = t0
+        offered = 0
+        while True:
+            t_next += rng.exponential(1.0 / rate)
+            if t_next - t0 > window:
+                break
+            while time.perf_counter() < t_next:
+                time.sleep(max(0.0, min(0.002, t_next - time.perf_counter())))
+            k, W, H, spp, m = draw()
+            s = sc[k]
+            R = s.render
+            t, _ = ctx.submit(handles[k], s.camera, W, H, spp, R["max_depth"], R["rr_depth"], 1, 1,
+                              int(rng.integers(1 << 30)), mlp_mode=m, want_image=keep_images)
+            reqs.append((t, W * H * spp))
+            offered += W * H * spp
+        st_window = ctx.serve_stats()
+        t_window = time.perf_counter() - t0
+        # drain: every request's result; otherwise those finished within the window
+        results = [ctx.wait(t) if drain else ctx.poll(t) for t, _ in reqs]
+        ctx.serve_stop(drain=drain)
+        ctx.forget_tickets()
+        lat = [r["done_ms"] - r["submit_ms"] for r in results if r["status"] == 2]
+        return dict(requests=len(reqs), completed=len(lat), offered_Mpaths_s=round(offered / window / 1e6, 1),
+                    served_Mpaths_s=round(st_window["served_paths"] / t_window / 1e6, 1),
+                    p50_ms=round(float(np.percentile(lat, 50)), 1) if lat else None,
+                    p99_ms=round(float(np.percentile(lat, 99)), 1) if lat else None)
+
+    sat = run(args.serve_rate, args.serve_window, False, False)
+    log(f"saturation run: {sat}")
+    mean_paths = np.mean([W * H for W, H in res_choices]) * np.mean(spp_choices)
+    sub_rate = max(1.0, 0.5 * sat["served_Mpaths_s"] * 1e6 / mean_paths)
+    sub = run(sub_rate, args.serve_window, True, True)
+    sub["rate_per_s"] = round(sub_rate, 1)
+    out = {"metric": "served Mpaths/s at saturation; p50/p99 request latency below saturation",
+           "value": sat["served_Mpaths_s"], "unit": "Mpaths/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "serving: Poisson arrivals over 8 resident scenes (3 Cornell variants, 3 1M-triangle "
+                                  "sphere scenes, 2 posed 200k-triangle tubes); res U{256^2,512^2,1280x720,1920x1080}, "
+                                  "spp U{4,16,64}, MLP on p=1/2 for the MLP-trained scene",
+                      "rate_per_s": args.serve_rate, "window_s": args.serve_window},
+           "saturation": sat, "below_saturation": sub,
+           "latency_ms": {"p50": sub["p50_ms"], "p99": sub["p99_ms"], "unit": "ms per request"}}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def measured_traffic(kernel):
     """Calibrated HBM bytes per launch of `kernel` from the committed counter summary."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -213,7 +426,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-row-step", type=int, default=8, help="CPU baseline renders every k-th row")
     ap.add_argument("--path-pool", type=int, default=0)
-    ap.add_argument("--mode", choices=["frame", "copy"], default="frame")
+    ap.add_argument("--mode", choices=["frame", "copy", "skin", "mlp", "serve"], default="frame")
+    ap.add_argument("--mlp-frames", type=int, default=32)
+    ap.add_argument("--mlp-steps", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-window", type=float, default=30.0)
+    ap.add_argument("--serve-seed", type=int, default=20261202)
     ap.add_argument("--pmc-summary", nargs=4, metavar="DIR")
     ap.add_argument("--kstats-summary", metavar="DIR")
     args = ap.parse_args()
@@ -231,6 +449,9 @@ def main():
               "image tiles or spp slices with one gather/all-reduce, not implemented); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.mode != "frame":
+        {"skin": skin_mode, "mlp": mlp_mode, "serve": serve_mode}[args.mode](args)
+        return
 
     from srt_amd import abi, scenes
 

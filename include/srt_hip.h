@@ -201,7 +201,8 @@ int srt_memcpy(srt_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t k
  * path pool until srt_serve_stop.  Requests are admitted first-come first-served
  * while their device film (28 bytes per pixel) fits in the HBM budget (a request
  * that can never fit ends with status SRT_E_OOM), and the requests admitted on
- * one scene share each chunk of `path_pool` paths (0: the "path_pool" option).
+ * one scene fill each chunk of `path_pool` paths in arrival order (0: the
+ * "path_pool" option).
  * A request's image is bit-identical to srt_render with the same arguments.
  * While serving, only srt_submit, srt_poll, srt_wait, srt_serve_stats,
  * srt_serve_stop, srt_last_error and srt_hbm_used may be called; every other
@@ -223,8 +224,10 @@ typedef struct srt_result {
 } srt_result;
 
 int srt_serve_start(srt_ctx* ctx, int64_t path_pool);
-/* waits until every queued and admitted request is done, then stops the thread */
-int srt_serve_stop(srt_ctx* ctx);
+/* drain = 1: waits until every queued and admitted request is done; drain = 0:
+ * cancels (status SRT_E_STATE) the requests that have no path scheduled yet and
+ * waits for the others.  Then stops the thread. */
+int srt_serve_stop(srt_ctx* ctx, int32_t drain);
 /* thread-safe; copies the request; the ticket stays valid until srt_serve_stop */
 int srt_submit(srt_ctx* ctx, const srt_request* request, uint64_t* ticket);
 int srt_poll(srt_ctx* ctx, uint64_t ticket, srt_result* out);

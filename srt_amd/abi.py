@@ -131,7 +131,7 @@ def load_library(path: str = LIB_PATH):
         "srt_mlp_train_frame": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32,
                                           C.c_int32, C.c_int64, C.c_int32, P, P]),
         "srt_serve_start": (C.c_int, [P, C.c_int64]),
-        "srt_serve_stop": (C.c_int, [P]),
+        "srt_serve_stop": (C.c_int, [P, C.c_int32]),
         "srt_submit": (C.c_int, [P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
         "srt_poll": (C.c_int, [P, C.c_uint64, C.POINTER(Result)]),
         "srt_wait": (C.c_int, [P, C.c_uint64, C.c_int32, C.POINTER(Result)]),
@@ -416,24 +416,30 @@ class Context:
         self.check(self.lib.srt_serve_start(self.h, path_pool))
         self._serve_out = {}
 
-    def serve_stop(self):
-        self.check(self.lib.srt_serve_stop(self.h))
+    def serve_stop(self, drain=True):
+        self.check(self.lib.srt_serve_stop(self.h, 1 if drain else 0))
 
     def submit(self, scene, cam: dict, width, height, spp, max_depth=16, rr_depth=4, use_nee=1, use_mis=1, seed=0,
-               mlp_mode=0):
-        """Queue a render request; returns (ticket, output array filled when the request is done)."""
-        out = np.zeros((width * height, 4), np.float32)
+               mlp_mode=0, want_image=True):
+        """Queue a render request; returns (ticket, output array filled when the request is done).
+        want_image=False: the image is not copied back (load generators); the array is None."""
+        out = np.zeros((width * height, 4), np.float32) if want_image else None
         r = Request()
         r.scene = scene
         r.camera = self.camera(cam)
         r.params = PtParams(spp, max_depth, rr_depth, use_nee, use_mis, seed, mlp_mode, 0)
         r.width = width
         r.height = height
-        r.rgba_out = out.ctypes.data
+        r.rgba_out = out.ctypes.data if out is not None else None
         t = C.c_uint64()
         self.check(self.lib.srt_submit(self.h, C.byref(r), C.byref(t)))
-        self._serve_out[t.value] = out      # keeps the buffer alive while the dispatcher writes it
+        if out is not None:
+            self._serve_out[t.value] = out      # keeps the buffer alive while the dispatcher writes it
         return t.value, out
+
+    def forget_tickets(self):
+        """Drop the binding's references to finished requests' buffers (after srt_serve_stop)."""
+        self._serve_out = {}
 
     def poll(self, ticket):
         res = Result()

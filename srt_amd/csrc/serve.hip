@@ -8,9 +8,10 @@
 //      (12 B accumulator + 16 B output per pixel) fits in the context's HBM
 //      budget; a request that can never fit fails with SRT_E_OOM;
 //   2. batching: each chunk takes the scene of the oldest admitted request and
-//      shares the path pool among every admitted request on that scene
-//      (equal shares, leftovers in arrival order), one segment per request;
-//      requests admitted while a chunk runs join the next chunk;
+//      fills the path pool with the remaining paths of the admitted requests on
+//      that scene in arrival order (first come, first served; small requests
+//      share a chunk), one segment per request; requests admitted while a
+//      chunk runs join the next chunk;
 //   3. completion: a request whose last paths were in the chunk has its film
 //      copied to the caller's buffer and its device memory released.
 // A request's paths run in increasing path-id order with its own PCG streams,
@@ -33,6 +34,7 @@ struct Request {
     DevBuf acc, out;
     double t_submit = 0, t_admit = 0, t_done = 0;
     int64_t chunks = 0;
+    bool inflight = false;            // part of the chunk the dispatcher is running
 };
 
 struct ServeState {
@@ -69,6 +71,7 @@ static void dispatcher(srt_ctx* c) {
     ServeState* S = (ServeState*)c->serve;
     (void)hipSetDevice(c->device);
     std::vector<Segment> segs;
+    std::vector<int64_t> take;                // paths of each batch request in the chunk
     for (;;) {
         std::vector<Request*> batch;
         SceneRec* scene = nullptr;
@@ -110,12 +113,21 @@ static void dispatcher(srt_ctx* c) {
                 S->cv.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
             }
-            // chunk: oldest request's scene, equal shares of the pool
+            // chunk: the oldest request's scene; the pool is filled with the
+            // remaining paths of that scene's requests in arrival order
             uint64_t sh = S->active.front()->req.scene;
             auto it = c->scenes.find(sh);
             scene = it == c->scenes.end() ? nullptr : it->second;
-            for (Request* q : S->active)
-                if (q->req.scene == sh && (int)batch.size() < kMaxSegments) batch.push_back(q);
+            int64_t left = S->pool;
+            take.clear();
+            for (Request* q : S->active) {
+                if (left == 0 || (int)batch.size() == kMaxSegments) break;
+                if (q->req.scene != sh) continue;
+                q->inflight = true;
+                batch.push_back(q);
+                take.push_back(std::min(q->total - q->next, left));
+                left -= take.back();
+            }
         }
         if (!scene) {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -127,22 +139,6 @@ static void dispatcher(srt_ctx* c) {
             }
             S->done_cv.notify_all();
             continue;
-        }
-        // water-filling of the pool over the batch, in arrival order
-        std::vector<int64_t> take(batch.size(), 0);
-        int64_t left = S->pool;
-        size_t open = batch.size();
-        while (left > 0 && open > 0) {
-            int64_t share = std::max<int64_t>(1, left / (int64_t)open);
-            open = 0;
-            for (size_t k = 0; k < batch.size() && left > 0; k++) {
-                int64_t rem = batch[k]->total - batch[k]->next - take[k];
-                if (rem <= 0) continue;
-                int64_t t = std::min(std::min(rem, share), left);
-                take[k] += t;
-                left -= t;
-                if (rem - t > 0) open++;
-            }
         }
         segs.clear();
         int64_t off = 0;
@@ -183,6 +179,7 @@ static void dispatcher(srt_ctx* c) {
             S->served_paths += off;
             for (size_t k = 0; k < batch.size(); k++) {
                 Request* q = batch[k];
+                q->inflight = false;
                 if (take[k] == 0) continue;
                 q->next += take[k];
                 q->chunks++;
@@ -244,13 +241,30 @@ int srt_serve_start(srt_ctx* c, int64_t path_pool) {
     return SRT_OK;
 }
 
-int srt_serve_stop(srt_ctx* c) {
+int srt_serve_stop(srt_ctx* c, int32_t drain) {
     if (!c) return SRT_E_INVALID;
     if (!c->serve) return set_err(c, SRT_E_STATE, "serving not started");
     ServeState* S = (ServeState*)c->serve;
     {
-        // let admitted and queued work finish
         std::unique_lock<std::mutex> lk(S->mu);
+        if (!drain) {
+            // cancel what has not started: queued requests and admitted ones
+            // with no path scheduled yet (their films are freed here)
+            for (Request* q : S->pending) finish(S, q, SRT_E_STATE);
+            S->pending.clear();
+            std::vector<Request*> keep;
+            for (Request* q : S->active) {
+                if (q->next == 0 && !q->inflight) {
+                    finish(S, q, SRT_E_STATE);
+                    dev_free(c, q->acc);
+                    dev_free(c, q->out);
+                } else {
+                    keep.push_back(q);
+                }
+            }
+            S->active.swap(keep);
+            S->done_cv.notify_all();
+        }
         S->done_cv.wait(lk, [&] { return S->pending.empty() && S->active.empty(); });
     }
     serve_free(c);

@@ -104,3 +104,20 @@ def test_admission_by_budget(served):
     for a, b in zip(res, res[1:]):
         assert b["admit_ms"] >= a["done_ms"]
     assert rb["status"] == -4
+
+
+def test_stop_without_drain_cancels_queued(served):
+    ctx, reqs, alone = served
+    h, cam, W, H, spp, d, rr, nee, mis, seed, m = reqs[0]
+    ctx.serve_start(0)
+    try:
+        first, img = ctx.submit(h, cam, W, H, spp, d, rr, nee, mis, seed)
+        rest = [ctx.submit(h, cam, 256, 256, 64, d, rr, nee, mis, seed + k)[0] for k in range(200)]
+        assert ctx.wait(first)["status"] == 2
+        assert np.array_equal(img, alone[0])
+    finally:
+        ctx.serve_stop(drain=False)
+    # the context is usable again and holds no request films
+    used = ctx.lib.srt_hbm_used(ctx.h)
+    assert np.array_equal(ctx.render(*reqs[0][:10]), alone[0])
+    assert ctx.lib.srt_hbm_used(ctx.h) == used
