@@ -85,16 +85,34 @@ SRT_D bool intersect_tri(const RayPre& r, f3 o, f3 v0, f3 v1, f3 v2, float tmin,
 
 constexpr float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013): conservative slab test
 
+// ---------------------------------------------------------------------------
+// wave-level queue append: one atomic per wave
+// ---------------------------------------------------------------------------
+SRT_D int wave_append(bool pred, uint32_t* counter) {
+    // works on the lanes active at the call: the lowest active lane does the
+    // atomic for all of them and readfirstlane broadcasts the base
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return -1;
+    const int lane = __lane_id();
+    const int first = __ffsll((unsigned long long)__builtin_amdgcn_read_exec()) - 1;
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __builtin_amdgcn_readfirstlane(base);
+    const uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    return pred ? (int)(base + prefix) : -1;
+}
+
 // One ray's traversal state, kept in registers while its lane walks the BVH.
 struct TravState {
     f3 o, inv;
     RayPre rp;
     float tmin, best_t, bu, bv;
     int32_t best_prim, best_pos;
-    int32_t cur;       // node (>= 0) or leaf (< 0) to visit next
+    int32_t cur;       // next node (>= 0) or leaf (< 0) to visit; valid while cur_ok
+    int32_t pend;      // postponed leaf; valid while has_pend
     int32_t sp;        // stack depth
     bool sx, sy, sz;   // direction signs select the near/far slab planes
-    bool live;         // false once the stack is empty (or, any-hit, a hit was found)
+    bool cur_ok, has_pend;
 };
 
 SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root) {
@@ -110,8 +128,10 @@ SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t
     st.best_pos = -1;
     st.bu = 0.0f; st.bv = 0.0f;
     st.cur = root;
+    st.pend = 0;
     st.sp = 0;
-    st.live = true;
+    st.cur_ok = true;
+    st.has_pend = false;
 }
 
 // Traversal stack: the first `lds_depth` entries of each lane live in LDS
@@ -132,55 +152,73 @@ SRT_D void stk_push(const TravStack& k, int32_t sp, int32_t v) {
 SRT_D int32_t stk_get(const TravStack& k, int32_t sp) {
     return (sp < k.lds_depth) ? k.lds[sp * k.bstride] : k.gst[(size_t)(sp - k.lds_depth) * k.gstride];
 }
-
-// Inner nodes until the lane reaches a leaf or the traversal ends.  Both child
-// boxes are tested with the robust slab test (Ize 2013); when both are hit the
-// nearer child is visited first and the other is pushed.
-template <bool COUNT>
-SRT_D void trav_inner(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nv) {
-    while (st.live && st.cur >= 0) {
-        if (COUNT) nv++;
-        const BvhNode* np = S.nodes + st.cur;
-        const float4 na = np->a, nb = np->b, nc = np->c;
-        const int4 nd = np->d;
-        const f3 o = st.o, inv = st.inv;
-        const bool sx = st.sx, sy = st.sy, sz = st.sz;
-        float lnx = ((sx ? na.y : na.x) - o.x) * inv.x, lfx = ((sx ? na.x : na.y) - o.x) * inv.x;
-        float lny = ((sy ? na.w : na.z) - o.y) * inv.y, lfy = ((sy ? na.z : na.w) - o.y) * inv.y;
-        float lnz = ((sz ? nc.y : nc.x) - o.z) * inv.z, lfz = ((sz ? nc.x : nc.y) - o.z) * inv.z;
-        float rnx = ((sx ? nb.y : nb.x) - o.x) * inv.x, rfx = ((sx ? nb.x : nb.y) - o.x) * inv.x;
-        float rny = ((sy ? nb.w : nb.z) - o.y) * inv.y, rfy = ((sy ? nb.z : nb.w) - o.y) * inv.y;
-        float rnz = ((sz ? nc.w : nc.z) - o.z) * inv.z, rfz = ((sz ? nc.z : nc.w) - o.z) * inv.z;
-        float tln = fmaxf(fmaxf(fmaxf(st.tmin, lnx), lny), lnz);
-        float tlf = fminf(fminf(fminf(st.best_t, lfx), lfy), lfz) * kRobust;
-        float trn = fmaxf(fmaxf(fmaxf(st.tmin, rnx), rny), rnz);
-        float trf = fminf(fminf(fminf(st.best_t, rfx), rfy), rfz) * kRobust;
-        const bool hl = tln <= tlf, hr = trn <= trf;
-        if (hl && hr) {
-            int32_t nearc = nd.x, farc = nd.y;
-            if (trn < tln) { nearc = nd.y; farc = nd.x; }
-            stk_push(k, st.sp, farc);
-            st.sp++;
-            st.cur = nearc;
-        } else if (hl) {
-            st.cur = nd.x;
-        } else if (hr) {
-            st.cur = nd.y;
-        } else if (st.sp == 0) {
-            st.live = false;
-        } else {
-            st.sp--;
-            st.cur = stk_get(k, st.sp);
-        }
+SRT_D void stk_pop(const TravStack& k, TravState& st) {
+    if (st.sp == 0) {
+        st.cur_ok = false;
+    } else {
+        st.sp--;
+        st.cur = stk_get(k, st.sp);
     }
 }
 
-// The triangles of leaf st.cur, then the next stack entry.  Closest hit:
-// (t, primitive id) lexicographic minimum, so the result does not depend on the
-// visiting order.  Any hit: the first hit ends the traversal; returns true.
+// Inner nodes, with speculative leaf postponement (Aila & Laine 2009): a lane
+// that reaches a leaf keeps it in `pend` and continues with the next stack
+// entry; it stops at its second leaf or when the stack is empty.  The wave
+// leaves the loop once every lane still in it holds a postponed leaf, so the
+// triangle tests that follow run with most lanes active.  Both child boxes are
+// tested with the robust slab test (Ize 2013); when both are hit the nearer
+// child is visited first and the other is pushed.
+template <bool COUNT>
+SRT_D void trav_inner(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nv) {
+    while (st.cur_ok) {
+        if (st.cur < 0) {
+            if (st.has_pend) break;               // second leaf: wait here
+            st.pend = st.cur;
+            st.has_pend = true;
+            stk_pop(k, st);
+        } else {
+            if (COUNT) nv++;
+            const BvhNode* np = S.nodes + st.cur;
+            const float4 na = np->a, nb = np->b, nc = np->c;
+            const int4 nd = np->d;
+            const f3 o = st.o, inv = st.inv;
+            const bool sx = st.sx, sy = st.sy, sz = st.sz;
+            float lnx = ((sx ? na.y : na.x) - o.x) * inv.x, lfx = ((sx ? na.x : na.y) - o.x) * inv.x;
+            float lny = ((sy ? na.w : na.z) - o.y) * inv.y, lfy = ((sy ? na.z : na.w) - o.y) * inv.y;
+            float lnz = ((sz ? nc.y : nc.x) - o.z) * inv.z, lfz = ((sz ? nc.x : nc.y) - o.z) * inv.z;
+            float rnx = ((sx ? nb.y : nb.x) - o.x) * inv.x, rfx = ((sx ? nb.x : nb.y) - o.x) * inv.x;
+            float rny = ((sy ? nb.w : nb.z) - o.y) * inv.y, rfy = ((sy ? nb.z : nb.w) - o.y) * inv.y;
+            float rnz = ((sz ? nc.w : nc.z) - o.z) * inv.z, rfz = ((sz ? nc.z : nc.w) - o.z) * inv.z;
+            float tln = fmaxf(fmaxf(fmaxf(st.tmin, lnx), lny), lnz);
+            float tlf = fminf(fminf(fminf(st.best_t, lfx), lfy), lfz) * kRobust;
+            float trn = fmaxf(fmaxf(fmaxf(st.tmin, rnx), rny), rnz);
+            float trf = fminf(fminf(fminf(st.best_t, rfx), rfy), rfz) * kRobust;
+            const bool hl = tln <= tlf, hr = trn <= trf;
+            if (hl && hr) {
+                int32_t nearc = nd.x, farc = nd.y;
+                if (trn < tln) { nearc = nd.y; farc = nd.x; }
+                stk_push(k, st.sp, farc);
+                st.sp++;
+                st.cur = nearc;
+            } else if (hl) {
+                st.cur = nd.x;
+            } else if (hr) {
+                st.cur = nd.y;
+            } else {
+                stk_pop(k, st);
+            }
+        }
+        if (__all(st.has_pend)) break;
+    }
+}
+
+// The triangles of the postponed leaf.  Closest hit: (t, primitive id)
+// lexicographic minimum, so the result does not depend on the visiting order.
+// Any hit: the first hit ends the traversal; returns true.
 template <bool ANY, bool COUNT>
-SRT_D bool trav_leaf(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nt) {
-    const uint32_t f = leaf_first(st.cur), c = leaf_count(st.cur);
+SRT_D bool trav_leaf(const DevScene& S, TravState& st, uint32_t& nt) {
+    st.has_pend = false;
+    const uint32_t f = leaf_first(st.pend), c = leaf_count(st.pend);
     for (uint32_t j = f; j < f + c; j++) {
         const float4* tp = S.tris + 3 * (size_t)j;
         const float4 A = tp[0], B = tp[1], C = tp[2];
@@ -190,7 +228,7 @@ SRT_D bool trav_leaf(const DevScene& S, TravState& st, const TravStack& k, uint3
                            &u, &v))
             continue;
         if (ANY) {
-            st.live = false;
+            st.cur_ok = false;
             return true;
         }
         const int32_t prim = __float_as_int(A.w);
@@ -198,42 +236,43 @@ SRT_D bool trav_leaf(const DevScene& S, TravState& st, const TravStack& k, uint3
             st.best_t = t; st.best_prim = prim; st.best_pos = (int32_t)j; st.bu = u; st.bv = v;
         }
     }
-    if (st.sp == 0) {
-        st.live = false;
-    } else {
-        st.sp--;
-        st.cur = stk_get(k, st.sp);
-    }
     return false;
 }
 
 // Persistent traversal with dynamic ray fetch.  Each lane holds one ray; when
-// at least `refill` lanes of the wave (or all of them) have finished, the wave
-// takes that many new ray indices from `fetch` with one atomic, so lanes do
-// not idle while the wave's slowest ray finishes.  IO supplies
+// at least `refill` lanes of the wave (or all of them) are idle, the wave
+// stores the finished lanes' results and takes that many new ray indices from
+// `fetch` with one atomic, so lanes do not idle while the wave's slowest ray
+// finishes, and the stores' appends and memory traffic happen once per refill.
+// IO supplies
 //   void load(uint32_t i, TravState&, int32_t root)
-//   void store(uint32_t i, const TravState&, bool occluded)
+//   void store(uint32_t i, const TravState&, bool occluded)   (on the storing lanes)
 // Every wave leaves the loop once `fetch` has passed `count` and its lanes
-// have finished.
+// have finished and stored.
 template <bool ANY, bool COUNT, class IO>
 SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, int refill, const TravStack& k, IO& io,
                             uint32_t& nv, uint32_t& nt) {
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
     TravState st;
-    st.live = false;
+    st.cur_ok = false;
+    st.has_pend = false;
     uint32_t ri = 0;
-    bool has = false, exhausted = false, occl = false;
+    bool has = false, done = false, exhausted = false, occl = false;
     for (;;) {
-        const bool need = !has && !exhausted;
-        const unsigned long long m = __ballot(need);
+        const bool idle = !has && !exhausted;
+        const unsigned long long m = __ballot(idle);
         unsigned long long act = __ballot(has);
         if (m != 0ull && (act == 0ull || __popcll(m) >= refill)) {
+            if (done) {
+                io.store(ri, st, occl);
+                done = false;
+            }
             const int leader = __ffsll(m) - 1;
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
             base = __shfl(base, leader);
-            if (need) {
+            if (idle) {
                 const uint32_t idx = base + (uint32_t)__popcll(m & below);
                 if (idx < count) {
                     ri = idx;
@@ -249,30 +288,13 @@ SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, 
         if (act == 0ull) break;
         if (has) {
             trav_inner<COUNT>(S, st, k, nv);
-            if (st.live) occl = trav_leaf<ANY, COUNT>(S, st, k, nt);
-            if (!st.live) {
-                io.store(ri, st, occl);
+            if (st.has_pend) occl = trav_leaf<ANY, COUNT>(S, st, nt);
+            if (occl || (!st.cur_ok && !st.has_pend)) {
                 has = false;
+                done = true;
             }
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// wave-level queue append: one atomic per wave
-// ---------------------------------------------------------------------------
-SRT_D int wave_append(bool pred, uint32_t* counter) {
-    unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return -1;
-    int lane = __lane_id();
-    unsigned long long below = (lane == 0) ? 0ull : (mask & ((1ull << lane) - 1ull));
-    uint32_t prefix = (uint32_t)__popcll(below);
-    uint32_t total = (uint32_t)__popcll(mask);
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, total);
-    base = __shfl(base, leader);
-    return pred ? (int)(base + prefix) : -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -329,7 +351,7 @@ __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* L
 }
 
 // Ray sources/sinks of the persistent traversal loop.
-struct QueueIO {       // closest hit of queue entries -> hit buffer
+struct QueueIO {       // closest hit of queue entries -> hit buffer (same index)
     PathQueue q;
     HitBuf h;
     SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
@@ -434,195 +456,212 @@ SRT_D bool finite3(f3 v) {
 }
 
 // One bounce for every entry of queue `qi` (path_tracer.cu:80-161).
-__global__ void __launch_bounds__(256) k_shade(DevScene S, const Segment* segs, int depth, PathQueue qin, HitBuf h,
-                                               PathQueue qout, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
-                                               uint32_t* counts, int qi, MlpIO mio) {
+// Shading of one hit (queue entry i, hit record i): hit reconstruction,
+// emitter term with MIS, next-event estimation, the radiance-MLP hooks, BSDF
+// sampling and Russian roulette.  Appends go to the lanes active at the call.
+SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const PathQueue& qin, const HitBuf& h,
+                     const PathQueue& qout, const ShadowQueue& sq, float* Lr, float* Lg, float* Lb, uint32_t* counts,
+                     int qi, const MlpIO& mio, uint32_t i) {
+    const uint32_t slot = qin.slot[i];
+    const f3 rd = mk(qin.dx[i], qin.dy[i], qin.dz[i]);
+    f3 beta = mk(qin.br[i], qin.bg[i], qin.bb[i]);
+    const int32_t tri = h.tri[i];
+    const uint32_t flags = qin.flags[i];
+    const bool specular = (flags & 1u) != 0u;
+    const uint32_t segk = flags >> 8;
+    const Segment& P = segs[segk];
+    // hit reconstruction (hit_info.h:27-65)
+    const float4* tp = S.tris + 3 * (size_t)tri;
+    const float4 A = tp[0], B = tp[1], C = tp[2];
+    const int32_t prim = __float_as_int(A.w);
+    const int32_t mesh = __float_as_int(B.w);
+    const f3 v0 = mk(A.x, A.y, A.z), v1 = mk(B.x, B.y, B.z), v2 = mk(C.x, C.y, C.z);
+    const float u = h.u[i], v = h.v[i];
+    const float w = 1.0f - u - v;
+    const f3 pos = v0 * w + v1 * u + v2 * v;
+    f3 normal = normalize(cross(v1 - v0, v2 - v0));
+    f3 sn = normal;
+    const uint32_t* ix = S.idx + 3 * (size_t)prim;
+    if (S.nrm) {
+        f3 n0 = ld3(S.nrm + 3 * (size_t)ix[0]), n1 = ld3(S.nrm + 3 * (size_t)ix[1]), n2 = ld3(S.nrm + 3 * (size_t)ix[2]);
+        sn = normalize(n0 * w + n1 * u + n2 * v);
+    }
+    if (dot(normal, sn) < 0.0f) normal = -normal;
+    const MatRec& m = S.mats[S.mesh_mat[mesh]];
+    const f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
+    if (mat_is_emissive(m)) {
+        // emitter hit with MIS (path_tracer.cu:88-106)
+        const float cos_i = dot(normal, -rd);
+        if (cos_i > 1e-4f) {
+            float mis = 1.0f;
+            if (P.use_nee && !specular) {
+                mis = 0.0f;
+                if (P.use_mis) {
+                    const f3 ro = mk(qin.ox[i], qin.oy[i], qin.oz[i]);
+                    const float t2 = dot(pos - ro, pos - ro);
+                    const float lpdf = (m.intensity / S.light_wsum) * t2 / cos_i;
+                    const float scatter_pdf = qin.pdf[i];
+                    mis = scatter_pdf / (lpdf + scatter_pdf);
+                }
+            }
+            const f3 Le = mk(m.emission_color[0], m.emission_color[1], m.emission_color[2]) * m.intensity;
+            const f3 add = beta * Le * mis;
+            Lr[slot] = Lr[slot] + add.x; Lg[slot] = Lg[slot] + add.y; Lb[slot] = Lb[slot] + add.z;
+        }
+        return;
+    }
+    f3 tg;
+    if (S.tan) {
+        f3 t0 = ld3(S.tan + 3 * (size_t)ix[0]), t1 = ld3(S.tan + 3 * (size_t)ix[1]), t2 = ld3(S.tan + 3 * (size_t)ix[2]);
+        tg = t0 * w + t1 * u + t2 * v;
+    } else {
+        tg = (fabsf(sn.x) > fabsf(sn.y)) ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    }
+    const Onb onb = make_onb(sn, tg);
+    if (dot(normal, -rd) < 0.0f) normal = -normal;
+    const bool last = (depth + 1 >= P.max_depth);
+    const int64_t pid = P.path_begin + (int64_t)(slot - (uint32_t)P.slot_offset);
+    const uint32_t sample = (uint32_t)(pid / P.wh);
+    Pcg32 rng = rng_resume(qin.rng[i], sample);
+    // next-event estimation (path_tracer.cu:113-142); the shadow ray and its
+    // would-be contribution go to the shadow queue
+    if (P.use_nee && !mat_is_specular(m)) {
+        f2 lr;
+        lr.x = rng.next_float();
+        lr.y = rng.next_float();
+        const LightSample ls = light_sample(S, lr);
+        const f3 sdir = normalize(ls.pos - pos);
+        const float cos_o = dot(normal, sdir);
+        const float cos_light = dot(ls.normal, -sdir);
+        const float t = length(ls.pos - pos);
+        const bool emit = (cos_light > 1e-4f) && (ls.pdf > 1e-4f) && (t > 1e-3f) && (cos_o > 0.0f || mat_is_transmissive(m));
+        f3 s_c = mk(0.0f);
+        if (emit) {
+            float mis = 1.0f;
+            const float lpdf = ls.pdf * t * t / cos_light;
+            if (P.use_mis) {
+                const float mpdf = mat_pdf(m, sdir, -rd, onb, color);
+                mis = lpdf / (lpdf + mpdf);
+            }
+            const f3 f = mat_eval(m, sdir, -rd, onb, color);
+            s_c = beta * f * fabsf(cos_o) * ls.emission * mis / lpdf;
+        }
+        const int js = wave_append(emit, counts + CNT_SHADOW);
+        if (js >= 0) {
+            sq.ox[js] = pos.x; sq.oy[js] = pos.y; sq.oz[js] = pos.z;
+            sq.dx[js] = sdir.x; sq.dy[js] = sdir.y; sq.dz[js] = sdir.z;
+            sq.tmax[js] = t - 1e-3f;
+            sq.cr[js] = s_c.x; sq.cg[js] = s_c.y; sq.cb[js] = s_c.z;
+            sq.slot[js] = slot;
+        }
+    }
+    if (P.mlp_mode != 0 && depth == 1 && !last) {
+        // radiance-MLP hooks at vertex 2; features: position in the scene
+        // box, (n+1)/2, (w_o+1)/2
+        float q9[9];
+        q9[0] = (pos.x - mio.lo[0]) * mio.inv_ext[0];
+        q9[1] = (pos.y - mio.lo[1]) * mio.inv_ext[1];
+        q9[2] = (pos.z - mio.lo[2]) * mio.inv_ext[2];
+        q9[3] = (normal.x + 1.0f) * 0.5f; q9[4] = (normal.y + 1.0f) * 0.5f; q9[5] = (normal.z + 1.0f) * 0.5f;
+        q9[6] = (-rd.x + 1.0f) * 0.5f; q9[7] = (-rd.y + 1.0f) * 0.5f; q9[8] = (-rd.z + 1.0f) * 0.5f;
+        if (P.mlp_mode == 1) {
+            // the path stops here; the prediction is added after the bounce's shadow rays
+            const int jq = wave_append(true, counts + CNT_MLPQ);
+            for (int k = 0; k < 9; k++) mio.q_in9[(size_t)jq * 9 + k] = q9[k];
+            mio.q_beta[(size_t)jq * 3] = beta.x;
+            mio.q_beta[(size_t)jq * 3 + 1] = beta.y;
+            mio.q_beta[(size_t)jq * 3 + 2] = beta.z;
+            mio.q_slot[jq] = slot;
+            return;
+        }
+        for (int k = 0; k < 9; k++) mio.rec_in9[(size_t)slot * 9 + k] = q9[k];
+        mio.rec_beta[(size_t)slot * 3] = beta.x;
+        mio.rec_beta[(size_t)slot * 3 + 1] = beta.y;
+        mio.rec_beta[(size_t)slot * 3 + 2] = beta.z;
+        mio.rec_valid[slot] = 1u;
+    }
+    if (last) return;
+    // BSDF sampling, throughput update, Russian roulette (path_tracer.cu:145-161)
+    f2 br;
+    br.x = rng.next_float();
+    br.y = rng.next_float();
+    const BxSample ms = mat_sample(m, -rd, br, onb, color);
+    bool alive = !(ms.pdf <= 1e-4f);
+    if (alive && dot(ms.wi, normal) < 0.0f && !mat_is_transmissive(m)) alive = false;
+    if (alive) {
+        beta = beta * (ms.f / ms.pdf);
+        if (!ms.delta) beta = beta * fabsf(dot(ms.wi, normal));
+        if (depth >= P.rr_depth) {
+            const float q = fmaxf(fmaxf(beta.x, fmaxf(beta.y, beta.z)), 0.05f);
+            if (rng.next_float() > q) alive = false;
+            else beta = beta / q;
+        }
+    }
+    const int jn = wave_append(alive, counts + (qi ^ 1));
+    if (jn >= 0) {
+        qout.ox[jn] = pos.x; qout.oy[jn] = pos.y; qout.oz[jn] = pos.z;
+        qout.dx[jn] = ms.wi.x; qout.dy[jn] = ms.wi.y; qout.dz[jn] = ms.wi.z;
+        qout.br[jn] = beta.x; qout.bg[jn] = beta.y; qout.bb[jn] = beta.z;
+        qout.pdf[jn] = ms.pdf;
+        qout.flags[jn] = (mat_is_specular(m) ? 1u : 0u) | (segk << 8);
+        qout.rng[jn] = rng.state;
+        qout.slot[jn] = slot;
+    }
+}
+
+// Per entry of queue `qi`: a miss adds beta * env (path_tracer.cu:81-85) and
+// ends the path; a hit is shaded.  About two thirds of the entries of an open
+// scene are misses, so each wave first gathers the indices of 64 hits in its
+// own LDS area (ballot + prefix count, no global atomics) and then shades them
+// with every lane busy.
+constexpr int kShadeBlock = 256;
+// orders one wave's LDS accesses across lanes (LDS executes a wave's operations
+// in order; this keeps the compiler from moving them across the point)
+SRT_D void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+__global__ void __launch_bounds__(kShadeBlock, 5) k_shade(DevScene S, const Segment* segs, int depth, PathQueue qin,
+                                                         HitBuf h, PathQueue qout, ShadowQueue sq, float* Lr,
+                                                         float* Lg, float* Lb, uint32_t* counts, int qi, MlpIO mio) {
+    __shared__ uint32_t gather[kShadeBlock / 64][128];
     const int lane = threadIdx.x & 63;
+    uint32_t* buf = gather[threadIdx.x >> 6];
+    const unsigned long long below = (1ull << lane) - 1ull;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
     const uint32_t count = counts[qi];
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_FETCH_T] = 0u;   // the next trace stage fetches from 0
-    for (uint32_t base = (uint32_t)wave * 64u; base < count; base += (uint32_t)nwaves * 64u) {
-        const uint32_t i = base + (uint32_t)lane;
-        bool emit_next = false, emit_shadow = false;
-        // outputs of this lane
-        f3 n_o = mk(0.0f), n_d = mk(0.0f), n_beta = mk(0.0f);
-        float n_pdf = 0.0f;
-        uint32_t n_flags = 0;
-        uint64_t n_rng = 0;
-        f3 s_d = mk(0.0f), s_c = mk(0.0f);
-        float s_tmax = 0.0f;
-        uint32_t slot = 0;
-        bool emit_query = false;
-        float q9[9];
-        f3 q_beta = mk(0.0f);
-        if (i < count) {
-            slot = qin.slot[i];
-            f3 ro = mk(qin.ox[i], qin.oy[i], qin.oz[i]);
-            f3 rd = mk(qin.dx[i], qin.dy[i], qin.dz[i]);
-            f3 beta = mk(qin.br[i], qin.bg[i], qin.bb[i]);
-            float scatter_pdf = qin.pdf[i];
-            const uint32_t flags = qin.flags[i];
-            bool specular = (flags & 1u) != 0u;
-            const uint32_t segk = flags >> 8;
-            const Segment& P = segs[segk];
-            const bool last = (depth + 1 >= P.max_depth);
-            int32_t tri = h.tri[i];
-            const int64_t pid = P.path_begin + (int64_t)(slot - (uint32_t)P.slot_offset);
-            const uint32_t sample = (uint32_t)(pid / P.wh);
-            Pcg32 rng = rng_resume(qin.rng[i], sample);
-            if (tri < 0) {
-                // miss (path_tracer.cu:81-85)
-                f3 env = mk(S.env[0], S.env[1], S.env[2]);
-                f3 add = beta * env;
+    const f3 env = mk(S.env[0], S.env[1], S.env[2]);
+    uint32_t nbuf = 0;                     // gathered hits (wave-uniform)
+    uint32_t base = (uint32_t)wave * 64u;
+    for (;;) {
+        // gather until 64 hits are waiting or the queue is exhausted
+        while (nbuf < 64u && base < count) {
+            const uint32_t i = base + (uint32_t)lane;
+            const bool valid = i < count;
+            const int32_t tri = valid ? h.tri[i] : -1;
+            if (valid && tri < 0) {
+                const uint32_t slot = qin.slot[i];
+                const f3 add = mk(qin.br[i], qin.bg[i], qin.bb[i]) * env;
                 Lr[slot] = Lr[slot] + add.x; Lg[slot] = Lg[slot] + add.y; Lb[slot] = Lb[slot] + add.z;
-            } else {
-                // hit reconstruction (hit_info.h:27-65)
-                const float4* tp = S.tris + 3 * (size_t)tri;
-                float4 A = tp[0], B = tp[1], C = tp[2];
-                const int32_t prim = __float_as_int(A.w);
-                const int32_t mesh = __float_as_int(B.w);
-                f3 v0 = mk(A.x, A.y, A.z), v1 = mk(B.x, B.y, B.z), v2 = mk(C.x, C.y, C.z);
-                float u = h.u[i], v = h.v[i];
-                float w = 1.0f - u - v;
-                f3 pos = v0 * w + v1 * u + v2 * v;
-                f3 normal = normalize(cross(v1 - v0, v2 - v0));
-                f3 sn = normal;
-                const uint32_t* ix = S.idx + 3 * (size_t)prim;
-                if (S.nrm) {
-                    f3 n0 = ld3(S.nrm + 3 * (size_t)ix[0]), n1 = ld3(S.nrm + 3 * (size_t)ix[1]), n2 = ld3(S.nrm + 3 * (size_t)ix[2]);
-                    sn = normalize(n0 * w + n1 * u + n2 * v);
-                }
-                if (dot(normal, sn) < 0.0f) normal = -normal;
-                f3 tg;
-                if (S.tan) {
-                    f3 t0 = ld3(S.tan + 3 * (size_t)ix[0]), t1 = ld3(S.tan + 3 * (size_t)ix[1]), t2 = ld3(S.tan + 3 * (size_t)ix[2]);
-                    tg = t0 * w + t1 * u + t2 * v;
-                } else {
-                    tg = (fabsf(sn.x) > fabsf(sn.y)) ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
-                }
-                const MatRec& m = S.mats[S.mesh_mat[mesh]];
-                f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
-                if (mat_is_emissive(m)) {
-                    // emitter hit with MIS (path_tracer.cu:88-106)
-                    float cos_i = dot(normal, -rd);
-                    if (cos_i > 1e-4f) {
-                        float mis = 1.0f;
-                        if (P.use_nee && !specular) {
-                            mis = 0.0f;
-                            if (P.use_mis) {
-                                float t2 = dot(pos - ro, pos - ro);
-                                float lpdf = (m.intensity / S.light_wsum) * t2 / cos_i;
-                                mis = scatter_pdf / (lpdf + scatter_pdf);
-                            }
-                        }
-                        f3 Le = mk(m.emission_color[0], m.emission_color[1], m.emission_color[2]) * m.intensity;
-                        f3 add = beta * Le * mis;
-                        Lr[slot] = Lr[slot] + add.x; Lg[slot] = Lg[slot] + add.y; Lb[slot] = Lb[slot] + add.z;
-                    }
-                } else {
-                    Onb onb = make_onb(sn, tg);
-                    float cos_i = dot(normal, -rd);
-                    if (cos_i < 0.0f) normal = -normal;
-                    // next-event estimation (path_tracer.cu:113-142); the shadow ray
-                    // and its would-be contribution go to the shadow queue
-                    if (P.use_nee && !mat_is_specular(m)) {
-                        f2 lr;
-                        lr.x = rng.next_float();
-                        lr.y = rng.next_float();
-                        LightSample ls = light_sample(S, lr);
-                        f3 sdir = normalize(ls.pos - pos);
-                        float cos_o = dot(normal, sdir);
-                        float cos_light = dot(ls.normal, -sdir);
-                        float t = length(ls.pos - pos);
-                        if ((cos_light > 1e-4f) && (ls.pdf > 1e-4f) && (t > 1e-3f) && (cos_o > 0.0f || mat_is_transmissive(m))) {
-                            float mis = 1.0f;
-                            float lpdf = ls.pdf * t * t / cos_light;
-                            if (P.use_mis) {
-                                float mpdf = mat_pdf(m, sdir, -rd, onb, color);
-                                mis = lpdf / (lpdf + mpdf);
-                            }
-                            f3 f = mat_eval(m, sdir, -rd, onb, color);
-                            s_c = beta * f * fabsf(cos_o) * ls.emission * mis / lpdf;
-                            s_d = sdir;
-                            s_tmax = t - 1e-3f;
-                            emit_shadow = true;
-                        }
-                    }
-                    bool mlp_here = (P.mlp_mode != 0) && depth == 1 && !last;
-                    if (mlp_here) {
-                        // vertex 2 features: position in the scene box, (n+1)/2, (w_o+1)/2
-                        f3 wo = -rd;
-                        q9[0] = (pos.x - mio.lo[0]) * mio.inv_ext[0];
-                        q9[1] = (pos.y - mio.lo[1]) * mio.inv_ext[1];
-                        q9[2] = (pos.z - mio.lo[2]) * mio.inv_ext[2];
-                        q9[3] = (normal.x + 1.0f) * 0.5f; q9[4] = (normal.y + 1.0f) * 0.5f; q9[5] = (normal.z + 1.0f) * 0.5f;
-                        q9[6] = (wo.x + 1.0f) * 0.5f; q9[7] = (wo.y + 1.0f) * 0.5f; q9[8] = (wo.z + 1.0f) * 0.5f;
-                        q_beta = beta;
-                        if (P.mlp_mode == 1) emit_query = true;
-                        else {
-                            for (int k = 0; k < 9; k++) mio.rec_in9[(size_t)slot * 9 + k] = q9[k];
-                            mio.rec_beta[(size_t)slot * 3] = beta.x;
-                            mio.rec_beta[(size_t)slot * 3 + 1] = beta.y;
-                            mio.rec_beta[(size_t)slot * 3 + 2] = beta.z;
-                            mio.rec_valid[slot] = 1u;
-                        }
-                    }
-                    if (!last && !emit_query) {
-                        // BSDF sampling, throughput update, Russian roulette (path_tracer.cu:145-161)
-                        f2 br;
-                        br.x = rng.next_float();
-                        br.y = rng.next_float();
-                        BxSample ms = mat_sample(m, -rd, br, onb, color);
-                        bool alive = !(ms.pdf <= 1e-4f);
-                        if (alive && dot(ms.wi, normal) < 0.0f && !mat_is_transmissive(m)) alive = false;
-                        if (alive) {
-                            beta = beta * (ms.f / ms.pdf);
-                            if (!ms.delta) beta = beta * fabsf(dot(ms.wi, normal));
-                            if (depth >= P.rr_depth) {
-                                float q = fmaxf(fmaxf(beta.x, fmaxf(beta.y, beta.z)), 0.05f);
-                                if (rng.next_float() > q) alive = false;
-                                else beta = beta / q;
-                            }
-                        }
-                        if (alive) {
-                            emit_next = true;
-                            n_o = pos; n_d = ms.wi; n_beta = beta; n_pdf = ms.pdf;
-                            n_flags = (mat_is_specular(m) ? 1u : 0u) | (segk << 8);
-                            n_rng = rng.state;
-                        }
-                    }
-                    if (emit_shadow) n_o = pos;
-                }
             }
+            const bool hit = valid && tri >= 0;
+            const unsigned long long m = __ballot(hit);
+            if (hit) buf[nbuf + (uint32_t)__popcll(m & below)] = i;
+            nbuf += (uint32_t)__popcll(m);
+            base += (uint32_t)nwaves * 64u;
         }
-        // compaction (all 64 lanes execute the ballots)
-        int js = wave_append(emit_shadow, counts + CNT_SHADOW);
-        if (js >= 0) {
-            sq.ox[js] = n_o.x; sq.oy[js] = n_o.y; sq.oz[js] = n_o.z;
-            sq.dx[js] = s_d.x; sq.dy[js] = s_d.y; sq.dz[js] = s_d.z;
-            sq.tmax[js] = s_tmax;
-            sq.cr[js] = s_c.x; sq.cg[js] = s_c.y; sq.cb[js] = s_c.z;
-            sq.slot[js] = slot;
-        }
-        int jq = wave_append(emit_query, counts + CNT_MLPQ);
-        if (jq >= 0) {
-            for (int k = 0; k < 9; k++) mio.q_in9[(size_t)jq * 9 + k] = q9[k];
-            mio.q_beta[(size_t)jq * 3] = q_beta.x;
-            mio.q_beta[(size_t)jq * 3 + 1] = q_beta.y;
-            mio.q_beta[(size_t)jq * 3 + 2] = q_beta.z;
-            mio.q_slot[jq] = slot;
-        }
-        int jn = wave_append(emit_next, counts + (qi ^ 1));
-        if (jn >= 0) {
-            qout.ox[jn] = n_o.x; qout.oy[jn] = n_o.y; qout.oz[jn] = n_o.z;
-            qout.dx[jn] = n_d.x; qout.dy[jn] = n_d.y; qout.dz[jn] = n_d.z;
-            qout.br[jn] = n_beta.x; qout.bg[jn] = n_beta.y; qout.bb[jn] = n_beta.z;
-            qout.pdf[jn] = n_pdf;
-            qout.flags[jn] = n_flags;
-            qout.rng[jn] = n_rng;
-            qout.slot[jn] = slot;
-        }
+        if (nbuf == 0u) break;
+        wave_sync_lds();
+        const uint32_t take = nbuf < 64u ? nbuf : 64u;
+        const uint32_t e = buf[lane];
+        const uint32_t rest = (lane + 64u < nbuf) ? buf[lane + 64] : 0u;
+        wave_sync_lds();
+        if (lane + 64u < nbuf) buf[lane] = rest;
+        nbuf -= take;
+        if ((uint32_t)lane < take) shade_hit(S, segs, depth, qin, h, qout, sq, Lr, Lg, Lb, counts, qi, mio, e);
+        wave_sync_lds();
     }
 }
 
@@ -999,7 +1038,8 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     for (int depth = 0; depth < max_depth; depth++) {
         int qi = depth & 1;
         prof_begin(c, "trace", &ev);
-        if (c->count_visits) k_trace<true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+        if (c->count_visits)
+            k_trace<true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
         else k_trace<false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
         prof_end(c, "trace", ev, 0);
         prof_begin(c, "shade", &ev);
