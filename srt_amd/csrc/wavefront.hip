@@ -138,19 +138,24 @@ SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t
 // (lane-interleaved: entry e of thread t at lds[e * blockDim + t]); deeper
 // entries go to a per-thread global area (entry e at gst[(e - lds_depth) *
 // gstride]).  The build guarantees a tree depth <= kMaxDepth.
+extern __shared__ int32_t trav_lds[];    // dynamic LDS of the traversal kernels: lds_depth * blockDim entries
 struct TravStack {
-    int32_t* lds;
     int32_t* gst;
     size_t gstride;
     int lds_depth;
     int bstride;
+    int lane0;           // this thread's first LDS entry (threadIdx.x)
 };
 SRT_D void stk_push(const TravStack& k, int32_t sp, int32_t v) {
-    if (sp < k.lds_depth) k.lds[sp * k.bstride] = v;
+    if (sp < k.lds_depth) trav_lds[k.lane0 + sp * k.bstride] = v;
     else k.gst[(size_t)(sp - k.lds_depth) * k.gstride] = v;
 }
 SRT_D int32_t stk_get(const TravStack& k, int32_t sp) {
-    return (sp < k.lds_depth) ? k.lds[sp * k.bstride] : k.gst[(size_t)(sp - k.lds_depth) * k.gstride];
+    if (sp < k.lds_depth) return trav_lds[k.lane0 + sp * k.bstride];
+    int32_t v = k.gst[(size_t)(sp - k.lds_depth) * k.gstride];
+    // keeps the compiler from merging the two loads into one generic (flat) load
+    __asm__ volatile("" : "+v"(v));
+    return v;
 }
 SRT_D void stk_pop(const TravStack& k, TravState& st) {
     if (st.sp == 0) {
@@ -194,19 +199,14 @@ SRT_D void trav_inner(const DevScene& S, TravState& st, const TravStack& k, uint
             float trn = fmaxf(fmaxf(fmaxf(st.tmin, rnx), rny), rnz);
             float trf = fminf(fminf(fminf(st.best_t, rfx), rfy), rfz) * kRobust;
             const bool hl = tln <= tlf, hr = trn <= trf;
+            const bool swap = hr && (!hl || trn < tln);        // right child first
+            const int32_t first = swap ? nd.y : nd.x, second = swap ? nd.x : nd.y;
             if (hl && hr) {
-                int32_t nearc = nd.x, farc = nd.y;
-                if (trn < tln) { nearc = nd.y; farc = nd.x; }
-                stk_push(k, st.sp, farc);
+                stk_push(k, st.sp, second);
                 st.sp++;
-                st.cur = nearc;
-            } else if (hl) {
-                st.cur = nd.x;
-            } else if (hr) {
-                st.cur = nd.y;
-            } else {
-                stk_pop(k, st);
             }
+            if (hl || hr) st.cur = first;
+            else stk_pop(k, st);
         }
         if (__all(st.has_pend)) break;
     }
@@ -380,10 +380,10 @@ struct ShadowIO {      // any hit in [1e-3, tmax]; unoccluded rays add their con
     }
 };
 
-SRT_D TravStack make_stack(int32_t* lds, int32_t* gstack, TraceCfg cfg) {
+SRT_D TravStack make_stack(int32_t* gstack, TraceCfg cfg) {
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     TravStack k;
-    k.lds = lds + threadIdx.x;
+    k.lane0 = (int)threadIdx.x;
     k.gst = gstack + gtid;
     k.gstride = (size_t)gridDim.x * blockDim.x;
     k.lds_depth = cfg.lds_depth;
@@ -398,7 +398,6 @@ SRT_D TravStack make_stack(int32_t* lds, int32_t* gstack, TraceCfg cfg) {
 template <bool COUNT>
 __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
                                                       int32_t* gstack, unsigned long long* totals, TraceCfg cfg) {
-    extern __shared__ int32_t lds_stack[];
     const uint32_t count = counts[qi];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         counts[qi ^ 1] = 0u;
@@ -407,7 +406,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, 
         counts[CNT_FETCH_S] = 0u;
         totals[0] += count;
     }
-    const TravStack k = make_stack(lds_stack, gstack, cfg);
+    const TravStack k = make_stack(gstack, cfg);
     QueueIO io{q, h};
     uint32_t nv = 0, nt = 0;
     trace_persistent<false, COUNT>(S, count, counts + CNT_FETCH_T, cfg.refill, k, io, nv, nt);
@@ -558,23 +557,27 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     if (P.mlp_mode != 0 && depth == 1 && !last) {
         // radiance-MLP hooks at vertex 2; features: position in the scene
         // box, (n+1)/2, (w_o+1)/2
-        float q9[9];
+        float* q9;
+        if (P.mlp_mode == 1) {
+            // the path stops here; the prediction is added after the bounce's shadow rays
+            const int jq = wave_append(true, counts + CNT_MLPQ);
+            q9 = mio.q_in9 + (size_t)jq * 9;
+        } else {
+            q9 = mio.rec_in9 + (size_t)slot * 9;
+        }
         q9[0] = (pos.x - mio.lo[0]) * mio.inv_ext[0];
         q9[1] = (pos.y - mio.lo[1]) * mio.inv_ext[1];
         q9[2] = (pos.z - mio.lo[2]) * mio.inv_ext[2];
         q9[3] = (normal.x + 1.0f) * 0.5f; q9[4] = (normal.y + 1.0f) * 0.5f; q9[5] = (normal.z + 1.0f) * 0.5f;
         q9[6] = (-rd.x + 1.0f) * 0.5f; q9[7] = (-rd.y + 1.0f) * 0.5f; q9[8] = (-rd.z + 1.0f) * 0.5f;
         if (P.mlp_mode == 1) {
-            // the path stops here; the prediction is added after the bounce's shadow rays
-            const int jq = wave_append(true, counts + CNT_MLPQ);
-            for (int k = 0; k < 9; k++) mio.q_in9[(size_t)jq * 9 + k] = q9[k];
+            const int jq = (int)((q9 - mio.q_in9) / 9);
             mio.q_beta[(size_t)jq * 3] = beta.x;
             mio.q_beta[(size_t)jq * 3 + 1] = beta.y;
             mio.q_beta[(size_t)jq * 3 + 2] = beta.z;
             mio.q_slot[jq] = slot;
             return;
         }
-        for (int k = 0; k < 9; k++) mio.rec_in9[(size_t)slot * 9 + k] = q9[k];
         mio.rec_beta[(size_t)slot * 3] = beta.x;
         mio.rec_beta[(size_t)slot * 3 + 1] = beta.y;
         mio.rec_beta[(size_t)slot * 3 + 2] = beta.z;
@@ -670,10 +673,9 @@ template <bool COUNT>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
                                                        uint32_t* counts, int32_t* gstack, unsigned long long* totals,
                                                        TraceCfg cfg) {
-    extern __shared__ int32_t lds_stack[];
     const uint32_t count = counts[CNT_SHADOW];
     if (blockIdx.x == 0 && threadIdx.x == 0) totals[1] += count;
-    const TravStack k = make_stack(lds_stack, gstack, cfg);
+    const TravStack k = make_stack(gstack, cfg);
     ShadowIO io{sq, Lr, Lg, Lb};
     uint32_t nv = 0, nt = 0;
     trace_persistent<true, COUNT>(S, count, counts + CNT_FETCH_S, cfg.refill, k, io, nv, nt);
@@ -780,8 +782,7 @@ struct RaysIO {        // standalone queries on flat arrays
 
 __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, uint32_t n, RaysIO io, uint32_t* fetch,
                                                            int32_t* gstack, TraceCfg cfg) {
-    extern __shared__ int32_t lds_stack[];
-    const TravStack k = make_stack(lds_stack, gstack, cfg);
+    const TravStack k = make_stack(gstack, cfg);
     uint32_t nv = 0, nt = 0;
     if (io.any) trace_persistent<true, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
     else trace_persistent<false, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
