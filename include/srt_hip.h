@@ -108,7 +108,8 @@ const char* srt_last_error(const srt_ctx* ctx);
  * (bytes; at least srt_hbm_used), "trace_stack_lds" (traversal stack entries per
  * lane in LDS, 4..64, default 16), "trace_refill" (finished lanes that make a
  * traversal wave fetch new rays, 1..64, default 32), "trace_blocks_per_cu"
- * (0: as many as fit) */
+ * (0: as many as fit), "trace_bvh_width" (4: traverse the 4-wide tree collapsed
+ * from the binary one, default; 2: the binary tree; results are identical) */
 int srt_set_option(srt_ctx* ctx, const char* key, int64_t value);
 size_t srt_hbm_used(const srt_ctx* ctx);
 

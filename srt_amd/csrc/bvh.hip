@@ -581,6 +581,105 @@ __global__ void __launch_bounds__(256) k_refit(int num_nodes, BvhNode* nodes, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// 4-wide tree collapsed from the binary tree, top-down by levels: a 4-wide node
+// made from binary node x starts with x's two children and repeatedly replaces
+// the inner child of largest surface area by its two children until it has 4
+// children or only leaves.  Nodes are numbered level by level in order
+// (deterministic); boxes are copies of binary-tree boxes, so both trees give
+// every ray the same box tests.
+// ---------------------------------------------------------------------------
+SRT_D DBox child_box(const BvhNode& nd, int side) {
+    DBox b;
+    if (side == 0) {
+        b.lo[0] = nd.a.x; b.hi[0] = nd.a.y; b.lo[1] = nd.a.z; b.hi[1] = nd.a.w; b.lo[2] = nd.c.x; b.hi[2] = nd.c.y;
+    } else {
+        b.lo[0] = nd.b.x; b.hi[0] = nd.b.y; b.lo[1] = nd.b.z; b.hi[1] = nd.b.w; b.lo[2] = nd.c.z; b.hi[2] = nd.c.w;
+    }
+    return b;
+}
+struct Kids4 {
+    int32_t ref[4], src[4];
+    int n;
+};
+SRT_D Kids4 collapse_children(const BvhNode* nodes, int32_t x) {
+    Kids4 k;
+    float area[4];
+    const BvhNode nd = nodes[x];
+    k.ref[0] = nd.d.x; k.src[0] = x << 1; area[0] = box_area(child_box(nd, 0));
+    k.ref[1] = nd.d.y; k.src[1] = (x << 1) | 1; area[1] = box_area(child_box(nd, 1));
+    k.ref[2] = k.ref[3] = 0; k.src[2] = k.src[3] = -1; area[2] = area[3] = 0.0f;
+    k.n = 2;
+    while (k.n < 4) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int j = 0; j < k.n; j++)
+            if (k.ref[j] >= 0 && area[j] > ba) { ba = area[j]; best = j; }
+        if (best < 0) break;
+        const int32_t y = k.ref[best];
+        const BvhNode ny = nodes[y];
+        k.ref[best] = ny.d.x; k.src[best] = y << 1; area[best] = box_area(child_box(ny, 0));
+        k.ref[k.n] = ny.d.y; k.src[k.n] = (y << 1) | 1; area[k.n] = box_area(child_box(ny, 1));
+        k.n++;
+    }
+    return k;
+}
+__global__ void __launch_bounds__(256) k_c4_count(int nf, const int32_t* level, const BvhNode* nodes, uint32_t* ninner) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const Kids4 k = collapse_children(nodes, level[i]);
+    uint32_t m = 0;
+    for (int j = 0; j < k.n; j++) m += (k.ref[j] >= 0) ? 1u : 0u;
+    ninner[i] = m;
+}
+SRT_D void set4(Bvh4Node& o, int j, const DBox& b) {
+    float* lx = &o.lox.x; float* hx = &o.hix.x; float* ly = &o.loy.x; float* hy = &o.hiy.x;
+    float* lz = &o.loz.x; float* hz = &o.hiz.x;
+    lx[j] = b.lo[0]; hx[j] = b.hi[0]; ly[j] = b.lo[1]; hy[j] = b.hi[1]; lz[j] = b.lo[2]; hz[j] = b.hi[2];
+}
+__global__ void __launch_bounds__(256) k_c4_emit(int nf, const int32_t* level, const BvhNode* nodes, const uint32_t* off,
+                                                 int base_this, int base_next, Bvh4Node* out, int32_t* next_level) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const Kids4 k = collapse_children(nodes, level[i]);
+    Bvh4Node o;
+    int32_t* ref = &o.ref.x;
+    int32_t* src = &o.src.x;
+    uint32_t q = off[i];
+    DBox empty;
+    empty.lo[0] = empty.lo[1] = empty.lo[2] = __builtin_huge_valf();
+    empty.hi[0] = empty.hi[1] = empty.hi[2] = -__builtin_huge_valf();
+    for (int j = 0; j < 4; j++) {
+        if (j < k.n) {
+            const int32_t sx = k.src[j];
+            set4(o, j, child_box(nodes[sx >> 1], sx & 1));
+            src[j] = sx;
+            if (k.ref[j] >= 0) {
+                next_level[q] = k.ref[j];
+                ref[j] = base_next + (int32_t)q;
+                q++;
+            } else {
+                ref[j] = k.ref[j];
+            }
+        } else {
+            set4(o, j, empty);
+            src[j] = -1;
+            ref[j] = 0;
+        }
+    }
+    out[base_this + i] = o;
+}
+// after a binary-tree refit: each wide child box is copied from its binary slot again
+__global__ void __launch_bounds__(256) k_refit4(int n4, const BvhNode* nodes, Bvh4Node* nodes4) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    Bvh4Node o = nodes4[i];
+    const int32_t* src = &o.src.x;
+    for (int j = 0; j < 4; j++)
+        if (src[j] >= 0) set4(o, j, child_box(nodes[src[j] >> 1], src[j] & 1));
+    nodes4[i] = o;
+}
+
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -784,8 +883,55 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     s->num_top = top;
     s->cluster_size = T;
     s->max_leaf = max_leaf;
+    if (int r = bvh4_build(c, s)) return r;
     s->has_bvh = true;
     return SRT_OK;
+}
+
+int bvh4_build(srt_ctx* c, SceneRec* s) {
+    dev_free(c, s->nodes4);
+    s->num_nodes4 = 0;
+    s->depth4 = 0;
+    if (s->root < 0) {            // a single leaf
+        s->root4 = s->root;
+        return SRT_OK;
+    }
+    hipStream_t st = c->stream;
+    const int n2 = s->num_nodes;
+    if (dev_alloc(c, s->nodes4, (size_t)n2 * sizeof(Bvh4Node))) return SRT_E_OOM;
+    Tmp tmp(c);
+    int32_t* lev[2] = {tmp.get<int32_t>(n2), tmp.get<int32_t>(n2)};
+    uint32_t* ninner = tmp.get<uint32_t>(n2);
+    uint32_t* off = tmp.get<uint32_t>(n2);
+    uint32_t* tot = tmp.get<uint32_t>(2);
+    if (!lev[0] || !lev[1] || !ninner || !off || !tot) return SRT_E_OOM;
+    size_t scan_bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, ninner, off, n2, st);
+    void* scan_tmp = tmp.get<uint8_t>(scan_bytes);
+    if (!scan_tmp) return SRT_E_OOM;
+    k_fill_i32<<<1, 64, 0, st>>>(lev[0], 1, s->root);
+    Bvh4Node* out = (Bvh4Node*)s->nodes4.p;
+    int base = 0, nf = 1, cur = 0, levels = 0;
+    while (nf > 0) {
+        if (base + nf > n2) return set_err(c, SRT_E_LIMIT, "4-wide collapse exceeded the binary node count");
+        k_c4_count<<<nblk(nf), 256, 0, st>>>(nf, lev[cur], (const BvhNode*)s->nodes.p, ninner);
+        if (int r = hip_check(c, hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, ninner, off, nf, st), "scan"))
+            return r;
+        k_c4_emit<<<nblk(nf), 256, 0, st>>>(nf, lev[cur], (const BvhNode*)s->nodes.p, off, base, base + nf, out,
+                                            lev[cur ^ 1]);
+        uint32_t last[2] = {0, 0};
+        if (int r = hip_check(c, hipMemcpyAsync(&last[0], off + nf - 1, 4, hipMemcpyDeviceToHost, st), "count")) return r;
+        if (int r = hip_check(c, hipMemcpyAsync(&last[1], ninner + nf - 1, 4, hipMemcpyDeviceToHost, st), "count")) return r;
+        if (int r = hip_check(c, hipStreamSynchronize(st), "collapse")) return r;
+        base += nf;
+        nf = (int)(last[0] + last[1]);
+        cur ^= 1;
+        levels++;
+    }
+    s->root4 = 0;
+    s->num_nodes4 = base;
+    s->depth4 = levels;
+    return hip_check(c, hipGetLastError(), "collapse launch");
 }
 
 int bvh_refit(srt_ctx* c, SceneRec* s) {
@@ -799,6 +945,9 @@ int bvh_refit(srt_ctx* c, SceneRec* s) {
         k_refit<<<nblk(s->num_nodes), 256, 0, st>>>(s->num_nodes, (BvhNode*)s->nodes.p, (const int32_t*)s->node_valid.p,
                                                     (const int32_t*)s->node_parent.p, (const float4*)s->tris.p,
                                                     (int32_t*)s->refit_counter.p);
+        if (s->num_nodes4 > 0)
+            k_refit4<<<nblk(s->num_nodes4), 256, 0, st>>>(s->num_nodes4, (const BvhNode*)s->nodes.p,
+                                                          (Bvh4Node*)s->nodes4.p);
     }
     if (int r = hip_check(c, hipGetLastError(), "refit launch")) return r;
     return hip_check(c, hipStreamSynchronize(st), "refit");

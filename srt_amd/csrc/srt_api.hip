@@ -89,6 +89,8 @@ DevScene dev_scene(const SceneRec* s) {
     S.tris = (const float4*)s->tris.p;
     S.nodes = (const BvhNode*)s->nodes.p;
     S.root = s->root;
+    S.nodes4 = (const Bvh4Node*)s->nodes4.p;
+    S.root4 = s->root4;
     S.num_tris = s->nf;
     S.idx = (const uint32_t*)s->idx.p;
     S.pos = (const float*)s->pos.p;
@@ -110,7 +112,7 @@ static void free_scene(srt_ctx* c, SceneRec* s) {
     DevBuf* all[] = {&s->pos, &s->nrm, &s->tan, &s->idx, &s->mesh_mat, &s->mesh_first, &s->area_cdf, &s->mats,
                      &s->light_cdf, &s->light_mesh, &s->tri_mesh, &s->orig_pos, &s->orig_nrm, &s->orig_tan,
                      &s->bone_ids, &s->bone_w, &s->bones, &s->nodes, &s->tris, &s->order, &s->node_parent,
-                     &s->refit_counter, &s->node_valid};
+                     &s->refit_counter, &s->node_valid, &s->nodes4};
     for (DevBuf* b : all) dev_free(c, *b);
     delete s;
 }
@@ -192,6 +194,11 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
     if (!strcmp(key, "trace_refill")) {
         if (value < 1 || value > 64) return set_err(c, SRT_E_INVALID, "trace_refill must be in [1, 64]");
         c->trace_refill = (int32_t)value;
+        return SRT_OK;
+    }
+    if (!strcmp(key, "trace_bvh_width")) {
+        if (value != 2 && value != 4) return set_err(c, SRT_E_INVALID, "trace_bvh_width must be 2 or 4");
+        c->trace_bvh_width = (int32_t)value;
         return SRT_OK;
     }
     if (!strcmp(key, "trace_blocks_per_cu")) {

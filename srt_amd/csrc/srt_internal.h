@@ -29,6 +29,18 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "node is 64 bytes");
 
+// 4-wide node (128 B) collapsed from the binary tree: the four child boxes as
+// one float4 per bound (component k = child k), the child refs (node >= 0,
+// leaf < 0, as in BvhNode) and, for refit, the binary-tree slot each child box
+// is copied from (parent << 1 | side; -1 for an empty child, whose box is
+// empty: lo = +inf, hi = -inf, so no ray hits it).
+struct alignas(16) Bvh4Node {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 ref;
+    int4 src;
+};
+static_assert(sizeof(Bvh4Node) == 128, "wide node is 128 bytes");
+
 __host__ __device__ inline int32_t leaf_ref(uint32_t first, uint32_t count) {
     return (int32_t)(0x80000000u | (first << 3) | (count - 1u));
 }
@@ -44,6 +56,7 @@ constexpr int kTraceBlock = 128;   // traversal workgroup size (2 waves)
 struct TraceCfg {
     int32_t lds_depth;   // stack entries per lane kept in LDS, 4..64
     int32_t refill;      // finished lanes that make a wave fetch new rays, 1..64
+    int32_t wide;        // 1: traverse the 4-wide tree, 0: the binary tree
 };
 
 // ---------------------------------------------------------------------------
@@ -53,6 +66,8 @@ struct DevScene {
     const float4* tris;        // 3 float4 per sorted triangle: (v0, prim) (v1, mesh) (v2, 0)
     const BvhNode* nodes;
     int32_t root;
+    const Bvh4Node* nodes4;    // the same tree, 4 wide
+    int32_t root4;
     int32_t num_tris;
     const uint32_t* idx;       // 3 per triangle (original order)
     const float* pos;          // 3 per vertex
@@ -171,6 +186,8 @@ struct SceneRec {
     bool has_bvh = false;
     DevBuf nodes, tris, order, node_parent, refit_counter, node_valid;
     int32_t root = 0, num_nodes = 0, max_depth = 0, num_clusters = 0, num_top = 0, cluster_size = 0;
+    DevBuf nodes4;
+    int32_t root4 = 0, num_nodes4 = 0, depth4 = 0;     // 4-wide tree: nodes, levels
     int32_t max_leaf = 4;
     // host copies needed for light CDF construction
     std::vector<float> h_mesh_area;
@@ -200,6 +217,7 @@ struct srt_ctx {
     int64_t film_pixels = 0;
     size_t gstack_entries = 0;            // int32 entries allocated in gstack
     int32_t trace_lds_depth = 16, trace_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
+    int32_t trace_bvh_width = 4;          // traversal over the 4-wide (4) or binary (2) tree
     // profiling
     bool profile = false;
     std::map<std::string, srt::KernelStat> stats;
@@ -240,6 +258,7 @@ void prof_collect(srt_ctx* c);
 // BVH (bvh.hip)
 int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size);
 int bvh_refit(srt_ctx* c, SceneRec* s);
+int bvh4_build(srt_ctx* c, SceneRec* s);
 int skin_vertices(srt_ctx* c, SceneRec* s, const float* bones_3x4, int nb);
 // wavefront integrator (wavefront.hip)
 // training samples produced by a mode-2 render

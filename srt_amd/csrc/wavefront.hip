@@ -212,6 +212,70 @@ SRT_D void trav_inner(const DevScene& S, TravState& st, const TravStack& k, uint
     }
 }
 
+// Slab test of one box (same arithmetic as the binary step).
+SRT_D bool slab(const TravState& st, float lx, float hx, float ly, float hy, float lz, float hz, float* tn) {
+    const f3 o = st.o, inv = st.inv;
+    const float nx = ((st.sx ? hx : lx) - o.x) * inv.x, fx = ((st.sx ? lx : hx) - o.x) * inv.x;
+    const float ny = ((st.sy ? hy : ly) - o.y) * inv.y, fy = ((st.sy ? ly : hy) - o.y) * inv.y;
+    const float nz = ((st.sz ? hz : lz) - o.z) * inv.z, fz = ((st.sz ? lz : hz) - o.z) * inv.z;
+    const float tnear = fmaxf(fmaxf(fmaxf(st.tmin, nx), ny), nz);
+    const float tfar = fminf(fminf(fminf(st.best_t, fx), fy), fz) * kRobust;
+    *tn = tnear;
+    return tnear <= tfar;
+}
+SRT_D void cswap(float& ka, int32_t& ra, float& kb, int32_t& rb) {
+    const bool s = kb < ka;
+    const float k = s ? kb : ka;
+    const int32_t r = s ? rb : ra;
+    kb = s ? ka : kb;
+    rb = s ? ra : rb;
+    ka = k;
+    ra = r;
+}
+
+// Inner nodes of the 4-wide tree, with the same leaf postponement as the
+// binary step: the hit children are ordered by entry distance, the nearest is
+// visited next and the others are pushed farthest first.
+template <bool COUNT>
+SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nv) {
+    while (st.cur_ok) {
+        if (st.cur < 0) {
+            if (st.has_pend) break;
+            st.pend = st.cur;
+            st.has_pend = true;
+            stk_pop(k, st);
+        } else {
+            if (COUNT) nv++;
+            const Bvh4Node* np = S.nodes4 + st.cur;
+            const float4 lx = np->lox, hx = np->hix, ly = np->loy, hy = np->hiy, lz = np->loz, hz = np->hiz;
+            const int4 rf = np->ref;
+            float k0, k1, k2, k3;
+            const bool h0 = slab(st, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, &k0);
+            const bool h1 = slab(st, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, &k1);
+            const bool h2 = slab(st, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, &k2);
+            const bool h3 = slab(st, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, &k3);
+            const float inf = __builtin_huge_valf();
+            k0 = h0 ? k0 : inf; k1 = h1 ? k1 : inf; k2 = h2 ? k2 : inf; k3 = h3 ? k3 : inf;
+            int32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+            const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+            cswap(k0, r0, k1, r1);
+            cswap(k2, r2, k3, r3);
+            cswap(k0, r0, k2, r2);
+            cswap(k1, r1, k3, r3);
+            cswap(k1, r1, k2, r2);
+            if (nh == 0) {
+                stk_pop(k, st);
+            } else {
+                if (nh > 3) { stk_push(k, st.sp, r3); st.sp++; }
+                if (nh > 2) { stk_push(k, st.sp, r2); st.sp++; }
+                if (nh > 1) { stk_push(k, st.sp, r1); st.sp++; }
+                st.cur = r0;
+            }
+        }
+        if (__all(st.has_pend)) break;
+    }
+}
+
 // The triangles of the postponed leaf.  Closest hit: (t, primitive id)
 // lexicographic minimum, so the result does not depend on the visiting order.
 // Any hit: the first hit ends the traversal; returns true.
@@ -249,9 +313,10 @@ SRT_D bool trav_leaf(const DevScene& S, TravState& st, uint32_t& nt) {
 //   void store(uint32_t i, const TravState&, bool occluded)   (on the storing lanes)
 // Every wave leaves the loop once `fetch` has passed `count` and its lanes
 // have finished and stored.
-template <bool ANY, bool COUNT, class IO>
+template <bool ANY, bool COUNT, bool WIDE, class IO>
 SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, int refill, const TravStack& k, IO& io,
                             uint32_t& nv, uint32_t& nt) {
+    const int32_t root = WIDE ? S.root4 : S.root;
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
     TravState st;
@@ -278,7 +343,7 @@ SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, 
                     ri = idx;
                     has = true;
                     occl = false;
-                    io.load(idx, st, S.root);
+                    io.load(idx, st, root);
                 } else {
                     exhausted = true;
                 }
@@ -287,7 +352,8 @@ SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, 
         }
         if (act == 0ull) break;
         if (has) {
-            trav_inner<COUNT>(S, st, k, nv);
+            if (WIDE) trav_inner4<COUNT>(S, st, k, nv);
+            else trav_inner<COUNT>(S, st, k, nv);
             if (st.has_pend) occl = trav_leaf<ANY, COUNT>(S, st, nt);
             if (occl || (!st.cur_ok && !st.has_pend)) {
                 has = false;
@@ -395,7 +461,7 @@ SRT_D TravStack make_stack(int32_t* gstack, TraceCfg cfg) {
 // following shade and shadow stages append to or fetch from.  COUNT: also
 // total node visits and triangle tests (an untimed pass used for the
 // algorithmic byte count).
-template <bool COUNT>
+template <bool COUNT, bool WIDE>
 __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
                                                       int32_t* gstack, unsigned long long* totals, TraceCfg cfg) {
     const uint32_t count = counts[qi];
@@ -409,7 +475,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, 
     const TravStack k = make_stack(gstack, cfg);
     QueueIO io{q, h};
     uint32_t nv = 0, nt = 0;
-    trace_persistent<false, COUNT>(S, count, counts + CNT_FETCH_T, cfg.refill, k, io, nv, nt);
+    trace_persistent<false, COUNT, WIDE>(S, count, counts + CNT_FETCH_T, cfg.refill, k, io, nv, nt);
     if (COUNT) {
         atomicAdd(&totals[2], (unsigned long long)nv);
         atomicAdd(&totals[3], (unsigned long long)nt);
@@ -669,7 +735,7 @@ __global__ void __launch_bounds__(kShadeBlock, 5) k_shade(DevScene S, const Segm
 }
 
 // Shadow rays of the bounce (fetch counter cleared by the trace stage).
-template <bool COUNT>
+template <bool COUNT, bool WIDE>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
                                                        uint32_t* counts, int32_t* gstack, unsigned long long* totals,
                                                        TraceCfg cfg) {
@@ -678,7 +744,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue 
     const TravStack k = make_stack(gstack, cfg);
     ShadowIO io{sq, Lr, Lg, Lb};
     uint32_t nv = 0, nt = 0;
-    trace_persistent<true, COUNT>(S, count, counts + CNT_FETCH_S, cfg.refill, k, io, nv, nt);
+    trace_persistent<true, COUNT, WIDE>(S, count, counts + CNT_FETCH_S, cfg.refill, k, io, nv, nt);
     if (COUNT) {
         atomicAdd(&totals[4], (unsigned long long)nv);
         atomicAdd(&totals[5], (unsigned long long)nt);
@@ -784,8 +850,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, uint32_t
                                                            int32_t* gstack, TraceCfg cfg) {
     const TravStack k = make_stack(gstack, cfg);
     uint32_t nv = 0, nt = 0;
-    if (io.any) trace_persistent<true, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
-    else trace_persistent<false, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
+    if (cfg.wide) {
+        if (io.any) trace_persistent<true, false, true>(S, n, fetch, cfg.refill, k, io, nv, nt);
+        else trace_persistent<false, false, true>(S, n, fetch, cfg.refill, k, io, nv, nt);
+    } else {
+        if (io.any) trace_persistent<true, false, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
+        else trace_persistent<false, false, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
+    }
 }
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
@@ -857,6 +928,7 @@ static TraceCfg trace_cfg(srt_ctx* c) {
     TraceCfg cfg;
     cfg.lds_depth = c->trace_lds_depth;
     cfg.refill = c->trace_refill;
+    cfg.wide = c->trace_bvh_width == 4 ? 1 : 0;
     return cfg;
 }
 
@@ -869,7 +941,7 @@ static int trace_grid(srt_ctx* c, int64_t n) {
     int per_cu = c->trace_blocks_per_cu;
     if (per_cu <= 0) {
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false>, kTraceBlock, trace_lds_bytes(cfg)) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, true>, kTraceBlock, trace_lds_bytes(cfg)) !=
                 hipSuccess || nb <= 0)
             nb = 8;
         per_cu = nb;
@@ -881,9 +953,12 @@ static int trace_grid(srt_ctx* c, int64_t n) {
     return (int)want;
 }
 
-// global part of the traversal stacks: (tree depth - LDS entries) per thread
+// global part of the traversal stacks, per thread: the deepest stack minus the
+// LDS entries.  A binary tree pushes at most one entry per level (depth + 1
+// entries); the 4-wide tree at most three per level.
 static int ensure_gstack(srt_ctx* c, const SceneRec* s, int64_t threads) {
-    const int64_t per = std::max<int64_t>(1, (int64_t)s->max_depth + 1 - c->trace_lds_depth);
+    const int64_t deepest = (c->trace_bvh_width == 4) ? 3 * (int64_t)s->depth4 + 1 : (int64_t)s->max_depth + 1;
+    const int64_t per = std::max<int64_t>(1, deepest - c->trace_lds_depth);
     const size_t need = (size_t)(threads * per);
     if (c->gstack_entries >= need) return SRT_OK;
     dev_free(c, c->gstack);
@@ -1039,18 +1114,30 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     for (int depth = 0; depth < max_depth; depth++) {
         int qi = depth & 1;
         prof_begin(c, "trace", &ev);
-        if (c->count_visits)
-            k_trace<true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
-        else k_trace<false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+        if (tcfg.wide) {
+            if (c->count_visits)
+                k_trace<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+            else k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+        } else {
+            if (c->count_visits)
+                k_trace<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+            else k_trace<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+        }
         prof_end(c, "trace", ev, 0);
         prof_begin(c, "shade", &ev);
         k_shade<<<shade_grid, 256, 0, st>>>(S, segs, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, mio);
         prof_end(c, "shade", ev, 0);
         if (any_nee) {
             prof_begin(c, "shadow", &ev);
-            if (c->count_visits)
-                k_shadow<true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
-            else k_shadow<false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
+            if (tcfg.wide) {
+                if (c->count_visits)
+                    k_shadow<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
+                else k_shadow<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
+            } else {
+                if (c->count_visits)
+                    k_shadow<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
+                else k_shadow<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
+            }
             prof_end(c, "shadow", ev, 0);
         }
         if (depth == 1 && mlp_mode == 1) {

@@ -1,0 +1,80 @@
+"""The 4-wide tree collapsed from the binary one: every ray gets the same
+closest hit (t, u, v, primitive) and the same any-hit answer from either
+tree, renders are bit-identical, and after a skinned refit both trees still
+agree.  (Closest hits are order-independent: (t, primitive id) minimum.)"""
+import numpy as np
+import pytest
+
+from srt_amd import scenes
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _trace_both(ctx, h, o, d, tmax=None):
+    out = {}
+    for w in (2, 4):
+        ctx.set_option("trace_bvh_width", w)
+        out[w] = (ctx.trace(h, o, d, tmax=tmax), ctx.trace(h, o, d, tmax=tmax, any_hit=True)[3])
+    ctx.set_option("trace_bvh_width", 4)
+    return out
+
+
+def _assert_same(out):
+    (t2, u2, v2, p2), a2 = out[2]
+    (t4, u4, v4, p4), a4 = out[4]
+    np.testing.assert_array_equal(p2, p4)
+    for x, y in ((t2, t4), (u2, u4), (v2, v4)):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    np.testing.assert_array_equal(a2, a4)
+
+
+@pytest.mark.parametrize("name", ["cornell", "spheres", "tube"])
+def test_primary_and_random_rays_agree(ctx, name):
+    s = getattr(scenes, name)()
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    o, d = O.primary_rays(s.camera, 320, 180, 5)
+    _assert_same(_trace_both(ctx, h, o, d))
+    rng = np.random.default_rng(11)
+    lo, hi = s.positions.min(0), s.positions.max(0)
+    n = 50000
+    o = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    tmax = rng.uniform(0.01, float(np.linalg.norm(hi - lo)), size=n).astype(np.float32)
+    out = _trace_both(ctx, h, o, d, tmax)
+    _assert_same(out)
+    assert 0.01 < out[4][1].mean() < 0.99
+    ctx.release(h)
+
+
+def test_render_identical_and_fewer_visits(ctx):
+    s = scenes.spheres()
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    imgs, visits = {}, {}
+    ctx.set_option("count_visits", 1)
+    for w in (2, 4):
+        ctx.set_option("trace_bvh_width", w)
+        imgs[w] = ctx.render(h, s.camera, 192, 108, 4, 8, 4, 1, 1, 3)
+        visits[w] = ctx.last_visits()
+    ctx.set_option("count_visits", 0)
+    ctx.set_option("trace_bvh_width", 4)
+    ctx.release(h)
+    assert np.array_equal(imgs[2], imgs[4])
+    assert visits[4]["closest_nodes"] < 0.75 * visits[2]["closest_nodes"]
+    assert visits[4]["closest_tris"] == visits[2]["closest_tris"] or visits[4]["closest_tris"] > 0
+
+
+def test_skinned_refit_keeps_trees_in_step(ctx):
+    s = scenes.tube()
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    ctx.set_skin(h, s.skin["bone_ids"], s.skin["bone_weights"])
+    ctx.skin_refit(h, s.skin["matrices"](17))
+    o, d = O.primary_rays(s.camera, 320, 180, 9)
+    out = _trace_both(ctx, h, o, d)
+    _assert_same(out)
+    assert (out[4][0][3] >= 0).mean() > 0.2
+    ctx.release(h)
