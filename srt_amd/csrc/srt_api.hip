@@ -91,6 +91,7 @@ DevScene dev_scene(const SceneRec* s) {
     S.root = s->root;
     S.nodes4 = (const Bvh4Node*)s->nodes4.p;
     S.root4 = s->root4;
+    S.num_nodes4 = s->num_nodes4;
     S.num_tris = s->nf;
     S.idx = (const uint32_t*)s->idx.p;
     S.pos = (const float*)s->pos.p;
@@ -194,6 +195,11 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
     if (!strcmp(key, "trace_refill")) {
         if (value < 1 || value > 64) return set_err(c, SRT_E_INVALID, "trace_refill must be in [1, 64]");
         c->trace_refill = (int32_t)value;
+        return SRT_OK;
+    }
+    if (!strcmp(key, "shadow_refill")) {
+        if (value < 1 || value > 64) return set_err(c, SRT_E_INVALID, "shadow_refill must be in [1, 64]");
+        c->shadow_refill = (int32_t)value;
         return SRT_OK;
     }
     if (!strcmp(key, "trace_bvh_width")) {

@@ -68,6 +68,7 @@ struct DevScene {
     int32_t root;
     const Bvh4Node* nodes4;    // the same tree, 4 wide
     int32_t root4;
+    int32_t num_nodes4;
     int32_t num_tris;
     const uint32_t* idx;       // 3 per triangle (original order)
     const float* pos;          // 3 per vertex
@@ -216,7 +217,7 @@ struct srt_ctx {
     srt::DevBuf q[2][14], hits[4], shq[11], Lbuf[3], counters, film_acc, film_out, gstack;
     int64_t film_pixels = 0;
     size_t gstack_entries = 0;            // int32 entries allocated in gstack
-    int32_t trace_lds_depth = 16, trace_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
+    int32_t trace_lds_depth = 16, trace_refill = 48, shadow_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
     int32_t trace_bvh_width = 4;          // traversal over the 4-wide (4) or binary (2) tree
     // profiling
     bool profile = false;

@@ -113,6 +113,7 @@ struct TravState {
     int32_t sp;        // stack depth
     bool sx, sy, sz;   // direction signs select the near/far slab planes
     bool cur_ok, has_pend;
+    uint32_t onx, ony, onz;   // byte offsets of the near planes in a Bvh4Node (far plane: offset ^ 16)
 };
 
 SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root) {
@@ -122,6 +123,9 @@ SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t
     // a ray lying in a slab plane with a zero direction component yields NaN
     // slab distances, which fmaxf/fminf ignore
     st.sx = st.inv.x < 0.0f; st.sy = st.inv.y < 0.0f; st.sz = st.inv.z < 0.0f;
+    st.onx = st.sx ? 16u : 0u;    // lox 0, hix 16, loy 32, hiy 48, loz 64, hiz 80
+    st.ony = st.sy ? 48u : 32u;
+    st.onz = st.sz ? 80u : 64u;
     st.tmin = tmin;
     st.best_t = tmax;
     st.best_prim = 0x7fffffff;
@@ -212,12 +216,13 @@ SRT_D void trav_inner(const DevScene& S, TravState& st, const TravStack& k, uint
     }
 }
 
-// Slab test of one box (same arithmetic as the binary step).
-SRT_D bool slab(const TravState& st, float lx, float hx, float ly, float hy, float lz, float hz, float* tn) {
+// Slab test of one box given its near and far planes (same arithmetic as the
+// binary step).
+SRT_D bool slab(const TravState& st, float bnx, float bfx, float bny, float bfy, float bnz, float bfz, float* tn) {
     const f3 o = st.o, inv = st.inv;
-    const float nx = ((st.sx ? hx : lx) - o.x) * inv.x, fx = ((st.sx ? lx : hx) - o.x) * inv.x;
-    const float ny = ((st.sy ? hy : ly) - o.y) * inv.y, fy = ((st.sy ? ly : hy) - o.y) * inv.y;
-    const float nz = ((st.sz ? hz : lz) - o.z) * inv.z, fz = ((st.sz ? lz : hz) - o.z) * inv.z;
+    const float nx = (bnx - o.x) * inv.x, fx = (bfx - o.x) * inv.x;
+    const float ny = (bny - o.y) * inv.y, fy = (bfy - o.y) * inv.y;
+    const float nz = (bnz - o.z) * inv.z, fz = (bfz - o.z) * inv.z;
     const float tnear = fmaxf(fmaxf(fmaxf(st.tmin, nx), ny), nz);
     const float tfar = fminf(fminf(fminf(st.best_t, fx), fy), fz) * kRobust;
     *tn = tnear;
@@ -233,11 +238,28 @@ SRT_D void cswap(float& ka, int32_t& ra, float& kb, int32_t& rb) {
     ra = r;
 }
 
+// 16-byte loads from the 4-wide node array through a buffer resource: one
+// SGPR base and a 32-bit per-lane byte offset (instead of a 64-bit address per
+// load); offsets outside the array read zeros.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+SRT_D __amdgpu_buffer_rsrc_t node4_rsrc(const DevScene& S) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)S.nodes4, (short)0, S.num_nodes4 * (int)sizeof(Bvh4Node), 0x00020000);
+}
+SRT_D float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+SRT_D int4 ld4i(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return make_int4((int)v.x, (int)v.y, (int)v.z, (int)v.w);
+}
+
 // Inner nodes of the 4-wide tree, with the same leaf postponement as the
 // binary step: the hit children are ordered by entry distance, the nearest is
 // visited next and the others are pushed farthest first.
 template <bool COUNT>
 SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nv) {
+    const __amdgpu_buffer_rsrc_t rsrc = node4_rsrc(S);
     while (st.cur_ok) {
         if (st.cur < 0) {
             if (st.has_pend) break;
@@ -246,14 +268,17 @@ SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uin
             stk_pop(k, st);
         } else {
             if (COUNT) nv++;
-            const Bvh4Node* np = S.nodes4 + st.cur;
-            const float4 lx = np->lox, hx = np->hix, ly = np->loy, hy = np->hiy, lz = np->loz, hz = np->hiz;
-            const int4 rf = np->ref;
+            // the ray's near and far planes are picked by address (per-ray offsets)
+            const uint32_t nb = (uint32_t)st.cur * (uint32_t)sizeof(Bvh4Node);
+            const float4 nx = ld4(rsrc, nb + st.onx), fx = ld4(rsrc, nb + (st.onx ^ 16u));
+            const float4 ny = ld4(rsrc, nb + st.ony), fy = ld4(rsrc, nb + (st.ony ^ 16u));
+            const float4 nz = ld4(rsrc, nb + st.onz), fz = ld4(rsrc, nb + (st.onz ^ 16u));
+            const int4 rf = ld4i(rsrc, nb + 96u);
             float k0, k1, k2, k3;
-            const bool h0 = slab(st, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, &k0);
-            const bool h1 = slab(st, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, &k1);
-            const bool h2 = slab(st, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, &k2);
-            const bool h3 = slab(st, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, &k3);
+            const bool h0 = slab(st, nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, &k0);
+            const bool h1 = slab(st, nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, &k1);
+            const bool h2 = slab(st, nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, &k2);
+            const bool h3 = slab(st, nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, &k3);
             const float inf = __builtin_huge_valf();
             k0 = h0 ? k0 : inf; k1 = h1 ? k1 : inf; k2 = h2 ? k2 : inf; k3 = h3 ? k3 : inf;
             int32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
@@ -377,9 +402,26 @@ SRT_D int find_segment(const Segment* segs, int nseg, uint32_t i) {
     return lo;
 }
 
+// Pixel order of the path ids: inside each sample plane, pixels follow 8x8
+// tiles (bands of 8 rows, each cut into blocks of 8 columns, row-major inside
+// a block; the last band and block may be narrower), so the 64 paths of a wave
+// cover a compact tile.  tile_pixel is a bijection of [0, W H).  Results do
+// not depend on the order: each path's random stream is
+// keyed by (pixel, sample) and each pixel sums its samples in sample order.
+SRT_D uint32_t tile_pixel(uint32_t rank, uint32_t W, uint32_t H) {
+    const uint32_t band = rank / (8u * W);
+    const uint32_t bh = (H - 8u * band < 8u) ? H - 8u * band : 8u;
+    const uint32_t r = rank - band * 8u * W;
+    const uint32_t b = r / (8u * bh);
+    const uint32_t bw = (W - 8u * b < 8u) ? W - 8u * b : 8u;
+    const uint32_t r2 = r - b * 8u * bh;
+    const uint32_t ly = r2 / bw, lx = r2 - ly * bw;
+    return (band * 8u + ly) * W + b * 8u + lx;
+}
+
 // One camera ray per slot (path_tracer.cu:66-68, camera.h:107-117, :27-32).
 // Slot i belongs to segment k: path id pid = path_begin + (i - slot_offset),
-// sample = pid / (W H), pixel = pid mod (W H).
+// sample = pid / (W H), pixel = tile_pixel(pid mod (W H)).
 __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* Lg, float* Lb, const Segment* segs, int nseg,
                                                 int n, uint32_t* counts) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -392,7 +434,7 @@ __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* L
     const Segment& sg = segs[k];
     int64_t pid = sg.path_begin + (i - sg.slot_offset);
     uint32_t sample = (uint32_t)(pid / sg.wh);
-    uint32_t pixel = (uint32_t)(pid - (int64_t)sample * sg.wh);
+    uint32_t pixel = tile_pixel((uint32_t)(pid - (int64_t)sample * sg.wh), (uint32_t)sg.width, (uint32_t)sg.height);
     int ix = (int)(pixel % (uint32_t)sg.width), iy = (int)(pixel / (uint32_t)sg.width);
     const DevCamera& cam = sg.cam;
     Pcg32 rng = path_rng(pixel, sg.seed, sample);
@@ -792,13 +834,14 @@ __global__ void __launch_bounds__(256) k_targets(int n, const float* Lr, const f
 // segment; a request's film is written when its last segment completes.
 __global__ void __launch_bounds__(256) k_finalize(const float* Lr, const float* Lg, const float* Lb, const Segment* segs) {
     const Segment& sg = segs[blockIdx.y];
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    int q = blockIdx.x * blockDim.x + threadIdx.x;     // tile rank of the pixel
     if (q >= sg.wh) return;
+    const uint32_t px = tile_pixel((uint32_t)q, (uint32_t)sg.width, (uint32_t)sg.height);
     const int64_t end = sg.path_begin + sg.n;
     int64_t s0 = (sg.path_begin - q + sg.wh - 1) / sg.wh;
     if (s0 < 0) s0 = 0;
     float* acc = sg.film_acc;
-    float ax = acc[3 * (size_t)q], ay = acc[3 * (size_t)q + 1], az = acc[3 * (size_t)q + 2];
+    float ax = acc[3 * (size_t)px], ay = acc[3 * (size_t)px + 1], az = acc[3 * (size_t)px + 2];
     const float fspp = (float)sg.spp;
     for (int64_t s = s0;; s++) {
         int64_t pid = s * sg.wh + q;
@@ -811,8 +854,8 @@ __global__ void __launch_bounds__(256) k_finalize(const float* Lr, const float* 
             az = az + L.z / fspp;
         }
     }
-    acc[3 * (size_t)q] = ax; acc[3 * (size_t)q + 1] = ay; acc[3 * (size_t)q + 2] = az;
-    if (sg.last && sg.film_out) sg.film_out[q] = make_float4(ax, ay, az, 1.0f);
+    acc[3 * (size_t)px] = ax; acc[3 * (size_t)px + 1] = ay; acc[3 * (size_t)px + 2] = az;
+    if (sg.last && sg.film_out) sg.film_out[px] = make_float4(ax, ay, az, 1.0f);
 }
 
 __global__ void k_zero(float* p, size_t n) {
@@ -1068,6 +1111,8 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     const int tgrid = trace_grid(c, n);
     if (int r = ensure_gstack(c, s, (int64_t)tgrid * kTraceBlock)) return r;
     const TraceCfg tcfg = trace_cfg(c);
+    TraceCfg scfg = tcfg;
+    scfg.refill = c->shadow_refill;
     const size_t tlds = trace_lds_bytes(tcfg);
     if (!c->segbuf.p) if (int r = dev_alloc(c, c->segbuf, sizeof(Segment) * kMaxSegments)) return r;
     hipStream_t st = c->stream;
@@ -1131,12 +1176,12 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
             prof_begin(c, "shadow", &ev);
             if (tcfg.wide) {
                 if (c->count_visits)
-                    k_shadow<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
-                else k_shadow<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
+                    k_shadow<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, scfg);
+                else k_shadow<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, scfg);
             } else {
                 if (c->count_visits)
-                    k_shadow<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
-                else k_shadow<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, tcfg);
+                    k_shadow<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, scfg);
+                else k_shadow<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, scfg);
             }
             prof_end(c, "shadow", ev, 0);
         }
