@@ -304,167 +304,208 @@ SRT_D void copy_item(const Items& s, int i, const Items& d, int j) {
     d.count[j] = s.count[i]; d.ref[j] = s.ref[i];
 }
 
-__global__ void __launch_bounds__(kTopThreads) k_top_sah(int m, Items it, Items tmp, int32_t* task_b, int32_t* task_e,
-                                                         int32_t* task_node, BvhNode* nodes) {
+// Level-parallel form: one workgroup per node of a level finds its split and
+// partitions its item range in place (stable: left items, then right items,
+// each in their previous order); k_top_link then numbers the inner children of
+// the level in order (breadth-first numbering, left before right) and writes
+// the parents' child boxes and refs.  Each node's split depends only on its own
+// items, so the tree is the one a sequential breadth-first build produces.
+struct TopSplit {
+    int32_t mid;
+    float box[2][6];        // child boxes: lo xyz, hi xyz
+};
+constexpr int kSplitThreads = 256;
+__global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp, const int32_t* task_b, const int32_t* task_e,
+                                                             TopSplit* out) {
     __shared__ uint32_t s_cb[6];
     __shared__ uint32_t s_bin[3][kBins][6];
     __shared__ int32_t s_cnt[3][kBins];
-    __shared__ int32_t s_scan[kTopThreads];
+    __shared__ int32_t s_scan[kSplitThreads];
     __shared__ uint32_t s_box[2][6];
-    __shared__ int32_t s_head, s_tail, s_next, s_axis, s_split, s_mid, s_run;
+    __shared__ int32_t s_axis, s_split, s_run;
     __shared__ float s_lo[3], s_ext[3];
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        task_b[0] = 0; task_e[0] = m; task_node[0] = 0;
-        s_head = 0; s_tail = 1; s_next = 1;
+    const int b = task_b[blockIdx.x], e = task_e[blockIdx.x];
+    if (tid < 6) s_cb[tid] = (tid < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
+    for (int j = tid; j < 3 * kBins * 6; j += kSplitThreads)
+        (&s_bin[0][0][0])[j] = ((j % 6) < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
+    for (int j = tid; j < 3 * kBins; j += kSplitThreads) (&s_cnt[0][0])[j] = 0;
+    if (tid < 12) (&s_box[0][0])[tid] = ((tid % 6) < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
+    __syncthreads();
+    // centroid bounds
+    for (int i = b + tid; i < e; i += kSplitThreads) {
+        for (int a = 0; a < 3; a++) {
+            uint32_t k = fkey(item_c(it, a, i));
+            atomicMin(&s_cb[a], k);
+            atomicMax(&s_cb[3 + a], k);
+        }
     }
     __syncthreads();
-    for (;;) {
-        __syncthreads();
-        if (s_head == s_tail) break;
-        const int b = task_b[s_head], e = task_e[s_head], node = task_node[s_head];
-        if (tid < 6) s_cb[tid] = (tid < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
-        for (int j = tid; j < 3 * kBins * 6; j += kTopThreads)
-            (&s_bin[0][0][0])[j] = ((j % 6) < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
-        for (int j = tid; j < 3 * kBins; j += kTopThreads) (&s_cnt[0][0])[j] = 0;
-        if (tid < 12) (&s_box[0][0])[tid] = ((tid % 6) < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
-        __syncthreads();
-        // centroid bounds
-        for (int i = b + tid; i < e; i += kTopThreads) {
-            for (int a = 0; a < 3; a++) {
-                uint32_t k = fkey(item_c(it, a, i));
-                atomicMin(&s_cb[a], k);
-                atomicMax(&s_cb[3 + a], k);
-            }
-        }
-        __syncthreads();
-        if (tid < 3) {
-            float lo = fromkey(s_cb[tid]), hi = fromkey(s_cb[3 + tid]);
-            s_lo[tid] = lo;
-            s_ext[tid] = hi - lo;
-        }
-        __syncthreads();
-        // bins for the three axes
-        for (int i = b + tid; i < e; i += kTopThreads) {
-            DBox ib = ldbox(it.box, i);
-            int cnt = it.count[i];
-            for (int a = 0; a < 3; a++) {
-                float ex = s_ext[a];
-                if (!(ex > 0.0f)) continue;
-                float scale = (float)kBins / ex;
-                int bi = bin_of(item_c(it, a, i), s_lo[a], scale);
-                for (int q = 0; q < 3; q++) {
-                    atomicMin(&s_bin[a][bi][q], fkey(ib.lo[q]));
-                    atomicMax(&s_bin[a][bi][3 + q], fkey(ib.hi[q]));
-                }
-                atomicAdd(&s_cnt[a][bi], cnt);
-            }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            float best = __builtin_huge_valf();
-            int best_axis = -1, best_split = -1;
-            for (int a = 0; a < 3; a++) {
-                if (!(s_ext[a] > 0.0f)) continue;
-                float la[kBins], ra[kBins];
-                int lc[kBins], rc[kBins];
-                DBox acc = empty_box();
-                int cnt = 0;
-                for (int k = 0; k < kBins; k++) {
-                    DBox bb;
-                    for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[a][k][q]); bb.hi[q] = fromkey(s_bin[a][k][3 + q]); }
-                    grow(acc, bb);
-                    cnt += s_cnt[a][k];
-                    la[k] = box_area(acc);
-                    lc[k] = cnt;
-                }
-                acc = empty_box();
-                cnt = 0;
-                for (int k = kBins - 1; k >= 0; k--) {
-                    DBox bb;
-                    for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[a][k][q]); bb.hi[q] = fromkey(s_bin[a][k][3 + q]); }
-                    grow(acc, bb);
-                    cnt += s_cnt[a][k];
-                    ra[k] = box_area(acc);
-                    rc[k] = cnt;
-                }
-                for (int k = 0; k < kBins - 1; k++) {
-                    if (lc[k] == 0 || rc[k + 1] == 0) continue;
-                    float cost = la[k] * (float)lc[k] + ra[k + 1] * (float)rc[k + 1];
-                    if (cost < best) { best = cost; best_axis = a; best_split = k; }
-                }
-            }
-            s_axis = best_axis;
-            s_split = best_split;
-            s_run = 0;
-        }
-        __syncthreads();
-        const int axis = s_axis, split = s_split;
-        int mid;
-        if (axis < 0) {
-            mid = (b + e) / 2;
-        } else {
-            const float scale = (float)kBins / s_ext[axis];
-            const float lo = s_lo[axis];
-            for (int pass = 0; pass < 2; pass++) {
-                const int base = (pass == 0) ? b : b + s_run;   // s_run holds nleft after pass 0
-                __syncthreads();
-                int run = 0;
-                for (int start = b; start < e; start += kTopThreads) {
-                    int i = start + tid;
-                    int flag = 0;
-                    if (i < e) {
-                        bool left = bin_of(item_c(it, axis, i), lo, scale) <= split;
-                        flag = (pass == 0) ? (left ? 1 : 0) : (left ? 0 : 1);
-                    }
-                    s_scan[tid] = flag;
-                    __syncthreads();
-                    for (int off = 1; off < kTopThreads; off <<= 1) {
-                        int v = (tid >= off) ? s_scan[tid - off] : 0;
-                        __syncthreads();
-                        s_scan[tid] += v;
-                        __syncthreads();
-                    }
-                    if (flag) copy_item(it, i, tmp, base + run + s_scan[tid] - 1);
-                    run += s_scan[kTopThreads - 1];
-                    __syncthreads();
-                }
-                if (pass == 0 && tid == 0) s_run = run;
-                __syncthreads();
-            }
-            mid = b + s_run;
-            __syncthreads();
-            for (int i = b + tid; i < e; i += kTopThreads) copy_item(tmp, i, it, i);
-            __syncthreads();
-        }
-        // child boxes
-        for (int i = b + tid; i < e; i += kTopThreads) {
-            int side = (i < mid) ? 0 : 1;
-            DBox ib = ldbox(it.box, i);
+    if (tid < 3) {
+        float lo = fromkey(s_cb[tid]), hi = fromkey(s_cb[3 + tid]);
+        s_lo[tid] = lo;
+        s_ext[tid] = hi - lo;
+    }
+    __syncthreads();
+    // bins for the three axes
+    for (int i = b + tid; i < e; i += kSplitThreads) {
+        DBox ib = ldbox(it.box, i);
+        int cnt = it.count[i];
+        for (int a = 0; a < 3; a++) {
+            float ex = s_ext[a];
+            if (!(ex > 0.0f)) continue;
+            float scale = (float)kBins / ex;
+            int bi = bin_of(item_c(it, a, i), s_lo[a], scale);
             for (int q = 0; q < 3; q++) {
-                atomicMin(&s_box[side][q], fkey(ib.lo[q]));
-                atomicMax(&s_box[side][3 + q], fkey(ib.hi[q]));
+                atomicMin(&s_bin[a][bi][q], fkey(ib.lo[q]));
+                atomicMax(&s_bin[a][bi][3 + q], fkey(ib.hi[q]));
+            }
+            atomicAdd(&s_cnt[a][bi], cnt);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // SAH over the 15 bin boundaries of each axis (cost = area x count per side)
+        float best = __builtin_huge_valf();
+        int best_axis = -1, best_split = -1;
+        for (int a = 0; a < 3; a++) {
+            if (!(s_ext[a] > 0.0f)) continue;
+            float la[kBins], ra[kBins];
+            int lc[kBins], rc[kBins];
+            DBox acc = empty_box();
+            int cnt = 0;
+            for (int k = 0; k < kBins; k++) {
+                DBox bb;
+                for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[a][k][q]); bb.hi[q] = fromkey(s_bin[a][k][3 + q]); }
+                grow(acc, bb);
+                cnt += s_cnt[a][k];
+                la[k] = box_area(acc);
+                lc[k] = cnt;
+            }
+            acc = empty_box();
+            cnt = 0;
+            for (int k = kBins - 1; k >= 0; k--) {
+                DBox bb;
+                for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[a][k][q]); bb.hi[q] = fromkey(s_bin[a][k][3 + q]); }
+                grow(acc, bb);
+                cnt += s_cnt[a][k];
+                ra[k] = box_area(acc);
+                rc[k] = cnt;
+            }
+            for (int k = 0; k < kBins - 1; k++) {
+                if (lc[k] == 0 || rc[k + 1] == 0) continue;
+                float cost = la[k] * (float)lc[k] + ra[k + 1] * (float)rc[k + 1];
+                if (cost < best) { best = cost; best_axis = a; best_split = k; }
             }
         }
+        s_axis = best_axis;
+        s_split = best_split;
+        s_run = 0;
+    }
+    __syncthreads();
+    const int axis = s_axis, split = s_split;
+    int mid;
+    if (axis < 0) {
+        mid = (b + e) / 2;
+    } else {
+        const float scale = (float)kBins / s_ext[axis];
+        const float lo = s_lo[axis];
+        for (int pass = 0; pass < 2; pass++) {
+            const int base = (pass == 0) ? b : b + s_run;   // s_run holds nleft after pass 0
+            __syncthreads();
+            int run = 0;
+            for (int start = b; start < e; start += kSplitThreads) {
+                int i = start + tid;
+                int flag = 0;
+                if (i < e) {
+                    bool left = bin_of(item_c(it, axis, i), lo, scale) <= split;
+                    flag = (pass == 0) ? (left ? 1 : 0) : (left ? 0 : 1);
+                }
+                s_scan[tid] = flag;
+                __syncthreads();
+                for (int off = 1; off < kSplitThreads; off <<= 1) {
+                    int v = (tid >= off) ? s_scan[tid - off] : 0;
+                    __syncthreads();
+                    s_scan[tid] += v;
+                    __syncthreads();
+                }
+                if (flag) copy_item(it, i, tmp, base + run + s_scan[tid] - 1);
+                run += s_scan[kSplitThreads - 1];
+                __syncthreads();
+            }
+            if (pass == 0 && tid == 0) s_run = run;
+            __syncthreads();
+        }
+        mid = b + s_run;
         __syncthreads();
-        if (tid == 0) {
-            BvhNode* nd = nodes + node;
-            int rb[2] = {b, mid}, re[2] = {mid, e};
+        for (int i = b + tid; i < e; i += kSplitThreads) copy_item(tmp, i, it, i);
+        __syncthreads();
+    }
+    // child boxes
+    for (int i = b + tid; i < e; i += kSplitThreads) {
+        int side = (i < mid) ? 0 : 1;
+        DBox ib = ldbox(it.box, i);
+        for (int q = 0; q < 3; q++) {
+            atomicMin(&s_box[side][q], fkey(ib.lo[q]));
+            atomicMax(&s_box[side][3 + q], fkey(ib.hi[q]));
+        }
+    }
+    __syncthreads();
+    if (tid < 12) out[blockIdx.x].box[tid / 6][tid % 6] = fromkey(s_box[tid / 6][tid % 6]);
+    if (tid == 0) out[blockIdx.x].mid = mid;
+}
+
+// Numbers the inner children of one level (one workgroup; `next_id` holds the
+// next free top-level node id) and writes the parents' children.
+__global__ void __launch_bounds__(kTopThreads) k_top_link(int ntask, const int32_t* task_b, const int32_t* task_e,
+                                                          const int32_t* task_node, const TopSplit* sp, const int32_t* item_ref,
+                                                          BvhNode* nodes, int32_t* next_b, int32_t* next_e,
+                                                          int32_t* next_node, int32_t* next_id, int32_t* next_count) {
+    __shared__ int32_t s_scan[kTopThreads];
+    __shared__ int32_t s_base, s_id;
+    const int tid = threadIdx.x;
+    if (tid == 0) { s_base = 0; s_id = *next_id; }
+    __syncthreads();
+    for (int start = 0; start < ntask; start += kTopThreads) {
+        const int t = start + tid;
+        int rb[2] = {0, 0}, re[2] = {0, 0}, inner = 0;
+        if (t < ntask) {
+            const int b = task_b[t], e = task_e[t], mid = sp[t].mid;
+            rb[0] = b; re[0] = mid; rb[1] = mid; re[1] = e;
+            inner = (re[0] - rb[0] > 1 ? 1 : 0) + (re[1] - rb[1] > 1 ? 1 : 0);
+        }
+        s_scan[tid] = inner;
+        __syncthreads();
+        for (int off = 1; off < kTopThreads; off <<= 1) {
+            int v = (tid >= off) ? s_scan[tid - off] : 0;
+            __syncthreads();
+            s_scan[tid] += v;
+            __syncthreads();
+        }
+        if (t < ntask) {
+            int q = s_base + s_scan[tid] - inner;
+            BvhNode* nd = nodes + task_node[t];
             for (int side = 0; side < 2; side++) {
                 DBox bx;
-                for (int q = 0; q < 3; q++) { bx.lo[q] = fromkey(s_box[side][q]); bx.hi[q] = fromkey(s_box[side][3 + q]); }
+                for (int k = 0; k < 3; k++) { bx.lo[k] = sp[t].box[side][k]; bx.hi[k] = sp[t].box[side][3 + k]; }
                 int32_t ref;
-                if (re[side] - rb[side] == 1) ref = it.ref[rb[side]];
-                else {
-                    ref = s_next++;
-                    task_b[s_tail] = rb[side]; task_e[s_tail] = re[side]; task_node[s_tail] = ref;
-                    s_tail++;
+                if (re[side] - rb[side] == 1) {
+                    ref = item_ref[rb[side]];
+                } else {
+                    ref = s_id + q;
+                    next_b[q] = rb[side]; next_e[q] = re[side]; next_node[q] = ref;
+                    q++;
                 }
                 set_child(nd, side, bx, ref);
             }
-            s_head++;
         }
         __syncthreads();
+        if (tid == 0) s_base += s_scan[kTopThreads - 1];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        *next_count = s_base;
+        *next_id = s_id + s_base;
     }
 }
 
@@ -848,12 +889,30 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
         };
         mk_items(it);
         mk_items(it2);
-        int32_t* tb_ = tmp.get<int32_t>(m);
-        int32_t* te_ = tmp.get<int32_t>(m);
-        int32_t* tn_ = tmp.get<int32_t>(m);
-        if (!tn_) return set_err(c, SRT_E_OOM, "top-level scratch");
+        int32_t* lb[2] = {tmp.get<int32_t>(m), tmp.get<int32_t>(m)};
+        int32_t* le[2] = {tmp.get<int32_t>(m), tmp.get<int32_t>(m)};
+        int32_t* ln[2] = {tmp.get<int32_t>(m), tmp.get<int32_t>(m)};
+        TopSplit* sp = tmp.get<TopSplit>(m);
+        int32_t* ctr = tmp.get<int32_t>(2);     // next free node id, next level size
+        if (!ctr) return set_err(c, SRT_E_OOM, "top-level scratch");
         k_items<<<nblk(m), 256, 0, st>>>(m, clusters, order, tb, K, kfirst, top, it);
-        k_top_sah<<<1, kTopThreads, 0, st>>>(m, it, it2, tb_, te_, tn_, nodes);
+        // level 0: the root (node 0) over all items; ids 1.. follow breadth-first
+        k_fill_i32<<<1, 64, 0, st>>>(lb[0], 1, 0);
+        k_fill_i32<<<1, 64, 0, st>>>(le[0], 1, m);
+        k_fill_i32<<<1, 64, 0, st>>>(ln[0], 1, 0);
+        k_fill_i32<<<1, 64, 0, st>>>(ctr, 1, 1);
+        int ntask = 1, cur = 0;
+        while (ntask > 0) {
+            k_top_split<<<ntask, kSplitThreads, 0, st>>>(it, it2, lb[cur], le[cur], sp);
+            k_top_link<<<1, kTopThreads, 0, st>>>(ntask, lb[cur], le[cur], ln[cur], sp, it.ref, nodes, lb[cur ^ 1],
+                                                  le[cur ^ 1], ln[cur ^ 1], ctr, ctr + 1);
+            int32_t next = 0;
+            if (int r = hip_check(c, hipMemcpyAsync(&next, ctr + 1, 4, hipMemcpyDeviceToHost, st), "top level")) return r;
+            if (int r = hip_check(c, hipStreamSynchronize(st), "top level")) return r;
+            if (next > m) return set_err(c, SRT_E_LIMIT, "top-level build overflow");
+            ntask = next;
+            cur ^= 1;
+        }
         k_fill_i32<<<nblk(top), 256, 0, st>>>(valid, top, 1);
         root = 0;
     } else if (n == 1) {
