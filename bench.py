@@ -323,7 +323,7 @@ def serve_mode(args):
     (trained on the first Cornell scene) on with probability 1/2 for that
     scene's requests.  Run 1: the offered rate (default 200/s) for a window,
     then queued requests are cancelled; reports served Mpaths/s.  Run 2: a
-    rate at about half the measured capacity; reports p50/p99 latency."""
+    rate at about 70 % of the measured capacity; reports p50/p99 latency."""
     from srt_amd import abi, scenes
     rng = np.random.default_rng(args.serve_seed)
     ctx = abi.Context(0)
@@ -390,7 +390,7 @@ def serve_mode(args):
     sat = run(args.serve_rate, args.serve_window, False, False)
     log(f"saturation run: {sat}")
     mean_paths = np.mean([W * H for W, H in res_choices]) * np.mean(spp_choices)
-    sub_rate = max(1.0, 0.5 * sat["served_Mpaths_s"] * 1e6 / mean_paths)
+    sub_rate = max(1.0, 0.7 * sat["served_Mpaths_s"] * 1e6 / mean_paths)
     sub = run(sub_rate, args.serve_window, True, True)
     sub["rate_per_s"] = round(sub_rate, 1)
     out = {"metric": "served Mpaths/s at saturation; p50/p99 request latency below saturation",

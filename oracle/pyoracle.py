@@ -48,6 +48,7 @@ def lib():
             "orc_render": (C.c_double, [P, P, P, P, P, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, P, P]),
+            "orc_last_counts": (None, [P]),
             "orc_tonemap": (None, [C.c_int, C.c_float, C.c_int, C.c_int, P]),
             "orc_accumulate": (None, [C.c_int, C.c_int, P, P]),
             "orc_skin": (None, [C.c_int, P, P, P, P, P, C.c_int, P, P, P, P]),
@@ -251,6 +252,13 @@ def render(scene: Scene, cam, W, H, spp, max_depth=16, rr_depth=4, use_nee=1, us
                             rr_depth, use_nee, use_mis, seed, 1 if fp64 else 0, threads, row_begin, row_step,
                             sample_begin, sample_end, _p(fo), _p(do))
     return out, secs
+
+
+def last_counts():
+    """(closest-hit traces, shadow rays issued) of the last render call."""
+    out = np.zeros(2, np.int64)
+    lib().orc_last_counts(_p(out))
+    return int(out[0]), int(out[1])
 
 
 TONEMAP = {"None": 0, "Clamp": 1, "Reinhard": 2, "Uncharted2": 3, "ACES": 4}

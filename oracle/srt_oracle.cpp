@@ -544,9 +544,11 @@ struct Tracer {
     }
 };
 
+// ntrace / nshadow count the closest-hit traces and the shadow rays issued
+// (the wavefront integrator's queue entries for the same paths).
 template <class R>
 static V3<R> render_pixel(const Scene& s, const Tracer& tr, const Camera& cam, const PtParams& p, int W, int H, int pixel,
-                          int sample_begin, int sample_end) {
+                          int sample_begin, int sample_end, int64_t* ntrace, int64_t* nshadow) {
     const int ix = pixel % W, iy = pixel / W;
     V3<R> result(R(0));
     V3<R> X = cvt<R>(cam.X), Y = cvt<R>(cam.Y), Z = cvt<R>(cam.Z), CP = cvt<R>(cam.pos);
@@ -574,6 +576,7 @@ static V3<R> render_pixel(const Scene& s, const Tracer& tr, const Camera& cam, c
         for (int depth = 0; depth < p.max_depth; depth++) {
             float of[3] = {(float)ro.x, (float)ro.y, (float)ro.z}, df[3] = {(float)rd.x, (float)rd.y, (float)rd.z};
             Hit h;
+            (*ntrace)++;
             bool hit = tr.closest(of, df, 1e-3f, 1e16f, &h);
             if (!hit) { L = L + beta * env; break; }
             HitInfo<R> info;
@@ -611,6 +614,7 @@ static V3<R> render_pixel(const Scene& s, const Tracer& tr, const Camera& cam, c
                     float so[3] = {(float)info.pos.x, (float)info.pos.y, (float)info.pos.z};
                     float sd[3] = {(float)sdir.x, (float)sdir.y, (float)sdir.z};
                     float tmaxs = (float)(t - (R)1e-3f);
+                    (*nshadow)++;
                     bool visible = !tr.any(so, sd, 1e-3f, tmaxs);
                     if (visible) {
                         R mis = R(1);
@@ -655,6 +659,7 @@ static V3<R> render_pixel(const Scene& s, const Tracer& tr, const Camera& cam, c
 using namespace orc;
 
 struct OrcScene { Scene s; };
+static int64_t g_last_counts[2] = {0, 0};
 struct OrcBvh { Bvh b; };
 
 extern "C" {
@@ -854,23 +859,27 @@ double orc_render(void* scene, void* bvh, const float* cam_pos, const float* cam
     if (sample_end <= 0) sample_end = spp;
     if (row_step <= 0) row_step = 1;
     std::atomic<int> next_row(row_begin);
+    std::atomic<int64_t> traces(0), shadows(0);
     auto worker = [&]() {
+        int64_t nt = 0, ns = 0;
         for (;;) {
             int row = next_row.fetch_add(row_step);
             if (row >= H) break;
             for (int x = 0; x < W; x++) {
                 int pixel = row * W + x;
                 if (fp64) {
-                    V3<double> r = render_pixel<double>(s, tr, cam, p, W, H, pixel, sample_begin, sample_end);
+                    V3<double> r = render_pixel<double>(s, tr, cam, p, W, H, pixel, sample_begin, sample_end, &nt, &ns);
                     out_d[4 * (size_t)pixel] = r.x; out_d[4 * (size_t)pixel + 1] = r.y; out_d[4 * (size_t)pixel + 2] = r.z;
                     out_d[4 * (size_t)pixel + 3] = 1.0;
                 } else {
-                    V3<float> r = render_pixel<float>(s, tr, cam, p, W, H, pixel, sample_begin, sample_end);
+                    V3<float> r = render_pixel<float>(s, tr, cam, p, W, H, pixel, sample_begin, sample_end, &nt, &ns);
                     out_f[4 * (size_t)pixel] = r.x; out_f[4 * (size_t)pixel + 1] = r.y; out_f[4 * (size_t)pixel + 2] = r.z;
                     out_f[4 * (size_t)pixel + 3] = 1.0f;
                 }
             }
         }
+        traces += nt;
+        shadows += ns;
     };
     if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
     auto t0 = std::chrono::steady_clock::now();
@@ -879,7 +888,15 @@ double orc_render(void* scene, void* bvh, const float* cam_pos, const float* cam
     worker();
     for (auto& th : pool) th.join();
     auto t1 = std::chrono::steady_clock::now();
+    g_last_counts[0] = traces.load();
+    g_last_counts[1] = shadows.load();
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// closest-hit traces and shadow rays of the last orc_render
+void orc_last_counts(int64_t* out2) {
+    out2[0] = g_last_counts[0];
+    out2[1] = g_last_counts[1];
 }
 
 // tonemapping.cu:14-68; exposure scale computed on the host as powf(2, exposure)

@@ -115,3 +115,34 @@ def test_converges_to_fp64_golden(ctx):
     rmse = np.sqrt(np.mean((img[:, :3].astype(np.float64) - g) ** 2))
     assert rmse < 0.03, rmse
     assert abs(img[:, :3].mean() / g.mean() - 1) < 0.01
+
+
+def test_nan_samples_dropped_like_the_oracle(ctx):
+    """R12: samples whose radiance is NaN/Inf are dropped but keep the 1/spp
+    weight; the GPU frame equals the oracle's on a scene with a NaN material."""
+    from test_oracle_closed_forms import nan_cornell
+    s = nan_cornell()
+    img, ref = _render_pair(ctx, s, 48, 48, 4, 5, seed=3)
+    assert np.isfinite(img).all()
+    assert np.array_equal(img, ref)
+    assert (img[:, :3] == 0).all(axis=1).sum() > 10
+
+
+@pytest.mark.parametrize("name,W,H,spp,depth", [("cornell", 64, 64, 8, 5), ("spheres", 192, 108, 2, 8),
+                                                ("tube", 128, 72, 2, 6)])
+def test_queue_counts_equal_megakernel_traces(ctx, name, W, H, spp, depth):
+    """The wavefront queues hold exactly the megakernel's rays: segments traced
+    = the oracle's closest-hit traces, shadow rays = its shadow tests, for the
+    same seeded paths (no entry lost or duplicated by compaction)."""
+    s = getattr(scenes, name)()
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    ctx.render(h, s.camera, W, H, spp, depth, 4, 1, 1, 2)
+    seg, sh = ctx.last_counts()
+    ctx.release(h)
+    osc = O.Scene(s)
+    O.render(osc, s.camera, W, H, spp, depth, 4, 1, 1, 2, bvh=O.Bvh(osc))
+    traces, shadows = O.last_counts()
+    assert seg == traces
+    assert sh == shadows
+    assert seg >= W * H * spp
