@@ -96,7 +96,7 @@ typedef struct srt_pt_params {
 
 typedef struct srt_bvh_opts {
     int32_t max_leaf;      /* 1..8, default 4 */
-    int32_t cluster_size;  /* 0: automatic (max(64, n/1024)) */
+    int32_t cluster_size;  /* 0: automatic (max(64, n/16384)) */
 } srt_bvh_opts;
 
 /* ---- context (replaces OptixRayTracer's context/stream, optix_ray_tracer.cu:37-56) ---- */

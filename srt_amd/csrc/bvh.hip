@@ -783,7 +783,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     if (n < 1) return set_err(c, SRT_E_INVALID, "scene has no triangles");
     if ((int64_t)n >= (1ll << 28)) return set_err(c, SRT_E_LIMIT, "too many triangles for the leaf encoding");
     hipStream_t st = c->stream;
-    const int T = cluster_size > 0 ? cluster_size : std::max(64, n / 1024);
+    const int T = cluster_size > 0 ? cluster_size : std::max(64, n / 16384);
     const float c_trav = 1.0f, c_isect = 1.0f;
     Tmp tmp(c);
     // boxes & centroids

@@ -356,22 +356,36 @@ SRT_D bool mat_is_transmissive(const MatRec& m) {
 
 struct BxSample { f3 f; f3 wi; float pdf; bool delta; };
 
-// bxdf.h:79-180
-SRT_D f3 bxdf_eval(const MatRec& m, f3 wi, f3 wo, f3 color, bool inner) {
-    switch (m.type) {
-    case T_DIFFUSE: return (wi.z <= 0.0f) ? mk(0.0f) : color * kInvPi;
-    case T_DIFFTRANS: return color * kInvPi;
-    case T_DIELECTRIC: return mk(0.0f);
-    case T_DISNEY: {
-        float eta = inner ? m.ior : 1.0f / m.ior;
+// Disney quantities that depend only on the material and its colour
+// (bxdf.h:93-99, :233-240); computed once per shading point.
+struct DisneyK {
+    float lum;
+    f3 ctint, spec_color, sheen_color;
+    float ax, ay;
+};
+SRT_D DisneyK disney_consts(const MatRec& m, f3 color) {
+    DisneyK k;
+    k.lum = luminance(color);
+    k.ctint = (k.lum > 0.0f) ? color / k.lum : mk(1.0f);
+    k.spec_color = lerp3((m.specular * 0.08f) * lerp3(mk(1.0f), k.ctint, m.specularTint), color, m.metallic);
+    k.sheen_color = lerp3(mk(1.0f), k.ctint, m.sheenTint);
+    float aspect = __builtin_sqrtf(1.0f - m.anisotropic * 0.9f);
+    float a2 = m.roughness * m.roughness;
+    k.ax = fmaxf(0.001f, a2 / aspect);
+    k.ay = fmaxf(0.001f, a2 * aspect);
+    return k;
+}
+
+// bxdf.h:93-174 (Disney evaluation)
+SRT_D f3 disney_eval(const MatRec& m, const DisneyK& K, f3 wi, f3 wo, f3 color, float eta) {
+    {
         f3 V = wo, L = wi;
         f3 H = (L.z > 0.0f) ? normalize(L + V) : normalize(L + V * eta);
         if (H.z <= 0.0f) H = -H;
         float V_H = dot(V, H), L_H = dot(L, H);
-        float lum = luminance(color);
-        f3 ctint = (lum > 0.0f) ? color / lum : mk(1.0f);
-        f3 spec_color = lerp3((m.specular * 0.08f) * lerp3(mk(1.0f), ctint, m.specularTint), color, m.metallic);
-        f3 sheen_color = lerp3(mk(1.0f), ctint, m.sheenTint);
+        const float lum = K.lum;
+        const f3 spec_color = K.spec_color;
+        const f3 sheen_color = K.sheen_color;
         float FM = fresnel_mix(m.metallic, eta, V_H);
         float diffuse_w = lum * (1.0f - m.metallic) * (1.0f - m.specTrans);
         float spec_reflect_w = luminance(lerp3(spec_color, mk(1.0f), FM));
@@ -394,10 +408,7 @@ SRT_D f3 bxdf_eval(const MatRec& m, f3 wi, f3 wo, f3 color, bool inner) {
         if (spec_reflect_w > 0.0f && L.z > 0.0f && V.z > 0.0f) {
             float _FM = fresnel_mix(m.metallic, eta, L_H);
             f3 F = lerp3(spec_color, mk(1.0f), _FM);
-            float aspect = __builtin_sqrtf(1.0f - m.anisotropic * 0.9f);
-            float a2 = m.roughness * m.roughness;
-            float ax = fmaxf(0.001f, a2 / aspect), ay = fmaxf(0.001f, a2 * aspect);
-            float D = GTR2_aniso(H.z, H.x, H.y, ax, ay);
+            float D = GTR2_aniso(H.z, H.x, H.y, K.ax, K.ay);
             float G = smithG_GGX(fabsf(L.z), m.roughness) * smithG_GGX(fabsf(V.z), m.roughness);
             f_spec_reflect = F * D * G / (4.0f * L.z * V.z);
         }
@@ -420,21 +431,27 @@ SRT_D f3 bxdf_eval(const MatRec& m, f3 wi, f3 wo, f3 color, bool inner) {
         }
         return f_diffuse + f_spec_reflect + f_spec_refract + f_clearcoat;
     }
+}
+
+// bxdf.h:79-180
+SRT_D f3 bxdf_eval(const MatRec& m, f3 wi, f3 wo, f3 color, bool inner) {
+    switch (m.type) {
+    case T_DIFFUSE: return (wi.z <= 0.0f) ? mk(0.0f) : color * kInvPi;
+    case T_DIFFTRANS: return color * kInvPi;
+    case T_DIELECTRIC: return mk(0.0f);
+    case T_DISNEY: return disney_eval(m, disney_consts(m, color), wi, wo, color, inner ? m.ior : 1.0f / m.ior);
     default: return mk(0.0f);
     }
 }
 
 struct Lobes { float dw, srw, stw, cw, ax, ay; };
 // bxdf.h:233-252
-SRT_D Lobes disney_lobes(const MatRec& m, f3 V, f3 color, float eta) {
+SRT_D Lobes disney_lobes(const MatRec& m, const DisneyK& K, f3 V, float eta) {
     Lobes w;
-    float lum = luminance(color);
-    f3 ctint = (lum > 0.0f) ? color / lum : mk(1.0f);
-    f3 spec_color = lerp3((m.specular * 0.08f) * lerp3(mk(1.0f), ctint, m.specularTint), color, m.metallic);
-    float aspect = __builtin_sqrtf(1.0f - m.anisotropic * 0.9f);
-    float a2 = m.roughness * m.roughness;
-    w.ax = fmaxf(0.001f, a2 / aspect);
-    w.ay = fmaxf(0.001f, a2 * aspect);
+    const float lum = K.lum;
+    const f3 spec_color = K.spec_color;
+    w.ax = K.ax;
+    w.ay = K.ay;
     float FM = fresnel_mix(m.metallic, eta, V.z);
     w.dw = lum * (1.0f - m.metallic) * (1.0f - m.specTrans);
     w.srw = luminance(lerp3(spec_color, mk(1.0f), FM));
@@ -464,6 +481,8 @@ SRT_D float disney_pdf(const MatRec& m, f3 L, f3 V, float eta, const Lobes& w) {
         pdf += w.cw * GTR1(H_reflect.z, m.clearcoatGloss) * H_reflect.z / (4.0f * dot(V, H_reflect));
     return pdf;
 }
+
+SRT_D BxSample disney_sample(const MatRec& m, const DisneyK& K, const Lobes& w, f3 wo, f2 rnd, f3 color, float eta);
 
 // bxdf.h:182-319
 SRT_D BxSample bxdf_sample(const MatRec& m, f3 wo, f2 rnd, f3 color, bool inner) {
@@ -498,8 +517,18 @@ SRT_D BxSample bxdf_sample(const MatRec& m, f3 wo, f2 rnd, f3 color, bool inner)
     }
     case T_DISNEY: {
         float eta = inner ? m.ior : 1.0f / m.ior;
+        const DisneyK K = disney_consts(m, color);
+        return disney_sample(m, K, disney_lobes(m, K, wo, eta), wo, rnd, color, eta);
+    }
+    default: return none;
+    }
+}
+
+// bxdf.h:228-313 (Disney sampling) with the shading point's constants and lobe weights
+SRT_D BxSample disney_sample(const MatRec& m, const DisneyK& K, const Lobes& w, f3 wo, f2 rnd, f3 color, float eta) {
+    BxSample none; none.f = mk(0.0f); none.wi = mk(0.0f); none.pdf = 0.0f; none.delta = false;
+    {
         f3 V = wo;
-        Lobes w = disney_lobes(m, V, color, eta);
         f3 L = mk(0.0f);
         if (rnd.x < w.dw) {
             rnd.x /= w.dw;
@@ -523,12 +552,10 @@ SRT_D BxSample bxdf_sample(const MatRec& m, f3 wo, f2 rnd, f3 color, bool inner)
         }
         BxSample r;
         r.pdf = disney_pdf(m, L, V, eta, w);
-        r.f = bxdf_eval(m, L, wo, color, inner);
+        r.f = disney_eval(m, K, L, wo, color, eta);
         r.wi = L;
         r.delta = false;
         return r;
-    }
-    default: return none;
     }
 }
 
@@ -540,8 +567,7 @@ SRT_D float bxdf_pdf(const MatRec& m, f3 wi, f3 wo, f3 color, bool inner) {
     case T_DIELECTRIC: return 0.0f;
     case T_DISNEY: {
         float eta = inner ? m.ior : 1.0f / m.ior;
-        Lobes w = disney_lobes(m, wo, color, eta);
-        return disney_pdf(m, wi, wo, eta, w);
+        return disney_pdf(m, wi, wo, eta, disney_lobes(m, disney_consts(m, color), wo, eta));
     }
     default: return 0.0f;
     }
@@ -568,6 +594,52 @@ SRT_D float mat_pdf(const MatRec& m, f3 wi, f3 wo, const Onb& onb, f3 color) {
     bool inner = false;
     if (wol.z <= 0.0f) { wol = -wol; wil = -wil; inner = true; }
     return bxdf_pdf(m, wil, wol, color, inner);
+}
+
+// One shading point: the outgoing direction in the local frame (flipped to
+// the upper hemisphere, gmaterial.h:59-104) and, for Disney materials, the
+// constants and lobe weights that every evaluation at this point shares.
+// sp_eval / sp_pdf / sp_sample give the same values as mat_eval / mat_pdf /
+// mat_sample with the same arguments.
+struct ShadePoint {
+    Onb onb;
+    f3 wol;
+    bool inner;
+    float eta;
+    DisneyK K;
+    Lobes w;
+};
+SRT_D ShadePoint shade_point(const MatRec& m, f3 wo, const Onb& onb, f3 color) {
+    ShadePoint p;
+    p.onb = onb;
+    p.wol = to_local(onb, wo);
+    p.inner = false;
+    if (p.wol.z <= 0.0f) { p.wol = -p.wol; p.inner = true; }
+    p.eta = p.inner ? m.ior : 1.0f / m.ior;
+    if (m.type == T_DISNEY) {
+        p.K = disney_consts(m, color);
+        p.w = disney_lobes(m, p.K, p.wol, p.eta);
+    }
+    return p;
+}
+SRT_D f3 sp_eval(const MatRec& m, const ShadePoint& p, f3 wi, f3 color) {
+    f3 wil = to_local(p.onb, wi);
+    if (p.inner) wil = -wil;
+    if (m.type == T_DISNEY) return disney_eval(m, p.K, wil, p.wol, color, p.eta);
+    return bxdf_eval(m, wil, p.wol, color, p.inner);
+}
+SRT_D float sp_pdf(const MatRec& m, const ShadePoint& p, f3 wi, f3 color) {
+    f3 wil = to_local(p.onb, wi);
+    if (p.inner) wil = -wil;
+    if (m.type == T_DISNEY) return disney_pdf(m, wil, p.wol, p.eta, p.w);
+    return bxdf_pdf(m, wil, p.wol, color, p.inner);
+}
+SRT_D BxSample sp_sample(const MatRec& m, const ShadePoint& p, f2 rnd, f3 color) {
+    BxSample s = (m.type == T_DISNEY) ? disney_sample(m, p.K, p.w, p.wol, rnd, color, p.eta)
+                                      : bxdf_sample(m, p.wol, rnd, color, p.inner);
+    if (p.inner) s.wi = -s.wi;
+    s.wi = to_world(p.onb, s.wi);
+    return s;
 }
 
 }  // namespace srt

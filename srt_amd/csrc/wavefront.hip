@@ -624,7 +624,7 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     } else {
         tg = (fabsf(sn.x) > fabsf(sn.y)) ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
     }
-    const Onb onb = make_onb(sn, tg);
+    const ShadePoint spt = shade_point(m, -rd, make_onb(sn, tg), color);
     if (dot(normal, -rd) < 0.0f) normal = -normal;
     const bool last = (depth + 1 >= P.max_depth);
     const int64_t pid = P.path_begin + (int64_t)(slot - (uint32_t)P.slot_offset);
@@ -647,10 +647,10 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
             float mis = 1.0f;
             const float lpdf = ls.pdf * t * t / cos_light;
             if (P.use_mis) {
-                const float mpdf = mat_pdf(m, sdir, -rd, onb, color);
+                const float mpdf = sp_pdf(m, spt, sdir, color);
                 mis = lpdf / (lpdf + mpdf);
             }
-            const f3 f = mat_eval(m, sdir, -rd, onb, color);
+            const f3 f = sp_eval(m, spt, sdir, color);
             s_c = beta * f * fabsf(cos_o) * ls.emission * mis / lpdf;
         }
         const int js = wave_append(emit, counts + CNT_SHADOW);
@@ -696,7 +696,7 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     f2 br;
     br.x = rng.next_float();
     br.y = rng.next_float();
-    const BxSample ms = mat_sample(m, -rd, br, onb, color);
+    const BxSample ms = sp_sample(m, spt, br, color);
     bool alive = !(ms.pdf <= 1e-4f);
     if (alive && dot(ms.wi, normal) < 0.0f && !mat_is_transmissive(m)) alive = false;
     if (alive) {
@@ -732,7 +732,7 @@ SRT_D void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
-__global__ void __launch_bounds__(kShadeBlock, 5) k_shade(DevScene S, const Segment* segs, int depth, PathQueue qin,
+__global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segment* segs, int depth, PathQueue qin,
                                                          HitBuf h, PathQueue qout, ShadowQueue sq, float* Lr,
                                                          float* Lg, float* Lb, uint32_t* counts, int qi, MlpIO mio) {
     __shared__ uint32_t gather[kShadeBlock / 64][128];
