@@ -110,7 +110,9 @@ const char* srt_last_error(const srt_ctx* ctx);
  * lanes that make a traversal wave fetch new rays, 1..64, defaults 48 / 32),
  * "trace_blocks_per_cu"
  * (0: as many as fit), "trace_bvh_width" (4: traverse the 4-wide tree collapsed
- * from the binary one, default; 2: the binary tree; results are identical) */
+ * from the binary one, default; 2: the binary tree; results are identical),
+ * "trace_inner_exit" (a traversal wave leaves its inner-node loop once at most
+ * this many lanes still search for a leaf, 0..63, default 8) */
 int srt_set_option(srt_ctx* ctx, const char* key, int64_t value);
 size_t srt_hbm_used(const srt_ctx* ctx);
 

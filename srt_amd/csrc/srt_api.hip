@@ -202,6 +202,11 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
         c->shadow_refill = (int32_t)value;
         return SRT_OK;
     }
+    if (!strcmp(key, "trace_inner_exit")) {
+        if (value < 0 || value > 63) return set_err(c, SRT_E_INVALID, "trace_inner_exit must be in [0, 63]");
+        c->trace_inner_exit = (int32_t)value;
+        return SRT_OK;
+    }
     if (!strcmp(key, "trace_bvh_width")) {
         if (value != 2 && value != 4) return set_err(c, SRT_E_INVALID, "trace_bvh_width must be 2 or 4");
         c->trace_bvh_width = (int32_t)value;

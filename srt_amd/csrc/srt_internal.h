@@ -57,6 +57,7 @@ struct TraceCfg {
     int32_t lds_depth;   // stack entries per lane kept in LDS, 4..64
     int32_t refill;      // finished lanes that make a wave fetch new rays, 1..64
     int32_t wide;        // 1: traverse the 4-wide tree, 0: the binary tree
+    int32_t inner_exit;  // leave the inner-node loop once at most this many lanes still search a leaf
 };
 
 // ---------------------------------------------------------------------------
@@ -219,6 +220,7 @@ struct srt_ctx {
     size_t gstack_entries = 0;            // int32 entries allocated in gstack
     int32_t trace_lds_depth = 16, trace_refill = 48, shadow_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
     int32_t trace_bvh_width = 4;          // traversal over the 4-wide (4) or binary (2) tree
+    int32_t trace_inner_exit = 8;         // stragglers resume after the leaf step
     // profiling
     bool profile = false;
     std::map<std::string, srt::KernelStat> stats;

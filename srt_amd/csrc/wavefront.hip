@@ -149,6 +149,7 @@ struct TravStack {
     int lds_depth;
     int bstride;
     int lane0;           // this thread's first LDS entry (threadIdx.x)
+    int inner_exit;
 };
 SRT_D void stk_push(const TravStack& k, int32_t sp, int32_t v) {
     if (sp < k.lds_depth) trav_lds[k.lane0 + sp * k.bstride] = v;
@@ -256,9 +257,12 @@ SRT_D int4 ld4i(__amdgpu_buffer_rsrc_t r, uint32_t off) {
 
 // Inner nodes of the 4-wide tree, with the same leaf postponement as the
 // binary step: the hit children are ordered by entry distance, the nearest is
-// visited next and the others are pushed farthest first.
+// visited next and the others are pushed farthest first.  The wave leaves the
+// loop once at most `inner_exit` of its lanes still search for a first leaf:
+// those lanes keep their state and continue after the leaf step, instead of
+// every other lane waiting for them.
 template <bool COUNT>
-SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nv) {
+SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nv, int inner_exit) {
     const __amdgpu_buffer_rsrc_t rsrc = node4_rsrc(S);
     while (st.cur_ok) {
         if (st.cur < 0) {
@@ -297,7 +301,7 @@ SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uin
                 st.cur = r0;
             }
         }
-        if (__all(st.has_pend)) break;
+        if (__popcll(__ballot(!st.has_pend)) <= (unsigned)inner_exit) break;
     }
 }
 
@@ -377,7 +381,7 @@ SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, 
         }
         if (act == 0ull) break;
         if (has) {
-            if (WIDE) trav_inner4<COUNT>(S, st, k, nv);
+            if (WIDE) trav_inner4<COUNT>(S, st, k, nv, k.inner_exit);
             else trav_inner<COUNT>(S, st, k, nv);
             if (st.has_pend) occl = trav_leaf<ANY, COUNT>(S, st, nt);
             if (occl || (!st.cur_ok && !st.has_pend)) {
@@ -495,6 +499,7 @@ SRT_D TravStack make_stack(int32_t* gstack, TraceCfg cfg) {
     k.gst = gstack + gtid;
     k.gstride = (size_t)gridDim.x * blockDim.x;
     k.lds_depth = cfg.lds_depth;
+    k.inner_exit = cfg.inner_exit;
     k.bstride = (int)blockDim.x;
     return k;
 }
@@ -972,6 +977,7 @@ static TraceCfg trace_cfg(srt_ctx* c) {
     cfg.lds_depth = c->trace_lds_depth;
     cfg.refill = c->trace_refill;
     cfg.wide = c->trace_bvh_width == 4 ? 1 : 0;
+    cfg.inner_exit = c->trace_inner_exit;
     return cfg;
 }
 
