@@ -107,7 +107,7 @@ const char* srt_last_error(const srt_ctx* ctx);
  * "count_visits" (0/1 count BVH node visits and triangle tests), "hbm_budget"
  * (bytes; at least srt_hbm_used), "trace_stack_lds" (traversal stack entries per
  * lane in LDS, 4..64, default 16), "trace_refill" / "shadow_refill" (finished
- * lanes that make a traversal wave fetch new rays, 1..64, defaults 48 / 32),
+ * lanes that make a traversal wave fetch new rays, 1..64, defaults 56 / 32),
  * "trace_blocks_per_cu"
  * (0: as many as fit), "trace_bvh_width" (4: traverse the 4-wide tree collapsed
  * from the binary one, default; 2: the binary tree; results are identical),

@@ -285,7 +285,7 @@ void bvh_build(const Scene& s, const BvhParams& prm, Bvh& out) {
             for (int a = 0; a < 3; a++) out.tris[12 * (size_t)k + 4 * v + a] = s.pos[3 * (size_t)s.idx[3 * (size_t)p + v] + a];
         out.tris[12 * (size_t)k + 3] = u2f(p);
         out.tris[12 * (size_t)k + 7] = u2f((uint32_t)s.tri_mesh[p]);
-        out.tris[12 * (size_t)k + 11] = 0.0f;
+        out.tris[12 * (size_t)k + 11] = u2f((uint32_t)s.mesh_mat[s.tri_mesh[p]]);   // material of the triangle
     }
 
     // 9. depth (internal nodes on the longest root-to-leaf path)

@@ -536,18 +536,49 @@ __global__ void __launch_bounds__(256) k_emit(int n, int T, int top, int max_lea
     valid[top + i] = 1;
 }
 
-__global__ void __launch_bounds__(256) k_tris(int n, const uint32_t* order, const float* pos, const uint32_t* idx,
-                                              const int32_t* tri_mesh, float4* tris) {
+// Sorted triangle records (and, for shading, copies of the vertex normals and
+// tangents in the same order), rewritten after every skinning step.
+struct TriSrc {
+    const float *pos, *nrm, *tan;
+    const uint32_t* idx;
+    const int32_t *tri_mesh, *mesh_mat;
+};
+__global__ void __launch_bounds__(256) k_tris(int n, const uint32_t* order, TriSrc src, float4* tris, float4* tri_sn,
+                                              float4* tri_st) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t p = order[k];
-    const uint32_t* ix = idx + 3 * (size_t)p;
-    const float* a = pos + 3 * (size_t)ix[0];
-    const float* b = pos + 3 * (size_t)ix[1];
-    const float* c = pos + 3 * (size_t)ix[2];
+    const uint32_t* ix = src.idx + 3 * (size_t)p;
+    const float* a = src.pos + 3 * (size_t)ix[0];
+    const float* b = src.pos + 3 * (size_t)ix[1];
+    const float* c = src.pos + 3 * (size_t)ix[2];
+    const int32_t mesh = src.tri_mesh[p];
     tris[3 * (size_t)k + 0] = make_float4(a[0], a[1], a[2], __uint_as_float(p));
-    tris[3 * (size_t)k + 1] = make_float4(b[0], b[1], b[2], __int_as_float(tri_mesh[p]));
-    tris[3 * (size_t)k + 2] = make_float4(c[0], c[1], c[2], 0.0f);
+    tris[3 * (size_t)k + 1] = make_float4(b[0], b[1], b[2], __int_as_float(mesh));
+    tris[3 * (size_t)k + 2] = make_float4(c[0], c[1], c[2], __int_as_float(src.mesh_mat[mesh]));
+    for (int v = 0; v < 3; v++) {
+        if (tri_sn) {
+            const float* q = src.nrm + 3 * (size_t)ix[v];
+            tri_sn[3 * (size_t)k + v] = make_float4(q[0], q[1], q[2], 0.0f);
+        }
+        if (tri_st) {
+            const float* q = src.tan + 3 * (size_t)ix[v];
+            tri_st[3 * (size_t)k + v] = make_float4(q[0], q[1], q[2], 0.0f);
+        }
+    }
+}
+static void launch_tris(SceneRec* s, hipStream_t st) {
+    TriSrc src;
+    src.pos = (const float*)s->pos.p;
+    src.nrm = (const float*)s->nrm.p;
+    src.tan = (const float*)s->tan.p;
+    src.idx = (const uint32_t*)s->idx.p;
+    src.tri_mesh = (const int32_t*)s->tri_mesh.p;
+    src.mesh_mat = (const int32_t*)s->mesh_mat.p;
+    const int n = s->nf;
+    k_tris<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, (const uint32_t*)s->order.p, src, (float4*)s->tris.p,
+                                                        s->has_nrm ? (float4*)s->tri_sn.p : nullptr,
+                                                        s->has_tan ? (float4*)s->tri_st.p : nullptr);
 }
 
 // parents in the final tree: parent index * 2 + side, -1 for the root
@@ -922,7 +953,11 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     }
     if (n > 1)
         k_emit<<<nblk(n - 1), 256, 0, st>>>(n, T, top, max_leaf, kl, kr, kfirst, par_int, order, tb, K, nodes, valid);
-    k_tris<<<nblk(n), 256, 0, st>>>(n, order, (const float*)s->pos.p, (const uint32_t*)s->idx.p, tri_mesh, (float4*)s->tris.p);
+    dev_free(c, s->tri_sn);
+    dev_free(c, s->tri_st);
+    if (s->has_nrm && dev_alloc(c, s->tri_sn, (size_t)n * 3 * sizeof(float4))) return SRT_E_OOM;
+    if (s->has_tan && dev_alloc(c, s->tri_st, (size_t)n * 3 * sizeof(float4))) return SRT_E_OOM;
+    launch_tris(s, st);
     int32_t* parent = (int32_t*)s->node_parent.p;
     k_fill_i32<<<nblk(num_nodes), 256, 0, st>>>(parent, num_nodes, -1);
     k_final_parents<<<nblk(num_nodes), 256, 0, st>>>(num_nodes, nodes, valid, parent);
@@ -997,8 +1032,7 @@ int bvh_refit(srt_ctx* c, SceneRec* s) {
     if (!s->has_bvh) return set_err(c, SRT_E_STATE, "refit before build");
     hipStream_t st = c->stream;
     const int n = s->nf;
-    k_tris<<<nblk(n), 256, 0, st>>>(n, (const uint32_t*)s->order.p, (const float*)s->pos.p, (const uint32_t*)s->idx.p,
-                                    (const int32_t*)s->tri_mesh.p, (float4*)s->tris.p);
+    launch_tris(s, st);
     if (s->root >= 0) {
         k_fill_i32<<<nblk(s->num_nodes), 256, 0, st>>>((int32_t*)s->refit_counter.p, s->num_nodes, 0);
         k_refit<<<nblk(s->num_nodes), 256, 0, st>>>(s->num_nodes, (BvhNode*)s->nodes.p, (const int32_t*)s->node_valid.p,

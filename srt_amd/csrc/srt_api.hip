@@ -87,6 +87,8 @@ void prof_collect(srt_ctx* c) {
 DevScene dev_scene(const SceneRec* s) {
     DevScene S;
     S.tris = (const float4*)s->tris.p;
+    S.tri_sn = s->has_nrm ? (const float4*)s->tri_sn.p : nullptr;
+    S.tri_st = s->has_tan ? (const float4*)s->tri_st.p : nullptr;
     S.nodes = (const BvhNode*)s->nodes.p;
     S.root = s->root;
     S.nodes4 = (const Bvh4Node*)s->nodes4.p;
@@ -113,7 +115,7 @@ static void free_scene(srt_ctx* c, SceneRec* s) {
     DevBuf* all[] = {&s->pos, &s->nrm, &s->tan, &s->idx, &s->mesh_mat, &s->mesh_first, &s->area_cdf, &s->mats,
                      &s->light_cdf, &s->light_mesh, &s->tri_mesh, &s->orig_pos, &s->orig_nrm, &s->orig_tan,
                      &s->bone_ids, &s->bone_w, &s->bones, &s->nodes, &s->tris, &s->order, &s->node_parent,
-                     &s->refit_counter, &s->node_valid, &s->nodes4};
+                     &s->refit_counter, &s->node_valid, &s->nodes4, &s->tri_sn, &s->tri_st};
     for (DevBuf* b : all) dev_free(c, *b);
     delete s;
 }

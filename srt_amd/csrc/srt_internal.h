@@ -64,7 +64,9 @@ struct TraceCfg {
 // device view of one resident scene
 // ---------------------------------------------------------------------------
 struct DevScene {
-    const float4* tris;        // 3 float4 per sorted triangle: (v0, prim) (v1, mesh) (v2, 0)
+    const float4* tris;        // 3 float4 per sorted triangle: (v0, prim) (v1, mesh) (v2, material)
+    const float4* tri_sn;      // 3 float4 per sorted triangle: vertex normals (xyz, 0), or null
+    const float4* tri_st;      // 3 float4 per sorted triangle: vertex tangents (xyz, 0), or null
     const BvhNode* nodes;
     int32_t root;
     const Bvh4Node* nodes4;    // the same tree, 4 wide
@@ -187,6 +189,7 @@ struct SceneRec {
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};   // scene box (MLP input normalisation)
     bool has_bvh = false;
     DevBuf nodes, tris, order, node_parent, refit_counter, node_valid;
+    DevBuf tri_sn, tri_st;       // per sorted triangle: copies of its vertex normals / tangents
     int32_t root = 0, num_nodes = 0, max_depth = 0, num_clusters = 0, num_top = 0, cluster_size = 0;
     DevBuf nodes4;
     int32_t root4 = 0, num_nodes4 = 0, depth4 = 0;     // 4-wide tree: nodes, levels
@@ -218,7 +221,7 @@ struct srt_ctx {
     srt::DevBuf q[2][14], hits[4], shq[11], Lbuf[3], counters, film_acc, film_out, gstack;
     int64_t film_pixels = 0;
     size_t gstack_entries = 0;            // int32 entries allocated in gstack
-    int32_t trace_lds_depth = 16, trace_refill = 48, shadow_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
+    int32_t trace_lds_depth = 16, trace_refill = 56, shadow_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
     int32_t trace_bvh_width = 4;          // traversal over the 4-wide (4) or binary (2) tree
     int32_t trace_inner_exit = 8;         // stragglers resume after the leaf step
     // profiling

@@ -585,21 +585,21 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     // hit reconstruction (hit_info.h:27-65)
     const float4* tp = S.tris + 3 * (size_t)tri;
     const float4 A = tp[0], B = tp[1], C = tp[2];
-    const int32_t prim = __float_as_int(A.w);
-    const int32_t mesh = __float_as_int(B.w);
+    const int32_t matid = __float_as_int(C.w);
     const f3 v0 = mk(A.x, A.y, A.z), v1 = mk(B.x, B.y, B.z), v2 = mk(C.x, C.y, C.z);
     const float u = h.u[i], v = h.v[i];
     const float w = 1.0f - u - v;
     const f3 pos = v0 * w + v1 * u + v2 * v;
     f3 normal = normalize(cross(v1 - v0, v2 - v0));
     f3 sn = normal;
-    const uint32_t* ix = S.idx + 3 * (size_t)prim;
-    if (S.nrm) {
-        f3 n0 = ld3(S.nrm + 3 * (size_t)ix[0]), n1 = ld3(S.nrm + 3 * (size_t)ix[1]), n2 = ld3(S.nrm + 3 * (size_t)ix[2]);
-        sn = normalize(n0 * w + n1 * u + n2 * v);
+    if (S.tri_sn) {
+        // the vertex normals of this triangle, copied in sorted order
+        const float4* q = S.tri_sn + 3 * (size_t)tri;
+        const float4 n0 = q[0], n1 = q[1], n2 = q[2];
+        sn = normalize(mk(n0.x, n0.y, n0.z) * w + mk(n1.x, n1.y, n1.z) * u + mk(n2.x, n2.y, n2.z) * v);
     }
     if (dot(normal, sn) < 0.0f) normal = -normal;
-    const MatRec& m = S.mats[S.mesh_mat[mesh]];
+    const MatRec& m = S.mats[matid];
     const f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
     if (mat_is_emissive(m)) {
         // emitter hit with MIS (path_tracer.cu:88-106)
@@ -623,9 +623,10 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
         return;
     }
     f3 tg;
-    if (S.tan) {
-        f3 t0 = ld3(S.tan + 3 * (size_t)ix[0]), t1 = ld3(S.tan + 3 * (size_t)ix[1]), t2 = ld3(S.tan + 3 * (size_t)ix[2]);
-        tg = t0 * w + t1 * u + t2 * v;
+    if (S.tri_st) {
+        const float4* q = S.tri_st + 3 * (size_t)tri;
+        const float4 t0 = q[0], t1 = q[1], t2 = q[2];
+        tg = mk(t0.x, t0.y, t0.z) * w + mk(t1.x, t1.y, t1.z) * u + mk(t2.x, t2.y, t2.z) * v;
     } else {
         tg = (fabsf(sn.x) > fabsf(sn.y)) ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
     }
