@@ -256,15 +256,24 @@ def skin_mode(args):
 
 def mlp_mode(args):
     """Radiance MLP against the plain tracer at equal wall time on the Cornell
-    box (256x256, depth 5): RMSE against the FP64 oracle's 4096-spp image.
-    The MLP is trained online from the tracer's own vertex-2 samples (2^18
-    per frame), then paths stop at vertex 2 and take its prediction."""
+    box (256x256, depth 5; default) or the 1M-triangle sphere scene (192x108,
+    depth 8; --mlp-scene spheres): RMSE against the FP64 oracle's 4096-spp
+    image.  The MLP is trained online from the tracer's own vertex-2 samples
+    (up to 2^18 per frame), then paths stop at vertex 2 and take its
+    prediction."""
     from srt_amd import abi, scenes
-    gold_path = os.path.join(ROOT, "tests", "golden", "cornell_256x256_fp64_4096spp.npy")
+    if args.mlp_scene == "spheres":
+        gold_path = os.path.join(ROOT, "tests", "golden", "spheres_192x108_fp64_4096spp.npy")
+        s = scenes.spheres()
+        W, H, depth, rr, train_spp = 192, 108, 8, 4, 64
+        workload = ("spheres-1M: 192x108, depth 8, NEE+MIS (RMSE vs FP64 4096-spp oracle); "
+                    "MLP 63->4x64->3 ReLU, Adam lr 1e-3")
+    else:
+        gold_path = os.path.join(ROOT, "tests", "golden", "cornell_256x256_fp64_4096spp.npy")
+        s = scenes.cornell()
+        W, H, depth, rr, train_spp = 256, 256, 5, 4, 8
+        workload = "cornell-36: 256x256, depth 5, NEE+MIS; MLP 63->4x64->3 ReLU, Adam lr 1e-3"
     gold = np.load(gold_path).astype(np.float64)
-    s = scenes.cornell()
-    W = H = 256
-    depth, rr = 5, 4
     ctx = abi.Context(0)
     h = ctx.upload(s)
     ctx.build_bvh(h)
@@ -278,13 +287,13 @@ def mlp_mode(args):
         img = ctx.render(h, s.camera, W, H, spp, depth, rr, 1, 1, seed + 1, mlp_mode=mode)
         return img, time.perf_counter() - t0
 
-    # online training: frames of 2^18 samples harvested from 256x256x8 tracer paths
+    # online training: frames of up to 2^18 samples harvested from W x H x train_spp tracer paths
     train_frames, steps = args.mlp_frames, args.mlp_steps
     ctx.mlp_create(seed=1, lr=1e-3)
     losses = []
     t0 = time.perf_counter()
     for f in range(train_frames):
-        loss, n = ctx.mlp_train_frame(h, s.camera, W, H, 8, depth, rr, seed=5000 + f, max_samples=1 << 18,
+        loss, n = ctx.mlp_train_frame(h, s.camera, W, H, train_spp, depth, rr, seed=5000 + f, max_samples=1 << 18,
                                       steps=steps)
         losses.append(loss)
     t_train = time.perf_counter() - t0
@@ -307,8 +316,8 @@ def mlp_mode(args):
     out = {"metric": "RMSE vs FP64 4096-spp oracle at equal wall time (radiance MLP vs plain tracer)",
            "unit": "RMSE", "higher_is_better": False, "dtype": "f32 tracer, bf16 MLP (fp32 accumulation)",
            "data": "synthetic",
-           "config": {"workload": "cornell-36: 256x256, depth 5, NEE+MIS; MLP 63->4x64->3 ReLU, Adam lr 1e-3, "
-                                  f"{train_frames} training frames x {steps} steps, 2^18 samples/frame",
+           "config": {"workload": workload + f", {train_frames} training frames x {steps} steps, up to 2^18 "
+                                             "samples/frame",
                       "golden": os.path.relpath(gold_path, ROOT)},
            "training": {"ms": round(t_train * 1e3, 1), "loss_first": round(float(losses[0]), 4),
                         "loss_last": round(float(losses[-1]), 4)},
@@ -431,6 +440,7 @@ def main():
     ap.add_argument("--mode", choices=["frame", "copy", "skin", "mlp", "serve"], default="frame")
     ap.add_argument("--mlp-frames", type=int, default=16)
     ap.add_argument("--mlp-steps", type=int, default=8)
+    ap.add_argument("--mlp-scene", choices=["cornell", "spheres"], default="cornell")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-window", type=float, default=30.0)
     ap.add_argument("--serve-seed", type=int, default=20261202)

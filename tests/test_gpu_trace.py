@@ -83,3 +83,28 @@ def test_watertight_and_axis_rays_match_brute_force(ctx):
     osc = O.Scene(s)
     np.testing.assert_array_equal(p, O.trace(osc, o, d)[3])
     ctx.release(h)
+
+
+@pytest.mark.parametrize("name", ["cornell", "spheres", "tube"])
+def test_scene_upload_round_trip(ctx, name):
+    """srt_scene_upload keeps the vertices bit for bit and builds the light and
+    area CDFs exactly as the oracle does (scene.cu:240-287, mesh.cpp:3-18)."""
+    s = getattr(scenes, name)()
+    h = ctx.upload(s)
+    nv = s.positions.shape[0]
+    has_n = s.normals is not None
+    has_t = getattr(s, "tangents", None) is not None
+    p, n, t = ctx.download_vertices(h, nv, normals=has_n, tangents=has_t)
+    assert np.array_equal(p.view(np.uint32), np.ascontiguousarray(s.positions, np.float32).view(np.uint32))
+    if has_n:
+        assert np.array_equal(n.view(np.uint32), np.ascontiguousarray(s.normals, np.float32).view(np.uint32))
+    if has_t:
+        assert np.array_equal(t.view(np.uint32), np.ascontiguousarray(s.tangents, np.float32).view(np.uint32))
+    osc = O.Scene(s)
+    got = ctx.lights(h, s.num_triangles, len(s.mesh_material))
+    ref = osc.lights()
+    assert len(got[0]) == len(ref[0]) > 0
+    for a, b in zip(got[:4], ref[:4]):
+        assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+    assert got[4] == ref[4]
+    ctx.release(h)
