@@ -143,20 +143,21 @@ SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t
 // entries go to a per-thread global area (entry e at gst[(e - lds_depth) *
 // gstride]).  The build guarantees a tree depth <= kMaxDepth.
 extern __shared__ int32_t trav_lds[];    // dynamic LDS of the traversal kernels: lds_depth * blockDim entries
+// The traversal kernels run kTraceBlock threads per workgroup, so entry sp of a
+// lane sits at trav_lds[sp * kTraceBlock + threadIdx.x] (a shift, no multiply).
 struct TravStack {
     int32_t* gst;
     size_t gstride;
     int lds_depth;
-    int bstride;
     int lane0;           // this thread's first LDS entry (threadIdx.x)
     int inner_exit;
 };
 SRT_D void stk_push(const TravStack& k, int32_t sp, int32_t v) {
-    if (sp < k.lds_depth) trav_lds[k.lane0 + sp * k.bstride] = v;
+    if (sp < k.lds_depth) trav_lds[k.lane0 + sp * kTraceBlock] = v;
     else k.gst[(size_t)(sp - k.lds_depth) * k.gstride] = v;
 }
 SRT_D int32_t stk_get(const TravStack& k, int32_t sp) {
-    if (sp < k.lds_depth) return trav_lds[k.lane0 + sp * k.bstride];
+    if (sp < k.lds_depth) return trav_lds[k.lane0 + sp * kTraceBlock];
     int32_t v = k.gst[(size_t)(sp - k.lds_depth) * k.gstride];
     // keeps the compiler from merging the two loads into one generic (flat) load
     __asm__ volatile("" : "+v"(v));
@@ -500,7 +501,6 @@ SRT_D TravStack make_stack(int32_t* gstack, TraceCfg cfg) {
     k.gstride = (size_t)gridDim.x * blockDim.x;
     k.lds_depth = cfg.lds_depth;
     k.inner_exit = cfg.inner_exit;
-    k.bstride = (int)blockDim.x;
     return k;
 }
 
