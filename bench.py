@@ -27,6 +27,7 @@ Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
        python bench.py --mode copy          (calibration kernel for the counters)
        python bench.py --pmc-summary FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
        python bench.py --kstats-summary DIR  (rocprofv3 --kernel-trace --stats output -> profiles/)
+       python bench.py --mlp-summary KS_DIR PMC_DIR  (MLP kernel times + MFMA counters -> profiles/)
 """
 import argparse
 import json
@@ -40,6 +41,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA, MI355X
+MLP_TRAIN_FLOP = 99072           # forward + backward + weight gradients per training sample (16,512 MACs x 6)
+MLP_FWD_FLOP = 33024             # forward per query (16,512 MACs x 2)
 METRIC = "Mpaths/s at equal RMSE vs FP64 oracle; p50/p99 render-request latency"
 
 # algorithmic bytes per unit (SoA layout of srt_amd/csrc/srt_internal.h)
@@ -194,6 +198,53 @@ def kstats_summary(d):
     print(text)
 
 
+MLP_PROFILE_FILE = os.path.join(ROOT, "profiles", "mlp_train_cornell_mfma.md")
+MLP_PMC = ["SQ_INSTS_VALU_MFMA_MOPS_BF16"]
+
+
+def mlp_summary(ks_dir, pmc_dir):
+    """MFMA summary of the radiance-MLP kernels: kernel times from a
+    `rocprofv3 --kernel-trace --stats` run and BF16 MFMA operation counts from
+    a `rocprofv3 --pmc` run of `bench.py --mode mlp --mlp-frames 4`."""
+    import glob
+    import sqlite3
+    times = {}
+    for f in glob.glob(os.path.join(ks_dir, "**", "*results.db"), recursive=True):
+        con = sqlite3.connect(f)
+        for n, c, t, a in con.execute("select name, total_calls, total_duration, average from top_kernels"):
+            times[n] = (int(c), float(t) * 1e3, float(a) * 1e3)     # ns
+        con.close()
+    if not times:
+        raise SystemExit(f"no kernel statistics under {ks_dir}")
+    mops = {}
+    for n, v in _pmc_rows(pmc_dir, MLP_PMC[0]):
+        mops.setdefault(n, []).append(v)
+    mlp_names = sorted((n for n in times if "mlp" in n), key=lambda n: -times[n][1])
+    lines = ["# Radiance-MLP kernels: time and MFMA counter (Cornell 256x256 training frames)", "",
+             "Commands (from the repository root; TMPDIR=/tmp):", "",
+             "    rocprofv3 --kernel-trace --stats -d gpurun_out/mlp_ks -o run -- python3 bench.py --mode mlp --mlp-frames 4",
+             "    rocprofv3 --pmc " + " ".join(MLP_PMC) + " -d gpurun_out/mlp_pmc -o run -- python3 bench.py --mode mlp "
+             "--mlp-frames 4",
+             "    python3 bench.py --mlp-summary gpurun_out/mlp_ks gpurun_out/mlp_pmc", "",
+             "The run trains on 6 frames (8 steps each, about 120k vertex-2 samples per step) and renders the",
+             "MLP form twice at 16 spp and three times at 256 spp (k_mlp_infer, one launch per render).",
+             "Counter GFLOP = SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 per dispatch, as read.  For k_mlp_infer it equals",
+             "the FLOP of the MFMA instructions the launch issues (36 per 32 queries); for k_mlp_train it reads",
+             "exactly one eighth of them (98 per 32 samples), so the algorithmic rates in `bench.py --mode mlp`",
+             "(HIP events, 99,072 FLOP per training sample, 33,024 per query) are the ones quoted.", "",
+             "| kernel | calls | average us | counter GFLOP/dispatch |",
+             "|---|---|---|---|"]
+    for n in mlp_names:
+        c, tot, avg = times[n]
+        short = n.split("(")[0].replace("void ", "")
+        flop = f"{sum(mops[n]) / len(mops[n]) * 512 / 1e9:.3f}" if n in mops else "-"
+        lines.append(f"| `{short}` | {c} | {avg / 1e3:.1f} | {flop} |")
+    text = "\n".join(lines) + "\n"
+    with open(MLP_PROFILE_FILE, "w") as fh:
+        fh.write(text)
+    print(text)
+
+
 def _stats_ms(st, name, steps):
     return round(st[name]["ms"] / steps, 3) if name in st else None
 
@@ -297,6 +348,29 @@ def mlp_mode(args):
                                       steps=steps)
         losses.append(loss)
     t_train = time.perf_counter() - t0
+
+    # MFMA roofline of the training step: two more training frames (not part of
+    # the comparison) with per-kernel HIP-event timing
+    ctx.set_option("profile", 1)
+    ctx.stats_reset()
+    n_prof = 0
+    for f in range(2):
+        _, n = ctx.mlp_train_frame(h, s.camera, W, H, train_spp, depth, rr, seed=9000 + f, max_samples=1 << 18,
+                                   steps=steps)
+        n_prof += n * steps
+    ctx.set_option("profile", 0)
+    st = ctx.stats()
+    train_k = [k for k in ("mlp_train",) if k in st]
+    k_ms = sum(st[k]["ms"] for k in train_k)
+    flop = MLP_TRAIN_FLOP * n_prof
+    achieved = flop / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
+    mlp_roofline = {"bound": "mfma", "kernel": "+".join(train_k), "achieved": round(achieved, 2),
+                    "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4),
+                    "flop_per_sample": MLP_TRAIN_FLOP, "samples": int(n_prof),
+                    "ms_per_step": round(k_ms / (2 * steps), 4),
+                    "adam_ms_per_step": round(st.get("mlp_adam", {}).get("ms", 0.0) / (2 * steps), 4),
+                    "per_kernel_ms": {k: round(st[k]["ms"] / (2 * steps), 4) for k in train_k}}
+    log(f"MLP training: {achieved:.1f} TFLOP/s over {train_k}, {mlp_roofline['ms_per_step']} ms/step")
     budgets = {}
     for spp in (16, 256):
         img_p, t_p = timed_render(spp, 100 + spp)                  # plain at this spp
@@ -313,6 +387,16 @@ def mlp_mode(args):
             "mlp_wins_equal_render_time": bool(rmse(img_m) < rmse(img_e)),
             "mlp_wins_counting_training": bool(rmse(img_m) < rmse(img_t)),
             "mlp_mean_ratio": round(float(img_m[:, :3].mean() / img_p[:, :3].mean()), 4)}
+    # MLP inference rate inside a 256-spp MLP-form render (per-kernel HIP events, query count read back)
+    ctx.set_option("profile", 1)
+    ctx.stats_reset()
+    ctx.render(h, s.camera, W, H, 256, depth, rr, 1, 1, 777, mlp_mode=1)
+    ctx.set_option("profile", 0)
+    q = ctx.stats().get("mlp_query", {"ms": 0.0, "items": 0})
+    q_tf = q["items"] * MLP_FWD_FLOP / (q["ms"] * 1e-3) / 1e12 if q["ms"] > 0 else 0.0
+    inference = {"queries": int(q["items"]), "ms": round(q["ms"], 4), "achieved": round(q_tf, 2), "unit": "TFLOP/s",
+                 "frac": round(q_tf / MFMA_BF16_PEAK_TFLOPS, 4), "flop_per_query": MLP_FWD_FLOP}
+    log(f"MLP inference: {q['items']} queries in {q['ms']:.3f} ms, {q_tf:.1f} TFLOP/s")
     out = {"metric": "RMSE vs FP64 4096-spp oracle at equal wall time (radiance MLP vs plain tracer)",
            "unit": "RMSE", "higher_is_better": False, "dtype": "f32 tracer, bf16 MLP (fp32 accumulation)",
            "data": "synthetic",
@@ -321,6 +405,8 @@ def mlp_mode(args):
                       "golden": os.path.relpath(gold_path, ROOT)},
            "training": {"ms": round(t_train * 1e3, 1), "loss_first": round(float(losses[0]), 4),
                         "loss_last": round(float(losses[-1]), 4)},
+           "roofline": mlp_roofline,
+           "inference": inference,
            "budgets": budgets}
     ctx.close()
     print(json.dumps(out), flush=True)
@@ -446,7 +532,11 @@ def main():
     ap.add_argument("--serve-seed", type=int, default=20261202)
     ap.add_argument("--pmc-summary", nargs=4, metavar="DIR")
     ap.add_argument("--kstats-summary", metavar="DIR")
+    ap.add_argument("--mlp-summary", nargs=2, metavar="DIR")
     args = ap.parse_args()
+    if args.mlp_summary:
+        mlp_summary(*args.mlp_summary)
+        return
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
         return
@@ -507,6 +597,19 @@ def main():
     value = paths * args.steps / elapsed / 1e6
     log(f"{value:.1f} Mpaths/s, {elapsed / args.steps * 1e3:.1f} ms/frame; kernels {json.dumps(st)}")
 
+    # the same frames handed back in host memory (film copied over PCIe each frame)
+    host_img = np.zeros((W * H, 4), np.float32)
+    n_host = max(1, min(args.steps, 3))
+    t1 = time.perf_counter()
+    for _ in range(n_host):
+        ctx.render(h, s.camera, W, H, spp, depth, R["rr_depth"], 1, 1, R["seed"], out=host_img)
+    host_s = (time.perf_counter() - t1) / n_host
+    pcie = {"value": round(paths / host_s / 1e6, 2), "unit": "Mpaths/s", "frames": n_host,
+            "ms_per_frame": round(host_s * 1e3, 2),
+            "note": "srt_render with a host rgba_out: the float32 RGBA film is copied to pageable host memory "
+                    "after every frame"}
+    log(f"host-output rate {pcie['value']} Mpaths/s ({pcie['ms_per_frame']} ms/frame)")
+
     # algorithmic bytes per frame per kernel
     emitted = segments - paths     # queue entries written by shade
     bytes_step = {
@@ -566,6 +669,7 @@ def main():
                        "p99": round(float(np.percentile(lat_sorted, 99)), 2), "unit": "ms per frame request"},
         "segments_per_path": round(segments / paths, 3), "shadow_rays_per_path": round(shadow_rays / paths, 3),
         "bvh_build_ms": round(bvh_ms, 1),
+        "pcie_inclusive": pcie,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -1,6 +1,7 @@
-// Radiance MLP for gfx950: BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32
-// accumulation, hand-written forward and backward kernels, Adam, and the
-// online-training / inference hooks of the path tracer.
+// Radiance MLP for gfx950: BF16 MFMA (v_mfma_f32_32x32x16_bf16 and
+// v_mfma_f32_16x16x32_bf16) with FP32 accumulation, hand-written forward and
+// backward kernels, Adam, and the online-training / inference hooks of the
+// path tracer.
 //
 // Replaces the reference's NeuralRadiosity pass network
 // (src/passes/npr/neural_radiosity.cu:6-52, tiny-cuda-nn HashGrid + CutlassMLP,
@@ -12,14 +13,17 @@
 //   loss    relative L2: mean (y - t)^2 / (y^2 + 0.01), y detached below
 //   Adam    Kingma & Ba 2015, beta1 0.9, beta2 0.999, eps 1e-8, lr 1e-3
 // Every activation and gradient that feeds an MFMA is rounded to BF16 (round
-// to nearest even); sums are FP32.  Weight gradients are reduced over fixed
-// sample chunks and then in chunk order, so training is bit-reproducible.
+// to nearest even); sums are FP32.  A training step is one kernel: weight
+// gradients are reduced over fixed 1024-sample blocks and then in block
+// order, so training is bit-reproducible.
 //
-// MFMA fragment layout used (16x16x32, wave64): lane l = 16 g + r holds
-// A[r][8g .. 8g+7] and B[8g .. 8g+7][r] (8 BF16 each), and receives
-// D[4g .. 4g+3][r] (4 FP32).  Only the row/column/lane-group structure matters:
-// A and B use the same K placement, so any K permutation inside the
-// instruction cancels.
+// MFMA fragment layouts used (wave64; only the row/column/lane-group
+// structure matters: A and B use the same K placement, so any K permutation
+// inside the instruction cancels):
+//   32x32x16: lane l = 32 g + r holds A[r][8g .. 8g+7] and B[8g .. 8g+7][r],
+//             and receives D[8q + 4g + p][r] in element 4q + p (q, p < 4);
+//   16x16x32: lane l = 16 g + r holds A[r][8g .. 8g+7] and B[8g .. 8g+7][r],
+//             and receives D[4g + i][r] in element i (i < 4).
 #include "srt_internal.h"
 #include <cstring>
 #include <cmath>
@@ -29,6 +33,7 @@ namespace srt {
 
 typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
 typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
+typedef float f32x16 __attribute__((__vector_size__(16 * sizeof(float))));
 
 constexpr int MLP_IN = 64;        // 63 encoded features + 1 zero pad
 constexpr int MLP_H = 64;
@@ -36,9 +41,7 @@ constexpr int MLP_OUTP = 16;      // 3 outputs padded to one MFMA tile
 constexpr int MLP_LAYERS = 5;
 constexpr int MLP_RAW = 9;
 constexpr int MLP_OCT = 3;
-constexpr int TILE = 16;          // samples per wave tile
 constexpr int MLP_BLOCK = 256;    // 4 waves
-constexpr int CHUNK = 4096;       // samples per weight-gradient partial
 
 // rows (outputs) of layer l, columns (inputs) are always 64
 __host__ __device__ constexpr int layer_rows(int l) { return l == 4 ? MLP_OUTP : MLP_H; }
@@ -70,261 +73,456 @@ struct MlpDev {
     float lo[3], inv_ext[3];   // position normalisation
 };
 
-// feature i of the encoding of raw input u[9]
-SRT_D float enc_feature(const float* u, int i) {
-    if (i < MLP_RAW) return u[i];
-    if (i >= MLP_RAW + 2 * MLP_RAW * MLP_OCT) return 0.0f;
-    int j = i - MLP_RAW;
-    int oct = j / (2 * MLP_RAW);
-    int r = j % (2 * MLP_RAW);
-    int c = r % MLP_RAW;
-    float x = u[c] * (float)(1 << oct) * kPi;
-    float s, co;
-    pt_sincosf(x, &s, &co);
-    return r < MLP_RAW ? s : co;
+// Encoding of the 9 raw inputs: [u, sin(2^k pi u), cos(2^k pi u), k = 0..2].
+// sin/cos of pi u come from one sincos per input; octaves 1 and 2 follow by
+// the double-angle identities (2^k pi u is 2^k times the float product pi u
+// exactly, so the identities restate the same angles).
+struct Enc {
+    float u[MLP_RAW], s[MLP_OCT][MLP_RAW], c[MLP_OCT][MLP_RAW];
+};
+SRT_D void enc_init(Enc& e, const float* u) {
+#pragma unroll
+    for (int i = 0; i < MLP_RAW; i++) {
+        e.u[i] = u[i];
+        float s, c;
+        pt_sincosf(u[i] * kPi, &s, &c);
+        e.s[0][i] = s; e.c[0][i] = c;
+#pragma unroll
+        for (int k = 1; k < MLP_OCT; k++) {
+            const float s2 = (2.0f * s) * c, c2 = (c - s) * (c + s);
+            s = s2; c = c2;
+            e.s[k][i] = s; e.c[k][i] = c;
+        }
+    }
+}
+// feature f (compile-time after unrolling) of the 64-wide encoded input
+SRT_D float enc_get(const Enc& e, int f) {
+    if (f < MLP_RAW) return e.u[f];
+    if (f >= MLP_RAW + 2 * MLP_RAW * MLP_OCT) return 0.0f;
+    const int j = f - MLP_RAW, oct = j / (2 * MLP_RAW), r = j % (2 * MLP_RAW);
+    return r < MLP_RAW ? e.s[oct][r] : e.c[oct][r - MLP_RAW];
 }
 
-// Per-wave tile machinery --------------------------------------------------
-// act: LDS [TILE][64] BF16 for the current layer input (row-major)
-// The wave computes out[TILE][rows] = act * W^T (+ bias), fragment by fragment.
-template <int ROWS>
-SRT_D void layer_mfma(const uint16_t* act, const uint16_t* wl, f32x4 acc[ROWS / 16], int lane) {
-    const int r = lane & 15, g = lane >> 4;
+SRT_D uint16_t bf(float x) {                 // round to nearest even (v_cvt_pk_bf16_f32)
+    __bf16 h = (__bf16)x;
+    uint16_t u;
+    __builtin_memcpy(&u, &h, 2);
+    return u;
+}
+SRT_D uint32_t bf2(float a, float b) {        // a in the low half (lower address)
+    return (uint32_t)bf(a) | ((uint32_t)bf(b) << 16);
+}
+SRT_D uint64_t pack4(const uint16_t v[4]) {
+    return (uint64_t)v[0] | ((uint64_t)v[1] << 16) | ((uint64_t)v[2] << 32) | ((uint64_t)v[3] << 48);
+}
+SRT_D uint16_t unpack4(uint64_t v, int p) { return (uint16_t)(v >> (16 * p)); }
+
+SRT_D f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+SRT_D f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+// ---------------------------------------------------------------------------
+// Wave tiles.  A wave owns 32 samples.  Its activation rows live in LDS as
+// act[sample][feature] (BF16, row stride AST), the operand form of the next
+// layer's product.  Layer outputs come out of v_mfma_f32_32x32x16_bf16 as
+// D[sample][feature]: lane (r = lane & 31, g = lane >> 5) holds feature r of
+// the 32-feature tile and samples 8q + 4g + p in element 4q + p.  That same
+// register form, four consecutive samples per 64-bit word, is what the weight
+// gradients need (sample-contiguous rows), so it is kept in registers (hpk)
+// and staged as [feature][sample] rows.
+// ---------------------------------------------------------------------------
+constexpr int T_WAVES = 4;
+constexpr int T_TILE = 32;                   // samples per wave
+constexpr int T_SUB = T_WAVES * T_TILE;      // samples per block step
+constexpr int T_CHUNK_MAX = 1024;            // samples per block (= per weight-gradient partial), at most
+constexpr int AST = 72;                      // row stride (BF16) of 64-wide LDS rows (+16 B against bank conflicts)
+constexpr int SST = T_SUB + 8;               // row stride of the staged [feature][sample] rows
+constexpr int WT4ST = 24;                    // row stride of W_4^T ([64][16])
+constexpr int WF_ELEMS = 4 * 64 * AST + MLP_OUTP * AST;
+constexpr int WT_ELEMS = 3 * 64 * AST + 64 * WT4ST;
+// samples per block for a step of n samples: a multiple of the block step,
+// about n / 256 (one block per CU of the 256-CU part), 128 .. 1024; a
+// function of n alone, so the reduction order does not depend on the device
+__host__ __device__ constexpr int train_chunk(int n) {
+    const int per = (n + 255) / 256;
+    const int c = (per + T_SUB - 1) / T_SUB * T_SUB;
+    return c < T_SUB ? T_SUB : (c > T_CHUNK_MAX ? T_CHUNK_MAX : c);
+}
+
+SRT_D void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// forward weights W_l[j][k] -> sWf[(64 l + j) AST + k]; bias -> sB
+SRT_D void load_forward_weights(const MlpDev& M, uint16_t* sWf, float* sB) {
+    for (int i = threadIdx.x; i < W_TOTAL / 8; i += blockDim.x) {
+        const int e = i * 8, l = e < 4 * 4096 ? e / 4096 : 4, o = e - w_off(l), j = o / 64, k = o % 64;
+        *(uint4*)&sWf[(l * 64 + j) * AST + k] = ((const uint4*)M.wf)[i];
+    }
+    for (int i = threadIdx.x; i < B_TOTAL; i += blockDim.x) sB[i] = M.bias[i];
+}
+
+// encode samples t0 .. t0+31 into act: lane (r, g) writes features 32g .. 32g+31 of sample t0 + r
+SRT_D void wave_encode(uint16_t* act, const float* in9, int t0, int n, int r, int g) {
+    const int s = t0 + r;
+    float u[MLP_RAW];
 #pragma unroll
-    for (int jb = 0; jb < ROWS / 16; jb++) acc[jb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < MLP_RAW; c++) u[c] = (s < n) ? in9[(size_t)s * MLP_RAW + c] : 0.0f;
+    Enc e;
+    enc_init(e, u);
+    uint32_t pk[16];
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-        bf16x8 a = ld_frag(act + r * 64 + 32 * q + 8 * g);
+    for (int i = 0; i < 16; i++) {
+        const float a0 = g ? enc_get(e, 32 + 2 * i) : enc_get(e, 2 * i);
+        const float a1 = g ? enc_get(e, 33 + 2 * i) : enc_get(e, 2 * i + 1);
+        pk[i] = bf2(a0, a1);
+    }
 #pragma unroll
-        for (int jb = 0; jb < ROWS / 16; jb++) {
-            bf16x8 b = ld_frag(wl + (16 * jb + r) * 64 + 32 * q + 8 * g);
-            acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[jb], 0, 0, 0);
+    for (int m = 0; m < 4; m++)
+        *(uint4*)&act[r * AST + 32 * g + 8 * m] = make_uint4(pk[4 * m], pk[4 * m + 1], pk[4 * m + 2], pk[4 * m + 3]);
+}
+
+// hidden layer l (0..3) on the wave's 32 samples: act (input rows) -> act
+// (output rows); the output's register form goes to h[jt][q]
+SRT_D void wave_hidden(uint16_t* act, const uint16_t* sWf, const float* sB, int l, uint64_t h[2][4], int r, int g) {
+    f32x16 acc[2];
+#pragma unroll
+    for (int jt = 0; jt < 2; jt++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[jt][i] = 0.0f;
+    const uint16_t* W = sWf + l * 64 * AST;
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) {
+        const bf16x8 a = ld_frag(act + r * AST + 16 * ks + 8 * g);
+#pragma unroll
+        for (int jt = 0; jt < 2; jt++) acc[jt] = mfma32(a, ld_frag(W + (32 * jt + r) * AST + 16 * ks + 8 * g), acc[jt]);
+    }
+#pragma unroll
+    for (int jt = 0; jt < 2; jt++) {
+        const float bj = sB[b_off(l) + 32 * jt + r];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint16_t v[4];
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const float z = acc[jt][4 * q + p] + bj;
+                v[p] = bf(z > 0.0f ? z : 0.0f);
+                act[(8 * q + 4 * g + p) * AST + 32 * jt + r] = v[p];
+            }
+            h[jt][q] = pack4(v);
         }
     }
 }
 
-// ---------------------------------------------------------------------------
-// forward: in9 [n][9] fp32 -> y [n][3] fp32; TRAIN also stores the BF16 layer
-// inputs H_l transposed ([l][64][n]) for the weight gradients.
-// ---------------------------------------------------------------------------
-template <bool TRAIN>
-__global__ void __launch_bounds__(MLP_BLOCK) k_mlp_fwd(const float* in9, int n_arg, const uint32_t* n_dev, MlpDev M, float* y,
-                                                       uint16_t* Ht) {
-    const int n = n_dev ? (int)*n_dev : n_arg;     // device-side count (integrator queries)
-    __shared__ __attribute__((aligned(16))) uint16_t sw[W_TOTAL];
-    __shared__ float sb[B_TOTAL];
-    __shared__ __attribute__((aligned(16))) uint16_t sact[MLP_BLOCK / 64][2][TILE * 64];
-    for (int i = threadIdx.x; i < W_TOTAL / 8; i += blockDim.x) ((uint4*)sw)[i] = ((const uint4*)M.wf)[i];
-    for (int i = threadIdx.x; i < B_TOTAL; i += blockDim.x) sb[i] = M.bias[i];
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int r = lane & 15, g = lane >> 4;
-    const int ntiles = (n + TILE - 1) / TILE;
-    constexpr int WAVES = MLP_BLOCK / 64;
-    // every wave of the block runs the same rounds (the layer barriers are
-    // block-wide); a wave past the last tile computes on zeros and stores nothing
-    for (int base = blockIdx.x * WAVES; base < ntiles; base += gridDim.x * WAVES) {
-        const int tile = base + wv;
-        const int s0 = tile * TILE;
-        uint16_t* a0 = sact[wv][0];
-        uint16_t* a1 = sact[wv][1];
-        // encode: lane (r, g) writes features 16g..16g+15 of sample r
-        {
-            float u[MLP_RAW];
-            const int s = s0 + r;
-            for (int c = 0; c < MLP_RAW; c++) u[c] = (s < n) ? in9[(size_t)s * MLP_RAW + c] : 0.0f;
-            for (int k = 0; k < 16; k++) {
-                int f = 16 * g + k;
-                uint16_t h = f2bf(s < n ? enc_feature(u, f) : 0.0f);
-                a0[r * 64 + f] = h;
-                if (TRAIN && s < n) Ht[((size_t)0 * 64 + f) * n + s] = h;
-            }
-        }
-        __syncthreads();
-        uint16_t* cur = a0;
-        uint16_t* nxt = a1;
-        for (int l = 0; l < 4; l++) {
-            f32x4 acc[4];
-            layer_mfma<64>(cur, sw + w_off(l), acc, lane);
-            // bias + ReLU -> BF16 -> next activation rows 4g..4g+3, column 16jb + r
+// output layer on act (H_4 rows): lane (r16, g16) gets y[16 rt + 4 g16 + i][channel r16] (bias not added)
+SRT_D void wave_output(const uint16_t* act, const uint16_t* sWf, f32x4 y[2], int lane) {
+    const int r16 = lane & 15, g16 = lane >> 4;
+    const uint16_t* W = sWf + 4 * 64 * AST;
 #pragma unroll
-            for (int jb = 0; jb < 4; jb++) {
-                int j = 16 * jb + r;
+    for (int rt = 0; rt < 2; rt++) {
+        y[rt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+            y[rt] = mfma16(ld_frag(act + (16 * rt + r16) * AST + 32 * ks + 8 * g16),
+                           ld_frag(W + r16 * AST + 32 * ks + 8 * g16), y[rt]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inference: in9 [n][9] -> y [n][3]; n from n_dev (device-side queue count)
+// when given.  Waves run independently over 32-sample tiles.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(MLP_BLOCK) k_mlp_infer(const float* in9, int n_arg, const uint32_t* n_dev, MlpDev M,
+                                                         float* y) {
+    const int n = n_dev ? (int)*n_dev : n_arg;
+    __shared__ __attribute__((aligned(16))) uint16_t sWf[WF_ELEMS];
+    __shared__ float sB[B_TOTAL];
+    __shared__ __attribute__((aligned(16))) uint16_t sAct[MLP_BLOCK / 64][T_TILE * AST];
+    load_forward_weights(M, sWf, sB);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = lane & 31, g = lane >> 5, r16 = lane & 15, g16 = lane >> 4;
+    uint16_t* act = sAct[w];
+    const int ntiles = (n + T_TILE - 1) / T_TILE;
+    for (int tile = blockIdx.x * (MLP_BLOCK / 64) + w; tile < ntiles; tile += gridDim.x * (MLP_BLOCK / 64)) {
+        const int t0 = tile * T_TILE;
+        wave_encode(act, in9, t0, n, r, g);
+        wave_lds_order();
+        uint64_t h[2][4];
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            wave_hidden(act, sWf, sB, l, h, r, g);
+            wave_lds_order();
+        }
+        f32x4 yv[2];
+        wave_output(act, sWf, yv, lane);
+        if (r16 < 3) {
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    int s = 4 * g + i;
-                    float z = acc[jb][i] + sb[b_off(l) + j];
-                    uint16_t h = f2bf(z > 0.0f ? z : 0.0f);
-                    nxt[s * 64 + j] = h;
-                    if (TRAIN && s0 + s < n) Ht[((size_t)(l + 1) * 64 + j) * n + (s0 + s)] = h;
+                    const int s = t0 + 16 * rt + 4 * g16 + i;
+                    if (s < n) y[(size_t)s * 3 + r16] = yv[rt][i] + sB[b_off(4) + r16];
+                }
+        }
+        wave_lds_order();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// training step (forward, loss gradient, backward, weight/bias gradients) in
+// one kernel.  Block b owns samples [b chunk, (b+1) chunk) and writes one
+// partial of every weight and bias gradient; k_mlp_adam sums the partials in
+// block order, so a step is bit-reproducible.  Per block step of 128 samples
+// (4 waves x 32): each wave encodes, runs the forward pass, the loss gradient
+// and the backward products on its own samples; the layer gradients G_l and
+// inputs H_l of all 128 samples are staged in LDS as [feature][sample] rows
+// and every wave accumulates one tile of dW_l = G_l^T H_l over them.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, const float* target, int n, int chunk,
+                                                             float inv_count, MlpDev M, float* wpart, float* bpart,
+                                                             float* loss_part) {
+    __shared__ __attribute__((aligned(16))) uint16_t sWf[WF_ELEMS];
+    __shared__ __attribute__((aligned(16))) uint16_t sWt[WT_ELEMS];     // W_l^T[k][j], layers 1..3, then layer 4
+    __shared__ __attribute__((aligned(16))) uint16_t sAct[T_WAVES][T_TILE * AST];
+    __shared__ __attribute__((aligned(16))) uint16_t sGt[64 * SST];     // G_l^T[feature][sample of the block step]
+    __shared__ __attribute__((aligned(16))) uint16_t sHt[64 * SST];     // H_l^T[feature][sample of the block step]
+    __shared__ float sB[B_TOTAL];
+    __shared__ float sRed[MLP_BLOCK];
+    load_forward_weights(M, sWf, sB);
+    for (int i = threadIdx.x; i < 3 * 64 * 8; i += MLP_BLOCK) {
+        const int l = 1 + i / 512, rem = i % 512, k = rem / 8, j = (rem % 8) * 8;
+        *(uint4*)&sWt[((l - 1) * 64 + k) * AST + j] = *(const uint4*)&M.wt[w_off(l) + k * 64 + j];
+    }
+    for (int i = threadIdx.x; i < 64 * 2; i += MLP_BLOCK) {
+        const int k = i / 2, j = (i % 2) * 8;
+        *(uint4*)&sWt[3 * 64 * AST + k * WT4ST + j] = *(const uint4*)&M.wt[w_off(4) + k * MLP_OUTP + j];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = lane & 31, g = lane >> 5, r16 = lane & 15, g16 = lane >> 4;
+    const int jt_w = w >> 1, kt_w = w & 1;      // this wave's 32x32 tile of each 64x64 weight gradient
+    const int dbj = threadIdx.x & 63;           // bias gradients: feature dbj over the samples of wave w
+    uint16_t* act = sAct[w];
+    f32x16 accW[4];
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) accW[l][i] = 0.0f;
+    f32x4 accW4 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float dbacc[MLP_LAYERS] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float lsum = 0.0f;
+    const int blk_begin = blockIdx.x * chunk;        // chunk: a multiple of T_SUB
+    const int blk_end = min(n, blk_begin + chunk);
+
+    // stage the register form of a 64-feature gradient or activation tile as [feature][sample] rows
+    auto stage = [&](uint16_t* dst, const uint64_t v[2][4]) {
+#pragma unroll
+        for (int jt = 0; jt < 2; jt++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) *(uint64_t*)&dst[(32 * jt + r) * SST + T_TILE * w + 8 * q + 4 * g] = v[jt][q];
+    };
+    // sum of the staged gradient row dbj over this wave's 32 samples, in sample order
+    auto bias_rows = [&](int rows) -> float {
+        float s = 0.0f;
+        if (dbj < rows) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const uint4 v = *(const uint4*)&sGt[dbj * SST + T_TILE * w + 8 * m];
+                const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    s += bf2f((uint16_t)(wd[e] & 0xffffu));
+                    s += bf2f((uint16_t)(wd[e] >> 16));
                 }
             }
-            __syncthreads();
-            uint16_t* t = cur; cur = nxt; nxt = t;
         }
-        f32x4 acc4[1];
-        layer_mfma<16>(cur, sw + w_off(4), acc4, lane);
-        if (r < 3) {
+        return s;
+    };
+
+    for (int sb = blk_begin; sb < blk_end; sb += T_SUB) {
+        const int t0 = sb + T_TILE * w;
+        uint64_t hpk[MLP_LAYERS][2][4];         // H_l (input of layer l), register form
+        wave_encode(act, in9, t0, n, r, g);
+        wave_lds_order();
+#pragma unroll
+        for (int jt = 0; jt < 2; jt++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                uint16_t v[4];
+#pragma unroll
+                for (int p = 0; p < 4; p++) v[p] = act[(8 * q + 4 * g + p) * AST + 32 * jt + r];
+                hpk[0][jt][q] = pack4(v);
+            }
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            wave_hidden(act, sWf, sB, l, hpk[l + 1], r, g);
+            wave_lds_order();
+        }
+        f32x4 yv[2];
+        wave_output(act, sWf, yv, lane);
+        wave_lds_order();
+        // loss gradient G_4 (channels 0..2 of 16): rows of act (columns 0..15) and staged rows
+        uint16_t gv[2][4];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                int s = s0 + 4 * g + i;
-                if (s < n) y[(size_t)s * 3 + r] = acc4[0][i] + sb[b_off(4) + r];
+                const int sl = 16 * rt + 4 * g16 + i, s = t0 + sl;
+                float gval = 0.0f;
+                if (r16 < 3 && s < n) {
+                    const float yy = yv[rt][i] + sB[b_off(4) + r16], tt = target[(size_t)s * 3 + r16];
+                    const float d = yy - tt, den = yy * yy + 0.01f;
+                    lsum += d * d / den;
+                    gval = 2.0f * d / den * inv_count;
+                }
+                gv[rt][i] = bf(gval);
+                act[sl * AST + r16] = gv[rt][i];
             }
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++) *(uint64_t*)&sGt[r16 * SST + T_TILE * w + 16 * rt + 4 * g16] = pack4(gv[rt]);
+        stage(sHt, hpk[4]);
+        wave_lds_order();
+        // G_3 = (G_4 W_4) masked by H_4 > 0
+        uint64_t gpk[2][4];
+        {
+            f32x16 acc[2];
+#pragma unroll
+            for (int kt = 0; kt < 2; kt++)
+#pragma unroll
+                for (int i = 0; i < 16; i++) acc[kt][i] = 0.0f;
+            const bf16x8 a = ld_frag(act + r * AST + 8 * g);
+#pragma unroll
+            for (int kt = 0; kt < 2; kt++)
+                acc[kt] = mfma32(a, ld_frag(sWt + 3 * 64 * AST + (32 * kt + r) * WT4ST + 8 * g), acc[kt]);
+#pragma unroll
+            for (int kt = 0; kt < 2; kt++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    uint16_t v[4];
+#pragma unroll
+                    for (int p = 0; p < 4; p++) v[p] = unpack4(hpk[4][kt][q], p) ? bf(acc[kt][4 * q + p]) : (uint16_t)0;
+                    gpk[kt][q] = pack4(v);
+                }
         }
         __syncthreads();
+        // dW_4 (rows 0..15, columns 16 w .. 16 w + 15) and db_4 over the block step
+#pragma unroll
+        for (int ks = 0; ks < T_SUB / 32; ks++)
+            accW4 = mfma16(ld_frag(sGt + r16 * SST + 32 * ks + 8 * g16),
+                           ld_frag(sHt + (16 * w + r16) * SST + 32 * ks + 8 * g16), accW4);
+        dbacc[4] += bias_rows(MLP_OUTP);
+        __syncthreads();
+#pragma unroll
+        for (int l = 3; l >= 0; l--) {
+            stage(sGt, gpk);
+            stage(sHt, hpk[l]);
+            if (l >= 1) {
+                // G_{l-1} = (G_l W_l) masked by H_l > 0; G_l rows into act as the product's operand
+#pragma unroll
+                for (int jt = 0; jt < 2; jt++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+#pragma unroll
+                        for (int p = 0; p < 4; p++) act[(8 * q + 4 * g + p) * AST + 32 * jt + r] = unpack4(gpk[jt][q], p);
+                wave_lds_order();
+                f32x16 acc[2];
+#pragma unroll
+                for (int kt = 0; kt < 2; kt++)
+#pragma unroll
+                    for (int i = 0; i < 16; i++) acc[kt][i] = 0.0f;
+                const uint16_t* Wt = sWt + (l - 1) * 64 * AST;
+#pragma unroll
+                for (int ks = 0; ks < 4; ks++) {
+                    const bf16x8 a = ld_frag(act + r * AST + 16 * ks + 8 * g);
+#pragma unroll
+                    for (int kt = 0; kt < 2; kt++)
+                        acc[kt] = mfma32(a, ld_frag(Wt + (32 * kt + r) * AST + 16 * ks + 8 * g), acc[kt]);
+                }
+#pragma unroll
+                for (int kt = 0; kt < 2; kt++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        uint16_t v[4];
+#pragma unroll
+                        for (int p = 0; p < 4; p++)
+                            v[p] = unpack4(hpk[l][kt][q], p) ? bf(acc[kt][4 * q + p]) : (uint16_t)0;
+                        gpk[kt][q] = pack4(v);
+                    }
+                wave_lds_order();
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < T_SUB / 16; ks++)
+                accW[l] = mfma32(ld_frag(sGt + (32 * jt_w + r) * SST + 16 * ks + 8 * g),
+                                 ld_frag(sHt + (32 * kt_w + r) * SST + 16 * ks + 8 * g), accW[l]);
+            dbacc[l] += bias_rows(64);
+            __syncthreads();
+        }
     }
-}
 
-// ---------------------------------------------------------------------------
-// loss gradient + backward through the layers.  Writes G_l^T ([l][rows][n],
-// BF16) = dL/dz of layer l's output, l = 0..4, and per-block loss partials.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(MLP_BLOCK) k_mlp_bwd(const float* y, const float* target, int n, float inv_count,
-                                                       MlpDev M, const uint16_t* Ht, uint16_t* Gt, float* loss_part) {
-    __shared__ __attribute__((aligned(16))) uint16_t swt[W_TOTAL];
-    __shared__ __attribute__((aligned(16))) uint16_t sg[MLP_BLOCK / 64][TILE * 64];
-    __shared__ float sloss[MLP_BLOCK];
-    for (int i = threadIdx.x; i < W_TOTAL / 8; i += blockDim.x) ((uint4*)swt)[i] = ((const uint4*)M.wt)[i];
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int r = lane & 15, g = lane >> 4;
-    const int ntiles = (n + TILE - 1) / TILE;
-    constexpr int WAVES = MLP_BLOCK / 64;
-    float lsum = 0.0f;
-    uint16_t* G = sg[wv];
-    for (int base = blockIdx.x * WAVES; base < ntiles; base += gridDim.x * WAVES) {
-        const int tile = base + wv;
-        const int s0 = tile * TILE;
-        // output gradient: sample 4g+i, channel r (< 3), padded to 16 columns
-        for (int i = 0; i < 4; i++) {
-            int s = 4 * g + i, gs = s0 + s;
-            float gval = 0.0f;
-            if (r < 3 && gs < n) {
-                float yy = y[(size_t)gs * 3 + r], tt = target[(size_t)gs * 3 + r];
-                float d = yy - tt, den = yy * yy + 0.01f;
-                lsum += d * d / den;
-                gval = 2.0f * d / den * inv_count;
-            }
-            uint16_t h = f2bf(gval);
-            // G rows are samples, columns outputs (16 used, stride 64)
-            G[s * 64 + r] = h;
-            if (gs < n) Gt[((size_t)4 * 64 + r) * n + gs] = h;
+    // partials of this block: dW_l tile (jt_w, kt_w): lane (r, g) holds D[32 jt_w + 8q + 4g + p][32 kt_w + r]
+    float* wp = wpart + (size_t)blockIdx.x * MLP_LAYERS * 4096;
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+                wp[l * 4096 + (32 * jt_w + 8 * q + 4 * g + p) * 64 + 32 * kt_w + r] = accW[l][4 * q + p];
+#pragma unroll
+    for (int i = 0; i < 4; i++) wp[4 * 4096 + (4 * g16 + i) * 64 + 16 * w + r16] = accW4[i];
+    // bias partials: the four waves' sums of each feature, added in wave order
+#pragma unroll
+    for (int l = 0; l < MLP_LAYERS; l++) {
+        sRed[threadIdx.x] = dbacc[l];
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int j = threadIdx.x;
+            const float v = ((sRed[j] + sRed[64 + j]) + sRed[128 + j]) + sRed[192 + j];
+            bpart[((size_t)blockIdx.x * MLP_LAYERS + l) * 64 + j] = v;
         }
         __syncthreads();
-        // layers 4..1: dh_l = G_l * W_l (W_l^T stored [64][rows]); z mask from H_l > 0
-        for (int l = 4; l >= 1; l--) {
-            const int rows = layer_rows(l);
-            f32x4 acc[4];
-#pragma unroll
-            for (int kb = 0; kb < 4; kb++) acc[kb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            const uint16_t* wt = swt + w_off(l);
-            for (int q = 0; q < rows / 32 + (rows % 32 ? 1 : 0); q++) {
-                // A = G (rows = samples, K = outputs of layer l); pad K beyond rows with 0
-                bf16x8 a;
-                if (32 * q + 8 * g < rows) a = ld_frag(G + r * 64 + 32 * q + 8 * g);
-                else { uint4 z = make_uint4(0, 0, 0, 0); __builtin_memcpy(&a, &z, 16); }
-#pragma unroll
-                for (int kb = 0; kb < 4; kb++) {
-                    bf16x8 b;
-                    if (32 * q + 8 * g < rows) {
-                        // B[kappa = j][n = k] = W_l[j][k] = Wt_l[k][j]: need 8 consecutive j for fixed k
-                        b = ld_frag(wt + (16 * kb + r) * rows + 32 * q + 8 * g);
-                    } else { uint4 z = make_uint4(0, 0, 0, 0); __builtin_memcpy(&b, &z, 16); }
-                    acc[kb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[kb], 0, 0, 0);
-                }
-            }
-            __syncthreads();
-            // dz_{l-1} = dh * [H_l > 0]; H_l is the input of layer l = output of layer l-1
-#pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-                int k = 16 * kb + r;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    int s = 4 * g + i, gs = s0 + s;
-                    float hv = (gs < n) ? bf2f(Ht[((size_t)l * 64 + k) * n + gs]) : 0.0f;
-                    float dz = hv > 0.0f ? acc[kb][i] : 0.0f;
-                    uint16_t h = f2bf(dz);
-                    G[s * 64 + k] = h;
-                    if (gs < n) Gt[((size_t)(l - 1) * 64 + k) * n + gs] = h;
-                }
-            }
-            __syncthreads();
-        }
     }
-    sloss[threadIdx.x] = lsum;
+    sRed[threadIdx.x] = lsum;
     __syncthreads();
     for (int st = MLP_BLOCK / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) sloss[threadIdx.x] += sloss[threadIdx.x + st];
+        if ((int)threadIdx.x < st) sRed[threadIdx.x] += sRed[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) loss_part[blockIdx.x] = sloss[0];
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = sRed[0];
 }
 
-// ---------------------------------------------------------------------------
-// weight gradients: partial dW_l[j][k] = sum over a chunk of samples of
-// G_l[j][s] * H_l[k][s] (both stored transposed, s contiguous).
-// grid: (chunks, 16 output blocks (jb, kb), layers); one wave per block.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) k_mlp_dw(int n, const uint16_t* Ht, const uint16_t* Gt, float* part) {
-    const int chunk = blockIdx.x, blk = blockIdx.y, l = blockIdx.z;
-    const int rows = layer_rows(l);
-    const int jb = blk >> 2, kb = blk & 3;
-    if (16 * jb >= rows) return;
-    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
-    const int s_begin = chunk * CHUNK;
-    const int s_end = min(n, s_begin + CHUNK);
-    const uint16_t* G = Gt + ((size_t)l * 64 + 16 * jb + r) * n;   // A rows: j
-    const uint16_t* H = Ht + ((size_t)l * 64 + 16 * kb + r) * n;   // B cols: k
-    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int s = s_begin; s < s_end; s += 32) {
-        int base = s + 8 * g;
-        bf16x8 a, b;
-        if (base + 8 <= s_end && (n % 8) == 0) {
-            a = ld_frag(G + base);
-            b = ld_frag(H + base);
-        } else {
-            uint16_t ta[8], tb[8];
-            for (int e = 0; e < 8; e++) {
-                ta[e] = (base + e < s_end) ? G[base + e] : (uint16_t)0;
-                tb[e] = (base + e < s_end) ? H[base + e] : (uint16_t)0;
-            }
-            __builtin_memcpy(&a, ta, 16);
-            __builtin_memcpy(&b, tb, 16);
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-    }
-    // D[j = 16jb + 4g + i][k = 16kb + r]
-    float* out = part + ((size_t)chunk * MLP_LAYERS + l) * (64 * 64);
-#pragma unroll
-    for (int i = 0; i < 4; i++) out[(16 * jb + 4 * g + i) * 64 + 16 * kb + r] = acc[i];
-}
-
-// bias gradients: db_l[j] partial over a chunk (fixed-order sums)
-__global__ void __launch_bounds__(64) k_mlp_db(int n, const uint16_t* Gt, float* part) {
-    const int chunk = blockIdx.x, l = blockIdx.z;
-    const int j = threadIdx.x;
-    if (j >= layer_rows(l)) return;
-    const int s_begin = chunk * CHUNK, s_end = min(n, s_begin + CHUNK);
-    const uint16_t* G = Gt + ((size_t)l * 64 + j) * n;
-    float acc = 0.0f;
-    for (int s = s_begin; s < s_end; s++) acc += bf2f(G[s]);
-    part[((size_t)chunk * MLP_LAYERS + l) * 64 + j] = acc;
-}
-
-// reduce partials in chunk order, Adam update, refresh BF16 copies
+// Sum the per-block partials, Adam update, refresh the BF16 copies.  Block b
+// handles parameters p = 64 b + (t & 63); thread group t >> 6 (of 4) sums the
+// partials t >> 6, t >> 6 + 4, ... in order and the 4 group sums are added in
+// group order, so the reduction order depends on the partial count alone.
+// Partial c of weight p is wpart[c * 5 * 4096 + p], of bias q bpart[c * 5 * 64 + q].
+constexpr int ADAM_PARAMS = 64;
 __global__ void __launch_bounds__(256) k_mlp_adam(int nchunks, const float* wpart, const float* bpart, float* W, float* B,
                                                   float* mW, float* vW, float* mB, float* vB, float lr, float b1, float b2,
                                                   float eps, float c1, float c2, int apply, float* grad_out, uint16_t* wf,
                                                   uint16_t* wt) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float sg[256];
+    const int t = threadIdx.x, grp = t >> 6;
+    const int p = blockIdx.x * ADAM_PARAMS + (t & 63);
+    float part = 0.0f;
+    if (p < W_TOTAL) {
+        for (int c = grp; c < nchunks; c += 4) part += wpart[(size_t)c * (MLP_LAYERS * 4096) + p];
+    } else if (p < W_TOTAL + B_TOTAL) {
+        for (int c = grp; c < nchunks; c += 4) part += bpart[(size_t)c * (MLP_LAYERS * 64) + (p - W_TOTAL)];
+    }
+    sg[t] = part;
+    __syncthreads();
+    if (grp != 0) return;
+    const float gtot = ((sg[t] + sg[t + 64]) + sg[t + 128]) + sg[t + 192];
     if (p < W_TOTAL) {
         int l = (p < 4 * 4096) ? p / 4096 : 4;
         int o = p - w_off(l);
         int j = o / 64, k = o % 64;
         bool live = !(l == 0 && k == 63) && !(l == 4 && j >= 3);
-        float gsum = 0.0f;
-        for (int c = 0; c < nchunks; c++) gsum += wpart[((size_t)c * MLP_LAYERS + l) * 4096 + j * 64 + k];
-        if (!live) gsum = 0.0f;
+        float gsum = live ? gtot : 0.0f;
         if (grad_out) grad_out[p] = gsum;
         float w = W[p];
         if (apply && live) {
@@ -345,9 +543,7 @@ __global__ void __launch_bounds__(256) k_mlp_adam(int nchunks, const float* wpar
         int l = (q < 256) ? q / 64 : 4;
         int j = q - b_off(l);
         bool live = !(l == 4 && j >= 3);
-        float gsum = 0.0f;
-        for (int c = 0; c < nchunks; c++) gsum += bpart[((size_t)c * MLP_LAYERS + l) * 64 + j];
-        if (!live) gsum = 0.0f;
+        float gsum = live ? gtot : 0.0f;
         if (grad_out) grad_out[p] = gsum;
         if (apply && live) {
             float m = b1 * mB[q] + (1.0f - b1) * gsum;
@@ -373,8 +569,8 @@ __global__ void k_loss_sum(const float* part, int nparts, float* out) {
 // ---------------------------------------------------------------------------
 struct MlpState {
     DevBuf W, B, mW, vW, mB, vB, wf, wt, bias_dev;
-    DevBuf Ht, Gt, y, wpart, bpart, loss_part, loss, grad, in_buf, tgt_buf, out_buf;
-    int64_t cap = 0;        // training sample capacity of Ht/Gt/y
+    DevBuf wpart, bpart, loss_part, loss, grad, in_buf, tgt_buf, out_buf;
+    int64_t cap = 0;        // training sample capacity of the partials and targets
     int64_t infer_cap = 0;
     int step = 0;
     float lr = 1e-3f, b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -386,8 +582,7 @@ static MlpState* mlp_of(srt_ctx* c) { return (MlpState*)c->mlp; }
 static void mlp_free_all(srt_ctx* c) {
     MlpState* m = mlp_of(c);
     if (!m) return;
-    DevBuf* all[] = {&m->W, &m->B, &m->mW, &m->vW, &m->mB, &m->vB, &m->wf, &m->wt, &m->bias_dev, &m->Ht, &m->Gt, &m->y,
-                     &m->wpart, &m->bpart, &m->loss_part, &m->loss, &m->grad, &m->in_buf, &m->tgt_buf, &m->out_buf};
+    DevBuf* all[] = {&m->W, &m->B, &m->mW, &m->vW, &m->mB, &m->vB, &m->wf, &m->wt, &m->bias_dev, &m->wpart, &m->bpart, &m->loss_part, &m->loss, &m->grad, &m->in_buf, &m->tgt_buf, &m->out_buf};
     for (DevBuf* b : all) dev_free(c, *b);
     delete m;
     c->mlp = nullptr;
@@ -405,7 +600,7 @@ static MlpDev mlp_dev(MlpState* m) {
 // refresh BF16 copies without an update (apply = 0)
 static int refresh_copies(srt_ctx* c, MlpState* m) {
     const int tot = W_TOTAL + B_TOTAL;
-    k_mlp_adam<<<(tot + 255) / 256, 256, 0, c->stream>>>(0, nullptr, nullptr, (float*)m->W.p, (float*)m->B.p, nullptr,
+    k_mlp_adam<<<(tot + ADAM_PARAMS - 1) / ADAM_PARAMS, 256, 0, c->stream>>>(0, nullptr, nullptr, (float*)m->W.p, (float*)m->B.p, nullptr,
                                                          nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0,
                                                          nullptr, (uint16_t*)m->wf.p, (uint16_t*)m->wt.p);
     return hip_check(c, hipGetLastError(), "mlp refresh");
@@ -413,13 +608,12 @@ static int refresh_copies(srt_ctx* c, MlpState* m) {
 
 static int ensure_train(srt_ctx* c, MlpState* m, int64_t n) {
     if (m->cap >= n) return SRT_OK;
-    DevBuf* bs[] = {&m->Ht, &m->Gt, &m->y, &m->wpart, &m->bpart, &m->tgt_buf};
+    DevBuf* bs[] = {&m->wpart, &m->bpart, &m->loss_part, &m->tgt_buf};
     for (DevBuf* b : bs) dev_free(c, *b);
-    int64_t chunks = (n + CHUNK - 1) / CHUNK;
-    int r = dev_alloc(c, m->Ht, (size_t)MLP_LAYERS * 64 * n * 2);
-    if (!r) r = dev_alloc(c, m->Gt, (size_t)MLP_LAYERS * 64 * n * 2);
-    if (!r) r = dev_alloc(c, m->y, (size_t)n * 12);
-    if (!r) r = dev_alloc(c, m->tgt_buf, (size_t)n * 12);
+    // blocks of any step of at most n samples: ceil(n' / train_chunk(n')) <= max(256, ceil(n / 1024))
+    const int64_t chunks = std::max<int64_t>(256, (n + T_CHUNK_MAX - 1) / T_CHUNK_MAX);
+    int r = dev_alloc(c, m->tgt_buf, (size_t)n * 12);
+    if (!r) r = dev_alloc(c, m->loss_part, (size_t)chunks * 4);
     if (!r) r = dev_alloc(c, m->wpart, (size_t)chunks * MLP_LAYERS * 4096 * 4);
     if (!r) r = dev_alloc(c, m->bpart, (size_t)chunks * MLP_LAYERS * 64 * 4);
     if (r) return r;
@@ -438,13 +632,14 @@ static int ensure_in(srt_ctx* c, MlpState* m, int64_t n) {
     return SRT_OK;
 }
 
-static int fwd_grid(srt_ctx* c, int64_t n) {
-    int64_t tiles = (n + TILE - 1) / TILE;
-    int64_t blocks = (tiles + 3) / 4;
+// inference grid: 32-sample tiles, 4 per block step, at most two blocks per CU (LDS)
+static int infer_grid(srt_ctx* c, int64_t n) {
+    int64_t tiles = (n + T_TILE - 1) / T_TILE;
+    int64_t blocks = (tiles + MLP_BLOCK / 64 - 1) / (MLP_BLOCK / 64);
     hipDeviceProp_t prop;
     int cus = 256;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
-    int64_t cap = (int64_t)cus * 3;
+    int64_t cap = (int64_t)cus * 2;
     return (int)std::max<int64_t>(1, std::min(blocks, cap));
 }
 
@@ -455,34 +650,21 @@ static int train_step_dev(srt_ctx* c, MlpState* m, const float* in9, const float
     if (int r = ensure_train(c, m, n)) return r;
     hipStream_t st = c->stream;
     MlpDev d = mlp_dev(m);
-    const int grid = fwd_grid(c, n);
-    if (!m->loss_part.p || m->loss_part.bytes < (size_t)grid * 4) {
-        dev_free(c, m->loss_part);
-        if (int r = dev_alloc(c, m->loss_part, (size_t)std::max(grid, 1024) * 4)) return r;
-    }
+    const int chunk = train_chunk(n), chunks = (n + chunk - 1) / chunk;
     if (!m->loss.p) if (int r = dev_alloc(c, m->loss, 16)) return r;
     if (!m->grad.p) if (int r = dev_alloc(c, m->grad, (size_t)(W_TOTAL + B_TOTAL) * 4)) return r;
     hipEvent_t ev;
-    prof_begin(c, "mlp_fwd", &ev);
-    k_mlp_fwd<true><<<grid, MLP_BLOCK, 0, st>>>(in9, n, nullptr, d, (float*)m->y.p, (uint16_t*)m->Ht.p);
-    prof_end(c, "mlp_fwd", ev, n);
-    prof_begin(c, "mlp_bwd", &ev);
-    k_mlp_bwd<<<grid, MLP_BLOCK, 0, st>>>((const float*)m->y.p, target, n, 1.0f / (3.0f * (float)n), d,
-                                          (const uint16_t*)m->Ht.p, (uint16_t*)m->Gt.p, (float*)m->loss_part.p);
-    prof_end(c, "mlp_bwd", ev, n);
-    k_loss_sum<<<1, 64, 0, st>>>((const float*)m->loss_part.p, grid, (float*)m->loss.p);
-    const int chunks = (n + CHUNK - 1) / CHUNK;
-    prof_begin(c, "mlp_dw", &ev);
-    k_mlp_dw<<<dim3(chunks, 16, MLP_LAYERS), 64, 0, st>>>(n, (const uint16_t*)m->Ht.p, (const uint16_t*)m->Gt.p,
-                                                            (float*)m->wpart.p);
-    k_mlp_db<<<dim3(chunks, 1, MLP_LAYERS), 64, 0, st>>>(n, (const uint16_t*)m->Gt.p, (float*)m->bpart.p);
-    prof_end(c, "mlp_dw", ev, n);
+    prof_begin(c, "mlp_train", &ev);
+    k_mlp_train<<<chunks, MLP_BLOCK, 0, st>>>(in9, target, n, chunk, 1.0f / (3.0f * (float)n), d, (float*)m->wpart.p,
+                                              (float*)m->bpart.p, (float*)m->loss_part.p);
+    prof_end(c, "mlp_train", ev, n);
+    k_loss_sum<<<1, 64, 0, st>>>((const float*)m->loss_part.p, chunks, (float*)m->loss.p);
     if (apply) m->step++;
     const float c1 = 1.0f - std::pow(m->b1, (float)std::max(m->step, 1));
     const float c2 = 1.0f - std::pow(m->b2, (float)std::max(m->step, 1));
     const int tot = W_TOTAL + B_TOTAL;
     prof_begin(c, "mlp_adam", &ev);
-    k_mlp_adam<<<(tot + 255) / 256, 256, 0, st>>>(chunks, (const float*)m->wpart.p, (const float*)m->bpart.p, (float*)m->W.p,
+    k_mlp_adam<<<(tot + ADAM_PARAMS - 1) / ADAM_PARAMS, 256, 0, st>>>(chunks, (const float*)m->wpart.p, (const float*)m->bpart.p, (float*)m->W.p,
                                                   (float*)m->B.p, (float*)m->mW.p, (float*)m->vW.p, (float*)m->mB.p,
                                                   (float*)m->vB.p, m->lr, m->b1, m->b2, m->eps, c1, c2, apply,
                                                   (float*)m->grad.p, (uint16_t*)m->wf.p, (uint16_t*)m->wt.p);
@@ -507,7 +689,7 @@ int mlp_infer_dev(srt_ctx* c, const float* in9, int n, float* out3) {
     if (n == 0) return SRT_OK;
     hipEvent_t ev;
     prof_begin(c, "mlp_infer", &ev);
-    k_mlp_fwd<false><<<fwd_grid(c, n), MLP_BLOCK, 0, c->stream>>>(in9, n, nullptr, mlp_dev(m), out3, nullptr);
+    k_mlp_infer<<<infer_grid(c, n), MLP_BLOCK, 0, c->stream>>>(in9, n, nullptr, mlp_dev(m), out3);
     prof_end(c, "mlp_infer", ev, n);
     return hip_check(c, hipGetLastError(), "mlp infer launch");
 }
@@ -518,8 +700,14 @@ int mlp_query_dev(srt_ctx* c, const float* in9, const uint32_t* count_dev, int m
     if (!m) return set_err(c, SRT_E_STATE, "no MLP: call srt_mlp_create first");
     hipEvent_t ev;
     prof_begin(c, "mlp_query", &ev);
-    k_mlp_fwd<false><<<fwd_grid(c, max_n), MLP_BLOCK, 0, c->stream>>>(in9, 0, count_dev, mlp_dev(m), out3, nullptr);
-    prof_end(c, "mlp_query", ev, 0);
+    k_mlp_infer<<<infer_grid(c, max_n), MLP_BLOCK, 0, c->stream>>>(in9, 0, count_dev, mlp_dev(m), out3);
+    uint32_t queries = 0;
+    if (c->profile) {
+        // profiling only: the query count, for the per-query rate in the statistics
+        if (int r = hip_check(c, hipMemcpyAsync(&queries, count_dev, 4, hipMemcpyDeviceToHost, c->stream), "count")) return r;
+        if (int r = hip_check(c, hipStreamSynchronize(c->stream), "count")) return r;
+    }
+    prof_end(c, "mlp_query", ev, queries);
     return hip_check(c, hipGetLastError(), "mlp query launch");
 }
 
