@@ -20,6 +20,10 @@ HOSTLIB = os.path.join(HERE, "libsrt_host.so")
 SRTRACER = os.path.join(HERE, "srtracer")
 
 HIP_SOURCES = ["srt_api.hip", "wavefront.hip", "bvh.hip", "mlp.hip", "serve.hip"]
+# per-source code-generation options: the MLP kernels keep MFMA accumulators in
+# ordinary VGPRs (their epilogues read every element; the AGPR form would add a
+# v_accvgpr_read per element)
+HIP_SOURCE_FLAGS = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 HOST_SOURCES = ["json.cpp", "scene_io.cpp", "passes.cpp", "renderer.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -57,7 +61,7 @@ def build_hip(verbose=False, jobs=4):
 
     def compile_one(pair):
         src, obj = pair
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        _run([HIPCC, *HIP_FLAGS, *HIP_SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj], verbose)
 
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(compile_one, zip(srcs, objs)))

@@ -74,9 +74,10 @@ struct MlpDev {
 };
 
 // Encoding of the 9 raw inputs: [u, sin(2^k pi u), cos(2^k pi u), k = 0..2].
-// sin/cos of pi u come from one sincos per input; octaves 1 and 2 follow by
-// the double-angle identities (2^k pi u is 2^k times the float product pi u
-// exactly, so the identities restate the same angles).
+// sin/cos of pi u come from the hardware sine and cosine (v_sin_f32 /
+// v_cos_f32 take the angle in turns: pi u = u/2 turns, u in [0, 1]); octaves
+// 1 and 2 follow by the double-angle identities.  The features feed BF16
+// operands (8 significant bits), far coarser than either step's error.
 struct Enc {
     float u[MLP_RAW], s[MLP_OCT][MLP_RAW], c[MLP_OCT][MLP_RAW];
 };
@@ -84,8 +85,7 @@ SRT_D void enc_init(Enc& e, const float* u) {
 #pragma unroll
     for (int i = 0; i < MLP_RAW; i++) {
         e.u[i] = u[i];
-        float s, c;
-        pt_sincosf(u[i] * kPi, &s, &c);
+        float s = __builtin_amdgcn_sinf(0.5f * u[i]), c = __builtin_amdgcn_cosf(0.5f * u[i]);
         e.s[0][i] = s; e.c[0][i] = c;
 #pragma unroll
         for (int k = 1; k < MLP_OCT; k++) {
@@ -103,19 +103,44 @@ SRT_D float enc_get(const Enc& e, int f) {
     return r < MLP_RAW ? e.s[oct][r] : e.c[oct][r - MLP_RAW];
 }
 
-SRT_D uint16_t bf(float x) {                 // round to nearest even (v_cvt_pk_bf16_f32)
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef short i16x2v __attribute__((ext_vector_type(2)));
+
+SRT_D uint16_t bf(float x) {                 // round to nearest even
     __bf16 h = (__bf16)x;
     uint16_t u;
     __builtin_memcpy(&u, &h, 2);
     return u;
 }
-SRT_D uint32_t bf2(float a, float b) {        // a in the low half (lower address)
-    return (uint32_t)bf(a) | ((uint32_t)bf(b) << 16);
+// two values rounded to BF16 (one v_cvt_pk_bf16_f32), a in the low half (lower address)
+SRT_D uint32_t bf2(float a, float b) {
+    const bf16x2v h = __builtin_convertvector((f32x2v){a, b}, bf16x2v);
+    uint32_t u;
+    __builtin_memcpy(&u, &h, 4);
+    return u;
 }
-SRT_D uint64_t pack4(const uint16_t v[4]) {
-    return (uint64_t)v[0] | ((uint64_t)v[1] << 16) | ((uint64_t)v[2] << 32) | ((uint64_t)v[3] << 48);
+// ReLU of two BF16 values: as 16-bit integers, negative values (sign bit set)
+// are negative and non-negative values keep their order, so max(x, 0) on the
+// integers is max(x, +0) on the floats (v_pk_max_i16)
+SRT_D uint32_t relu2(uint32_t v) {
+    i16x2v x;
+    __builtin_memcpy(&x, &v, 4);
+    x = __builtin_elementwise_max(x, (i16x2v){0, 0});
+    __builtin_memcpy(&v, &x, 4);
+    return v;
 }
-SRT_D uint16_t unpack4(uint64_t v, int p) { return (uint16_t)(v >> (16 * p)); }
+// g where h != 0, else 0, per 16-bit half; h holds non-negative BF16 values.
+// Two packed integer instructions: min(h, 1) per half is 0 or 1, and the
+// low-half product applies it (written out: the compiler would otherwise
+// expand the pair into compares and selects)
+SRT_D uint32_t mask2(uint32_t g, uint32_t h) {
+    uint32_t m, out;
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(m) : "v"(h), "v"(0x00010001u));
+    asm("v_pk_mul_lo_u16 %0, %1, %2" : "=v"(out) : "v"(g), "v"(m));
+    return out;
+}
+SRT_D uint64_t pack64(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
 
 SRT_D f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
 SRT_D f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
@@ -152,6 +177,20 @@ SRT_D void wave_lds_order() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the four BF16 values of one register group (samples 8q + 4g + p, p = 0..3,
+// of one feature) to / from rows base, base + AST, base + 2 AST, base + 3 AST of act
+SRT_D void put_rows(uint16_t* act, int base, uint32_t lo, uint32_t hi) {
+    act[base] = (uint16_t)lo;
+    act[base + AST] = (uint16_t)(lo >> 16);
+    act[base + 2 * AST] = (uint16_t)hi;
+    act[base + 3 * AST] = (uint16_t)(hi >> 16);
+}
+SRT_D uint64_t get_rows(const uint16_t* act, int base) {
+    const uint32_t lo = (uint32_t)act[base] | ((uint32_t)act[base + AST] << 16);
+    const uint32_t hi = (uint32_t)act[base + 2 * AST] | ((uint32_t)act[base + 3 * AST] << 16);
+    return pack64(lo, hi);
 }
 
 // forward weights W_l[j][k] -> sWf[(64 l + j) AST + k]; bias -> sB
@@ -203,14 +242,11 @@ SRT_D void wave_hidden(uint16_t* act, const uint16_t* sWf, const float* sB, int 
         const float bj = sB[b_off(l) + 32 * jt + r];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            uint16_t v[4];
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                const float z = acc[jt][4 * q + p] + bj;
-                v[p] = bf(z > 0.0f ? z : 0.0f);
-                act[(8 * q + 4 * g + p) * AST + 32 * jt + r] = v[p];
-            }
-            h[jt][q] = pack4(v);
+            // ReLU after rounding equals rounding after ReLU (rounding keeps the sign)
+            const uint32_t lo = relu2(bf2(acc[jt][4 * q] + bj, acc[jt][4 * q + 1] + bj));
+            const uint32_t hi = relu2(bf2(acc[jt][4 * q + 2] + bj, acc[jt][4 * q + 3] + bj));
+            h[jt][q] = pack64(lo, hi);
+            put_rows(act, (8 * q + 4 * g) * AST + 32 * jt + r, lo, hi);
         }
     }
 }
@@ -303,7 +339,6 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = lane & 31, g = lane >> 5, r16 = lane & 15, g16 = lane >> 4;
     const int jt_w = w >> 1, kt_w = w & 1;      // this wave's 32x32 tile of each 64x64 weight gradient
-    const int dbj = threadIdx.x & 63;           // bias gradients: feature dbj over the samples of wave w
     uint16_t* act = sAct[w];
     f32x16 accW[4];
 #pragma unroll
@@ -311,7 +346,15 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
 #pragma unroll
         for (int i = 0; i < 16; i++) accW[l][i] = 0.0f;
     f32x4 accW4 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    float dbacc[MLP_LAYERS] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // bias gradients: rows 16w .. 16w + 15 of each layer (layer 4: wave 0) as a product with a ones column
+    f32x4 dbm[MLP_LAYERS];
+#pragma unroll
+    for (int l = 0; l < MLP_LAYERS; l++) dbm[l] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 ones;
+    {
+        const uint4 o = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);     // BF16 1.0
+        __builtin_memcpy(&ones, &o, 16);
+    }
     float lsum = 0.0f;
     const int blk_begin = blockIdx.x * chunk;        // chunk: a multiple of T_SUB
     const int blk_end = min(n, blk_begin + chunk);
@@ -323,22 +366,10 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
 #pragma unroll
             for (int q = 0; q < 4; q++) *(uint64_t*)&dst[(32 * jt + r) * SST + T_TILE * w + 8 * q + 4 * g] = v[jt][q];
     };
-    // sum of the staged gradient row dbj over this wave's 32 samples, in sample order
-    auto bias_rows = [&](int rows) -> float {
-        float s = 0.0f;
-        if (dbj < rows) {
+    // the staged gradient rows 16w .. 16w + 15 summed over the block step's samples
+    auto bias_rows = [&](f32x4& acc) {
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
-                const uint4 v = *(const uint4*)&sGt[dbj * SST + T_TILE * w + 8 * m];
-                const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    s += bf2f((uint16_t)(wd[e] & 0xffffu));
-                    s += bf2f((uint16_t)(wd[e] >> 16));
-                }
-            }
-        }
-        return s;
+        for (int ks = 0; ks < T_SUB / 32; ks++) acc = mfma16(ld_frag(sGt + (16 * w + r16) * SST + 32 * ks + 8 * g16), ones, acc);
     };
 
     for (int sb = blk_begin; sb < blk_end; sb += T_SUB) {
@@ -349,12 +380,7 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
 #pragma unroll
         for (int jt = 0; jt < 2; jt++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                uint16_t v[4];
-#pragma unroll
-                for (int p = 0; p < 4; p++) v[p] = act[(8 * q + 4 * g + p) * AST + 32 * jt + r];
-                hpk[0][jt][q] = pack4(v);
-            }
+            for (int q = 0; q < 4; q++) hpk[0][jt][q] = get_rows(act, (8 * q + 4 * g) * AST + 32 * jt + r);
 #pragma unroll
         for (int l = 0; l < 4; l++) {
             wave_hidden(act, sWf, sB, l, hpk[l + 1], r, g);
@@ -364,24 +390,24 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
         wave_output(act, sWf, yv, lane);
         wave_lds_order();
         // loss gradient G_4 (channels 0..2 of 16): rows of act (columns 0..15) and staged rows
-        uint16_t gv[2][4];
 #pragma unroll
-        for (int rt = 0; rt < 2; rt++)
+        for (int rt = 0; rt < 2; rt++) {
+            float gval[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int sl = 16 * rt + 4 * g16 + i, s = t0 + sl;
-                float gval = 0.0f;
+                const int s = t0 + 16 * rt + 4 * g16 + i;
+                gval[i] = 0.0f;
                 if (r16 < 3 && s < n) {
                     const float yy = yv[rt][i] + sB[b_off(4) + r16], tt = target[(size_t)s * 3 + r16];
-                    const float d = yy - tt, den = yy * yy + 0.01f;
-                    lsum += d * d / den;
-                    gval = 2.0f * d / den * inv_count;
+                    const float d = yy - tt, inv = 1.0f / (yy * yy + 0.01f);
+                    lsum += (d * d) * inv;
+                    gval[i] = ((2.0f * d) * inv) * inv_count;
                 }
-                gv[rt][i] = bf(gval);
-                act[sl * AST + r16] = gv[rt][i];
             }
-#pragma unroll
-        for (int rt = 0; rt < 2; rt++) *(uint64_t*)&sGt[r16 * SST + T_TILE * w + 16 * rt + 4 * g16] = pack4(gv[rt]);
+            const uint32_t lo = bf2(gval[0], gval[1]), hi = bf2(gval[2], gval[3]);
+            put_rows(act, (16 * rt + 4 * g16) * AST + r16, lo, hi);
+            *(uint64_t*)&sGt[r16 * SST + T_TILE * w + 16 * rt + 4 * g16] = pack64(lo, hi);
+        }
         stage(sHt, hpk[4]);
         wave_lds_order();
         // G_3 = (G_4 W_4) masked by H_4 > 0
@@ -399,20 +425,18 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
 #pragma unroll
             for (int kt = 0; kt < 2; kt++)
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    uint16_t v[4];
-#pragma unroll
-                    for (int p = 0; p < 4; p++) v[p] = unpack4(hpk[4][kt][q], p) ? bf(acc[kt][4 * q + p]) : (uint16_t)0;
-                    gpk[kt][q] = pack4(v);
-                }
+                for (int q = 0; q < 4; q++)
+                    gpk[kt][q] = pack64(mask2(bf2(acc[kt][4 * q], acc[kt][4 * q + 1]), (uint32_t)hpk[4][kt][q]),
+                                        mask2(bf2(acc[kt][4 * q + 2], acc[kt][4 * q + 3]), (uint32_t)(hpk[4][kt][q] >> 32)));
         }
         __syncthreads();
         // dW_4 (rows 0..15, columns 16 w .. 16 w + 15) and db_4 over the block step
 #pragma unroll
-        for (int ks = 0; ks < T_SUB / 32; ks++)
-            accW4 = mfma16(ld_frag(sGt + r16 * SST + 32 * ks + 8 * g16),
-                           ld_frag(sHt + (16 * w + r16) * SST + 32 * ks + 8 * g16), accW4);
-        dbacc[4] += bias_rows(MLP_OUTP);
+        for (int ks = 0; ks < T_SUB / 32; ks++) {
+            const bf16x8 a = ld_frag(sGt + r16 * SST + 32 * ks + 8 * g16);
+            accW4 = mfma16(a, ld_frag(sHt + (16 * w + r16) * SST + 32 * ks + 8 * g16), accW4);
+            if (w == 0) dbm[4] = mfma16(a, ones, dbm[4]);
+        }
         __syncthreads();
 #pragma unroll
         for (int l = 3; l >= 0; l--) {
@@ -424,8 +448,7 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
                 for (int jt = 0; jt < 2; jt++)
 #pragma unroll
                     for (int q = 0; q < 4; q++)
-#pragma unroll
-                        for (int p = 0; p < 4; p++) act[(8 * q + 4 * g + p) * AST + 32 * jt + r] = unpack4(gpk[jt][q], p);
+                        put_rows(act, (8 * q + 4 * g) * AST + 32 * jt + r, (uint32_t)gpk[jt][q], (uint32_t)(gpk[jt][q] >> 32));
                 wave_lds_order();
                 f32x16 acc[2];
 #pragma unroll
@@ -443,13 +466,10 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
 #pragma unroll
                 for (int kt = 0; kt < 2; kt++)
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        uint16_t v[4];
-#pragma unroll
-                        for (int p = 0; p < 4; p++)
-                            v[p] = unpack4(hpk[l][kt][q], p) ? bf(acc[kt][4 * q + p]) : (uint16_t)0;
-                        gpk[kt][q] = pack4(v);
-                    }
+                    for (int q = 0; q < 4; q++)
+                        gpk[kt][q] = pack64(mask2(bf2(acc[kt][4 * q], acc[kt][4 * q + 1]), (uint32_t)hpk[l][kt][q]),
+                                            mask2(bf2(acc[kt][4 * q + 2], acc[kt][4 * q + 3]),
+                                                  (uint32_t)(hpk[l][kt][q] >> 32)));
                 wave_lds_order();
             }
             __syncthreads();
@@ -457,7 +477,7 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
             for (int ks = 0; ks < T_SUB / 16; ks++)
                 accW[l] = mfma32(ld_frag(sGt + (32 * jt_w + r) * SST + 16 * ks + 8 * g),
                                  ld_frag(sHt + (32 * kt_w + r) * SST + 16 * ks + 8 * g), accW[l]);
-            dbacc[l] += bias_rows(64);
+            bias_rows(dbm[l]);
             __syncthreads();
         }
     }
@@ -473,17 +493,17 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
                 wp[l * 4096 + (32 * jt_w + 8 * q + 4 * g + p) * 64 + 32 * kt_w + r] = accW[l][4 * q + p];
 #pragma unroll
     for (int i = 0; i < 4; i++) wp[4 * 4096 + (4 * g16 + i) * 64 + 16 * w + r16] = accW4[i];
-    // bias partials: the four waves' sums of each feature, added in wave order
+    // bias partials: lane (0, g16) of wave w holds rows 16w + 4 g16 + i (every column of the product is the same)
+    if (r16 == 0) {
+        float* bp = bpart + (size_t)blockIdx.x * MLP_LAYERS * 64;
 #pragma unroll
-    for (int l = 0; l < MLP_LAYERS; l++) {
-        sRed[threadIdx.x] = dbacc[l];
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            const int j = threadIdx.x;
-            const float v = ((sRed[j] + sRed[64 + j]) + sRed[128 + j]) + sRed[192 + j];
-            bpart[((size_t)blockIdx.x * MLP_LAYERS + l) * 64 + j] = v;
+        for (int l = 0; l < 4; l++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) bp[l * 64 + 16 * w + 4 * g16 + i] = dbm[l][i];
+        if (w == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) bp[4 * 64 + 4 * g16 + i] = dbm[4][i];
         }
-        __syncthreads();
     }
     sRed[threadIdx.x] = lsum;
     __syncthreads();
@@ -495,28 +515,32 @@ __global__ void __launch_bounds__(MLP_BLOCK, 1) k_mlp_train(const float* in9, co
 }
 
 // Sum the per-block partials, Adam update, refresh the BF16 copies.  Block b
-// handles parameters p = 64 b + (t & 63); thread group t >> 6 (of 4) sums the
-// partials t >> 6, t >> 6 + 4, ... in order and the 4 group sums are added in
-// group order, so the reduction order depends on the partial count alone.
+// handles parameters p = 64 b + (t & 63); thread group t >> 6 (of 16) sums the
+// partials t >> 6, t >> 6 + 16, ... in order and the 16 group sums are added
+// in group order, so the reduction order depends on the partial count alone.
 // Partial c of weight p is wpart[c * 5 * 4096 + p], of bias q bpart[c * 5 * 64 + q].
 constexpr int ADAM_PARAMS = 64;
-__global__ void __launch_bounds__(256) k_mlp_adam(int nchunks, const float* wpart, const float* bpart, float* W, float* B,
+constexpr int ADAM_GROUPS = 16;
+constexpr int ADAM_BLOCK = ADAM_PARAMS * ADAM_GROUPS;
+__global__ void __launch_bounds__(ADAM_BLOCK) k_mlp_adam(int nchunks, const float* wpart, const float* bpart, float* W, float* B,
                                                   float* mW, float* vW, float* mB, float* vB, float lr, float b1, float b2,
                                                   float eps, float c1, float c2, int apply, float* grad_out, uint16_t* wf,
                                                   uint16_t* wt) {
-    __shared__ float sg[256];
+    __shared__ float sg[ADAM_BLOCK];
     const int t = threadIdx.x, grp = t >> 6;
     const int p = blockIdx.x * ADAM_PARAMS + (t & 63);
     float part = 0.0f;
     if (p < W_TOTAL) {
-        for (int c = grp; c < nchunks; c += 4) part += wpart[(size_t)c * (MLP_LAYERS * 4096) + p];
+        for (int c = grp; c < nchunks; c += ADAM_GROUPS) part += wpart[(size_t)c * (MLP_LAYERS * 4096) + p];
     } else if (p < W_TOTAL + B_TOTAL) {
-        for (int c = grp; c < nchunks; c += 4) part += bpart[(size_t)c * (MLP_LAYERS * 64) + (p - W_TOTAL)];
+        for (int c = grp; c < nchunks; c += ADAM_GROUPS) part += bpart[(size_t)c * (MLP_LAYERS * 64) + (p - W_TOTAL)];
     }
     sg[t] = part;
     __syncthreads();
     if (grp != 0) return;
-    const float gtot = ((sg[t] + sg[t + 64]) + sg[t + 128]) + sg[t + 192];
+    float gtot = sg[t];
+#pragma unroll
+    for (int k = 1; k < ADAM_GROUPS; k++) gtot += sg[t + 64 * k];
     if (p < W_TOTAL) {
         int l = (p < 4 * 4096) ? p / 4096 : 4;
         int o = p - w_off(l);
@@ -600,7 +624,7 @@ static MlpDev mlp_dev(MlpState* m) {
 // refresh BF16 copies without an update (apply = 0)
 static int refresh_copies(srt_ctx* c, MlpState* m) {
     const int tot = W_TOTAL + B_TOTAL;
-    k_mlp_adam<<<(tot + ADAM_PARAMS - 1) / ADAM_PARAMS, 256, 0, c->stream>>>(0, nullptr, nullptr, (float*)m->W.p, (float*)m->B.p, nullptr,
+    k_mlp_adam<<<(tot + ADAM_PARAMS - 1) / ADAM_PARAMS, ADAM_BLOCK, 0, c->stream>>>(0, nullptr, nullptr, (float*)m->W.p, (float*)m->B.p, nullptr,
                                                          nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0,
                                                          nullptr, (uint16_t*)m->wf.p, (uint16_t*)m->wt.p);
     return hip_check(c, hipGetLastError(), "mlp refresh");
@@ -664,7 +688,7 @@ static int train_step_dev(srt_ctx* c, MlpState* m, const float* in9, const float
     const float c2 = 1.0f - std::pow(m->b2, (float)std::max(m->step, 1));
     const int tot = W_TOTAL + B_TOTAL;
     prof_begin(c, "mlp_adam", &ev);
-    k_mlp_adam<<<(tot + ADAM_PARAMS - 1) / ADAM_PARAMS, 256, 0, st>>>(chunks, (const float*)m->wpart.p, (const float*)m->bpart.p, (float*)m->W.p,
+    k_mlp_adam<<<(tot + ADAM_PARAMS - 1) / ADAM_PARAMS, ADAM_BLOCK, 0, st>>>(chunks, (const float*)m->wpart.p, (const float*)m->bpart.p, (float*)m->W.p,
                                                   (float*)m->B.p, (float*)m->mW.p, (float*)m->vW.p, (float*)m->mB.p,
                                                   (float*)m->vB.p, m->lr, m->b1, m->b2, m->eps, c1, c2, apply,
                                                   (float*)m->grad.p, (uint16_t*)m->wf.p, (uint16_t*)m->wt.p);
