@@ -150,6 +150,18 @@ int srt_trace_rays(srt_ctx* ctx, srt_scene scene, int32_t n, const float* origin
  * the context (see srt_film_download). */
 int srt_render(srt_ctx* ctx, srt_scene scene, const srt_camera* camera, const srt_pt_params* params, int32_t width,
                int32_t height, float* rgba_out, int32_t out_on_device);
+
+/* ---- Simple NPR pass (replaces Simple::render + its OptiX programs,
+ * passes/npr/simple.cu:14-33, shader/npr/simple.cu:42-97) ----
+ * samples_per_pixel jittered camera rays per pixel (the path tracer's camera
+ * ray streams for the same seed), closest hit, and one first-hit colouring:
+ * type 0 Depth (distance from the camera scaled between the scene box's
+ * nearest and farthest distance, 0.2 .. 1.0), 1 Normal ((n + 1) / 2), 2
+ * BaseColor, 3 Ambient (base colour x (|n . d| / 2 + 1/2)), 4 FaceOrientation;
+ * misses are white.  Pixel = sum over samples of L / samples_per_pixel, alpha
+ * 1.  Output as for srt_render (rgba_out == NULL is an error here). */
+int srt_render_simple(srt_ctx* ctx, srt_scene scene, const srt_camera* camera, int32_t type, int32_t samples_per_pixel,
+                      uint32_t seed, int32_t width, int32_t height, float* rgba_out, int32_t out_on_device);
 int srt_film_download(srt_ctx* ctx, float* rgba_host, int64_t num_pixels);
 
 /* ---- post-processing (Accumulate::render accumulate.cu:10-29,

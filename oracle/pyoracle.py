@@ -45,6 +45,8 @@ def lib():
             "orc_trace": (None, [P, P, C.c_int, C.c_int, P, P, C.c_float, P, P, P, P, P, P, P]),
             "orc_camera_basis": (None, [P, P, P, C.c_float, C.c_int, C.c_int, P]),
             "orc_primary_rays": (None, [P, P, P, C.c_float, C.c_int, C.c_int, C.c_uint32, C.c_uint32, P, P]),
+            "orc_render_simple": (None, [P, P, P, P, P, C.c_float, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int,
+                                         P, P]),
             "orc_render": (C.c_double, [P, P, P, P, P, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, P, P]),
@@ -252,6 +254,19 @@ def render(scene: Scene, cam, W, H, spp, max_depth=16, rr_depth=4, use_nee=1, us
                             rr_depth, use_nee, use_mis, seed, 1 if fp64 else 0, threads, row_begin, row_step,
                             sample_begin, sample_end, _p(fo), _p(do))
     return out, secs
+
+
+SIMPLE = {"Depth": 0, "Normal": 1, "BaseColor": 2, "Ambient": 3, "FaceOrientation": 4}
+
+
+def render_simple(scene: Scene, cam, W, H, type="Depth", spp=16, seed=0, bvh: Bvh | None = None):
+    """Simple NPR pass (float).  Returns (image (H*W,4), (min_depth, max_depth))."""
+    out = np.zeros((W * H, 4), np.float32)
+    rng2 = np.zeros(2, np.float32)
+    t = SIMPLE[type] if isinstance(type, str) else int(type)
+    lib().orc_render_simple(scene.h, None if bvh is None else bvh.h, _p(_f3(cam["position"])), _p(_f3(cam["target"])),
+                            _p(_f3(cam["up"])), C.c_float(cam["fov"]), W, H, spp, seed, t, _p(out), _p(rng2))
+    return out, (float(rng2[0]), float(rng2[1]))
 
 
 def last_counts():

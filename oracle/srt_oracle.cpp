@@ -893,6 +893,84 @@ double orc_render(void* scene, void* bvh, const float* cam_pos, const float* cam
     return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// Simple NPR pass, float (shader/npr/simple.cu:42-97, passes/npr/simple.cu:14-33):
+// samples 0..spp-1 of each pixel take the path tracer's camera ray (same
+// stream), the closest hit, and one first-hit colouring (type 0 Depth, 1
+// Normal, 2 BaseColor, 3 Ambient, 4 FaceOrientation; misses white); the pixel
+// is the running sum of L / spp in sample order, alpha 1.  Depth scales the
+// camera distance between the vertex box's min_distance and max_distance
+// (aabb.h:24-41), returned in range2.
+void orc_render_simple(void* scene, void* bvh, const float* cam_pos, const float* cam_target, const float* cam_up,
+                       float fov, int W, int H, int spp, uint32_t seed, int type, float* out, float* range2) {
+    Scene& s = ((OrcScene*)scene)->s;
+    Tracer tr{&s, bvh ? &((OrcBvh*)bvh)->b : nullptr};
+    Camera cam = make_camera(cam_pos, cam_target, cam_up, fov, W, H);
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; a++) { lo[a] = s.pos[a]; hi[a] = s.pos[a]; }
+    for (int i = 0; i < s.nv; i++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], s.pos[3 * (size_t)i + a]);
+            hi[a] = std::max(hi[a], s.pos[3 * (size_t)i + a]);
+        }
+    bool inside = true;
+    V3<float> far3, near3;
+    float fa[3], na[3];
+    for (int a = 0; a < 3; a++) {
+        const float dlo = std::fabs(cam.pos[a] - lo[a]), dhi = std::fabs(cam.pos[a] - hi[a]);
+        fa[a] = std::fmax(dlo, dhi);
+        na[a] = std::fmin(dlo, dhi);
+        inside = inside && cam.pos[a] >= lo[a] && cam.pos[a] <= hi[a];
+    }
+    far3 = V3<float>(fa[0], fa[1], fa[2]);
+    near3 = V3<float>(na[0], na[1], na[2]);
+    const float max_depth = length(far3);
+    const float min_depth = inside ? 0.0f : length(near3);
+    if (range2) { range2[0] = min_depth; range2[1] = max_depth; }
+    V3<float> X = cvt<float>(cam.X), Y = cvt<float>(cam.Y), Z = cvt<float>(cam.Z), CP = cvt<float>(cam.pos);
+    for (int pixel = 0; pixel < W * H; pixel++) {
+        V3<float> result(0.0f);
+        for (int smp = 0; smp < spp; smp++) {
+            Pcg32 rng;
+            path_rng(rng, (uint32_t)pixel, seed, (uint32_t)smp);
+            float r0 = rng.next_float();
+            float xx = ((float)(pixel % W) + r0) / (float)W;
+            float r1 = rng.next_float();
+            float yy = ((float)(pixel / W) + r1) / (float)H;
+            float px = (xx - 0.5f) * cam.nw, py = (yy - 0.5f) * cam.nh;
+            V3<float> dir = normalize(V3<float>(px, py, 1.0f));
+            V3<float> zero(0.0f);
+            V3<float> ro = zero.x * X + zero.y * Y + zero.z * Z + CP;
+            V3<float> rd = dir.x * X + dir.y * Y + dir.z * Z;
+            float of[3] = {ro.x, ro.y, ro.z}, df[3] = {rd.x, rd.y, rd.z};
+            Hit h;
+            V3<float> L(1.0f);
+            if (tr.closest(of, df, 1e-3f, 1e16f, &h)) {
+                HitInfo<float> info;
+                get_hitinfo<float>(s, h.prim, h.u, h.v, info);
+                if (type == 0) {
+                    const float d = length(info.pos - CP);
+                    const float ratio = (d - min_depth) / (max_depth - min_depth);
+                    L = V3<float>(ratio * 0.8f + 0.2f);
+                } else if (type == 1) {
+                    L = (info.normal + V3<float>(1.0f)) * 0.5f;
+                } else if (type == 2) {
+                    L = info.color;
+                } else if (type == 3) {
+                    L = info.color * (std::fabs(dot(info.normal, rd)) * 0.5f + 0.5f);
+                } else {
+                    L = dot(info.normal, rd) > 0.0f ? V3<float>(1.0f, 0.85f, 0.0f) : V3<float>(0.34f, 0.73f, 0.76f);
+                }
+            }
+            const float fs = (float)spp;
+            result = V3<float>(result.x + L.x / fs, result.y + L.y / fs, result.z + L.z / fs);
+        }
+        out[4 * (size_t)pixel] = result.x;
+        out[4 * (size_t)pixel + 1] = result.y;
+        out[4 * (size_t)pixel + 2] = result.z;
+        out[4 * (size_t)pixel + 3] = 1.0f;
+    }
+}
+
 // closest-hit traces and shadow rays of the last orc_render
 void orc_last_counts(int64_t* out2) {
     out2[0] = g_last_counts[0];

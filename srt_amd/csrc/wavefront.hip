@@ -424,9 +424,28 @@ SRT_D uint32_t tile_pixel(uint32_t rank, uint32_t W, uint32_t H) {
     return (band * 8u + ly) * W + b * 8u + lx;
 }
 
-// One camera ray per slot (path_tracer.cu:66-68, camera.h:107-117, :27-32).
-// Slot i belongs to segment k: path id pid = path_begin + (i - slot_offset),
-// sample = pid / (W H), pixel = tile_pixel(pid mod (W H)).
+// Camera ray of one sample of one pixel (path_tracer.cu:66-68,
+// camera.h:107-117, :27-32): two jitter draws from the (pixel, sample)
+// stream; rng is left after them.
+SRT_D void camera_ray(const DevCamera& cam, int width, int height, uint32_t pixel, uint32_t seed, uint32_t sample, f3& ro,
+                      f3& rd, Pcg32& rng) {
+    const int ix = (int)(pixel % (uint32_t)width), iy = (int)(pixel / (uint32_t)width);
+    rng = path_rng(pixel, seed, sample);
+    float r0 = rng.next_float();
+    float xx = ((float)ix + r0) / (float)width;
+    float r1 = rng.next_float();
+    float yy = ((float)iy + r1) / (float)height;
+    float px = (xx - 0.5f) * cam.nw, py = (yy - 0.5f) * cam.nh;
+    f3 dir = normalize(mk(px, py, 1.0f));
+    f3 X = ld3(cam.X), Y = ld3(cam.Y), Z = ld3(cam.Z), CP = ld3(cam.pos);
+    f3 zero = mk(0.0f);
+    ro = zero.x * X + zero.y * Y + zero.z * Z + CP;
+    rd = dir.x * X + dir.y * Y + dir.z * Z;
+}
+
+// One camera ray per slot.  Slot i belongs to segment k: path id
+// pid = path_begin + (i - slot_offset), sample = pid / (W H),
+// pixel = tile_pixel(pid mod (W H)).
 __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* Lg, float* Lb, const Segment* segs, int nseg,
                                                 int n, uint32_t* counts) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -440,19 +459,9 @@ __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* L
     int64_t pid = sg.path_begin + (i - sg.slot_offset);
     uint32_t sample = (uint32_t)(pid / sg.wh);
     uint32_t pixel = tile_pixel((uint32_t)(pid - (int64_t)sample * sg.wh), (uint32_t)sg.width, (uint32_t)sg.height);
-    int ix = (int)(pixel % (uint32_t)sg.width), iy = (int)(pixel / (uint32_t)sg.width);
-    const DevCamera& cam = sg.cam;
-    Pcg32 rng = path_rng(pixel, sg.seed, sample);
-    float r0 = rng.next_float();
-    float xx = ((float)ix + r0) / (float)sg.width;
-    float r1 = rng.next_float();
-    float yy = ((float)iy + r1) / (float)sg.height;
-    float px = (xx - 0.5f) * cam.nw, py = (yy - 0.5f) * cam.nh;
-    f3 dir = normalize(mk(px, py, 1.0f));
-    f3 X = ld3(cam.X), Y = ld3(cam.Y), Z = ld3(cam.Z), CP = ld3(cam.pos);
-    f3 zero = mk(0.0f);
-    f3 ro = zero.x * X + zero.y * Y + zero.z * Z + CP;
-    f3 rd = dir.x * X + dir.y * Y + dir.z * Z;
+    f3 ro, rd;
+    Pcg32 rng;
+    camera_ray(sg.cam, sg.width, sg.height, pixel, sg.seed, sample, ro, rd, rng);
     q.ox[i] = ro.x; q.oy[i] = ro.y; q.oz[i] = ro.z;
     q.dx[i] = rd.x; q.dy[i] = rd.y; q.dz[i] = rd.z;
     q.br[i] = 1.0f; q.bg[i] = 1.0f; q.bb[i] = 1.0f;
@@ -571,6 +580,35 @@ SRT_D bool finite3(f3 v) {
 // Shading of one hit (queue entry i, hit record i): hit reconstruction,
 // emitter term with MIS, next-event estimation, the radiance-MLP hooks, BSDF
 // sampling and Russian roulette.  Appends go to the lanes active at the call.
+// First-hit geometry of sorted triangle tri at barycentrics (u, v)
+// (hit_info.h:27-65, identity instance transform): position, the geometric
+// normal turned to the side of the interpolated shading normal, the shading
+// normal, and the material id.
+struct HitGeom {
+    f3 pos, normal, sn;
+    float w;
+    int32_t matid;
+};
+SRT_D HitGeom hit_geometry(const DevScene& S, int32_t tri, float u, float v) {
+    HitGeom g;
+    const float4* tp = S.tris + 3 * (size_t)tri;
+    const float4 A = tp[0], B = tp[1], C = tp[2];
+    g.matid = __float_as_int(C.w);
+    const f3 v0 = mk(A.x, A.y, A.z), v1 = mk(B.x, B.y, B.z), v2 = mk(C.x, C.y, C.z);
+    g.w = 1.0f - u - v;
+    g.pos = v0 * g.w + v1 * u + v2 * v;
+    g.normal = normalize(cross(v1 - v0, v2 - v0));
+    g.sn = g.normal;
+    if (S.tri_sn) {
+        // the vertex normals of this triangle, copied in sorted order
+        const float4* q = S.tri_sn + 3 * (size_t)tri;
+        const float4 n0 = q[0], n1 = q[1], n2 = q[2];
+        g.sn = normalize(mk(n0.x, n0.y, n0.z) * g.w + mk(n1.x, n1.y, n1.z) * u + mk(n2.x, n2.y, n2.z) * v);
+    }
+    if (dot(g.normal, g.sn) < 0.0f) g.normal = -g.normal;
+    return g;
+}
+
 SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const PathQueue& qin, const HitBuf& h,
                      const PathQueue& qout, const ShadowQueue& sq, float* Lr, float* Lg, float* Lb, uint32_t* counts,
                      int qi, const MlpIO& mio, uint32_t i) {
@@ -582,24 +620,12 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     const bool specular = (flags & 1u) != 0u;
     const uint32_t segk = flags >> 8;
     const Segment& P = segs[segk];
-    // hit reconstruction (hit_info.h:27-65)
-    const float4* tp = S.tris + 3 * (size_t)tri;
-    const float4 A = tp[0], B = tp[1], C = tp[2];
-    const int32_t matid = __float_as_int(C.w);
-    const f3 v0 = mk(A.x, A.y, A.z), v1 = mk(B.x, B.y, B.z), v2 = mk(C.x, C.y, C.z);
     const float u = h.u[i], v = h.v[i];
-    const float w = 1.0f - u - v;
-    const f3 pos = v0 * w + v1 * u + v2 * v;
-    f3 normal = normalize(cross(v1 - v0, v2 - v0));
-    f3 sn = normal;
-    if (S.tri_sn) {
-        // the vertex normals of this triangle, copied in sorted order
-        const float4* q = S.tri_sn + 3 * (size_t)tri;
-        const float4 n0 = q[0], n1 = q[1], n2 = q[2];
-        sn = normalize(mk(n0.x, n0.y, n0.z) * w + mk(n1.x, n1.y, n1.z) * u + mk(n2.x, n2.y, n2.z) * v);
-    }
-    if (dot(normal, sn) < 0.0f) normal = -normal;
-    const MatRec& m = S.mats[matid];
+    const HitGeom hg = hit_geometry(S, tri, u, v);
+    const float w = hg.w;
+    const f3 pos = hg.pos, sn = hg.sn;
+    f3 normal = hg.normal;
+    const MatRec& m = S.mats[hg.matid];
     const f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
     if (mat_is_emissive(m)) {
         // emitter hit with MIS (path_tracer.cu:88-106)
@@ -906,6 +932,76 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, uint32_t
         if (io.any) trace_persistent<true, false, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
         else trace_persistent<false, false, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
     }
+}
+
+// camera rays of one sample for every pixel (index = pixel) -> hit buffer
+struct SimpleIO {
+    DevCamera cam;
+    int width, height;
+    uint32_t seed, sample;
+    HitBuf h;
+    SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
+        f3 ro, rd;
+        Pcg32 rng;
+        camera_ray(cam, width, height, i, seed, sample, ro, rd, rng);
+        trav_begin(st, ro, rd, 1e-3f, 1e16f, root);
+    }
+    SRT_D void store(uint32_t i, const TravState& st, bool) const {
+        const bool hit = st.best_pos >= 0;
+        h.t[i] = hit ? st.best_t : 0.0f;
+        h.u[i] = hit ? st.bu : 0.0f;
+        h.v[i] = hit ? st.bv : 0.0f;
+        h.tri[i] = hit ? st.best_pos : -1;
+    }
+};
+
+template <bool WIDE>
+__global__ void __launch_bounds__(kTraceBlock) k_simple_trace(DevScene S, uint32_t n, SimpleIO io, uint32_t* fetch,
+                                                             int32_t* gstack, TraceCfg cfg) {
+    const TravStack k = make_stack(gstack, cfg);
+    uint32_t nv = 0, nt = 0;
+    trace_persistent<false, false, WIDE>(S, n, fetch, cfg.refill, k, io, nv, nt);
+}
+
+// the Simple pass types (SimpleParams::Type, params/npr_params.h)
+enum SimpleType { kSimpleDepth = 0, kSimpleNormal = 1, kSimpleBaseColor = 2, kSimpleAmbient = 3, kSimpleFace = 4 };
+
+// first-hit colouring of pixel i's ray of sample io.sample, added as L / spp
+// to the pixel (simple.cu:62-95); misses are white
+__global__ void __launch_bounds__(256) k_simple_accum(DevScene S, SimpleIO io, int spp, int type, float min_depth,
+                                                     float max_depth, float4* acc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)io.width * (uint32_t)io.height) return;
+    f3 ro, rd;
+    Pcg32 rng;
+    camera_ray(io.cam, io.width, io.height, i, io.seed, io.sample, ro, rd, rng);
+    f3 L = mk(1.0f);
+    const int32_t tri = io.h.tri[i];
+    if (tri >= 0) {
+        const HitGeom hg = hit_geometry(S, tri, io.h.u[i], io.h.v[i]);
+        const MatRec& m = S.mats[hg.matid];
+        const f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
+        if (type == kSimpleDepth) {
+            const float d = length(hg.pos - ld3(io.cam.pos));
+            const float ratio = (d - min_depth) / (max_depth - min_depth);
+            L = mk(ratio * 0.8f + 0.2f);
+        } else if (type == kSimpleNormal) {
+            L = (hg.normal + mk(1.0f)) * 0.5f;
+        } else if (type == kSimpleBaseColor) {
+            L = color;
+        } else if (type == kSimpleAmbient) {
+            L = color * (fabsf(dot(hg.normal, rd)) * 0.5f + 0.5f);
+        } else {
+            L = dot(hg.normal, rd) > 0.0f ? mk(1.0f, 0.85f, 0.0f) : mk(0.34f, 0.73f, 0.76f);
+        }
+    }
+    const float fs = (float)spp;
+    float4 a = io.sample == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : acc[i];
+    a.x = a.x + L.x / fs;
+    a.y = a.y + L.y / fs;
+    a.z = a.z + L.z / fs;
+    a.w = 1.0f;
+    acc[i] = a;
 }
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
@@ -1272,6 +1368,55 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
     for (int k = 0; k < 4; k++) c->last_visits[k] = (int64_t)tot[2 + k];
     prof_collect(c);
     return SRT_OK;
+}
+
+// Simple NPR pass (shader/npr/simple.cu:42-97, passes/npr/simple.cu:14-33):
+// per sample, the camera rays of every pixel go through the persistent
+// traversal, then each pixel adds L / spp of its first-hit colouring, in
+// sample order.  min_depth / max_depth: the scene box's distances from the
+// camera (aabb.h:24-41, computed on the host as passes/npr/simple.cu:28-30).
+int render_simple(srt_ctx* c, SceneRec* s, const DevCamera& cam, int type, int spp, uint32_t seed, int width, int height,
+                  float* out_dev) {
+    const int64_t n = (int64_t)width * height;
+    if (int r = ensure_pool(c, n)) return r;
+    DevScene S = dev_scene(s);
+    const int grid = trace_grid(c, n);
+    if (int r = ensure_gstack(c, s, (int64_t)grid * kTraceBlock)) return r;
+    const TraceCfg cfg = trace_cfg(c);
+    float box_far[3], box_near[3];
+    bool inside = true;
+    for (int a = 0; a < 3; a++) {
+        const float dlo = std::fabs(cam.pos[a] - s->bounds_lo[a]), dhi = std::fabs(cam.pos[a] - s->bounds_hi[a]);
+        box_far[a] = std::fmax(dlo, dhi);
+        box_near[a] = std::fmin(dlo, dhi);
+        inside = inside && cam.pos[a] >= s->bounds_lo[a] && cam.pos[a] <= s->bounds_hi[a];
+    }
+    const float max_depth = std::sqrt(box_far[0] * box_far[0] + box_far[1] * box_far[1] + box_far[2] * box_far[2]);
+    const float min_depth =
+        inside ? 0.0f : std::sqrt(box_near[0] * box_near[0] + box_near[1] * box_near[1] + box_near[2] * box_near[2]);
+    SimpleIO io;
+    io.cam = cam;
+    io.width = width;
+    io.height = height;
+    io.seed = seed;
+    io.h.t = (float*)c->hits[0].p; io.h.u = (float*)c->hits[1].p; io.h.v = (float*)c->hits[2].p;
+    io.h.tri = (int32_t*)c->hits[3].p;
+    uint32_t* fetch = (uint32_t*)c->counters.p + CNT_FETCH_Q;
+    hipEvent_t ev;
+    prof_begin(c, "simple", &ev);
+    for (int smp = 0; smp < spp; smp++) {
+        io.sample = (uint32_t)smp;
+        k_set_u32<<<1, 1, 0, c->stream>>>(fetch, 0u);
+        if (cfg.wide) k_simple_trace<true><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch,
+                                                                                                  (int32_t*)c->gstack.p, cfg);
+        else k_simple_trace<false><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch,
+                                                                                          (int32_t*)c->gstack.p, cfg);
+        k_simple_accum<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(S, io, spp, type, min_depth, max_depth,
+                                                                          (float4*)out_dev);
+    }
+    prof_end(c, "simple", ev, n * spp);
+    if (int r = hip_check(c, hipGetLastError(), "simple launch")) return r;
+    return hip_check(c, hipStreamSynchronize(c->stream), "simple");
 }
 
 int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin, const float* tmax_dev,

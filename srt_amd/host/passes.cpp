@@ -70,6 +70,41 @@ Json PathTracer::params() const {
 // ---------------------------------------------------------------------------
 // Accumulate (accumulate.cu:10-29)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Simple (passes/npr/simple.cu:14-33)
+// ---------------------------------------------------------------------------
+REGISTER_RENDER_PASS_CPP(Simple);
+
+void Simple::init() {
+    if (!scene || !scene->ctx || !scene->handle) throw RenderError("Simple needs a scene uploaded to the device");
+}
+
+void Simple::render(std::shared_ptr<Film> film) {
+    if (!enable) return;
+    srt_camera cam{};
+    for (int i = 0; i < 3; i++) {
+        cam.position[i] = scene->camera->position[i];
+        cam.target[i] = scene->camera->target[i];
+        cam.up[i] = scene->camera->up[i];
+    }
+    cam.fov_deg = scene->camera->fov;
+    const uint32_t s = seed + frame;
+    frame++;
+    check_srt(scene->ctx,
+              srt_render_simple(scene->ctx, scene->handle, &cam, (int32_t)type, samples_per_pixel, s, width, height,
+                                film->get_pixels(), 1),
+              "Simple");
+}
+
+Json Simple::params() const {
+    static const char* names[] = {"Depth", "Normal", "BaseColor", "Ambient", "FaceOrientation"};
+    Json j = Json::object();
+    j["type"] = Json(std::string(names[(int)type]));
+    j["samples_per_pixel"] = Json(samples_per_pixel);
+    j["seed"] = Json((double)seed);
+    return j;
+}
+
 REGISTER_RENDER_PASS_CPP(Accumulate);
 
 Accumulate::~Accumulate() {

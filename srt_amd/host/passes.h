@@ -2,6 +2,7 @@
 //
 //   PathTracer  src/passes/integrator/path_tracer.{h,cu}  (megakernel pass)
 //   Wavefront   src/passes/wavefront/wavefront.{h,cu}     (stage-per-kernel pass)
+//   Simple      src/passes/npr/simple.{h,cu}                (first-hit NPR pass)
 //   Accumulate  src/passes/postprocess/accumulate.{h,cu}
 //   ToneMapping src/passes/postprocess/tonemapping.{h,cu}
 //
@@ -51,6 +52,44 @@ class Wavefront : public PathTracer {
 
 public:
     friend void from_json(const Json& j, Wavefront& p) { from_json(j, (PathTracer&)p); }
+};
+
+// Simple: `type` Depth | Normal | BaseColor | Ambient | FaceOrientation
+// (unknown names read as Depth, simple.h:24-31), samples_per_pixel 16; frame k
+// renders with seed + k (the reference draws a random seed per frame).
+class Simple : public RenderPass {
+public:
+    REGISTER_RENDER_PASS(Simple);
+    enum class Type { Depth, Normal, BaseColor, Ambient, FaceOrientation };
+
+private:
+    Type type{Type::Depth};
+    int samples_per_pixel{16};
+    uint32_t seed{0};
+    uint32_t frame{0};
+
+public:
+    Simple() {}
+    void init() override;
+    void render(std::shared_ptr<Film> film) override;
+    void update() override {}          // the scene refits its shared BVH in Scene::update
+    Json params() const override;
+
+    static Type string_to_type(const std::string& str) {
+        if (str == "Depth") return Type::Depth;
+        if (str == "Normal") return Type::Normal;
+        if (str == "BaseColor") return Type::BaseColor;
+        if (str == "Ambient") return Type::Ambient;
+        if (str == "FaceOrientation") return Type::FaceOrientation;
+        return Type::Depth;
+    }
+
+    friend void from_json(const Json& j, Simple& p) {
+        if (j.is_null()) return;
+        p.type = Simple::string_to_type(j.value("type", "Depth"));
+        p.samples_per_pixel = j.value("samples_per_pixel", 16);
+        p.seed = (uint32_t)j.value("seed", 0);
+    }
 };
 
 class Accumulate : public RenderPass {

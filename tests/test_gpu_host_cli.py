@@ -55,6 +55,23 @@ def test_image_pathtracer_matches_c_abi(ctx, tmp_path):
     assert np.array_equal(img, film_to_image(ref, W, H))
 
 
+def test_image_simple_pass_matches_c_abi(ctx, tmp_path):
+    s = scenes.cornell()
+    scenes.save_srtscene(s, str(tmp_path / "cornell.srtscene"))
+    W, H = 80, 60
+    cfg = {"render": {"type": "image", "output": "simple.pfm", "resolution": [W, H]},
+           "passes": [{"name": "Simple", "params": {"type": "Ambient", "samples_per_pixel": 4, "seed": 5}}],
+           "scene": {"model": "cornell.srtscene",
+                     "camera": {"type": "perspective", "position": list(s.camera["position"]),
+                                "target": list(s.camera["target"]), "up": list(s.camera["up"]), "fov": 40}}}
+    run(cfg, tmp_path)
+    img = read_pfm(tmp_path / "simple.pfm")
+    h = ctx.upload(s)
+    ref = ctx.render_simple(h, s.camera, W, H, type="Ambient", spp=4, seed=5)
+    ctx.release(h)
+    assert np.array_equal(img, film_to_image(ref, W, H))
+
+
 def test_interactive_accumulate_tonemap(ctx, tmp_path):
     s = scenes.cornell()
     scenes.save_srtscene(s, str(tmp_path / "c.srtscene"))

@@ -1,6 +1,6 @@
 """The C++ pass layer mirrors the reference's RenderPass interface: pass names,
 JSON keys and defaults (path_tracer.h:34-38, tonemapping.h:39-41,
-wavefront.h:66-70), unknown-pass errors, and config parsing with comments
+wavefront.h:66-70, simple.h:33-39), unknown-pass errors, and config parsing with comments
 (main.cu:5-17).  CPU only: no device is touched."""
 import ctypes
 import json
@@ -46,6 +46,17 @@ def test_tonemapping_defaults_and_names(host):
         assert json.loads(params(host, "ToneMapping", {"type": t, "exposure": 1.5})[1])["type"] == t
     # unknown operator names fall back to None (tonemapping.h:27-35)
     assert json.loads(params(host, "ToneMapping", {"type": "Filmic"})[1])["type"] == "None"
+
+
+def test_simple_defaults_and_names(host):
+    r, out = params(host, "Simple")
+    assert r > 0
+    assert json.loads(out) == {"type": "Depth", "samples_per_pixel": 16, "seed": 0}
+    for t in ["Normal", "BaseColor", "Ambient", "FaceOrientation"]:
+        assert json.loads(params(host, "Simple", {"type": t, "samples_per_pixel": 4})[1]) == \
+            {"type": t, "samples_per_pixel": 4, "seed": 0}
+    # unknown names read as Depth (simple.h:24-31)
+    assert json.loads(params(host, "Simple", {"type": "Curvature"})[1])["type"] == "Depth"
 
 
 def test_accumulate_and_unknown_pass(host):
