@@ -170,6 +170,13 @@ int srt_film_download(srt_ctx* ctx, float* rgba_host, int64_t num_pixels);
 int srt_tonemap(srt_ctx* ctx, float* rgba, int64_t num_pixels, int32_t type, float exposure, int32_t use_gamma,
                 int32_t on_device);
 int srt_accumulate(srt_ctx* ctx, float* acc, float* rgba, int64_t num_pixels, int32_t frame_count, int32_t on_device);
+/* Denoise pass (replaces Denoise::render, passes/postprocess/denoise.cu:54-114,
+ * which runs the closed OptiX HDR denoiser on the colour alone).  In place on
+ * width*height float4 pixels: five passes of the edge-avoiding a-trous
+ * wavelet filter (Dammertz et al. 2010, B3-spline taps at spacing 1..16, the
+ * colour-difference weight on log(1 + c)); alpha is kept.
+ * on_device as srt_accumulate. */
+int srt_denoise(srt_ctx* ctx, float* rgba, int32_t width, int32_t height, int32_t on_device);
 
 /* ---- radiance MLP (replaces the NeuralRadiosity network, src/passes/npr/
  *      neural_radiosity.cu:6-52, whose training is a stub at :165-169) ----

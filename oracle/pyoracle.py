@@ -47,6 +47,7 @@ def lib():
             "orc_primary_rays": (None, [P, P, P, C.c_float, C.c_int, C.c_int, C.c_uint32, C.c_uint32, P, P]),
             "orc_render_simple": (None, [P, P, P, P, P, C.c_float, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int,
                                          P, P]),
+            "orc_denoise": (None, [C.c_int, C.c_int, P]),
             "orc_render": (C.c_double, [P, P, P, P, P, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, P, P]),
@@ -267,6 +268,13 @@ def render_simple(scene: Scene, cam, W, H, type="Depth", spp=16, seed=0, bvh: Bv
     lib().orc_render_simple(scene.h, None if bvh is None else bvh.h, _p(_f3(cam["position"])), _p(_f3(cam["target"])),
                             _p(_f3(cam["up"])), C.c_float(cam["fov"]), W, H, spp, seed, t, _p(out), _p(rng2))
     return out, (float(rng2[0]), float(rng2[1]))
+
+
+def denoise(rgba, W, H):
+    """The a-trous denoiser (float), on a copy."""
+    a = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4).copy()
+    lib().orc_denoise(W, H, _p(a))
+    return a
 
 
 def last_counts():

@@ -971,6 +971,50 @@ void orc_render_simple(void* scene, void* bvh, const float* cam_pos, const float
     }
 }
 
+// Denoise pass of this build (the reference's is the closed OptiX denoiser,
+// passes/postprocess/denoise.cu:54-114): edge-avoiding a-trous wavelet filter
+// (Dammertz et al., HPG 2010) on the colour alone, in place: 5 passes of the
+// B3-spline kernel (1, 4, 6, 4, 1) / 16 at tap spacing 2^i, tap weight
+// exp(-|g_p - g_q|^2 / sigma_i^2) on g = log(1 + c) per channel of the pass's
+// input, sigma_i = 0.2 * 2^-i; taps outside the image skipped; alpha kept.
+void orc_denoise(int W, int H, float* rgba) {
+    const float h[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+    std::vector<float> a(rgba, rgba + (size_t)W * H * 4), b((size_t)W * H * 4);
+    std::vector<V3<float>> gbuf((size_t)W * H);
+    for (int it = 0; it < 5; it++) {
+        for (size_t p = 0; p < (size_t)W * H; p++)
+            gbuf[p] = V3<float>(pt_logf(1.0f + a[4 * p]), pt_logf(1.0f + a[4 * p + 1]), pt_logf(1.0f + a[4 * p + 2]));
+        auto guide = [&](const float* c) { return gbuf[(size_t)(c - a.data()) / 4]; };
+        const float sigma = 0.2f * std::ldexp(1.0f, -it);
+        const float inv_s2 = 1.0f / (sigma * sigma);
+        const int step = 1 << it;
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++) {
+                const float* cp = &a[4 * ((size_t)y * W + x)];
+                const V3<float> gp = guide(cp);
+                V3<float> sum(0.0f);
+                float wsum = 0.0f;
+                for (int j = -2; j <= 2; j++) {
+                    const int yy = y + j * step;
+                    if (yy < 0 || yy >= H) continue;
+                    for (int i = -2; i <= 2; i++) {
+                        const int xx = x + i * step;
+                        if (xx < 0 || xx >= W) continue;
+                        const float* cq = &a[4 * ((size_t)yy * W + xx)];
+                        const V3<float> d = gp - guide(cq);
+                        const float w = (h[j + 2] * h[i + 2]) * pt_expf(-(dot(d, d) * inv_s2));
+                        sum = sum + V3<float>(cq[0], cq[1], cq[2]) * w;
+                        wsum = wsum + w;
+                    }
+                }
+                float* o = &b[4 * ((size_t)y * W + x)];
+                o[0] = sum.x / wsum; o[1] = sum.y / wsum; o[2] = sum.z / wsum; o[3] = cp[3];
+            }
+        a.swap(b);
+    }
+    std::copy(a.begin(), a.end(), rgba);
+}
+
 // closest-hit traces and shadow rays of the last orc_render
 void orc_last_counts(int64_t* out2) {
     out2[0] = g_last_counts[0];

@@ -26,6 +26,7 @@ EXPORTED = [
     "srt_scene_build_bvh", "srt_scene_bvh_info", "srt_scene_bvh_download",
     "srt_scene_set_skin", "srt_scene_skin_refit", "srt_scene_download_vertices",
     "srt_trace_rays", "srt_render", "srt_render_simple", "srt_film_download", "srt_tonemap", "srt_accumulate",
+    "srt_denoise",
     "srt_stats", "srt_stats_reset", "srt_last_counts", "srt_last_visits",
     "srt_device_alloc", "srt_device_free", "srt_memcpy",
     "srt_mlp_num_params", "srt_mlp_create", "srt_mlp_set_params", "srt_mlp_get_params", "srt_mlp_set_hparams",
@@ -114,6 +115,7 @@ def load_library(path: str = LIB_PATH):
         "srt_film_download": (C.c_int, [P, P, C.c_int64]),
         "srt_tonemap": (C.c_int, [P, P, C.c_int64, C.c_int32, C.c_float, C.c_int32, C.c_int32]),
         "srt_accumulate": (C.c_int, [P, P, P, C.c_int64, C.c_int32, C.c_int32]),
+        "srt_denoise": (C.c_int, [P, P, C.c_int32, C.c_int32, C.c_int32]),
         "srt_stats": (C.c_int, [P, P, C.c_int32, P, P, P, C.c_int32]),
         "srt_stats_reset": (None, [P]),
         "srt_last_counts": (C.c_int, [P, P, P]),
@@ -336,6 +338,11 @@ class Context:
         a = np.ascontiguousarray(rgba, np.float32).copy()
         t = TONEMAP[type] if isinstance(type, str) else int(type)
         self.check(self.lib.srt_tonemap(self.h, _ptr(a), a.shape[0], t, C.c_float(exposure), 1 if use_gamma else 0, 0))
+        return a
+
+    def denoise(self, rgba, width, height):
+        a = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4).copy()
+        self.check(self.lib.srt_denoise(self.h, _ptr(a), width, height, 0))
         return a
 
     def accumulate(self, acc, rgba, frame_count):

@@ -628,6 +628,26 @@ int srt_accumulate(srt_ctx* c, float* acc, float* rgba, int64_t n, int32_t frame
     return r;
 }
 
+int srt_denoise(srt_ctx* c, float* rgba, int32_t W, int32_t H, int32_t on_device) {
+    SRT_IDLE(c);
+    if (!c || !rgba || W <= 0 || H <= 0) return SRT_E_INVALID;
+    const int64_t n = (int64_t)W * H;
+    if (n > (1ll << 28)) return set_err(c, SRT_E_LIMIT, "resolution too large");
+    DevBuf tmp, tp;
+    int r = dev_alloc(c, tmp, (size_t)n * 32);
+    if (r) return r;
+    if (on_device) r = denoise_dev(c, rgba, (float*)tmp.p, W, H);
+    else {
+        r = post_buffer(c, rgba, n, tp);
+        if (!r) r = denoise_dev(c, (float*)tp.p, (float*)tmp.p, W, H);
+        if (!r) r = hip_check(c, hipMemcpy(rgba, tp.p, (size_t)n * 16, hipMemcpyDeviceToHost), "download");
+        dev_free(c, tp);
+    }
+    dev_free(c, tmp);
+    prof_collect(c);
+    return r;
+}
+
 int srt_device_alloc(srt_ctx* c, size_t bytes, void** out) {
     SRT_IDLE(c);
     if (!c || !out) return SRT_E_INVALID;

@@ -288,6 +288,7 @@ int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* 
                int any, float* t, float* u, float* v, int32_t* prim);
 int tonemap_dev(srt_ctx* c, float* rgba, int n, int type, float exposure, int use_gamma);
 int accumulate_dev(srt_ctx* c, float* acc, float* rgba, int n, int frame_count);
+int denoise_dev(srt_ctx* c, float* rgba, float* tmp, int W, int H);
 DevScene dev_scene(const SceneRec* s);
 DevCamera make_camera(const float pos[3], const float target[3], const float up[3], float fov, int W, int H);
 }  // namespace srt

@@ -1056,6 +1056,53 @@ __global__ void k_accumulate(float4* acc, float4* px, int64_t n, int frame_count
     px[i] = r;
 }
 
+// Denoise pass.  The reference runs the OptiX HDR denoiser (a closed network)
+// on the colour alone, no albedo/normal guide layers
+// (passes/postprocess/denoise.cu:54-114); this pass is the edge-avoiding
+// a-trous wavelet filter of Dammertz et al. (HPG 2010) with the same single
+// input: kDenoisePasses passes of the B3-spline kernel (1, 4, 6, 4, 1) / 16 at
+// tap spacing 2^i, each tap weighted by exp(-|g_p - g_q|^2 / sigma_i^2) on the
+// log colour g = log(1 + c) of the pass's input (HDR-robust: a difference is
+// a ratio), sigma_i = kDenoiseSigma 2^-i; taps outside the image are skipped
+// and the weights renormalised.  Alpha is kept.
+constexpr int kDenoisePasses = 5;
+constexpr float kDenoiseSigma = 0.2f;
+
+// the edge-stopping guide of one pass: log(1 + c) per channel
+__global__ void __launch_bounds__(256) k_denoise_guide(const float4* in, float4* g, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = in[i];
+    g[i] = make_float4(pt_logf(1.0f + c.x), pt_logf(1.0f + c.y), pt_logf(1.0f + c.z), 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_atrous(const float4* in, const float4* g, float4* out, int W, int H, int step,
+                                               float inv_s2) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
+    const float h[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+    const float4 cp = in[(size_t)y * W + x];
+    const float4 g4 = g[(size_t)y * W + x];
+    const f3 gp = mk(g4.x, g4.y, g4.z);
+    f3 sum = mk(0.0f);
+    float wsum = 0.0f;
+    for (int j = -2; j <= 2; j++) {
+        const int yy = y + j * step;
+        if (yy < 0 || yy >= H) continue;
+        for (int i = -2; i <= 2; i++) {
+            const int xx = x + i * step;
+            if (xx < 0 || xx >= W) continue;
+            const float4 cq = in[(size_t)yy * W + xx];
+            const float4 gq = g[(size_t)yy * W + xx];
+            const f3 d = gp - mk(gq.x, gq.y, gq.z);
+            const float w = (h[j + 2] * h[i + 2]) * pt_expf(-(dot(d, d) * inv_s2));
+            sum = sum + mk(cq.x, cq.y, cq.z) * w;
+            wsum = wsum + w;
+        }
+    }
+    out[(size_t)y * W + x] = make_float4(sum.x / wsum, sum.y / wsum, sum.z / wsum, cp.w);
+}
+
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
@@ -1444,6 +1491,31 @@ int tonemap_dev(srt_ctx* c, float* rgba, int n, int type, float exposure, int us
     prof_end(c, "tonemap", ev, n);
     if (int r = hip_check(c, hipGetLastError(), "tonemap launch")) return r;
     return hip_check(c, hipStreamSynchronize(c->stream), "tonemap");
+}
+
+// in place on rgba (W x H float4); tmp: 2 W x H float4 scratch (ping-pong image, guide)
+int denoise_dev(srt_ctx* c, float* rgba, float* tmp, int W, int H) {
+    const dim3 grid((unsigned)((W + 15) / 16), (unsigned)((H + 15) / 16));
+    const int64_t n = (int64_t)W * H;
+    hipEvent_t ev;
+    prof_begin(c, "denoise", &ev);
+    float4* a = (float4*)rgba;
+    float4* b = (float4*)tmp;
+    float4* g = (float4*)tmp + n;
+    for (int it = 0; it < kDenoisePasses; it++) {
+        const float sigma = kDenoiseSigma * std::ldexp(1.0f, -it);
+        const float inv_s2 = 1.0f / (sigma * sigma);
+        k_denoise_guide<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(a, g, n);
+        k_atrous<<<grid, 256, 0, c->stream>>>(a, g, b, W, H, 1 << it, inv_s2);
+        std::swap(a, b);
+    }
+    // an odd pass count leaves the result in tmp
+    if (a != (float4*)rgba)
+        if (int r = hip_check(c, hipMemcpyAsync(rgba, a, (size_t)W * H * 16, hipMemcpyDeviceToDevice, c->stream), "denoise"))
+            return r;
+    prof_end(c, "denoise", ev, (int64_t)W * H);
+    if (int r = hip_check(c, hipGetLastError(), "denoise launch")) return r;
+    return hip_check(c, hipStreamSynchronize(c->stream), "denoise");
 }
 
 int accumulate_dev(srt_ctx* c, float* acc, float* rgba, int n, int frame_count) {

@@ -136,6 +136,16 @@ Json Accumulate::params() const { return Json::object(); }
 // ---------------------------------------------------------------------------
 // ToneMapping (tonemapping.cu:5-70)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Denoise (passes/postprocess/denoise.cu:54-114)
+// ---------------------------------------------------------------------------
+REGISTER_RENDER_PASS_CPP(Denoise);
+
+void Denoise::render(std::shared_ptr<Film> film) {
+    if (!enable) return;
+    check_srt(scene->ctx, srt_denoise(scene->ctx, film->get_pixels(), width, height, 1), "Denoise");
+}
+
 REGISTER_RENDER_PASS_CPP(ToneMapping);
 
 void ToneMapping::render(std::shared_ptr<Film> film) {

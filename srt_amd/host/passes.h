@@ -4,6 +4,7 @@
 //   Wavefront   src/passes/wavefront/wavefront.{h,cu}     (stage-per-kernel pass)
 //   Simple      src/passes/npr/simple.{h,cu}                (first-hit NPR pass)
 //   Accumulate  src/passes/postprocess/accumulate.{h,cu}
+//   Denoise     src/passes/postprocess/denoise.{h,cu}       (a-trous filter in place of the OptiX denoiser)
 //   ToneMapping src/passes/postprocess/tonemapping.{h,cu}
 //
 // PathTracer and Wavefront read the same JSON keys with the same defaults
@@ -106,6 +107,16 @@ public:
     int get_frame_count() const { return frame_count; }
     Json params() const override;
     friend void from_json(const Json&, Accumulate&) {}
+};
+
+// Denoise: no parameters (denoise.h:31); filters the film in place.
+class Denoise : public RenderPass {
+    REGISTER_RENDER_PASS(Denoise);
+
+public:
+    Denoise() {}
+    void render(std::shared_ptr<Film> film) override;
+    friend void from_json(const Json&, Denoise&) {}
 };
 
 class ToneMapping : public RenderPass {

@@ -1,0 +1,47 @@
+"""GPU denoiser (srt_denoise) against the oracle's restatement of the same
+a-trous filter, bit for bit, on a noisy path-traced Cornell frame; and the
+filter lowering the frame's error against the FP64 4096-spp golden image."""
+import os
+
+import numpy as np
+import pytest
+
+from srt_amd import scenes
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cornell_256x256_fp64_4096spp.npy")
+
+
+def test_matches_oracle_bit_for_bit(ctx):
+    s = scenes.cornell()
+    h = ctx.upload(s)
+    W, H = 96, 72
+    img = ctx.render(h, s.camera, W, H, 2, 5, 4, 1, 1, 11)
+    ctx.release(h)
+    gpu = ctx.denoise(img, W, H)
+    ref = O.denoise(img, W, H)
+    assert np.array_equal(gpu, ref), np.abs(gpu - ref).max()
+    assert np.array_equal(gpu[:, 3], img[:, 3])
+
+
+def test_lowers_error_against_the_fp64_golden(ctx):
+    s = scenes.cornell()
+    h = ctx.upload(s)
+    W = H = 256
+    img = ctx.render(h, s.camera, W, H, 16, 5, 4, 1, 1, 21)
+    ctx.release(h)
+    gold = np.load(GOLD)
+    raw = np.sqrt(np.mean((img[:, :3].astype(np.float64) - gold) ** 2))
+    den = np.sqrt(np.mean((ctx.denoise(img, W, H)[:, :3].astype(np.float64) - gold) ** 2))
+    # the Cornell frame's error is mostly bright, isolated paths that an edge-avoiding filter keeps:
+    # the gain is modest (0.925 of the raw error on the oracle's 16-spp frame)
+    assert den < 0.97 * raw, (raw, den)
+
+
+def test_odd_sizes_and_device_film(ctx):
+    rng = np.random.default_rng(4)
+    W, H = 17, 5
+    a = np.ones((W * H, 4), np.float32)
+    a[:, :3] = rng.gamma(2.0, 0.3, size=(W * H, 3))
+    assert np.array_equal(ctx.denoise(a, W, H), O.denoise(a, W, H))

@@ -59,10 +59,11 @@ def test_simple_defaults_and_names(host):
     assert json.loads(params(host, "Simple", {"type": "Curvature"})[1])["type"] == "Depth"
 
 
-def test_accumulate_and_unknown_pass(host):
+def test_accumulate_denoise_and_unknown_pass(host):
     assert params(host, "Accumulate")[0] >= 0
-    r, out = params(host, "Denoise")
-    assert r == -1 and "Render pass Denoise not found!" in out
+    assert params(host, "Denoise")[0] >= 0
+    r, out = params(host, "Bloom")
+    assert r == -1 and "Render pass Bloom not found!" in out
 
 
 def test_config_comments_and_bad_json(host):
