@@ -162,6 +162,18 @@ int srt_render(srt_ctx* ctx, srt_scene scene, const srt_camera* camera, const sr
  * 1.  Output as for srt_render (rgba_out == NULL is an error here). */
 int srt_render_simple(srt_ctx* ctx, srt_scene scene, const srt_camera* camera, int32_t type, int32_t samples_per_pixel,
                       uint32_t seed, int32_t width, int32_t height, float* rgba_out, int32_t out_on_device);
+
+/* ---- LightTracer pass (replaces LightTracer::render + its OptiX programs,
+ * passes/integrator/light_tracer.cu:13-37, shader/integrator/light_tracer.cu:57-176) ----
+ * width*height*samples_per_pixel particles from the area lights, splatted
+ * through the pinhole camera by camera connections at non-specular vertices
+ * (the reference's weights, aspect ratio and previous-direction pdf
+ * included); uses samples_per_pixel, max_depth, rr_depth and seed of params.
+ * The film is cleared to (0, 0, 0, 1) first.  Splats are float atomic adds:
+ * a pixel's value can differ in the last bits between runs.  A scene without
+ * area lights is an error.  Output as for srt_render (rgba_out != NULL). */
+int srt_render_light(srt_ctx* ctx, srt_scene scene, const srt_camera* camera, const srt_pt_params* params, int32_t width,
+                     int32_t height, float* rgba_out, int32_t out_on_device);
 int srt_film_download(srt_ctx* ctx, float* rgba_host, int64_t num_pixels);
 
 /* ---- post-processing (Accumulate::render accumulate.cu:10-29,

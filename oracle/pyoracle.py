@@ -48,6 +48,9 @@ def lib():
             "orc_render_simple": (None, [P, P, P, P, P, C.c_float, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int,
                                          P, P]),
             "orc_denoise": (None, [C.c_int, C.c_int, P]),
+            "orc_camera_pixel": (None, [P, P, P, C.c_float, C.c_int, C.c_int, C.c_int, P, P]),
+            "orc_render_light": (None, [P, P, P, P, P, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_uint32, P]),
             "orc_render": (C.c_double, [P, P, P, P, P, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, P, P]),
@@ -275,6 +278,23 @@ def denoise(rgba, W, H):
     a = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4).copy()
     lib().orc_denoise(W, H, _p(a))
     return a
+
+
+def camera_pixel(cam, W, H, dirs):
+    """Pixel index of each direction pointing away from the image (-1: outside)."""
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros(d.shape[0], np.int32)
+    lib().orc_camera_pixel(_p(_f3(cam["position"])), _p(_f3(cam["target"])), _p(_f3(cam["up"])),
+                           C.c_float(cam["fov"]), W, H, d.shape[0], _p(d), _p(out))
+    return out
+
+
+def render_light(scene: Scene, cam, W, H, spp=1, max_depth=16, rr_depth=4, seed=0, bvh: Bvh | None = None):
+    """LightTracer pass (float), splats in (launch index, sample, depth) order."""
+    out = np.zeros((W * H, 4), np.float32)
+    lib().orc_render_light(scene.h, None if bvh is None else bvh.h, _p(_f3(cam["position"])), _p(_f3(cam["target"])),
+                           _p(_f3(cam["up"])), C.c_float(cam["fov"]), W, H, spp, max_depth, rr_depth, seed, _p(out))
+    return out
 
 
 def last_counts():

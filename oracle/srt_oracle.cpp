@@ -1015,6 +1015,124 @@ void orc_denoise(int W, int H, float* rgba) {
     std::copy(a.begin(), a.end(), rgba);
 }
 
+// camera.h:143-148 get_xy and the pixel index of light_tracer.cu:100-101
+// (x < 1 that rounds x * W up to W goes to the last column / row)
+static bool camera_pixel(const Camera& cam, V3<float> dir, int W, int H, int* idx) {
+    const V3<float> d = -dir;
+    const V3<float> l(dot(d, cvt<float>(cam.X)), dot(d, cvt<float>(cam.Y)), dot(d, cvt<float>(cam.Z)));
+    const float px = l.x / l.z, py = l.y / l.z;
+    const float x = px / cam.nw + 0.5f, y = py / cam.nh + 0.5f;
+    if (!(x >= 0.0f && x < 1.0f && y >= 0.0f && y < 1.0f)) return false;
+    const int ix = std::min((int)(x * (float)W), W - 1), iy = std::min((int)(y * (float)H), H - 1);
+    *idx = ix + iy * W;
+    return true;
+}
+
+// pixel index of each direction (-1 outside the image): the inverse of the camera rays
+void orc_camera_pixel(const float* pos, const float* target, const float* up, float fov, int W, int H, int n,
+                      const float* dirs, int32_t* out) {
+    Camera cam = make_camera(pos, target, up, fov, W, H);
+    for (int i = 0; i < n; i++) {
+        int idx = -1;
+        if (!camera_pixel(cam, V3<float>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), W, H, &idx)) idx = -1;
+        out[i] = idx;
+    }
+}
+
+// LightTracer, float (shader/integrator/light_tracer.cu:57-176): particle
+// (launch index idx, sample) uses the (idx, seed, sample) stream; splats are
+// added in (idx, sample, depth) order into out (float4, cleared to (0,0,0,1)).
+void orc_render_light(void* scene, void* bvh, const float* cam_pos, const float* cam_target, const float* cam_up,
+                      float fov, int W, int H, int spp, int max_depth, int rr_depth, uint32_t seed, float* out) {
+    Scene& s = ((OrcScene*)scene)->s;
+    Tracer tr{&s, bvh ? &((OrcBvh*)bvh)->b : nullptr};
+    Camera cam = make_camera(cam_pos, cam_target, cam_up, fov, W, H);
+    const V3<float> CP = cvt<float>(cam.pos);
+    const float aspect = (float)W / (float)H;
+    for (int i = 0; i < W * H; i++) { out[4 * i] = 0.0f; out[4 * i + 1] = 0.0f; out[4 * i + 2] = 0.0f; out[4 * i + 3] = 1.0f; }
+    auto splat = [&](int idx, V3<float> L) {
+        out[4 * idx] = out[4 * idx] + L.x;
+        out[4 * idx + 1] = out[4 * idx + 1] + L.y;
+        out[4 * idx + 2] = out[4 * idx + 2] + L.z;
+    };
+    for (int idx = 0; idx < W * H; idx++) {
+        for (int smp = 0; smp < spp; smp++) {
+            Pcg32 rng;
+            path_rng(rng, (uint32_t)idx, seed, (uint32_t)smp);
+            V2<float> lr;
+            lr.x = rng.next_float();
+            lr.y = rng.next_float();
+            LightSample<float> ls = light_sample<float>(s, lr);
+            V2<float> sr;
+            sr.x = rng.next_float();
+            sr.y = rng.next_float();
+            V3<float> wi = normalize(ls.normal + uniform_sample_sphere(sr));
+            float cos_theta = dot(wi, ls.normal);
+            float pw = cosine_hemisphere_pdf(cos_theta);
+            V3<float> beta = ls.emission * cos_theta / ls.pdf;
+            bool specular = true;
+            V3<float> ro = ls.pos, rd = wi;
+            for (int depth = 0; depth < max_depth; depth++) {
+                float of[3] = {ro.x, ro.y, ro.z}, df[3] = {rd.x, rd.y, rd.z};
+                Hit h;
+                bool hit = tr.closest(of, df, 1e-3f, 1e16f, &h);
+                HitInfo<float> info;
+                if (hit) get_hitinfo<float>(s, h.prim, h.u, h.v, info);
+                if (specular) {
+                    V3<float> to_camera = CP - ro;
+                    const float t = length(to_camera);
+                    to_camera = V3<float>(to_camera.x / t, to_camera.y / t, to_camera.z / t);
+                    if ((!hit || t < length(info.pos - ro)) && dot(to_camera, rd) > 1.0f - 1e-6f) {
+                        int px;
+                        if (camera_pixel(cam, rd, W, H, &px)) {
+                            V3<float> b = V3<float>(beta.x / pw, beta.y / pw, beta.z / pw);
+                            splat(px, V3<float>(b.x / (float)spp, b.y / (float)spp, b.z / (float)spp));
+                        }
+                    }
+                }
+                if (!hit) break;
+                const MaterialRec& m = s.mats[info.mat];
+                if (mat_is_emissive(m)) break;
+                BxDF<float> bx(m);
+                if (dot(-rd, info.normal) < 0.0f) info.normal = -info.normal;
+                if (!mat_is_specular(m)) {
+                    const V3<float> sdir = normalize(CP - info.pos);
+                    int px;
+                    if (camera_pixel(cam, sdir, W, H, &px)) {
+                        const float t = length(CP - info.pos);
+                        float so[3] = {info.pos.x, info.pos.y, info.pos.z}, sd[3] = {sdir.x, sdir.y, sdir.z};
+                        if (!tr.any(so, sd, 1e-3f, t - 1e-3f)) {
+                            const V3<float> f = mat_eval(bx, sdir, -rd, info.onb, info.color);
+                            V3<float> L = aspect * beta * f * dot(sdir, info.normal);
+                            L = V3<float>(L.x / t, L.y / t, L.z / t);
+                            L = V3<float>(L.x / t, L.y / t, L.z / t);
+                            L = V3<float>(L.x / pw, L.y / pw, L.z / pw);
+                            splat(px, V3<float>(L.x / (float)spp, L.y / (float)spp, L.z / (float)spp));
+                        }
+                    }
+                }
+                V2<float> br;
+                br.x = rng.next_float();
+                br.y = rng.next_float();
+                BxDFSample<float> ms = mat_sample(bx, -rd, br, info.onb, info.color);
+                if (ms.pdf <= 1e-5f) break;
+                const V3<float> f = mat_eval(bx, ms.wi, -rd, info.onb, info.color);
+                const V3<float> g = f * std::fabs(dot(ms.wi, info.normal));
+                beta = beta * V3<float>(g.x / pw, g.y / pw, g.z / pw);
+                pw = ms.pdf;
+                specular = mat_is_specular(m);
+                ro = info.pos;
+                rd = ms.wi;
+                if (depth >= rr_depth) {
+                    const float p = std::fmin(std::fmax(beta.x, std::fmax(beta.y, beta.z)), 0.95f);
+                    if (rng.next_float() > p) break;
+                    beta = V3<float>(beta.x / p, beta.y / p, beta.z / p);
+                }
+            }
+        }
+    }
+}
+
 // closest-hit traces and shadow rays of the last orc_render
 void orc_last_counts(int64_t* out2) {
     out2[0] = g_last_counts[0];

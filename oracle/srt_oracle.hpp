@@ -310,6 +310,13 @@ template <class R> inline V3<R> cosine_sample_hemisphere(V2<R> s) {
     return V3<R>(d.x, d.y, z);
 }
 template <class R> inline R cosine_hemisphere_pdf(R c) { return c * C<R>::inv_pi(); }
+// basic.h:83-89
+template <class R> inline V3<R> uniform_sample_sphere(V2<R> s) {
+    R z = R(1) - R(2) * s.x;
+    R r = M<R>::sqrt(R(1) - z * z);
+    R phi = R(2) * C<R>::pi() * s.y;
+    return V3<R>(r * M<R>::cos(phi), r * M<R>::sin(phi), z);
+}
 // basic.h:139-143
 template <class R> inline V2<R> uniform_sample_triangle(V2<R> s) {
     R sx = M<R>::sqrt(s.x);

@@ -25,7 +25,7 @@ EXPORTED = [
     "srt_scene_upload", "srt_scene_release", "srt_scene_lights",
     "srt_scene_build_bvh", "srt_scene_bvh_info", "srt_scene_bvh_download",
     "srt_scene_set_skin", "srt_scene_skin_refit", "srt_scene_download_vertices",
-    "srt_trace_rays", "srt_render", "srt_render_simple", "srt_film_download", "srt_tonemap", "srt_accumulate",
+    "srt_trace_rays", "srt_render", "srt_render_simple", "srt_render_light", "srt_film_download", "srt_tonemap", "srt_accumulate",
     "srt_denoise",
     "srt_stats", "srt_stats_reset", "srt_last_counts", "srt_last_visits",
     "srt_device_alloc", "srt_device_free", "srt_memcpy",
@@ -110,6 +110,8 @@ def load_library(path: str = LIB_PATH):
         "srt_trace_rays": (C.c_int, [P, C.c_uint64, C.c_int32, P, P, C.c_float, P, C.c_int32, P, P, P, P]),
         "srt_render": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32, C.c_int32, P,
                                  C.c_int32]),
+        "srt_render_light": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32, C.c_int32, P,
+                                       C.c_int32]),
         "srt_render_simple": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32, C.c_int32,
                                         C.c_int32, P, C.c_int32]),
         "srt_film_download": (C.c_int, [P, P, C.c_int64]),
@@ -316,6 +318,18 @@ class Context:
         p = PtParams(spp, max_depth, rr_depth, use_nee, use_mis, seed, mlp_mode, 0)
         c = self.camera(cam)
         self.check(self.lib.srt_render(self.h, scene, C.byref(c), C.byref(p), width, height, None, 0))
+
+    def render_light(self, scene, cam: dict, width, height, spp=1, max_depth=16, rr_depth=4, seed=0, device_ptr=None):
+        """LightTracer pass: a (H*W, 4) float32 host array, or written to `device_ptr`."""
+        p = PtParams(spp, max_depth, rr_depth, 0, 0, seed, 0, 0)
+        c = self.camera(cam)
+        if device_ptr is not None:
+            self.check(self.lib.srt_render_light(self.h, scene, C.byref(c), C.byref(p), width, height,
+                                                 C.c_void_p(device_ptr), 1))
+            return None
+        out = np.zeros((width * height, 4), np.float32)
+        self.check(self.lib.srt_render_light(self.h, scene, C.byref(c), C.byref(p), width, height, _ptr(out), 0))
+        return out
 
     def render_simple(self, scene, cam: dict, width, height, type="Depth", spp=16, seed=0, device_ptr=None):
         """Simple NPR pass: a (H*W, 4) float32 host array, or written to `device_ptr`."""

@@ -238,6 +238,15 @@ SRT_D f3 cosine_sample_hemisphere(f2 s) {
     return mk(d.x, d.y, z);
 }
 SRT_D float cosine_hemisphere_pdf(float c) { return c * kInvPi; }
+// basic.h:83-89
+SRT_D f3 uniform_sample_sphere(f2 s) {
+    float z = 1.0f - 2.0f * s.x;
+    float r = __builtin_sqrtf(1.0f - z * z);
+    float phi = 2.0f * kPi * s.y;
+    float sp, cp;
+    pt_sincosf(phi, &sp, &cp);
+    return mk(r * cp, r * sp, z);
+}
 // basic.h:139-143
 SRT_D f2 uniform_sample_triangle(f2 s) {
     float sx = __builtin_sqrtf(s.x);

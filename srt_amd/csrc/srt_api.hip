@@ -536,6 +536,39 @@ int srt_render_simple(srt_ctx* c, srt_scene h, const srt_camera* cam, int32_t ty
     return SRT_OK;
 }
 
+int srt_render_light(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_params* p, int32_t W, int32_t H,
+                     float* rgba_out, int32_t out_on_device) {
+    SRT_IDLE(c);
+    if (!c || !cam || !p || !rgba_out) return SRT_E_INVALID;
+    SceneRec* s = find_scene(c, h);
+    if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (W <= 0 || H <= 0) return set_err(c, SRT_E_INVALID, "resolution must be positive");
+    if ((int64_t)W * H > (1ll << 28)) return set_err(c, SRT_E_LIMIT, "resolution too large");
+    if (p->samples_per_pixel <= 0) return set_err(c, SRT_E_INVALID, "samples_per_pixel must be positive");
+    if ((int64_t)W * H * p->samples_per_pixel > (1ll << 40)) return set_err(c, SRT_E_LIMIT, "too many paths");
+    if (p->max_depth < 0 || p->rr_depth < 0) return set_err(c, SRT_E_INVALID, "depths must be >= 0");
+    if (!(cam->fov_deg > 0.0f && cam->fov_deg < 180.0f)) return set_err(c, SRT_E_INVALID, "fov must be in (0, 180)");
+    if (s->nlights == 0) return set_err(c, SRT_E_INVALID, "the light tracer needs an area light in the scene");
+    if (!s->has_bvh) {
+        if (int r = bvh_build(c, s, 4, 0)) return r;
+    }
+    DevCamera dc = make_camera(cam->position, cam->target, cam->up, cam->fov_deg, W, H);
+    const int64_t wh = (int64_t)W * H;
+    float* film = nullptr;
+    if (out_on_device) film = rgba_out;
+    else {
+        if (!c->film_out.p || c->film_out.bytes < (size_t)wh * 16) {
+            dev_free(c, c->film_out);
+            if (int r = dev_alloc(c, c->film_out, (size_t)wh * 16)) return r;
+        }
+        film = (float*)c->film_out.p;
+    }
+    if (int r = render_light(c, s, dc, p->samples_per_pixel, p->max_depth, p->rr_depth, p->seed, W, H, film)) return r;
+    prof_collect(c);
+    if (!out_on_device) return hip_check(c, hipMemcpy(rgba_out, film, (size_t)wh * 16, hipMemcpyDeviceToHost), "film download");
+    return SRT_OK;
+}
+
 int srt_mlp_train_frame(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_params* p, int32_t W, int32_t H,
                         int64_t max_samples, int32_t steps, float* loss_out, int64_t* samples_out) {
     SRT_IDLE(c);

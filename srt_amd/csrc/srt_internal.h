@@ -279,6 +279,8 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
 int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, TrainOut* train = nullptr);
 int render_simple(srt_ctx* c, SceneRec* s, const DevCamera& cam, int type, int spp, uint32_t seed, int width, int height,
                   float* out_dev);
+int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max_depth, int rr_depth, uint32_t seed,
+                 int width, int height, float* film_dev);
 // MLP (mlp.hip)
 int mlp_infer_dev(srt_ctx* c, const float* in9, int n, float* out3);
 int mlp_query_dev(srt_ctx* c, const float* in9, const uint32_t* count_dev, int max_n, float* out3);

@@ -1004,6 +1004,200 @@ __global__ void __launch_bounds__(256) k_simple_accum(DevScene S, SimpleIO io, i
     acc[i] = a;
 }
 
+// ---------------------------------------------------------------------------
+// LightTracer pass (shader/integrator/light_tracer.cu:57-176,
+// passes/integrator/light_tracer.cu:13-37): particles leave the area lights
+// (cosine-distributed about the light normal), bounce through the scene, and
+// splat into the film through the pinhole camera: at every non-specular
+// vertex by a camera connection (a shadow ray toward the camera), and a
+// specular particle that passes through the camera position directly.  The
+// splat weights are the reference's own (aspect ratio, the previous
+// direction's pdf pw), reproduced as written.  Splats are float atomic adds,
+// so a pixel's sum order varies between runs (the reference's as well).
+// ---------------------------------------------------------------------------
+struct LtParams {
+    DevCamera cam;
+    int32_t width, height, spp, rr_depth;
+    uint32_t seed;
+    int64_t path_begin;      // path id of slot 0 of the chunk: pid = sample * W * H + launch index
+    float aspect;            // width / height
+};
+
+// camera.h:143-148 get_xy; the pixel the direction `dir` (pointing away from
+// the camera's image) lands in.  x < 1 can still round x * W up to W: such
+// splats go to the last column / row.
+SRT_D bool camera_pixel(const DevCamera& cam, f3 dir, int W, int H, int* idx) {
+    const f3 d = -dir;
+    const f3 l = mk(dot(d, ld3(cam.X)), dot(d, ld3(cam.Y)), dot(d, ld3(cam.Z)));
+    const float px = l.x / l.z, py = l.y / l.z;
+    const float x = px / cam.nw + 0.5f, y = py / cam.nh + 0.5f;
+    if (!(x >= 0.0f && x < 1.0f && y >= 0.0f && y < 1.0f)) return false;
+    const int ix = min((int)(x * (float)W), W - 1), iy = min((int)(y * (float)H), H - 1);
+    *idx = ix + iy * W;
+    return true;
+}
+
+SRT_D void splat(float4* film, int idx, f3 L) {
+    atomicAdd(&film[idx].x, L.x);
+    atomicAdd(&film[idx].y, L.y);
+    atomicAdd(&film[idx].z, L.z);
+}
+
+__global__ void __launch_bounds__(256) k_lt_raygen(DevScene S, LtParams P, PathQueue q, int n, uint32_t* counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        counts[CNT_Q0] = (uint32_t)n;
+        counts[CNT_FETCH_T] = 0u;
+    }
+    if (i >= n) return;
+    const int64_t wh = (int64_t)P.width * P.height;
+    const int64_t pid = P.path_begin + i;
+    const uint32_t sample = (uint32_t)(pid / wh);
+    const uint32_t idx = (uint32_t)(pid - (int64_t)sample * wh);
+    Pcg32 rng = path_rng(idx, P.seed, sample);
+    f2 lr;
+    lr.x = rng.next_float();
+    lr.y = rng.next_float();
+    const LightSample ls = light_sample(S, lr);
+    f2 sr;
+    sr.x = rng.next_float();
+    sr.y = rng.next_float();
+    const f3 wi = normalize(ls.normal + uniform_sample_sphere(sr));
+    const float cos_theta = dot(wi, ls.normal);
+    const float pw = cosine_hemisphere_pdf(cos_theta);
+    const f3 beta = ls.emission * cos_theta / ls.pdf;
+    q.ox[i] = ls.pos.x; q.oy[i] = ls.pos.y; q.oz[i] = ls.pos.z;
+    q.dx[i] = wi.x; q.dy[i] = wi.y; q.dz[i] = wi.z;
+    q.br[i] = beta.x; q.bg[i] = beta.y; q.bb[i] = beta.z;
+    q.pdf[i] = pw;
+    q.flags[i] = 1u;                 // bit 0: the last scattering was specular (the light counts as one)
+    q.rng[i] = rng.state;
+    q.slot[i] = (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int depth, PathQueue qin, HitBuf h,
+                                                 PathQueue qout, ShadowQueue sq, float4* film, uint32_t* counts, int qi) {
+    const uint32_t count = counts[qi];
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_FETCH_T] = 0u;   // the next trace stage fetches from 0
+    const f3 cam_pos = ld3(P.cam.pos);
+    const int64_t wh = (int64_t)P.width * P.height;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const f3 ro = mk(qin.ox[i], qin.oy[i], qin.oz[i]), rd = mk(qin.dx[i], qin.dy[i], qin.dz[i]);
+        f3 beta = mk(qin.br[i], qin.bg[i], qin.bb[i]);
+        float pw = qin.pdf[i];
+        const uint32_t slot = qin.slot[i];
+        const uint32_t sample = (uint32_t)((P.path_begin + slot) / wh);
+        const int32_t tri = h.tri[i];
+        const bool hit = tri >= 0;
+        HitGeom hg;
+        if (hit) hg = hit_geometry(S, tri, h.u[i], h.v[i]);
+        // a specular particle through the camera position (light_tracer.cu:89-110)
+        if (qin.flags[i] & 1u) {
+            f3 to_camera = cam_pos - ro;
+            const float t = length(to_camera);
+            to_camera = to_camera / t;
+            if ((!hit || t < length(hg.pos - ro)) && dot(to_camera, rd) > 1.0f - 1e-6f) {
+                int px;
+                if (camera_pixel(P.cam, rd, P.width, P.height, &px)) splat(film, px, beta / pw / (float)P.spp);
+            }
+        }
+        if (!hit) continue;
+        const MatRec& m = S.mats[hg.matid];
+        if (mat_is_emissive(m)) continue;
+        f3 normal = hg.normal;
+        if (dot(-rd, normal) < 0.0f) normal = -normal;
+        const f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
+        f3 tg;
+        if (S.tri_st) {
+            const float4* q = S.tri_st + 3 * (size_t)tri;
+            const float4 t0 = q[0], t1 = q[1], t2 = q[2];
+            const float u = h.u[i], v = h.v[i];
+            tg = mk(t0.x, t0.y, t0.z) * hg.w + mk(t1.x, t1.y, t1.z) * u + mk(t2.x, t2.y, t2.z) * v;
+        } else {
+            tg = (fabsf(hg.sn.x) > fabsf(hg.sn.y)) ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+        }
+        const ShadePoint spt = shade_point(m, -rd, make_onb(hg.sn, tg), color);
+        Pcg32 rng = rng_resume(qin.rng[i], sample);
+        // camera connection (light_tracer.cu:124-142): the contribution waits in the shadow queue
+        bool emit = false;
+        f3 sdir = mk(0.0f), L = mk(0.0f);
+        float t = 0.0f;
+        int px = 0;
+        if (!mat_is_specular(m)) {
+            sdir = normalize(cam_pos - hg.pos);
+            if (camera_pixel(P.cam, sdir, P.width, P.height, &px)) {
+                t = length(cam_pos - hg.pos);
+                const f3 f = sp_eval(m, spt, sdir, color);
+                L = P.aspect * beta * f * dot(sdir, normal) / t / t / pw;
+                L = L / (float)P.spp;
+                emit = true;
+            }
+        }
+        const int js = wave_append(emit, counts + CNT_SHADOW);
+        if (js >= 0) {
+            sq.ox[js] = hg.pos.x; sq.oy[js] = hg.pos.y; sq.oz[js] = hg.pos.z;
+            sq.dx[js] = sdir.x; sq.dy[js] = sdir.y; sq.dz[js] = sdir.z;
+            sq.tmax[js] = t - 1e-3f;
+            sq.cr[js] = L.x; sq.cg[js] = L.y; sq.cb[js] = L.z;
+            sq.slot[js] = (uint32_t)px;
+        }
+        // next direction (light_tracer.cu:145-165)
+        f2 br;
+        br.x = rng.next_float();
+        br.y = rng.next_float();
+        const BxSample ms = sp_sample(m, spt, br, color);
+        bool alive = !(ms.pdf <= 1e-5f);
+        bool spec = false;
+        if (alive) {
+            const f3 f = sp_eval(m, spt, ms.wi, color);
+            beta = beta * (f * fabsf(dot(ms.wi, normal)) / pw);
+            pw = ms.pdf;
+            spec = mat_is_specular(m);
+            if (depth >= P.rr_depth) {
+                const float p = fminf(fmaxf(beta.x, fmaxf(beta.y, beta.z)), 0.95f);
+                if (rng.next_float() > p) alive = false;
+                else beta = beta / p;
+            }
+        }
+        const int jn = wave_append(alive, counts + (qi ^ 1));
+        if (jn >= 0) {
+            qout.ox[jn] = hg.pos.x; qout.oy[jn] = hg.pos.y; qout.oz[jn] = hg.pos.z;
+            qout.dx[jn] = ms.wi.x; qout.dy[jn] = ms.wi.y; qout.dz[jn] = ms.wi.z;
+            qout.br[jn] = beta.x; qout.bg[jn] = beta.y; qout.bb[jn] = beta.z;
+            qout.pdf[jn] = pw;
+            qout.flags[jn] = spec ? 1u : 0u;
+            qout.rng[jn] = rng.state;
+            qout.slot[jn] = slot;
+        }
+    }
+}
+
+struct LtShadowIO {    // camera connections: unoccluded ones splat their contribution
+    ShadowQueue sq;
+    float4* film;
+    SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
+        trav_begin(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root);
+    }
+    SRT_D void store(uint32_t i, const TravState&, bool occluded) const {
+        if (!occluded) splat(film, (int)sq.slot[i], mk(sq.cr[i], sq.cg[i], sq.cb[i]));
+    }
+};
+
+template <bool WIDE>
+__global__ void __launch_bounds__(kTraceBlock) k_lt_shadow(DevScene S, ShadowQueue sq, float4* film, uint32_t* counts,
+                                                          int32_t* gstack, TraceCfg cfg) {
+    const uint32_t count = counts[CNT_SHADOW];
+    const TravStack k = make_stack(gstack, cfg);
+    LtShadowIO io{sq, film};
+    uint32_t nv = 0, nt = 0;
+    trace_persistent<true, false, WIDE>(S, count, counts + CNT_FETCH_S, cfg.refill, k, io, nv, nt);
+}
+
+__global__ void __launch_bounds__(256) k_film_clear(float4* film, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) film[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+}
+
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 // ---------------------------------------------------------------------------
@@ -1464,6 +1658,61 @@ int render_simple(srt_ctx* c, SceneRec* s, const DevCamera& cam, int type, int s
     prof_end(c, "simple", ev, n * spp);
     if (int r = hip_check(c, hipGetLastError(), "simple launch")) return r;
     return hip_check(c, hipStreamSynchronize(c->stream), "simple");
+}
+
+// LightTracer pass: W x H x spp particles (launch index x sample), in chunks
+// of the path pool; film_dev is cleared to (0, 0, 0, 1) first.
+int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max_depth, int rr_depth, uint32_t seed,
+                 int width, int height, float* film_dev) {
+    const int64_t wh = (int64_t)width * height;
+    const int64_t total = wh * spp;
+    int64_t pool = std::min<int64_t>(std::min<int64_t>(total, c->pool_request), 0x7fffffff);
+    if (int r = ensure_pool(c, pool)) return r;
+    const int tgrid = trace_grid(c, pool);
+    DevScene S = dev_scene(s);
+    if (int r = ensure_gstack(c, s, (int64_t)tgrid * kTraceBlock)) return r;
+    const TraceCfg tcfg = trace_cfg(c);
+    TraceCfg scfg = tcfg;
+    scfg.refill = c->shadow_refill;
+    const size_t tlds = trace_lds_bytes(tcfg);
+    hipStream_t st = c->stream;
+    uint32_t* counts = (uint32_t*)c->counters.p;
+    unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + 64);
+    PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
+    HitBuf H;
+    H.t = (float*)c->hits[0].p; H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
+    ShadowQueue SQ;
+    {
+        auto f = [&](int j) { return (float*)c->shq[j].p; };
+        SQ.ox = f(0); SQ.oy = f(1); SQ.oz = f(2); SQ.dx = f(3); SQ.dy = f(4); SQ.dz = f(5); SQ.tmax = f(6);
+        SQ.cr = f(7); SQ.cg = f(8); SQ.cb = f(9); SQ.slot = (uint32_t*)c->shq[10].p;
+    }
+    int32_t* gst = (int32_t*)c->gstack.p;
+    float4* film = (float4*)film_dev;
+    LtParams P;
+    P.cam = cam;
+    P.width = width; P.height = height; P.spp = spp; P.rr_depth = rr_depth; P.seed = seed;
+    P.aspect = (float)width / (float)height;
+    hipEvent_t ev;
+    prof_begin(c, "light_tracer", &ev);
+    k_film_clear<<<(unsigned)((wh + 255) / 256), 256, 0, st>>>(film, wh);
+    for (int64_t p0 = 0; p0 < total; p0 += pool) {
+        const int n = (int)std::min<int64_t>(pool, total - p0);
+        P.path_begin = p0;
+        const int shade_grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)num_cus(c) * 8);
+        k_lt_raygen<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(S, P, Q[0], n, counts);
+        for (int depth = 0; depth < max_depth; depth++) {
+            const int qi = depth & 1;
+            if (tcfg.wide) k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+            else k_trace<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+            k_lt_shade<<<shade_grid, 256, 0, st>>>(S, P, depth, Q[qi], H, Q[qi ^ 1], SQ, film, counts, qi);
+            if (tcfg.wide) k_lt_shadow<true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, gst, scfg);
+            else k_lt_shadow<false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, gst, scfg);
+        }
+    }
+    prof_end(c, "light_tracer", ev, total);
+    if (int r = hip_check(c, hipGetLastError(), "light tracer launch")) return r;
+    return hip_check(c, hipStreamSynchronize(st), "light tracer");
 }
 
 int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin, const float* tmax_dev,

@@ -71,6 +71,43 @@ Json PathTracer::params() const {
 // Accumulate (accumulate.cu:10-29)
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
+// LightTracer (passes/integrator/light_tracer.cu:13-37)
+// ---------------------------------------------------------------------------
+REGISTER_RENDER_PASS_CPP(LightTracer);
+
+void LightTracer::init() {
+    if (!scene || !scene->ctx || !scene->handle) throw RenderError("LightTracer needs a scene uploaded to the device");
+}
+
+void LightTracer::render(std::shared_ptr<Film> film) {
+    if (!enable) return;
+    srt_camera cam{};
+    for (int i = 0; i < 3; i++) {
+        cam.position[i] = scene->camera->position[i];
+        cam.target[i] = scene->camera->target[i];
+        cam.up[i] = scene->camera->up[i];
+    }
+    cam.fov_deg = scene->camera->fov;
+    srt_pt_params p{};
+    p.samples_per_pixel = samples_per_pixel;
+    p.max_depth = max_depth;
+    p.rr_depth = rr_depth;
+    p.seed = seed + frame;
+    frame++;
+    check_srt(scene->ctx, srt_render_light(scene->ctx, scene->handle, &cam, &p, width, height, film->get_pixels(), 1),
+              "LightTracer");
+}
+
+Json LightTracer::params() const {
+    Json j = Json::object();
+    j["samples_per_pixel"] = Json(samples_per_pixel);
+    j["max_depth"] = Json(max_depth);
+    j["rr_depth"] = Json(rr_depth);
+    j["seed"] = Json((double)seed);
+    return j;
+}
+
+// ---------------------------------------------------------------------------
 // Simple (passes/npr/simple.cu:14-33)
 // ---------------------------------------------------------------------------
 REGISTER_RENDER_PASS_CPP(Simple);

@@ -2,6 +2,7 @@
 //
 //   PathTracer  src/passes/integrator/path_tracer.{h,cu}  (megakernel pass)
 //   Wavefront   src/passes/wavefront/wavefront.{h,cu}     (stage-per-kernel pass)
+//   LightTracer src/passes/integrator/light_tracer.{h,cu}  (particle tracing from the lights)
 //   Simple      src/passes/npr/simple.{h,cu}                (first-hit NPR pass)
 //   Accumulate  src/passes/postprocess/accumulate.{h,cu}
 //   Denoise     src/passes/postprocess/denoise.{h,cu}       (a-trous filter in place of the OptiX denoiser)
@@ -53,6 +54,33 @@ class Wavefront : public PathTracer {
 
 public:
     friend void from_json(const Json& j, Wavefront& p) { from_json(j, (PathTracer&)p); }
+};
+
+// LightTracer: samples_per_pixel 1, max_depth 16, rr_depth 4
+// (light_tracer.h:15-17, :28-35); frame k renders with seed + k.
+class LightTracer : public RenderPass {
+    REGISTER_RENDER_PASS(LightTracer);
+
+    int samples_per_pixel{1};
+    int max_depth{16};
+    int rr_depth{4};
+    uint32_t seed{0};
+    uint32_t frame{0};
+
+public:
+    LightTracer() {}
+    void init() override;
+    void render(std::shared_ptr<Film> film) override;
+    void update() override {}
+    Json params() const override;
+
+    friend void from_json(const Json& j, LightTracer& p) {
+        if (j.is_null()) return;
+        p.samples_per_pixel = j.value("samples_per_pixel", 1);
+        p.max_depth = j.value("max_depth", 16);
+        p.rr_depth = j.value("rr_depth", 4);
+        p.seed = (uint32_t)j.value("seed", 0);
+    }
 };
 
 // Simple: `type` Depth | Normal | BaseColor | Ambient | FaceOrientation

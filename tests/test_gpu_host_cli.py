@@ -72,6 +72,24 @@ def test_image_simple_pass_matches_c_abi(ctx, tmp_path):
     assert np.array_equal(img, film_to_image(ref, W, H))
 
 
+def test_image_light_tracer_pass_matches_c_abi(ctx, tmp_path):
+    s = scenes.cornell()
+    scenes.save_srtscene(s, str(tmp_path / "cornell.srtscene"))
+    W, H = 48, 36
+    cfg = {"render": {"type": "image", "output": "light.pfm", "resolution": [W, H]},
+           "passes": [{"name": "LightTracer", "params": {"samples_per_pixel": 2, "max_depth": 4, "seed": 9}}],
+           "scene": {"model": "cornell.srtscene",
+                     "camera": {"type": "perspective", "position": list(s.camera["position"]),
+                                "target": list(s.camera["target"]), "up": list(s.camera["up"]), "fov": 40}}}
+    run(cfg, tmp_path)
+    img = read_pfm(tmp_path / "light.pfm")
+    h = ctx.upload(s)
+    ref = film_to_image(ctx.render_light(h, s.camera, W, H, spp=2, max_depth=4, seed=9), W, H)
+    ctx.release(h)
+    # float atomic splats: equal up to the order of the sums
+    assert np.all(np.abs(img - ref) <= 1e-4 * np.maximum(np.abs(img), np.abs(ref)) + 1e-6 * ref.max())
+
+
 def test_interactive_accumulate_tonemap(ctx, tmp_path):
     s = scenes.cornell()
     scenes.save_srtscene(s, str(tmp_path / "c.srtscene"))

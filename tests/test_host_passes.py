@@ -48,6 +48,14 @@ def test_tonemapping_defaults_and_names(host):
     assert json.loads(params(host, "ToneMapping", {"type": "Filmic"})[1])["type"] == "None"
 
 
+def test_light_tracer_defaults(host):
+    r, out = params(host, "LightTracer")
+    assert r > 0
+    assert json.loads(out) == {"samples_per_pixel": 1, "max_depth": 16, "rr_depth": 4, "seed": 0}
+    assert json.loads(params(host, "LightTracer", {"samples_per_pixel": 8, "max_depth": 5})[1]) == \
+        {"samples_per_pixel": 8, "max_depth": 5, "rr_depth": 4, "seed": 0}
+
+
 def test_simple_defaults_and_names(host):
     r, out = params(host, "Simple")
     assert r > 0
