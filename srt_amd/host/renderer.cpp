@@ -78,7 +78,17 @@ void Renderer::load_scene(const Json& config) {
         if (t == "constant") vec3(e.at("color"), scene->background);
         else throw RenderError("Unknown environment type: " + t);
     }
-    if (config.contains("animation")) scene->enable_animation = config.at("animation").get_bool();
+    if (config.contains("animation")) {
+        // a list of keyframed node animations (renderer.cu:71-121), or a flag
+        // that enables the skinning animation stored in an .srtscene
+        const Json& a = config.at("animation");
+        if (a.is_array()) {
+            for (size_t i = 0; i < a.size(); i++) scene->node_anims.push_back(parse_node_animation(a.at(i)));
+            scene->enable_animation = true;
+        } else {
+            scene->enable_animation = a.get_bool();
+        }
+    }
     scene->build_gscene(ctx);
     film = std::make_shared<Film>(ctx, width, height);
 }
@@ -233,6 +243,18 @@ int srt_host_save_image(const char* path, int width, int height, const float* rg
         return 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "%s\n", e.what());
+        return -1;
+    }
+}
+
+// 3x4 row-major transform at time t of one node animation given as the JSON
+// object of the scene config's "animation" list.  0, or -1 (message in err).
+int srt_host_node_transform(const char* animation_json, float t, float* out12, char* err, int err_len) {
+    try {
+        srt::parse_node_animation(srt::Json::parse(animation_json)).transform(t, out12);
+        return 0;
+    } catch (const std::exception& e) {
+        if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", e.what());
         return -1;
     }
 }

@@ -49,10 +49,136 @@ void Scene::build_gscene(srt_ctx* c) {
     check(ctx, srt_scene_upload(ctx, &d, &handle), "scene upload");
     check(ctx, srt_scene_build_bvh(ctx, handle, nullptr), "BVH build");
     if (num_bones > 0) check(ctx, srt_scene_set_skin(ctx, handle, bone_ids.data(), bone_weights.data()), "skin weights");
+    if (!node_anims.empty()) {
+        if (num_bones > 0) throw RenderError("a skinned scene cannot also have node animations");
+        // every vertex follows the animation of its mesh's name (rigid: one bone, weight 1)
+        const size_t nv = positions.size() / 3;
+        std::vector<int32_t> ids(4 * nv, -1), owner(nv, -1);
+        std::vector<float> w(4 * nv, 0.0f);
+        for (size_t v = 0; v < nv; v++) { ids[4 * v] = 0; w[4 * v] = 1.0f; }
+        for (size_t a = 0; a < node_anims.size(); a++) {
+            bool found = false;
+            for (size_t m = 0; m + 1 < mesh_first.size(); m++) {
+                if (mesh_names[m] != node_anims[a].name) continue;
+                found = true;
+                for (int32_t f = mesh_first[m]; f < mesh_first[m + 1]; f++)
+                    for (int k = 0; k < 3; k++) {
+                        const uint32_t v = indices[3 * (size_t)f + k];
+                        if (owner[v] >= 0 && owner[v] != (int32_t)a)
+                            throw RenderError("a vertex is shared by two animated meshes: " + node_anims[a].name);
+                        owner[v] = (int32_t)a;
+                        ids[4 * v] = (int32_t)(a + 1);
+                    }
+            }
+            if (!found) throw RenderError("animation of unknown mesh " + node_anims[a].name);
+        }
+        check(ctx, srt_scene_set_skin(ctx, handle, ids.data(), w.data()), "node animation weights");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// node animation (animation.cpp, quaternion.h:104-117, transform.h:148-167)
+// ---------------------------------------------------------------------------
+namespace {
+struct Quat { float x, y, z, w; };
+Quat qadd(Quat a, Quat b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+Quat qscale(Quat a, float s) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
+float qdot(Quat a, Quat b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+Quat qnormalize(Quat a) { return qscale(a, 1.0f / std::sqrt(qdot(a, a))); }
+Quat slerp(Quat q1, Quat q2, float t) {
+    const float cos_theta = std::min(std::max(qdot(q1, q2), -1.0f), 1.0f);
+    if (std::fabs(cos_theta) > 1.0f - 1e-6f) return qadd(qscale(q1, 1.0f - t), qscale(q2, t));
+    if (cos_theta < 0.0f) return slerp(q1, qscale(q2, -1.0f), t);
+    const float theta = std::acos(cos_theta), theta_p = theta * t;
+    const Quat qperp = qnormalize(qadd(q2, qscale(q1, -cos_theta)));
+    return qadd(qscale(q1, std::cos(theta_p)), qscale(qperp, std::sin(theta_p)));
+}
+// key interval of time t over stamps ts: (index, factor); the scale channel
+// compares with '<' where the others use '<=' (animation.cpp:37-81)
+template <int N>
+void interp(const std::vector<float>& ts, const std::vector<float>& v, float t, bool strict, const float* dflt, float* out) {
+    const size_t n = ts.size();
+    if (n == 0) { for (int k = 0; k < N; k++) out[k] = dflt[k]; return; }
+    if (n == 1) { for (int k = 0; k < N; k++) out[k] = v[k]; return; }
+    size_t idx = 0;
+    for (; idx < n - 1; idx++)
+        if (strict ? (t < ts[idx + 1]) : (t <= ts[idx + 1])) break;
+    if (idx == n - 1) { for (int k = 0; k < N; k++) out[k] = v[N * idx + k]; return; }
+    const float f = (t - ts[idx]) / (ts[idx + 1] - ts[idx]);
+    if (N == 4) {
+        const Quat a{v[4 * idx], v[4 * idx + 1], v[4 * idx + 2], v[4 * idx + 3]};
+        const Quat b{v[4 * idx + 4], v[4 * idx + 5], v[4 * idx + 6], v[4 * idx + 7]};
+        const Quat q = slerp(a, b, f);
+        out[0] = q.x; out[1] = q.y; out[2] = q.z; out[3] = q.w;
+        return;
+    }
+    for (int k = 0; k < N; k++) out[k] = v[N * idx + k] * (1.0f - f) + v[N * (idx + 1) + k] * f;
+}
+void mat4_mul(const float a[16], const float b[16], float o[16]) {
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float s = 0.0f;
+            for (int k = 0; k < 4; k++) s += a[4 * i + k] * b[4 * k + j];
+            o[4 * i + j] = s;
+        }
+}
+}  // namespace
+
+void NodeAnimation::transform(float t, float out[12]) const {
+    t = std::fmod(t, duration);                                    // Extrapolation::Repeat
+    static const float zero3[3] = {0, 0, 0}, one3[3] = {1, 1, 1}, idq[4] = {0, 0, 0, 1};
+    float T[3], R[4], S[3];
+    interp<3>(t_keys, translations, t, false, zero3, T);
+    interp<4>(r_keys, rotations, t, false, idq, R);
+    interp<3>(s_keys, scales, t, true, one3, S);
+    const float x = R[0], y = R[1], z = R[2], w = R[3];
+    const float x2 = x * x, y2 = y * y, z2 = z * z, xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y, wz = w * z;
+    const float Mt[16] = {1, 0, 0, T[0], 0, 1, 0, T[1], 0, 0, 1, T[2], 0, 0, 0, 1};
+    const float Mr[16] = {1.0f - 2.0f * (y2 + z2), 2.0f * (xy - wz), 2.0f * (xz + wy), 0.0f,
+                          2.0f * (xy + wz), 1.0f - 2.0f * (x2 + z2), 2.0f * (yz - wx), 0.0f,
+                          2.0f * (xz - wy), 2.0f * (yz + wx), 1.0f - 2.0f * (x2 + y2), 0.0f,
+                          0.0f, 0.0f, 0.0f, 1.0f};
+    const float Ms[16] = {S[0], 0, 0, 0, 0, S[1], 0, 0, 0, 0, S[2], 0, 0, 0, 0, 1};
+    float TR[16], M[16];
+    mat4_mul(Mt, Mr, TR);
+    mat4_mul(TR, Ms, M);
+    for (int i = 0; i < 12; i++) out[i] = M[i];
+}
+
+NodeAnimation parse_node_animation(const Json& c) {
+    NodeAnimation a;
+    a.name = c.at("name").get_string();
+    const Json& keys = c.at("keys");
+    if (!keys.is_array() || keys.size() == 0) throw RenderError("animation " + a.name + ": 'keys' must be a non-empty list");
+    std::vector<float> ts;
+    for (size_t i = 0; i < keys.size(); i++) ts.push_back(keys.at(i).get_float());
+    auto channel = [&](const char* key, int n, std::vector<float>& stamps, std::vector<float>& vals) {
+        if (!c.contains(key)) return;
+        const Json& v = c.at(key);
+        if (v.size() != ts.size()) throw RenderError("animation " + a.name + ": '" + key + "' needs one entry per key");
+        for (size_t i = 0; i < v.size(); i++) {
+            if ((int)v.at(i).size() != n) throw RenderError("animation " + a.name + ": bad '" + key + "' entry");
+            for (int k = 0; k < n; k++) vals.push_back(v.at(i).at((size_t)k).get_float());
+        }
+        stamps = ts;
+    };
+    channel("translations", 3, a.t_keys, a.translations);
+    channel("rotations", 4, a.r_keys, a.rotations);
+    channel("scales", 3, a.s_keys, a.scales);
+    a.duration = ts.back();
+    return a;
 }
 
 void Scene::update(float t) {
     if (!has_animation() || !enable_animation || !handle) return;
+    if (!node_anims.empty()) {
+        // bone 0: the identity for meshes without an animation
+        std::vector<float> bones(12 * (node_anims.size() + 1), 0.0f);
+        bones[0] = bones[5] = bones[10] = 1.0f;
+        for (size_t i = 0; i < node_anims.size(); i++) node_anims[i].transform(t, &bones[12 * (i + 1)]);
+        check(ctx, srt_scene_skin_refit(ctx, handle, bones.data(), (int32_t)(node_anims.size() + 1)), "node animation");
+        return;
+    }
     int f = (int)std::floor(t * 30.0f + 1e-4f) % num_frames;
     if (f < 0) f += num_frames;
     if (f == current_frame) return;

@@ -2,7 +2,7 @@
 //
 // Mirrors the parts of the reference's Scene (src/scene/scene.h:65-130) that
 // the path-tracing passes read: meshes, materials, camera, background,
-// skinning animation, and the upload to the device (Scene::build_gscene,
+// skinning and keyframed node animation, and the upload to the device (Scene::build_gscene,
 // src/scene/scene.cu:84-288), which here is srt_scene_upload +
 // srt_scene_build_bvh: one BVH per scene shared by every pass.
 #pragma once
@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "json.h"
 #include "srt_hip.h"
 
 namespace srt {
@@ -33,6 +34,19 @@ struct Camera {
     void set_moved(bool m) { moved = m; }
 };
 
+// A keyframed rigid animation of the meshes of one name (animation.h:24-53,
+// animation.cpp, the "animation" list of the scene config, renderer.cu:71-121):
+// linear interpolation of translation and scale, slerp of rotation,
+// repeat extrapolation over the last key's time, T * R * S.
+struct NodeAnimation {
+    std::string name;
+    float duration = 0.0f;
+    std::vector<float> t_keys, r_keys, s_keys;             // time stamps of each channel
+    std::vector<float> translations, rotations, scales;    // 3, 4 (x, y, z, w), 3 floats per key
+    void transform(float t, float out[12]) const;          // 3x4 row-major
+};
+NodeAnimation parse_node_animation(const Json& c);
+
 class Scene {
 public:
     std::shared_ptr<Camera> camera;
@@ -53,6 +67,8 @@ public:
     std::vector<int32_t> bone_ids;
     std::vector<float> bone_weights;
     std::vector<float> bone_frames;
+    // rigid node animations: bone 0 is the identity, bone i + 1 moves the meshes named node_anims[i].name
+    std::vector<NodeAnimation> node_anims;
 
     // device residency
     srt_ctx* ctx = nullptr;
@@ -60,7 +76,7 @@ public:
     int current_frame = -1;
 
     ~Scene();
-    bool has_animation() const { return num_frames > 0; }
+    bool has_animation() const { return num_frames > 0 || !node_anims.empty(); }
     bool is_static() const { return (!has_animation() || !enable_animation) && !camera->moved; }   // scene.h:93
     int num_triangles() const { return (int)(indices.size() / 3); }
     void add_mesh(const std::string& name, const std::vector<float>& pos, const std::vector<uint32_t>& tris, int32_t material);

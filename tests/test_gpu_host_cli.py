@@ -144,7 +144,7 @@ def test_example_config_layout_with_denoise(ctx, tmp_path):
     """The reference's example/example.json layout (interactive; PathTracer,
     Accumulate, Denoise, ToneMapping; OBJ Cornell box; constant black
     environment) runs as written, on this repository's own OBJ export of the
-    Cornell box; the image equals the same chain through the C ABI."""
+    Cornell box, and writes a finite image."""
     s = scenes.cornell()
     scenes.save_obj(s, str(tmp_path / "cornell_box.obj"))
     W, H, F = 64, 48, 3
@@ -163,6 +163,53 @@ def test_example_config_layout_with_denoise(ctx, tmp_path):
     img = read_pfm(tmp_path / "example.pfm")
     assert img.shape == (H, W, 3) and np.all(np.isfinite(img))
     assert 0.02 < img.mean() < 1.0
+
+
+def test_node_animation_moves_the_named_mesh(ctx, tmp_path):
+    """The scene config's "animation" list (renderer.cu:71-121): frame k of a
+    video is rendered at t = k / 30 with the named mesh moved by its keyframed
+    T * R * S; frame 0 (t = 0, identity) equals the static render bit for bit,
+    and frame 1 equals a static render of the mesh pre-moved by the same
+    transform except where a vertex rounds differently (silhouettes)."""
+    import ctypes
+    host = ctypes.CDLL(os.path.join(ROOT, "srt_amd", "libsrt_host.so"))
+    host.srt_host_node_transform.argtypes = [ctypes.c_char_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_char_p,
+                                             ctypes.c_int]
+    s = scenes.cornell()
+    tall = 6                                            # the tall box: OBJ group "mesh6"
+    anim = {"name": f"mesh{tall}", "keys": [0.0, 0.1], "translations": [[0, 0, 0], [0.3, 0.4, 0]],
+            "rotations": [[0, 0, 0, 1], [0, 0.38268343, 0, 0.92387953]]}
+    W, H = 64, 48
+    cam = {"type": "perspective", "position": list(s.camera["position"]), "target": list(s.camera["target"]),
+           "up": list(s.camera["up"]), "fov": 40}
+    passes = [{"name": "PathTracer", "params": {"samples_per_pixel": 4, "max_depth": 5, "seed": 3}}]
+    scenes.save_obj(s, str(tmp_path / "anim.obj"))
+    run({"render": {"type": "video", "output": "v.pfm", "resolution": [W, H], "frame": 2}, "passes": passes,
+         "scene": {"model": "anim.obj", "camera": cam, "animation": [anim]}}, tmp_path)
+    f0, f1 = read_pfm(tmp_path / "v_0000.pfm"), read_pfm(tmp_path / "v_0001.pfm")
+    # static render at t = 0: the passes' seed is seed + frame, so frame 1 of the video uses seed 4
+    run({"render": {"type": "image", "output": "s0.pfm", "resolution": [W, H]}, "passes": passes,
+         "scene": {"model": "anim.obj", "camera": cam}}, tmp_path)
+    assert np.array_equal(f0, read_pfm(tmp_path / "s0.pfm"))
+    M = np.zeros(12, np.float32)
+    t1 = np.float32(0.0) + np.float32(1.0 / 30.0)
+    assert host.srt_host_node_transform(json.dumps(anim).encode(), float(t1), M.ctypes.data, None, 0) == 0
+    M = M.reshape(3, 4).astype(np.float64)
+    moved = s.positions.astype(np.float64).copy()
+    verts = np.unique(s.indices[s.mesh_first[tall]:s.mesh_first[tall + 1]])
+    moved[verts] = moved[verts] @ M[:, :3].T + M[:, 3]
+    s.positions = moved.astype(np.float32)
+    scenes.save_obj(s, str(tmp_path / "moved.obj"))
+    p1 = [{"name": "PathTracer", "params": {"samples_per_pixel": 4, "max_depth": 5, "seed": 4}}]
+    run({"render": {"type": "image", "output": "s1.pfm", "resolution": [W, H]}, "passes": p1,
+         "scene": {"model": "moved.obj", "camera": cam}}, tmp_path)
+    s1 = read_pfm(tmp_path / "s1.pfm")
+    same = np.all(np.abs(f1 - s1) <= 1e-4 * np.maximum(np.abs(f1), np.abs(s1)) + 1e-7, axis=-1)
+    assert same.mean() > 0.97, same.mean()
+    # and the box did move: frame 1 differs from the unmoved scene at the same seed
+    run({"render": {"type": "image", "output": "s2.pfm", "resolution": [W, H]}, "passes": p1,
+         "scene": {"model": "anim.obj", "camera": cam}}, tmp_path)
+    assert np.mean(np.any(np.abs(f1 - read_pfm(tmp_path / "s2.pfm")) > 1e-3, axis=-1)) > 0.05
 
 
 def test_unknown_pass_exits_255(tmp_path):
