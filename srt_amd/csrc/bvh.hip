@@ -536,15 +536,15 @@ __global__ void __launch_bounds__(256) k_emit(int n, int T, int top, int max_lea
     valid[top + i] = 1;
 }
 
-// Sorted triangle records (and, for shading, copies of the vertex normals and
-// tangents in the same order), rewritten after every skinning step.
+// Sorted triangle records for the traversal and the 128-byte shading records
+// of the same triangles (positions, material, vertex normals and tangents),
+// rewritten after every skinning step.
 struct TriSrc {
     const float *pos, *nrm, *tan;
     const uint32_t* idx;
     const int32_t *tri_mesh, *mesh_mat;
 };
-__global__ void __launch_bounds__(256) k_tris(int n, const uint32_t* order, TriSrc src, float4* tris, float4* tri_sn,
-                                              float4* tri_st) {
+__global__ void __launch_bounds__(256) k_tris(int n, const uint32_t* order, TriSrc src, float4* tris, float4* rec) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t p = order[k];
@@ -553,32 +553,39 @@ __global__ void __launch_bounds__(256) k_tris(int n, const uint32_t* order, TriS
     const float* b = src.pos + 3 * (size_t)ix[1];
     const float* c = src.pos + 3 * (size_t)ix[2];
     const int32_t mesh = src.tri_mesh[p];
+    const int32_t mat = src.mesh_mat[mesh];
     tris[3 * (size_t)k + 0] = make_float4(a[0], a[1], a[2], __uint_as_float(p));
     tris[3 * (size_t)k + 1] = make_float4(b[0], b[1], b[2], __int_as_float(mesh));
-    tris[3 * (size_t)k + 2] = make_float4(c[0], c[1], c[2], __int_as_float(src.mesh_mat[mesh]));
+    tris[3 * (size_t)k + 2] = make_float4(c[0], c[1], c[2], __int_as_float(mat));
+    // shading record (layout at DevScene::tri_rec)
+    float nv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int v = 0; v < 3; v++) {
-        if (tri_sn) {
-            const float* q = src.nrm + 3 * (size_t)ix[v];
-            tri_sn[3 * (size_t)k + v] = make_float4(q[0], q[1], q[2], 0.0f);
-        }
-        if (tri_st) {
-            const float* q = src.tan + 3 * (size_t)ix[v];
-            tri_st[3 * (size_t)k + v] = make_float4(q[0], q[1], q[2], 0.0f);
+        for (int j = 0; j < 3; j++) {
+            if (src.nrm) nv[3 * v + j] = src.nrm[3 * (size_t)ix[v] + j];
+            if (src.tan) t[3 * v + j] = src.tan[3 * (size_t)ix[v] + j];
         }
     }
+    float4* r = rec + 8 * (size_t)k;
+    r[0] = make_float4(a[0], a[1], a[2], __int_as_float(mat));
+    r[1] = make_float4(b[0], b[1], b[2], nv[0]);
+    r[2] = make_float4(c[0], c[1], c[2], nv[1]);
+    r[3] = make_float4(nv[2], nv[3], nv[4], nv[5]);
+    r[4] = make_float4(nv[6], nv[7], nv[8], t[0]);
+    r[5] = make_float4(t[1], t[2], t[3], t[4]);
+    r[6] = make_float4(t[5], t[6], t[7], t[8]);
+    r[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 static void launch_tris(SceneRec* s, hipStream_t st) {
     TriSrc src;
     src.pos = (const float*)s->pos.p;
-    src.nrm = (const float*)s->nrm.p;
-    src.tan = (const float*)s->tan.p;
+    src.nrm = s->has_nrm ? (const float*)s->nrm.p : nullptr;
+    src.tan = s->has_tan ? (const float*)s->tan.p : nullptr;
     src.idx = (const uint32_t*)s->idx.p;
     src.tri_mesh = (const int32_t*)s->tri_mesh.p;
     src.mesh_mat = (const int32_t*)s->mesh_mat.p;
     const int n = s->nf;
     k_tris<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, (const uint32_t*)s->order.p, src, (float4*)s->tris.p,
-                                                        s->has_nrm ? (float4*)s->tri_sn.p : nullptr,
-                                                        s->has_tan ? (float4*)s->tri_st.p : nullptr);
+                                                        (float4*)s->tri_rec.p);
 }
 
 // parents in the final tree: parent index * 2 + side, -1 for the root
@@ -953,10 +960,8 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     }
     if (n > 1)
         k_emit<<<nblk(n - 1), 256, 0, st>>>(n, T, top, max_leaf, kl, kr, kfirst, par_int, order, tb, K, nodes, valid);
-    dev_free(c, s->tri_sn);
-    dev_free(c, s->tri_st);
-    if (s->has_nrm && dev_alloc(c, s->tri_sn, (size_t)n * 3 * sizeof(float4))) return SRT_E_OOM;
-    if (s->has_tan && dev_alloc(c, s->tri_st, (size_t)n * 3 * sizeof(float4))) return SRT_E_OOM;
+    dev_free(c, s->tri_rec);
+    if (dev_alloc(c, s->tri_rec, (size_t)n * 8 * sizeof(float4))) return SRT_E_OOM;
     launch_tris(s, st);
     int32_t* parent = (int32_t*)s->node_parent.p;
     k_fill_i32<<<nblk(num_nodes), 256, 0, st>>>(parent, num_nodes, -1);

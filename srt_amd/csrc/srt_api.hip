@@ -87,8 +87,8 @@ void prof_collect(srt_ctx* c) {
 DevScene dev_scene(const SceneRec* s) {
     DevScene S;
     S.tris = (const float4*)s->tris.p;
-    S.tri_sn = s->has_nrm ? (const float4*)s->tri_sn.p : nullptr;
-    S.tri_st = s->has_tan ? (const float4*)s->tri_st.p : nullptr;
+    S.tri_rec = (const float4*)s->tri_rec.p;
+    S.rec_flags = (s->has_nrm ? REC_NORMALS : 0) | (s->has_tan ? REC_TANGENTS : 0);
     S.nodes = (const BvhNode*)s->nodes.p;
     S.root = s->root;
     S.nodes4 = (const Bvh4Node*)s->nodes4.p;
@@ -115,7 +115,7 @@ static void free_scene(srt_ctx* c, SceneRec* s) {
     DevBuf* all[] = {&s->pos, &s->nrm, &s->tan, &s->idx, &s->mesh_mat, &s->mesh_first, &s->area_cdf, &s->mats,
                      &s->light_cdf, &s->light_mesh, &s->tri_mesh, &s->orig_pos, &s->orig_nrm, &s->orig_tan,
                      &s->bone_ids, &s->bone_w, &s->bones, &s->nodes, &s->tris, &s->order, &s->node_parent,
-                     &s->refit_counter, &s->node_valid, &s->nodes4, &s->tri_sn, &s->tri_st};
+                     &s->refit_counter, &s->node_valid, &s->nodes4, &s->tri_rec};
     for (DevBuf* b : all) dev_free(c, *b);
     delete s;
 }
@@ -149,8 +149,8 @@ int srt_create(int device, size_t hbm_budget_bytes, srt_ctx** out) {
     (void)hipMemGetInfo(&free_b, &total_b);
     c->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(free_b * 0.9);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SRT_E_HIP; }
-    if (dev_alloc(c, c->counters, 256)) { (void)hipStreamDestroy(c->stream); delete c; return SRT_E_OOM; }
-    (void)hipMemset(c->counters.p, 0, 256);
+    if (dev_alloc(c, c->counters, kCountersBytes)) { (void)hipStreamDestroy(c->stream); delete c; return SRT_E_OOM; }
+    (void)hipMemset(c->counters.p, 0, kCountersBytes);
     *out = c;
     return SRT_OK;
 }

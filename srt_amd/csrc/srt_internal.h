@@ -63,10 +63,18 @@ struct TraceCfg {
 // ---------------------------------------------------------------------------
 // device view of one resident scene
 // ---------------------------------------------------------------------------
+// Shading record of sorted triangle k: 8 float4 at tri_rec + 8 k, one
+// 128-byte line, so a hit reads its triangle with one line fetch.
+//   [0] v0.xyz, material id   [1] v1.xyz, n0.x    [2] v2.xyz, n0.y
+//   [3] n0.z, n1.xyz          [4] n2.xyz, t0.x    [5] t0.yz, t1.xy
+//   [6] t1.z, t2.xyz          [7] unused (0)
+// (v: vertex positions, n: vertex normals, t: vertex tangents; the parts
+// whose flag is missing are 0)
+enum { REC_NORMALS = 1, REC_TANGENTS = 2 };
 struct DevScene {
     const float4* tris;        // 3 float4 per sorted triangle: (v0, prim) (v1, mesh) (v2, material)
-    const float4* tri_sn;      // 3 float4 per sorted triangle: vertex normals (xyz, 0), or null
-    const float4* tri_st;      // 3 float4 per sorted triangle: vertex tangents (xyz, 0), or null
+    const float4* tri_rec;     // 8 float4 (one 128-byte line) per sorted triangle: its shading record
+    int32_t rec_flags;         // REC_NORMALS | REC_TANGENTS: which parts of the records are filled
     const BvhNode* nodes;
     int32_t root;
     const Bvh4Node* nodes4;    // the same tree, 4 wide
@@ -159,14 +167,27 @@ struct ShadowQueue {
     uint32_t* slot;
 };
 
-// counters that live in device memory between kernels of one chunk
+// Counters that live in device memory between the kernels of one chunk.  Each
+// sits on its own 256-byte line: atomics on one line from the whole GPU are
+// serialised (about 88 M per second on MI355X, whatever the number of
+// counters on the line), so counters used together must not share a line.
+// A fetch counter is split into kFetchPieces counters, one per contiguous
+// piece of its queue (trace_persistent).
+constexpr int kCntLine = 64;           // 32-bit words per 256-byte line
+constexpr int kFetchPieces = 32;
 enum {
-    CNT_Q0 = 0, CNT_Q1 = 1, CNT_SHADOW = 2, CNT_MLPQ = 3,
-    CNT_FETCH_T = 5,     // next queue entry to trace (cleared by raygen and shade)
-    CNT_FETCH_S = 6,     // next shadow entry to trace (cleared by trace)
-    CNT_FETCH_Q = 7,     // next standalone ray query
-    CNT_MAX = 16
+    CNT_Q0 = 0 * kCntLine, CNT_Q1 = 1 * kCntLine,    // path queue sizes (cnt_q)
+    CNT_SHADOW = 2 * kCntLine, CNT_MLPQ = 3 * kCntLine,
+    CNT_FETCH_T = 4 * kCntLine,                        // next queue entry to trace (cleared by raygen and shade)
+    CNT_FETCH_S = CNT_FETCH_T + kFetchPieces * kCntLine,   // next shadow entry to trace (cleared by trace)
+    CNT_FETCH_Q = CNT_FETCH_S + kFetchPieces * kCntLine,   // next standalone ray query
+    CNT_WORDS = CNT_FETCH_Q + kFetchPieces * kCntLine
 };
+// statistics (unsigned 64-bit) after the counters, then the end of the buffer
+constexpr size_t kTotalsOffset = (size_t)CNT_WORDS * 4;
+constexpr size_t kCountersBytes = kTotalsOffset + 256;
+// size counter of path queue qi (0 or 1)
+__host__ __device__ inline int cnt_q(int qi) { return qi * kCntLine; }
 
 // ---------------------------------------------------------------------------
 // host-side records
@@ -189,7 +210,7 @@ struct SceneRec {
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};   // scene box (MLP input normalisation)
     bool has_bvh = false;
     DevBuf nodes, tris, order, node_parent, refit_counter, node_valid;
-    DevBuf tri_sn, tri_st;       // per sorted triangle: copies of its vertex normals / tangents
+    DevBuf tri_rec;              // per sorted triangle: 128-byte shading record (positions, material, normals, tangents)
     int32_t root = 0, num_nodes = 0, max_depth = 0, num_clusters = 0, num_top = 0, cluster_size = 0;
     DevBuf nodes4;
     int32_t root4 = 0, num_nodes4 = 0, depth4 = 0;     // 4-wide tree: nodes, levels

@@ -102,6 +102,12 @@ SRT_D int wave_append(bool pred, uint32_t* counter) {
     return pred ? (int)(base + prefix) : -1;
 }
 
+// Zeroes the kFetchPieces counters of a split fetch counter; run by the
+// threads 0..kFetchPieces-1 of one workgroup.
+SRT_D void clear_fetch(uint32_t* fetch) {
+    if (threadIdx.x < (unsigned)kFetchPieces) fetch[threadIdx.x * kCntLine] = 0u;
+}
+
 // One ray's traversal state, kept in registers while its lane walks the BVH.
 struct TravState {
     f3 o, inv;
@@ -338,17 +344,26 @@ SRT_D bool trav_leaf(const DevScene& S, TravState& st, uint32_t& nt) {
 // stores the finished lanes' results and takes that many new ray indices from
 // `fetch` with one atomic, so lanes do not idle while the wave's slowest ray
 // finishes, and the stores' appends and memory traffic happen once per refill.
+// [0, count) is cut into P = min(kFetchPieces, grid size) contiguous pieces;
+// workgroup b works on piece b mod P with that piece's counter at
+// fetch + piece * kCntLine, so the fetch atomics spread over P lines.
 // IO supplies
 //   void load(uint32_t i, TravState&, int32_t root)
 //   void store(uint32_t i, const TravState&, bool occluded)   (on the storing lanes)
-// Every wave leaves the loop once `fetch` has passed `count` and its lanes
-// have finished and stored.
+// Every wave leaves the loop once its piece's counter has passed the piece's
+// end and its lanes have finished and stored.
 template <bool ANY, bool COUNT, bool WIDE, class IO>
 SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, int refill, const TravStack& k, IO& io,
                             uint32_t& nv, uint32_t& nt) {
     const int32_t root = WIDE ? S.root4 : S.root;
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
+    // this workgroup's piece of [0, count) and the piece's own counter
+    const uint32_t npieces = gridDim.x < (unsigned)kFetchPieces ? gridDim.x : (uint32_t)kFetchPieces;
+    const uint32_t piece = blockIdx.x % npieces;
+    const uint32_t lo = (uint32_t)((uint64_t)count * piece / npieces);
+    const uint32_t hi = (uint32_t)((uint64_t)count * (piece + 1u) / npieces);
+    fetch += piece * kCntLine;
     TravState st;
     st.cur_ok = false;
     st.has_pend = false;
@@ -368,8 +383,8 @@ SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, 
             if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
             base = __shfl(base, leader);
             if (idle) {
-                const uint32_t idx = base + (uint32_t)__popcll(m & below);
-                if (idx < count) {
+                const uint32_t idx = lo + base + (uint32_t)__popcll(m & below);
+                if (idx < hi) {
                     ri = idx;
                     has = true;
                     occl = false;
@@ -449,10 +464,8 @@ SRT_D void camera_ray(const DevCamera& cam, int width, int height, uint32_t pixe
 __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* Lg, float* Lb, const Segment* segs, int nseg,
                                                 int n, uint32_t* counts) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        counts[CNT_Q0] = (uint32_t)n;
-        counts[CNT_FETCH_T] = 0u;
-    }
+    if (i == 0) counts[CNT_Q0] = (uint32_t)n;
+    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_T);
     if (i >= n) return;
     const int k = find_segment(segs, nseg, (uint32_t)i);
     const Segment& sg = segs[k];
@@ -487,6 +500,14 @@ struct QueueIO {       // closest hit of queue entries -> hit buffer (same index
         h.tri[i] = hit ? st.best_pos : -1;
     }
 };
+// L[slot] += c for the three channels; the three loads go out before the stores
+SRT_D void add_rgb(float* Lr, float* Lg, float* Lb, uint32_t slot, float r, float g, float b) {
+    const float r0 = Lr[slot], g0 = Lg[slot], b0 = Lb[slot];
+    Lr[slot] = r0 + r;
+    Lg[slot] = g0 + g;
+    Lb[slot] = b0 + b;
+}
+
 struct ShadowIO {      // any hit in [1e-3, tmax]; unoccluded rays add their contribution
     ShadowQueue sq;
     float *Lr, *Lg, *Lb;
@@ -496,9 +517,7 @@ struct ShadowIO {      // any hit in [1e-3, tmax]; unoccluded rays add their con
     SRT_D void store(uint32_t i, const TravState&, bool occluded) const {
         if (occluded) return;
         const uint32_t slot = sq.slot[i];     // one shadow ray per slot per bounce: no write conflicts
-        Lr[slot] = Lr[slot] + sq.cr[i];
-        Lg[slot] = Lg[slot] + sq.cg[i];
-        Lb[slot] = Lb[slot] + sq.cb[i];
+        add_rgb(Lr, Lg, Lb, slot, sq.cr[i], sq.cg[i], sq.cb[i]);
     }
 };
 
@@ -520,12 +539,12 @@ SRT_D TravStack make_stack(int32_t* gstack, TraceCfg cfg) {
 template <bool COUNT, bool WIDE>
 __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
                                                       int32_t* gstack, unsigned long long* totals, TraceCfg cfg) {
-    const uint32_t count = counts[qi];
+    const uint32_t count = counts[cnt_q(qi)];
+    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_S);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counts[qi ^ 1] = 0u;
+        counts[cnt_q(qi ^ 1)] = 0u;
         counts[CNT_SHADOW] = 0u;
         counts[CNT_MLPQ] = 0u;
-        counts[CNT_FETCH_S] = 0u;
         totals[0] += count;
     }
     const TravStack k = make_stack(gstack, cfg);
@@ -591,22 +610,64 @@ struct HitGeom {
 };
 SRT_D HitGeom hit_geometry(const DevScene& S, int32_t tri, float u, float v) {
     HitGeom g;
-    const float4* tp = S.tris + 3 * (size_t)tri;
-    const float4 A = tp[0], B = tp[1], C = tp[2];
-    g.matid = __float_as_int(C.w);
-    const f3 v0 = mk(A.x, A.y, A.z), v1 = mk(B.x, B.y, B.z), v2 = mk(C.x, C.y, C.z);
+    // the first five parts of the shading record, read together (one line)
+    const float4* r = S.tri_rec + 8 * (size_t)tri;
+    const float4 R0 = r[0], R1 = r[1], R2 = r[2], R3 = r[3], R4 = r[4];
+    g.matid = __float_as_int(R0.w);
+    const f3 v0 = mk(R0.x, R0.y, R0.z), v1 = mk(R1.x, R1.y, R1.z), v2 = mk(R2.x, R2.y, R2.z);
     g.w = 1.0f - u - v;
     g.pos = v0 * g.w + v1 * u + v2 * v;
     g.normal = normalize(cross(v1 - v0, v2 - v0));
     g.sn = g.normal;
-    if (S.tri_sn) {
-        // the vertex normals of this triangle, copied in sorted order
-        const float4* q = S.tri_sn + 3 * (size_t)tri;
-        const float4 n0 = q[0], n1 = q[1], n2 = q[2];
-        g.sn = normalize(mk(n0.x, n0.y, n0.z) * g.w + mk(n1.x, n1.y, n1.z) * u + mk(n2.x, n2.y, n2.z) * v);
+    if (S.rec_flags & REC_NORMALS) {
+        const f3 n0 = mk(R1.w, R2.w, R3.x), n1 = mk(R3.y, R3.z, R3.w), n2 = mk(R4.x, R4.y, R4.z);
+        g.sn = normalize(n0 * g.w + n1 * u + n2 * v);
     }
     if (dot(g.normal, g.sn) < 0.0f) g.normal = -g.normal;
     return g;
+}
+
+// The leading 64 bytes of a segment (path range and depth settings), read
+// together as eight 8-byte parts.
+struct SegHead {
+    int64_t path_begin;
+    int32_t n, slot_offset, wh, width, height, spp;
+    uint32_t seed;
+    int32_t max_depth, rr_depth, use_nee, use_mis, mlp_mode, last;
+    float cam0;
+};
+static_assert(sizeof(SegHead) == 64 && offsetof(Segment, last) == offsetof(SegHead, last) &&
+                  offsetof(Segment, mlp_mode) == offsetof(SegHead, mlp_mode) && sizeof(Segment) >= 64 &&
+                  alignof(Segment) >= 8,
+              "SegHead mirrors the head of Segment");
+SRT_D SegHead load_seg_head(const Segment* segs, uint32_t k) {
+    const uint2* p = reinterpret_cast<const uint2*>(segs + k);
+    const uint2 q[8] = {p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]};
+    SegHead h;
+    __builtin_memcpy(&h, q, sizeof(SegHead));
+    return h;
+}
+
+// Tangent for the shading frame (hit_info.h:51-55): the interpolated vertex
+// tangent when the scene has tangents, else an axis picked from the shading
+// normal.
+SRT_D f3 hit_tangent(const DevScene& S, int32_t tri, const HitGeom& g, float u, float v) {
+    if (S.rec_flags & REC_TANGENTS) {
+        const float4* r = S.tri_rec + 8 * (size_t)tri;
+        const float4 R4 = r[4], R5 = r[5], R6 = r[6];
+        const f3 t0 = mk(R4.w, R5.x, R5.y), t1 = mk(R5.z, R5.w, R6.x), t2 = mk(R6.y, R6.z, R6.w);
+        return t0 * g.w + t1 * u + t2 * v;
+    }
+    return (fabsf(g.sn.x) > fabsf(g.sn.y)) ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+}
+
+// A material record by value: its five 16-byte parts are read together.
+SRT_D MatRec load_mat(const MatRec* mats, int32_t id) {
+    const float4* p = reinterpret_cast<const float4*>(mats + id);
+    const float4 q[5] = {p[0], p[1], p[2], p[3], p[4]};
+    MatRec m;
+    __builtin_memcpy(&m, q, sizeof(MatRec));
+    return m;
 }
 
 SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const PathQueue& qin, const HitBuf& h,
@@ -619,13 +680,13 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     const uint32_t flags = qin.flags[i];
     const bool specular = (flags & 1u) != 0u;
     const uint32_t segk = flags >> 8;
-    const Segment& P = segs[segk];
+    const SegHead P = load_seg_head(segs, segk);
     const float u = h.u[i], v = h.v[i];
     const HitGeom hg = hit_geometry(S, tri, u, v);
     const float w = hg.w;
     const f3 pos = hg.pos, sn = hg.sn;
     f3 normal = hg.normal;
-    const MatRec& m = S.mats[hg.matid];
+    const MatRec m = load_mat(S.mats, hg.matid);
     const f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
     if (mat_is_emissive(m)) {
         // emitter hit with MIS (path_tracer.cu:88-106)
@@ -644,18 +705,11 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
             }
             const f3 Le = mk(m.emission_color[0], m.emission_color[1], m.emission_color[2]) * m.intensity;
             const f3 add = beta * Le * mis;
-            Lr[slot] = Lr[slot] + add.x; Lg[slot] = Lg[slot] + add.y; Lb[slot] = Lb[slot] + add.z;
+            add_rgb(Lr, Lg, Lb, slot, add.x, add.y, add.z);
         }
         return;
     }
-    f3 tg;
-    if (S.tri_st) {
-        const float4* q = S.tri_st + 3 * (size_t)tri;
-        const float4 t0 = q[0], t1 = q[1], t2 = q[2];
-        tg = mk(t0.x, t0.y, t0.z) * w + mk(t1.x, t1.y, t1.z) * u + mk(t2.x, t2.y, t2.z) * v;
-    } else {
-        tg = (fabsf(sn.x) > fabsf(sn.y)) ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
-    }
+    const f3 tg = hit_tangent(S, tri, hg, u, v);
     const ShadePoint spt = shade_point(m, -rd, make_onb(sn, tg), color);
     if (dot(normal, -rd) < 0.0f) normal = -normal;
     const bool last = (depth + 1 >= P.max_depth);
@@ -740,7 +794,7 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
             else beta = beta / q;
         }
     }
-    const int jn = wave_append(alive, counts + (qi ^ 1));
+    const int jn = wave_append(alive, counts + cnt_q(qi ^ 1));
     if (jn >= 0) {
         qout.ox[jn] = pos.x; qout.oy[jn] = pos.y; qout.oz[jn] = pos.z;
         qout.dx[jn] = ms.wi.x; qout.dy[jn] = ms.wi.y; qout.dz[jn] = ms.wi.z;
@@ -773,8 +827,8 @@ __global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segm
     const unsigned long long below = (1ull << lane) - 1ull;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
-    const uint32_t count = counts[qi];
-    if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_FETCH_T] = 0u;   // the next trace stage fetches from 0
+    const uint32_t count = counts[cnt_q(qi)];
+    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_T);   // the next trace stage fetches from 0
     const f3 env = mk(S.env[0], S.env[1], S.env[2]);
     uint32_t nbuf = 0;                     // gathered hits (wave-uniform)
     uint32_t base = (uint32_t)wave * 64u;
@@ -787,7 +841,7 @@ __global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segm
             if (valid && tri < 0) {
                 const uint32_t slot = qin.slot[i];
                 const f3 add = mk(qin.br[i], qin.bg[i], qin.bb[i]) * env;
-                Lr[slot] = Lr[slot] + add.x; Lg[slot] = Lg[slot] + add.y; Lb[slot] = Lb[slot] + add.z;
+                add_rgb(Lr, Lg, Lb, slot, add.x, add.y, add.z);
             }
             const bool hit = valid && tri >= 0;
             const unsigned long long m = __ballot(hit);
@@ -1045,10 +1099,8 @@ SRT_D void splat(float4* film, int idx, f3 L) {
 
 __global__ void __launch_bounds__(256) k_lt_raygen(DevScene S, LtParams P, PathQueue q, int n, uint32_t* counts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        counts[CNT_Q0] = (uint32_t)n;
-        counts[CNT_FETCH_T] = 0u;
-    }
+    if (i == 0) counts[CNT_Q0] = (uint32_t)n;
+    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_T);
     if (i >= n) return;
     const int64_t wh = (int64_t)P.width * P.height;
     const int64_t pid = P.path_begin + i;
@@ -1077,8 +1129,8 @@ __global__ void __launch_bounds__(256) k_lt_raygen(DevScene S, LtParams P, PathQ
 
 __global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int depth, PathQueue qin, HitBuf h,
                                                  PathQueue qout, ShadowQueue sq, float4* film, uint32_t* counts, int qi) {
-    const uint32_t count = counts[qi];
-    if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_FETCH_T] = 0u;   // the next trace stage fetches from 0
+    const uint32_t count = counts[cnt_q(qi)];
+    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_T);   // the next trace stage fetches from 0
     const f3 cam_pos = ld3(P.cam.pos);
     const int64_t wh = (int64_t)P.width * P.height;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
@@ -1107,15 +1159,7 @@ __global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int de
         f3 normal = hg.normal;
         if (dot(-rd, normal) < 0.0f) normal = -normal;
         const f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
-        f3 tg;
-        if (S.tri_st) {
-            const float4* q = S.tri_st + 3 * (size_t)tri;
-            const float4 t0 = q[0], t1 = q[1], t2 = q[2];
-            const float u = h.u[i], v = h.v[i];
-            tg = mk(t0.x, t0.y, t0.z) * hg.w + mk(t1.x, t1.y, t1.z) * u + mk(t2.x, t2.y, t2.z) * v;
-        } else {
-            tg = (fabsf(hg.sn.x) > fabsf(hg.sn.y)) ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
-        }
+        const f3 tg = hit_tangent(S, tri, hg, h.u[i], h.v[i]);
         const ShadePoint spt = shade_point(m, -rd, make_onb(hg.sn, tg), color);
         Pcg32 rng = rng_resume(qin.rng[i], sample);
         // camera connection (light_tracer.cu:124-142): the contribution waits in the shadow queue
@@ -1159,7 +1203,7 @@ __global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int de
                 else beta = beta / p;
             }
         }
-        const int jn = wave_append(alive, counts + (qi ^ 1));
+        const int jn = wave_append(alive, counts + cnt_q(qi ^ 1));
         if (jn >= 0) {
             qout.ox[jn] = hg.pos.x; qout.oy[jn] = hg.pos.y; qout.oz[jn] = hg.pos.z;
             qout.dx[jn] = ms.wi.x; qout.dy[jn] = ms.wi.y; qout.dz[jn] = ms.wi.z;
@@ -1198,7 +1242,7 @@ __global__ void __launch_bounds__(256) k_film_clear(float4* film, int64_t n) {
     if (i < n) film[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
 }
 
-__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+__global__ void k_clear_fetch(uint32_t* fetch) { clear_fetch(fetch); }
 
 // ---------------------------------------------------------------------------
 // post-processing (tonemapping.cu:14-68, accumulate.cu:22-26)
@@ -1481,7 +1525,7 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     }
     DevScene S = dev_scene(s);
     uint32_t* counts = (uint32_t*)c->counters.p;
-    unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + 64);
+    unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + kTotalsOffset);
     PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
     HitBuf H;
     H.t = (float*)c->hits[0].p; H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
@@ -1579,7 +1623,7 @@ int render_wavefront(srt_ctx* c, SceneRec* s, const DevCamera& cam, const DevPar
         c->film_pixels = wh;
     }
     hipStream_t st = c->stream;
-    unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + 64);
+    unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + kTotalsOffset);
     {
         size_t n3 = (size_t)wh * 3;
         k_zero<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>((float*)c->film_acc.p, n3);
@@ -1647,7 +1691,7 @@ int render_simple(srt_ctx* c, SceneRec* s, const DevCamera& cam, int type, int s
     prof_begin(c, "simple", &ev);
     for (int smp = 0; smp < spp; smp++) {
         io.sample = (uint32_t)smp;
-        k_set_u32<<<1, 1, 0, c->stream>>>(fetch, 0u);
+        k_clear_fetch<<<1, kFetchPieces, 0, c->stream>>>(fetch);
         if (cfg.wide) k_simple_trace<true><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch,
                                                                                                   (int32_t*)c->gstack.p, cfg);
         else k_simple_trace<false><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch,
@@ -1677,7 +1721,7 @@ int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max
     const size_t tlds = trace_lds_bytes(tcfg);
     hipStream_t st = c->stream;
     uint32_t* counts = (uint32_t*)c->counters.p;
-    unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + 64);
+    unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + kTotalsOffset);
     PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
     HitBuf H;
     H.t = (float*)c->hits[0].p; H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
@@ -1725,7 +1769,7 @@ int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* 
     io.o3 = o_dev; io.d3 = d_dev; io.tmax = tmax_dev; io.tmin = tmin;
     io.t = t; io.u = u; io.v = v; io.prim = prim; io.tris = S.tris; io.any = any;
     uint32_t* fetch = (uint32_t*)c->counters.p + CNT_FETCH_Q;
-    k_set_u32<<<1, 1, 0, c->stream>>>(fetch, 0u);
+    k_clear_fetch<<<1, kFetchPieces, 0, c->stream>>>(fetch);
     k_trace_rays<<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch, (int32_t*)c->gstack.p,
                                                                          cfg);
     if (int r = hip_check(c, hipGetLastError(), "trace launch")) return r;
