@@ -171,23 +171,47 @@ struct ShadowQueue {
 // sits on its own 256-byte line: atomics on one line from the whole GPU are
 // serialised (about 88 M per second on MI355X, whatever the number of
 // counters on the line), so counters used together must not share a line.
-// A fetch counter is split into kFetchPieces counters, one per contiguous
-// piece of its queue (trace_persistent).
+//
+// The path queues, their hit records and the shadow queue are cut into kParts
+// partitions with a size counter each: partition p holds entries
+// [p cap, p cap + size_p), cap = part_cap(chunk paths).  A stage reads its
+// input queue as the concatenation of the partitions, cut into kParts
+// contiguous pieces of equal length (QueueView): workgroup b works on piece
+// b mod kParts and appends what it produces to partition b mod kParts of the
+// next queue.  The appends of a stage thus spread over kParts counters, the
+// pieces keep the work balanced and spatially coherent (camera paths fill the
+// partitions in slot order), and no partition can outgrow cap: a piece holds
+// at most ceil(paths / kParts) <= cap entries and each entry appends at most
+// one entry to each queue.  Fetch counters have one counter per piece.
 constexpr int kCntLine = 64;           // 32-bit words per 256-byte line
-constexpr int kFetchPieces = 32;
+constexpr int kParts = 32;             // queue partitions = fetch pieces
 enum {
-    CNT_Q0 = 0 * kCntLine, CNT_Q1 = 1 * kCntLine,    // path queue sizes (cnt_q)
-    CNT_SHADOW = 2 * kCntLine, CNT_MLPQ = 3 * kCntLine,
-    CNT_FETCH_T = 4 * kCntLine,                        // next queue entry to trace (cleared by raygen and shade)
-    CNT_FETCH_S = CNT_FETCH_T + kFetchPieces * kCntLine,   // next shadow entry to trace (cleared by trace)
-    CNT_FETCH_Q = CNT_FETCH_S + kFetchPieces * kCntLine,   // next standalone ray query
-    CNT_WORDS = CNT_FETCH_Q + kFetchPieces * kCntLine
+    CNT_Q0 = 0,                                        // kParts lines: sizes of path queue 0 (cnt_q)
+    CNT_Q1 = CNT_Q0 + kParts * kCntLine,               // kParts lines: path queue 1
+    CNT_SHADOW = CNT_Q1 + kParts * kCntLine,           // kParts lines: shadow queue
+    CNT_FETCH_T = CNT_SHADOW + kParts * kCntLine,      // kParts lines: next queue entry to trace (cleared by raygen and shade)
+    CNT_FETCH_S = CNT_FETCH_T + kParts * kCntLine,     // kParts lines: next shadow entry to trace (cleared by trace)
+    CNT_FETCH_Q = CNT_FETCH_S + kParts * kCntLine,     // kParts lines: next standalone ray query
+    CNT_MLPQ = CNT_FETCH_Q + kParts * kCntLine,        // one line: MLP query queue (dense)
+    CNT_WORDS = CNT_MLPQ + kCntLine
 };
 // statistics (unsigned 64-bit) after the counters, then the end of the buffer
 constexpr size_t kTotalsOffset = (size_t)CNT_WORDS * 4;
 constexpr size_t kCountersBytes = kTotalsOffset + 256;
-// size counter of path queue qi (0 or 1)
-__host__ __device__ inline int cnt_q(int qi) { return qi * kCntLine; }
+// first size counter of path queue qi (0 or 1)
+__host__ __device__ inline int cnt_q(int qi) { return qi ? CNT_Q1 : CNT_Q0; }
+// entries per partition for a chunk of n paths (a multiple of 64)
+__host__ __device__ inline uint32_t part_cap(uint32_t n) {
+    const uint32_t turns = (n + 63u) / 64u;
+    return (turns + (uint32_t)kParts - 1u) / (uint32_t)kParts * 64u;
+}
+// camera paths in partition p: slots [p cap, min((p + 1) cap, n)), at their own index
+__host__ __device__ inline uint32_t part_size(uint32_t n, uint32_t cap, uint32_t p) {
+    const uint64_t first = (uint64_t)p * cap;
+    if (first >= n) return 0u;
+    const uint64_t left = n - first;
+    return left < cap ? (uint32_t)left : cap;
+}
 
 // ---------------------------------------------------------------------------
 // host-side records

@@ -102,11 +102,13 @@ SRT_D int wave_append(bool pred, uint32_t* counter) {
     return pred ? (int)(base + prefix) : -1;
 }
 
-// Zeroes the kFetchPieces counters of a split fetch counter; run by the
-// threads 0..kFetchPieces-1 of one workgroup.
-SRT_D void clear_fetch(uint32_t* fetch) {
-    if (threadIdx.x < (unsigned)kFetchPieces) fetch[threadIdx.x * kCntLine] = 0u;
+// Zeroes the kParts counters (one line apart) that start at `first`; run by
+// the threads 0..kParts-1 of one workgroup.
+SRT_D void clear_parts(uint32_t* first) {
+    if (threadIdx.x < (unsigned)kParts) first[threadIdx.x * kCntLine] = 0u;
 }
+// Partition of this workgroup in a partitioned queue (grid: a multiple of kParts)
+SRT_D uint32_t my_part() { return blockIdx.x % (uint32_t)kParts; }
 
 // One ray's traversal state, kept in registers while its lane walks the BVH.
 struct TravState {
@@ -344,26 +346,21 @@ SRT_D bool trav_leaf(const DevScene& S, TravState& st, uint32_t& nt) {
 // stores the finished lanes' results and takes that many new ray indices from
 // `fetch` with one atomic, so lanes do not idle while the wave's slowest ray
 // finishes, and the stores' appends and memory traffic happen once per refill.
-// [0, count) is cut into P = min(kFetchPieces, grid size) contiguous pieces;
-// workgroup b works on piece b mod P with that piece's counter at
-// fetch + piece * kCntLine, so the fetch atomics spread over P lines.
+// The wave works on indices [lo, hi), taken from `fetch` (lo + fetch), of
+// entries view.entry(index); the workgroups of a grid work on separate ranges
+// with separate counters (queue_piece, dense_range), so the fetch atomics
+// spread over lines.
 // IO supplies
 //   void load(uint32_t i, TravState&, int32_t root)
 //   void store(uint32_t i, const TravState&, bool occluded)   (on the storing lanes)
-// Every wave leaves the loop once its piece's counter has passed the piece's
-// end and its lanes have finished and stored.
-template <bool ANY, bool COUNT, bool WIDE, class IO>
-SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, int refill, const TravStack& k, IO& io,
-                            uint32_t& nv, uint32_t& nt) {
+// Every wave leaves the loop once `fetch` has passed hi - lo and its lanes
+// have finished and stored.
+template <bool ANY, bool COUNT, bool WIDE, class IO, class View>
+SRT_D void trace_persistent(const DevScene& S, const View& view, uint32_t lo, uint32_t hi, uint32_t* fetch, int refill,
+                            const TravStack& k, IO& io, uint32_t& nv, uint32_t& nt) {
     const int32_t root = WIDE ? S.root4 : S.root;
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
-    // this workgroup's piece of [0, count) and the piece's own counter
-    const uint32_t npieces = gridDim.x < (unsigned)kFetchPieces ? gridDim.x : (uint32_t)kFetchPieces;
-    const uint32_t piece = blockIdx.x % npieces;
-    const uint32_t lo = (uint32_t)((uint64_t)count * piece / npieces);
-    const uint32_t hi = (uint32_t)((uint64_t)count * (piece + 1u) / npieces);
-    fetch += piece * kCntLine;
     TravState st;
     st.cur_ok = false;
     st.has_pend = false;
@@ -385,10 +382,10 @@ SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, 
             if (idle) {
                 const uint32_t idx = lo + base + (uint32_t)__popcll(m & below);
                 if (idx < hi) {
-                    ri = idx;
+                    ri = view.entry(idx);
                     has = true;
                     occl = false;
-                    io.load(idx, st, root);
+                    io.load(ri, st, root);
                 } else {
                     exhausted = true;
                 }
@@ -406,6 +403,54 @@ SRT_D void trace_persistent(const DevScene& S, uint32_t count, uint32_t* fetch, 
             }
         }
     }
+}
+
+// A partitioned queue read as the concatenation of its partitions: prefix[p]
+// entries come before partition p, prefix[kParts] is the total (in LDS).
+struct QueueView {
+    const uint32_t* prefix;
+    uint32_t cap;
+    uint32_t p0;        // partition of the first entry of this workgroup's piece
+    // queue entry of concatenated index j (prefix[p0] <= j < total)
+    SRT_D uint32_t entry(uint32_t j) const {
+        uint32_t p = p0;
+        while (j >= prefix[p + 1]) ++p;
+        return p * cap + (j - prefix[p]);
+    }
+};
+// Workgroup prologue, run by all its threads: the prefix sums of the kParts
+// size counters at counts[cnt] into lds[0..kParts], and this workgroup's piece
+// [lo, hi) of the concatenation (piece blockIdx mod kParts of kParts equal ones).
+SRT_D QueueView queue_piece(const uint32_t* counts, int cnt, uint32_t cap, uint32_t* lds, uint32_t& lo, uint32_t& hi) {
+    if (threadIdx.x < 64u) {
+        uint32_t v = threadIdx.x < (unsigned)kParts ? counts[cnt + threadIdx.x * kCntLine] : 0u;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(v, d);
+            if ((int)threadIdx.x >= d) v += t;
+        }
+        if (threadIdx.x < (unsigned)kParts) lds[threadIdx.x + 1] = v;
+        if (threadIdx.x == 0) lds[0] = 0u;
+    }
+    __syncthreads();
+    const uint32_t total = lds[kParts];
+    const uint32_t piece = my_part();
+    lo = (uint32_t)((uint64_t)total * piece / kParts);
+    hi = (uint32_t)((uint64_t)total * (piece + 1u) / kParts);
+    QueueView v{lds, cap, 0u};
+    while (v.p0 + 1u < (unsigned)kParts && lo >= lds[v.p0 + 1]) ++v.p0;
+    return v;
+}
+struct IdentityEntry {
+    SRT_D uint32_t entry(uint32_t j) const { return j; }
+};
+// A dense array [0, n): workgroup b works on piece b mod P of P = min(kParts,
+// grid size) contiguous pieces, with the piece's counter at fetch + piece lines.
+SRT_D uint32_t* dense_range(uint32_t n, uint32_t* fetch, uint32_t& lo, uint32_t& hi) {
+    const uint32_t np = gridDim.x < (unsigned)kParts ? gridDim.x : (uint32_t)kParts;
+    const uint32_t piece = blockIdx.x % np;
+    lo = (uint32_t)((uint64_t)n * piece / np);
+    hi = (uint32_t)((uint64_t)n * (piece + 1u) / np);
+    return fetch + piece * kCntLine;
 }
 
 // ---------------------------------------------------------------------------
@@ -458,14 +503,17 @@ SRT_D void camera_ray(const DevCamera& cam, int width, int height, uint32_t pixe
     rd = dir.x * X + dir.y * Y + dir.z * Z;
 }
 
-// One camera ray per slot.  Slot i belongs to segment k: path id
-// pid = path_begin + (i - slot_offset), sample = pid / (W H),
+// One camera ray per slot, at queue entry = slot.  Slot i belongs to
+// segment k: path id pid = path_begin + (i - slot_offset), sample = pid / (W H),
 // pixel = tile_pixel(pid mod (W H)).
 __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* Lg, float* Lb, const Segment* segs, int nseg,
-                                                int n, uint32_t* counts) {
+                                                int n, uint32_t cap, uint32_t* counts) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) counts[CNT_Q0] = (uint32_t)n;
-    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_T);
+    if (blockIdx.x == 0) {
+        clear_parts(counts + CNT_FETCH_T);
+        if (threadIdx.x < (unsigned)kParts)
+            counts[CNT_Q0 + threadIdx.x * kCntLine] = part_size((uint32_t)n, cap, threadIdx.x);
+    }
     if (i >= n) return;
     const int k = find_segment(segs, nseg, (uint32_t)i);
     const Segment& sg = segs[k];
@@ -475,13 +523,14 @@ __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* L
     f3 ro, rd;
     Pcg32 rng;
     camera_ray(sg.cam, sg.width, sg.height, pixel, sg.seed, sample, ro, rd, rng);
-    q.ox[i] = ro.x; q.oy[i] = ro.y; q.oz[i] = ro.z;
-    q.dx[i] = rd.x; q.dy[i] = rd.y; q.dz[i] = rd.z;
-    q.br[i] = 1.0f; q.bg[i] = 1.0f; q.bb[i] = 1.0f;
-    q.pdf[i] = 1.0f;
-    q.flags[i] = 1u | ((uint32_t)k << 8);        // bit 0: specular, bits 8..: segment
-    q.rng[i] = rng.state;
-    q.slot[i] = (uint32_t)i;
+    const uint32_t e = (uint32_t)i;           // partition i / cap, in slot order
+    q.ox[e] = ro.x; q.oy[e] = ro.y; q.oz[e] = ro.z;
+    q.dx[e] = rd.x; q.dy[e] = rd.y; q.dz[e] = rd.z;
+    q.br[e] = 1.0f; q.bg[e] = 1.0f; q.bb[e] = 1.0f;
+    q.pdf[e] = 1.0f;
+    q.flags[e] = 1u | ((uint32_t)k << 8);        // bit 0: specular, bits 8..: segment
+    q.rng[e] = rng.state;
+    q.slot[e] = (uint32_t)i;
     Lr[i] = 0.0f; Lg[i] = 0.0f; Lb[i] = 0.0f;
 }
 
@@ -538,19 +587,23 @@ SRT_D TravStack make_stack(int32_t* gstack, TraceCfg cfg) {
 // algorithmic byte count).
 template <bool COUNT, bool WIDE>
 __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
-                                                      int32_t* gstack, unsigned long long* totals, TraceCfg cfg) {
-    const uint32_t count = counts[cnt_q(qi)];
-    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_S);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counts[cnt_q(qi ^ 1)] = 0u;
-        counts[CNT_SHADOW] = 0u;
-        counts[CNT_MLPQ] = 0u;
-        totals[0] += count;
+                                                      uint32_t cap, int32_t* gstack, unsigned long long* totals,
+                                                      TraceCfg cfg) {
+    __shared__ uint32_t prefix[kParts + 1];
+    uint32_t lo, hi;
+    const QueueView view = queue_piece(counts, cnt_q(qi), cap, prefix, lo, hi);
+    uint32_t* fetch = counts + CNT_FETCH_T + my_part() * kCntLine;
+    if (blockIdx.x == 0) {
+        clear_parts(counts + CNT_FETCH_S);
+        clear_parts(counts + cnt_q(qi ^ 1));
+        clear_parts(counts + CNT_SHADOW);
+        if (threadIdx.x == 0) counts[CNT_MLPQ] = 0u;
     }
+    if (blockIdx.x < (unsigned)kParts && threadIdx.x == 0) atomicAdd(&totals[0], (unsigned long long)(hi - lo));
     const TravStack k = make_stack(gstack, cfg);
     QueueIO io{q, h};
     uint32_t nv = 0, nt = 0;
-    trace_persistent<false, COUNT, WIDE>(S, count, counts + CNT_FETCH_T, cfg.refill, k, io, nv, nt);
+    trace_persistent<false, COUNT, WIDE>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
     if (COUNT) {
         atomicAdd(&totals[2], (unsigned long long)nv);
         atomicAdd(&totals[3], (unsigned long long)nt);
@@ -672,7 +725,7 @@ SRT_D MatRec load_mat(const MatRec* mats, int32_t id) {
 
 SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const PathQueue& qin, const HitBuf& h,
                      const PathQueue& qout, const ShadowQueue& sq, float* Lr, float* Lg, float* Lb, uint32_t* counts,
-                     int qi, const MlpIO& mio, uint32_t i) {
+                     int qi, uint32_t part, uint32_t cap, const MlpIO& mio, uint32_t i) {
     const uint32_t slot = qin.slot[i];
     const f3 rd = mk(qin.dx[i], qin.dy[i], qin.dz[i]);
     f3 beta = mk(qin.br[i], qin.bg[i], qin.bb[i]);
@@ -739,8 +792,9 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
             const f3 f = sp_eval(m, spt, sdir, color);
             s_c = beta * f * fabsf(cos_o) * ls.emission * mis / lpdf;
         }
-        const int js = wave_append(emit, counts + CNT_SHADOW);
-        if (js >= 0) {
+        const int ja = wave_append(emit, counts + CNT_SHADOW + part * kCntLine);
+        if (ja >= 0) {
+            const uint32_t js = part * cap + (uint32_t)ja;
             sq.ox[js] = pos.x; sq.oy[js] = pos.y; sq.oz[js] = pos.z;
             sq.dx[js] = sdir.x; sq.dy[js] = sdir.y; sq.dz[js] = sdir.z;
             sq.tmax[js] = t - 1e-3f;
@@ -794,8 +848,9 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
             else beta = beta / q;
         }
     }
-    const int jn = wave_append(alive, counts + cnt_q(qi ^ 1));
-    if (jn >= 0) {
+    const int ja = wave_append(alive, counts + cnt_q(qi ^ 1) + part * kCntLine);
+    if (ja >= 0) {
+        const uint32_t jn = part * cap + (uint32_t)ja;
         qout.ox[jn] = pos.x; qout.oy[jn] = pos.y; qout.oz[jn] = pos.z;
         qout.dx[jn] = ms.wi.x; qout.dy[jn] = ms.wi.y; qout.dz[jn] = ms.wi.z;
         qout.br[jn] = beta.x; qout.bg[jn] = beta.y; qout.bb[jn] = beta.z;
@@ -820,23 +875,31 @@ SRT_D void wave_sync_lds() {
 }
 __global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segment* segs, int depth, PathQueue qin,
                                                          HitBuf h, PathQueue qout, ShadowQueue sq, float* Lr,
-                                                         float* Lg, float* Lb, uint32_t* counts, int qi, MlpIO mio) {
+                                                         float* Lg, float* Lb, uint32_t* counts, int qi, uint32_t cap,
+                                                         MlpIO mio) {
     __shared__ uint32_t gather[kShadeBlock / 64][128];
     const int lane = threadIdx.x & 63;
     uint32_t* buf = gather[threadIdx.x >> 6];
     const unsigned long long below = (1ull << lane) - 1ull;
-    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
-    const uint32_t count = counts[cnt_q(qi)];
-    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_T);   // the next trace stage fetches from 0
+    // the waves of the workgroups of one piece share its entries; their
+    // appends go to partition `part`
+    __shared__ uint32_t prefix[kParts + 1];
+    uint32_t lo, hi;
+    const QueueView view = queue_piece(counts, cnt_q(qi), cap, prefix, lo, hi);
+    const uint32_t part = my_part();
+    const uint32_t count = hi - lo;
+    const int wave = (int)((blockIdx.x / (unsigned)kParts) * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int nwaves = (int)((gridDim.x / (unsigned)kParts) * (blockDim.x >> 6));
+    if (blockIdx.x == 0) clear_parts(counts + CNT_FETCH_T);   // the next trace stage fetches from 0
     const f3 env = mk(S.env[0], S.env[1], S.env[2]);
     uint32_t nbuf = 0;                     // gathered hits (wave-uniform)
     uint32_t base = (uint32_t)wave * 64u;
     for (;;) {
-        // gather until 64 hits are waiting or the queue is exhausted
+        // gather until 64 hits are waiting or the partition is exhausted
         while (nbuf < 64u && base < count) {
-            const uint32_t i = base + (uint32_t)lane;
-            const bool valid = i < count;
+            const uint32_t r = base + (uint32_t)lane;
+            const bool valid = r < count;
+            const uint32_t i = valid ? view.entry(lo + r) : 0u;
             const int32_t tri = valid ? h.tri[i] : -1;
             if (valid && tri < 0) {
                 const uint32_t slot = qin.slot[i];
@@ -857,7 +920,7 @@ __global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segm
         wave_sync_lds();
         if (lane + 64u < nbuf) buf[lane] = rest;
         nbuf -= take;
-        if ((uint32_t)lane < take) shade_hit(S, segs, depth, qin, h, qout, sq, Lr, Lg, Lb, counts, qi, mio, e);
+        if ((uint32_t)lane < take) shade_hit(S, segs, depth, qin, h, qout, sq, Lr, Lg, Lb, counts, qi, part, cap, mio, e);
         wave_sync_lds();
     }
 }
@@ -865,14 +928,17 @@ __global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segm
 // Shadow rays of the bounce (fetch counter cleared by the trace stage).
 template <bool COUNT, bool WIDE>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
-                                                       uint32_t* counts, int32_t* gstack, unsigned long long* totals,
-                                                       TraceCfg cfg) {
-    const uint32_t count = counts[CNT_SHADOW];
-    if (blockIdx.x == 0 && threadIdx.x == 0) totals[1] += count;
+                                                       uint32_t* counts, uint32_t cap, int32_t* gstack,
+                                                       unsigned long long* totals, TraceCfg cfg) {
+    __shared__ uint32_t prefix[kParts + 1];
+    uint32_t lo, hi;
+    const QueueView view = queue_piece(counts, CNT_SHADOW, cap, prefix, lo, hi);
+    uint32_t* fetch = counts + CNT_FETCH_S + my_part() * kCntLine;
+    if (blockIdx.x < (unsigned)kParts && threadIdx.x == 0) atomicAdd(&totals[1], (unsigned long long)(hi - lo));
     const TravStack k = make_stack(gstack, cfg);
     ShadowIO io{sq, Lr, Lg, Lb};
     uint32_t nv = 0, nt = 0;
-    trace_persistent<true, COUNT, WIDE>(S, count, counts + CNT_FETCH_S, cfg.refill, k, io, nv, nt);
+    trace_persistent<true, COUNT, WIDE>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
     if (COUNT) {
         atomicAdd(&totals[4], (unsigned long long)nv);
         atomicAdd(&totals[5], (unsigned long long)nt);
@@ -978,13 +1044,15 @@ struct RaysIO {        // standalone queries on flat arrays
 __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, uint32_t n, RaysIO io, uint32_t* fetch,
                                                            int32_t* gstack, TraceCfg cfg) {
     const TravStack k = make_stack(gstack, cfg);
-    uint32_t nv = 0, nt = 0;
+    uint32_t nv = 0, nt = 0, lo, hi;
+    fetch = dense_range(n, fetch, lo, hi);
+    const IdentityEntry view;
     if (cfg.wide) {
-        if (io.any) trace_persistent<true, false, true>(S, n, fetch, cfg.refill, k, io, nv, nt);
-        else trace_persistent<false, false, true>(S, n, fetch, cfg.refill, k, io, nv, nt);
+        if (io.any) trace_persistent<true, false, true>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
+        else trace_persistent<false, false, true>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
     } else {
-        if (io.any) trace_persistent<true, false, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
-        else trace_persistent<false, false, false>(S, n, fetch, cfg.refill, k, io, nv, nt);
+        if (io.any) trace_persistent<true, false, false>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
+        else trace_persistent<false, false, false>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
     }
 }
 
@@ -1013,8 +1081,9 @@ template <bool WIDE>
 __global__ void __launch_bounds__(kTraceBlock) k_simple_trace(DevScene S, uint32_t n, SimpleIO io, uint32_t* fetch,
                                                              int32_t* gstack, TraceCfg cfg) {
     const TravStack k = make_stack(gstack, cfg);
-    uint32_t nv = 0, nt = 0;
-    trace_persistent<false, false, WIDE>(S, n, fetch, cfg.refill, k, io, nv, nt);
+    uint32_t nv = 0, nt = 0, lo, hi;
+    fetch = dense_range(n, fetch, lo, hi);
+    trace_persistent<false, false, WIDE>(S, IdentityEntry{}, lo, hi, fetch, cfg.refill, k, io, nv, nt);
 }
 
 // the Simple pass types (SimpleParams::Type, params/npr_params.h)
@@ -1097,10 +1166,14 @@ SRT_D void splat(float4* film, int idx, f3 L) {
     atomicAdd(&film[idx].z, L.z);
 }
 
-__global__ void __launch_bounds__(256) k_lt_raygen(DevScene S, LtParams P, PathQueue q, int n, uint32_t* counts) {
+__global__ void __launch_bounds__(256) k_lt_raygen(DevScene S, LtParams P, PathQueue q, int n, uint32_t cap,
+                                                   uint32_t* counts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) counts[CNT_Q0] = (uint32_t)n;
-    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_T);
+    if (blockIdx.x == 0) {
+        clear_parts(counts + CNT_FETCH_T);
+        if (threadIdx.x < (unsigned)kParts)
+            counts[CNT_Q0 + threadIdx.x * kCntLine] = part_size((uint32_t)n, cap, threadIdx.x);
+    }
     if (i >= n) return;
     const int64_t wh = (int64_t)P.width * P.height;
     const int64_t pid = P.path_begin + i;
@@ -1118,22 +1191,30 @@ __global__ void __launch_bounds__(256) k_lt_raygen(DevScene S, LtParams P, PathQ
     const float cos_theta = dot(wi, ls.normal);
     const float pw = cosine_hemisphere_pdf(cos_theta);
     const f3 beta = ls.emission * cos_theta / ls.pdf;
-    q.ox[i] = ls.pos.x; q.oy[i] = ls.pos.y; q.oz[i] = ls.pos.z;
-    q.dx[i] = wi.x; q.dy[i] = wi.y; q.dz[i] = wi.z;
-    q.br[i] = beta.x; q.bg[i] = beta.y; q.bb[i] = beta.z;
-    q.pdf[i] = pw;
-    q.flags[i] = 1u;                 // bit 0: the last scattering was specular (the light counts as one)
-    q.rng[i] = rng.state;
-    q.slot[i] = (uint32_t)i;
+    const uint32_t e = (uint32_t)i;           // partition i / cap, in path order
+    q.ox[e] = ls.pos.x; q.oy[e] = ls.pos.y; q.oz[e] = ls.pos.z;
+    q.dx[e] = wi.x; q.dy[e] = wi.y; q.dz[e] = wi.z;
+    q.br[e] = beta.x; q.bg[e] = beta.y; q.bb[e] = beta.z;
+    q.pdf[e] = pw;
+    q.flags[e] = 1u;                 // bit 0: the last scattering was specular (the light counts as one)
+    q.rng[e] = rng.state;
+    q.slot[e] = (uint32_t)i;
 }
 
 __global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int depth, PathQueue qin, HitBuf h,
-                                                 PathQueue qout, ShadowQueue sq, float4* film, uint32_t* counts, int qi) {
-    const uint32_t count = counts[cnt_q(qi)];
-    if (blockIdx.x == 0) clear_fetch(counts + CNT_FETCH_T);   // the next trace stage fetches from 0
+                                                 PathQueue qout, ShadowQueue sq, float4* film, uint32_t* counts, int qi,
+                                                 uint32_t cap) {
+    __shared__ uint32_t prefix[kParts + 1];
+    uint32_t lo, hi;
+    const QueueView view = queue_piece(counts, cnt_q(qi), cap, prefix, lo, hi);
+    const uint32_t part = my_part();
+    const uint32_t first = part * cap;     // of the output partition
+    if (blockIdx.x == 0) clear_parts(counts + CNT_FETCH_T);   // the next trace stage fetches from 0
     const f3 cam_pos = ld3(P.cam.pos);
     const int64_t wh = (int64_t)P.width * P.height;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const uint32_t step = (gridDim.x / (unsigned)kParts) * blockDim.x;
+    for (uint32_t r = (blockIdx.x / (unsigned)kParts) * blockDim.x + threadIdx.x; r < hi - lo; r += step) {
+        const uint32_t i = view.entry(lo + r);
         const f3 ro = mk(qin.ox[i], qin.oy[i], qin.oz[i]), rd = mk(qin.dx[i], qin.dy[i], qin.dz[i]);
         f3 beta = mk(qin.br[i], qin.bg[i], qin.bb[i]);
         float pw = qin.pdf[i];
@@ -1177,8 +1258,9 @@ __global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int de
                 emit = true;
             }
         }
-        const int js = wave_append(emit, counts + CNT_SHADOW);
-        if (js >= 0) {
+        const int ja = wave_append(emit, counts + CNT_SHADOW + part * kCntLine);
+        if (ja >= 0) {
+            const uint32_t js = first + (uint32_t)ja;
             sq.ox[js] = hg.pos.x; sq.oy[js] = hg.pos.y; sq.oz[js] = hg.pos.z;
             sq.dx[js] = sdir.x; sq.dy[js] = sdir.y; sq.dz[js] = sdir.z;
             sq.tmax[js] = t - 1e-3f;
@@ -1203,8 +1285,9 @@ __global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int de
                 else beta = beta / p;
             }
         }
-        const int jn = wave_append(alive, counts + cnt_q(qi ^ 1));
-        if (jn >= 0) {
+        const int ja2 = wave_append(alive, counts + cnt_q(qi ^ 1) + part * kCntLine);
+        if (ja2 >= 0) {
+            const uint32_t jn = first + (uint32_t)ja2;
             qout.ox[jn] = hg.pos.x; qout.oy[jn] = hg.pos.y; qout.oz[jn] = hg.pos.z;
             qout.dx[jn] = ms.wi.x; qout.dy[jn] = ms.wi.y; qout.dz[jn] = ms.wi.z;
             qout.br[jn] = beta.x; qout.bg[jn] = beta.y; qout.bb[jn] = beta.z;
@@ -1229,12 +1312,15 @@ struct LtShadowIO {    // camera connections: unoccluded ones splat their contri
 
 template <bool WIDE>
 __global__ void __launch_bounds__(kTraceBlock) k_lt_shadow(DevScene S, ShadowQueue sq, float4* film, uint32_t* counts,
-                                                          int32_t* gstack, TraceCfg cfg) {
-    const uint32_t count = counts[CNT_SHADOW];
+                                                          uint32_t cap, int32_t* gstack, TraceCfg cfg) {
+    __shared__ uint32_t prefix[kParts + 1];
+    uint32_t lo, hi;
+    const QueueView view = queue_piece(counts, CNT_SHADOW, cap, prefix, lo, hi);
+    uint32_t* fetch = counts + CNT_FETCH_S + my_part() * kCntLine;
     const TravStack k = make_stack(gstack, cfg);
     LtShadowIO io{sq, film};
     uint32_t nv = 0, nt = 0;
-    trace_persistent<true, false, WIDE>(S, count, counts + CNT_FETCH_S, cfg.refill, k, io, nv, nt);
+    trace_persistent<true, false, WIDE>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
 }
 
 __global__ void __launch_bounds__(256) k_film_clear(float4* film, int64_t n) {
@@ -1242,7 +1328,7 @@ __global__ void __launch_bounds__(256) k_film_clear(float4* film, int64_t n) {
     if (i < n) film[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
 }
 
-__global__ void k_clear_fetch(uint32_t* fetch) { clear_fetch(fetch); }
+__global__ void k_clear_parts(uint32_t* first) { clear_parts(first); }
 
 // ---------------------------------------------------------------------------
 // post-processing (tonemapping.cu:14-68, accumulate.cu:22-26)
@@ -1399,6 +1485,13 @@ static int ensure_gstack(srt_ctx* c, const SceneRec* s, int64_t threads) {
     return SRT_OK;
 }
 
+// Grids of the kernels on partitioned queues: a multiple of kParts, so every
+// partition has the same number of workgroups.
+static int part_grid(int64_t g) {
+    if (g < kParts) g = kParts;
+    return (int)((g + kParts - 1) / kParts * kParts);
+}
+
 static int ensure_pool(srt_ctx* c, int64_t slots) {
     if (c->pool_cap >= slots) return SRT_OK;
     for (int k = 0; k < 2; k++)
@@ -1408,15 +1501,16 @@ static int ensure_pool(srt_ctx* c, int64_t slots) {
     for (auto& b : c->Lbuf) dev_free(c, b);
     c->pool_cap = 0;
     const size_t n = (size_t)slots;
+    const size_t qn = (size_t)kParts * part_cap((uint32_t)slots);   // partitioned queue entries (>= n)
     for (int k = 0; k < 2; k++) {
         for (int j = 0; j < 14; j++) {
-            size_t bytes = (j == 11) ? 8 * n : 4 * n;   // 11: rng (u64)
-            if (j == 13) bytes = 0;                       // unused spare
+            size_t bytes = (j == 11) ? 8 * qn : 4 * qn;   // 11: rng (u64)
+            if (j == 13) bytes = 0;                         // unused spare
             if (bytes && dev_alloc(c, c->q[k][j], bytes)) return SRT_E_OOM;
         }
     }
-    for (int j = 0; j < 4; j++) if (dev_alloc(c, c->hits[j], 4 * n)) return SRT_E_OOM;
-    for (int j = 0; j < 11; j++) if (dev_alloc(c, c->shq[j], 4 * n)) return SRT_E_OOM;
+    for (int j = 0; j < 4; j++) if (dev_alloc(c, c->hits[j], 4 * qn)) return SRT_E_OOM;
+    for (int j = 0; j < 11; j++) if (dev_alloc(c, c->shq[j], 4 * qn)) return SRT_E_OOM;
     for (int j = 0; j < 3; j++) if (dev_alloc(c, c->Lbuf[j], 4 * n)) return SRT_E_OOM;
     c->pool_cap = slots;
     return SRT_OK;
@@ -1496,7 +1590,8 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     }
     if (n <= 0 || n > 0x7fffffff) return set_err(c, SRT_E_LIMIT, "chunk of %lld paths", (long long)n);
     if (int r = ensure_pool(c, n)) return r;
-    const int tgrid = trace_grid(c, n);
+    const int tgrid = part_grid(trace_grid(c, n));
+    const uint32_t cap = part_cap((uint32_t)n);
     if (int r = ensure_gstack(c, s, (int64_t)tgrid * kTraceBlock)) return r;
     const TraceCfg tcfg = trace_cfg(c);
     TraceCfg scfg = tcfg;
@@ -1537,11 +1632,11 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     }
     float* Lr = (float*)c->Lbuf[0].p; float* Lg = (float*)c->Lbuf[1].p; float* Lb = (float*)c->Lbuf[2].p;
     int32_t* gst = (int32_t*)c->gstack.p;
-    const int shade_grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)num_cus(c) * 8);
+    const int shade_grid = part_grid(std::min<int64_t>((n + 255) / 256, (int64_t)num_cus(c) * 8));
     const int nblk = (int)((n + 255) / 256);
     hipEvent_t ev;
     prof_begin(c, "raygen", &ev);
-    k_raygen<<<nblk, 256, 0, st>>>(Q[0], Lr, Lg, Lb, segs, nseg, (int)n, counts);
+    k_raygen<<<nblk, 256, 0, st>>>(Q[0], Lr, Lg, Lb, segs, nseg, (int)n, cap, counts);
     prof_end(c, "raygen", ev, n);
     if (mlp_mode == 2) k_zero<<<nblk, 256, 0, st>>>((float*)mio.rec_valid, (size_t)n);
     for (int depth = 0; depth < max_depth; depth++) {
@@ -1549,27 +1644,27 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
         prof_begin(c, "trace", &ev);
         if (tcfg.wide) {
             if (c->count_visits)
-                k_trace<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
-            else k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+                k_trace<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+            else k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
         } else {
             if (c->count_visits)
-                k_trace<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
-            else k_trace<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
+                k_trace<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+            else k_trace<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
         }
         prof_end(c, "trace", ev, 0);
         prof_begin(c, "shade", &ev);
-        k_shade<<<shade_grid, 256, 0, st>>>(S, segs, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, mio);
+        k_shade<<<shade_grid, 256, 0, st>>>(S, segs, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, cap, mio);
         prof_end(c, "shade", ev, 0);
         if (any_nee) {
             prof_begin(c, "shadow", &ev);
             if (tcfg.wide) {
                 if (c->count_visits)
-                    k_shadow<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, scfg);
-                else k_shadow<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, scfg);
+                    k_shadow<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, cap, gst, totals, scfg);
+                else k_shadow<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, cap, gst, totals, scfg);
             } else {
                 if (c->count_visits)
-                    k_shadow<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, scfg);
-                else k_shadow<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, gst, totals, scfg);
+                    k_shadow<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, cap, gst, totals, scfg);
+                else k_shadow<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, cap, gst, totals, scfg);
             }
             prof_end(c, "shadow", ev, 0);
         }
@@ -1691,7 +1786,7 @@ int render_simple(srt_ctx* c, SceneRec* s, const DevCamera& cam, int type, int s
     prof_begin(c, "simple", &ev);
     for (int smp = 0; smp < spp; smp++) {
         io.sample = (uint32_t)smp;
-        k_clear_fetch<<<1, kFetchPieces, 0, c->stream>>>(fetch);
+        k_clear_parts<<<1, kParts, 0, c->stream>>>(fetch);
         if (cfg.wide) k_simple_trace<true><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch,
                                                                                                   (int32_t*)c->gstack.p, cfg);
         else k_simple_trace<false><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch,
@@ -1712,7 +1807,7 @@ int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max
     const int64_t total = wh * spp;
     int64_t pool = std::min<int64_t>(std::min<int64_t>(total, c->pool_request), 0x7fffffff);
     if (int r = ensure_pool(c, pool)) return r;
-    const int tgrid = trace_grid(c, pool);
+    const int tgrid = part_grid(trace_grid(c, pool));
     DevScene S = dev_scene(s);
     if (int r = ensure_gstack(c, s, (int64_t)tgrid * kTraceBlock)) return r;
     const TraceCfg tcfg = trace_cfg(c);
@@ -1743,15 +1838,17 @@ int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max
     for (int64_t p0 = 0; p0 < total; p0 += pool) {
         const int n = (int)std::min<int64_t>(pool, total - p0);
         P.path_begin = p0;
-        const int shade_grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)num_cus(c) * 8);
-        k_lt_raygen<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(S, P, Q[0], n, counts);
+        const int shade_grid = part_grid(std::min<int64_t>((n + 255) / 256, (int64_t)num_cus(c) * 8));
+        const uint32_t cap = part_cap((uint32_t)n);
+        k_lt_raygen<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(S, P, Q[0], n, cap, counts);
         for (int depth = 0; depth < max_depth; depth++) {
             const int qi = depth & 1;
-            if (tcfg.wide) k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
-            else k_trace<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, gst, totals, tcfg);
-            k_lt_shade<<<shade_grid, 256, 0, st>>>(S, P, depth, Q[qi], H, Q[qi ^ 1], SQ, film, counts, qi);
-            if (tcfg.wide) k_lt_shadow<true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, gst, scfg);
-            else k_lt_shadow<false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, gst, scfg);
+            if (tcfg.wide)
+                k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+            else k_trace<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+            k_lt_shade<<<shade_grid, 256, 0, st>>>(S, P, depth, Q[qi], H, Q[qi ^ 1], SQ, film, counts, qi, cap);
+            if (tcfg.wide) k_lt_shadow<true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, cap, gst, scfg);
+            else k_lt_shadow<false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, cap, gst, scfg);
         }
     }
     prof_end(c, "light_tracer", ev, total);
@@ -1769,7 +1866,7 @@ int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* 
     io.o3 = o_dev; io.d3 = d_dev; io.tmax = tmax_dev; io.tmin = tmin;
     io.t = t; io.u = u; io.v = v; io.prim = prim; io.tris = S.tris; io.any = any;
     uint32_t* fetch = (uint32_t*)c->counters.p + CNT_FETCH_Q;
-    k_clear_fetch<<<1, kFetchPieces, 0, c->stream>>>(fetch);
+    k_clear_parts<<<1, kParts, 0, c->stream>>>(fetch);
     k_trace_rays<<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch, (int32_t*)c->gstack.p,
                                                                          cfg);
     if (int r = hip_check(c, hipGetLastError(), "trace launch")) return r;
