@@ -184,8 +184,9 @@ def kstats_summary(d):
              "Command (from the repository root; TMPDIR=/tmp):", "",
              "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_ks -o run -- python3 bench.py --steps 2 --warmup 1 --no-cpu",
              "    python3 bench.py --kstats-summary gpurun_out/prof_ks", "",
-             "The run renders 4 frames: the untimed counting pass (k_trace<true, true>, k_shadow<true, true>), 1 warmup",
-             "and 2 timed frames; the BVH build (k_top_split/k_top_link, k_bottom_up, sorts, 4-wide collapse) runs once.", "",
+             "The run renders 6 frames: the untimed counting pass (k_trace<true, true>, k_shadow<true, true>), 1 warmup,",
+             "2 timed frames and 2 frames copied to host memory; the BVH build (k_top_split/k_top_link, k_bottom_up,",
+             "sorts, 4-wide collapse) runs once.", "",
              "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
     for n, c, t, a in rows[:16]:
         short = n.split("(")[0].replace("void ", "")
