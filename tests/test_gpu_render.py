@@ -57,6 +57,24 @@ def test_chunking_independent(ctx):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("pool", [64, 65, 100, 2047, 2049])
+def test_queue_partition_shapes(ctx, pool):
+    """Chunks of 64 to 2049 paths leave most of the 32 queue partitions empty or
+    ragged (partition capacity = 64 * ceil(ceil(n / 64) / 32)): the image and
+    the queue totals equal the single-chunk render's."""
+    s = scenes.cornell()
+    h = ctx.upload(s)
+    a = ctx.render(h, s.camera, 40, 30, 4, 5, 4, 1, 1, 11)
+    ca = ctx.last_counts()
+    ctx.set_option("path_pool", pool)
+    b = ctx.render(h, s.camera, 40, 30, 4, 5, 4, 1, 1, 11)
+    cb = ctx.last_counts()
+    ctx.set_option("path_pool", 1 << 27)
+    ctx.release(h)
+    assert np.array_equal(a, b)
+    assert ca == cb
+
+
 def test_spheres_small(ctx):
     s = scenes.spheres()
     img, ref = _render_pair(ctx, s, 192, 108, 2, 8)
