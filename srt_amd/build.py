@@ -24,7 +24,7 @@ HIP_SOURCES = ["srt_api.hip", "wavefront.hip", "bvh.hip", "mlp.hip", "serve.hip"
 # ordinary VGPRs (their epilogues read every element; the AGPR form would add a
 # v_accvgpr_read per element)
 HIP_SOURCE_FLAGS = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
-HOST_SOURCES = ["json.cpp", "scene_io.cpp", "passes.cpp", "renderer.cpp"]
+HOST_SOURCES = ["json.cpp", "scene_io.cpp", "image_out.cpp", "passes.cpp", "renderer.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SRT_OFFLOAD_ARCH", "gfx950")

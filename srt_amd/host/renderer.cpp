@@ -235,8 +235,8 @@ int srt_host_pass_params(const char* name, const char* params_json, char* out, i
     return rc < 0 ? rc : (int)res.size();
 }
 
-// Write width*height RGBA floats (film layout) to path (.exr, .pfm or .ppm by
-// extension).  Returns 0, or -1 with the message on stderr.
+// Write width*height RGBA floats (film layout) to path (.exr, .hdr, .png, .bmp,
+// .jpg, .pfm or .ppm by extension).  Returns 0, or -1 with the message on stderr.
 int srt_host_save_image(const char* path, int width, int height, const float* rgba) {
     try {
         srt::save_image(path, width, height, rgba);

@@ -1,5 +1,7 @@
 #include "scene_io.h"
 
+#include "image_out.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -385,87 +387,12 @@ void load_srtscene(const std::string& path, Scene& s) {
 // as Image::save_to_file does, texture.cu:219-230)
 // ---------------------------------------------------------------------------
 
-// float -> IEEE half, round to nearest even (overflow to infinity, NaN kept)
-static uint16_t to_half(float f) {
-    uint32_t x;
-    std::memcpy(&x, &f, 4);
-    const uint32_t sign = (x >> 16) & 0x8000u;
-    const uint32_t ax = x & 0x7fffffffu;
-    if (ax >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
-    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);               // rounds to >= 65520: infinity
-    if (ax < 0x38800000u) {                                                   // below the smallest normal half
-        if (ax < 0x33000000u) return (uint16_t)sign;                          // below half the smallest subnormal
-        const uint32_t e = ax >> 23, m = (ax & 0x7fffffu) | 0x800000u;
-        const uint32_t shift = 126u - e;                                      // 14..24
-        uint32_t h = m >> shift;
-        const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
-        if (rem > half || (rem == half && (h & 1u))) h++;
-        return (uint16_t)(sign | h);
-    }
-    uint32_t h = ((ax - 0x38000000u) >> 13);
-    const uint32_t rem = ax & 0x1fffu;
-    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
-    return (uint16_t)(sign | h);
-}
-
-// OpenEXR 2.0 scanline file, no compression, HALF channels (the reference
-// writes HALF, texture.cu:96-117); channel names in the file's required
-// alphabetical order A, B, G, R; rows top to bottom.
-static void write_exr(std::ofstream& f, int W, int H, const std::function<const float*(int, int)>& px) {
-    std::string hdr;
-    auto put32 = [&](std::string& s, int32_t v) { s.append((const char*)&v, 4); };
-    auto attr = [&](const char* name, const char* type, const std::string& value) {
-        hdr += name; hdr += '\0';
-        hdr += type; hdr += '\0';
-        put32(hdr, (int32_t)value.size());
-        hdr += value;
-    };
-    std::string ch;
-    for (const char* name : {"A", "B", "G", "R"}) {
-        ch += name; ch += '\0';
-        put32(ch, 1);                        // HALF
-        ch += std::string(4, '\0');           // pLinear + reserved
-        put32(ch, 1); put32(ch, 1);          // x / y sampling
-    }
-    ch += '\0';
-    std::string box;
-    put32(box, 0); put32(box, 0); put32(box, W - 1); put32(box, H - 1);
-    const float one = 1.0f, zero = 0.0f;
-    attr("channels", "chlist", ch);
-    attr("compression", "compression", std::string(1, '\0'));
-    attr("dataWindow", "box2i", box);
-    attr("displayWindow", "box2i", box);
-    attr("lineOrder", "lineOrder", std::string(1, '\0'));
-    attr("pixelAspectRatio", "float", std::string((const char*)&one, 4));
-    std::string center((const char*)&zero, 4);
-    center.append((const char*)&zero, 4);
-    attr("screenWindowCenter", "v2f", center);
-    attr("screenWindowWidth", "float", std::string((const char*)&one, 4));
-    hdr += '\0';
-    const uint32_t magic = 20000630u, version = 2u;
-    f.write((const char*)&magic, 4);
-    f.write((const char*)&version, 4);
-    f.write(hdr.data(), (std::streamsize)hdr.size());
-    const uint64_t line_bytes = 8ull * W;    // 4 channels x 2 bytes
-    const uint64_t start = 8 + hdr.size() + 8ull * H;
-    for (int y = 0; y < H; y++) {
-        const uint64_t off = start + (uint64_t)y * (8 + line_bytes);
-        f.write((const char*)&off, 8);
-    }
-    std::vector<uint16_t> line(4 * (size_t)W);
-    static const int chan[4] = {3, 2, 1, 0};    // A, B, G, R from RGBA
-    for (int y = 0; y < H; y++) {
-        for (int k = 0; k < 4; k++)
-            for (int x = 0; x < W; x++) line[(size_t)k * W + x] = to_half(px(y, x)[chan[k]]);
-        const int32_t yy = y, size = (int32_t)line_bytes;
-        f.write((const char*)&yy, 4);
-        f.write((const char*)&size, 4);
-        f.write((const char*)line.data(), (std::streamsize)line_bytes);
-    }
-}
-
 void save_image(const std::string& path, int W, int H, const float* rgba) {
     std::string ext = path.substr(path.find_last_of('.') + 1);
+    static const char* known[] = {"pfm", "ppm", "exr", "hdr", "png", "bmp", "jpg"};
+    if (std::find(std::begin(known), std::end(known), ext) == std::end(known))
+        throw RenderError("Unsupported image format: " + ext + " (use .exr, .hdr, .png, .bmp, .jpg, .pfm or .ppm)");
+    if (ext == "jpg" && (W > 65535 || H > 65535)) throw RenderError("image too large for JPEG: " + path);
     std::ofstream f(path, std::ios::binary);
     if (!f.is_open()) throw RenderError("Failed to open file for writing: " + path);
     auto px = [&](int row_from_top, int col) {
@@ -490,8 +417,14 @@ void save_image(const std::string& path, int W, int H, const float* rgba) {
             }
     } else if (ext == "exr") {
         write_exr(f, W, H, px);
+    } else if (ext == "hdr") {
+        write_hdr(f, W, H, px);
+    } else if (ext == "png") {
+        write_png(f, W, H, px);
+    } else if (ext == "bmp") {
+        write_bmp(f, W, H, px);
     } else {
-        throw RenderError("unsupported image format '." + ext + "' (use .exr, .pfm or .ppm)");
+        write_jpg(f, W, H, px);
     }
 }
 
