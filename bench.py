@@ -28,6 +28,7 @@ Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
        python bench.py --pmc-summary FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
        python bench.py --kstats-summary DIR  (rocprofv3 --kernel-trace --stats output -> profiles/)
        python bench.py --mlp-summary KS_DIR PMC_DIR  (MLP kernel times + MFMA counters -> profiles/)
+       python bench.py --valu-summary DIR  (VALU issue counters of the frame kernels -> profiles/)
 """
 import argparse
 import json
@@ -59,6 +60,8 @@ MATERIAL = 80        # material record read by shade
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_spheres1m_hbm_traffic.md")
+VALU_FILE = os.path.join(ROOT, "profiles", "trace_spheres1m_valu_issue.md")
+MI355X_SIMDS, MI355X_XCDS = 256 * 4, 8
 COPY_ELEMS = 1 << 28          # calibration copy: 1 GiB read + 1 GiB written (float32)
 # kernels of the frame, as rocprofv3 names them
 KERNEL_KEYS = {"trace": "k_trace<false, true>", "shadow": "k_shadow<false, true>", "shade": "k_shade("}
@@ -154,6 +157,66 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
     with open(TRAFFIC_FILE, "w") as fh:
         fh.write(text)
     print(text)
+
+
+def valu_summary(d):
+    """VALU issue of the frame kernels from one rocprofv3 --pmc pass: the SIMDs'
+    busy fraction 4 SQ_ACTIVE_INST_VALU / (SIMDs x GRBM_GUI_ACTIVE / XCDs)
+    (a wave64 VALU instruction holds its SIMD for one quad-cycle; GRBM_GUI_ACTIVE
+    sums the cycles of the 8 XCDs) and the active lanes per VALU instruction,
+    SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)."""
+    names = ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY",
+             "GRBM_GUI_ACTIVE"]
+    per = {}
+    for cn in names:
+        for name, v in _pmc_rows(d, cn):
+            for k, key in KERNEL_KEYS.items():
+                if key in name:
+                    per.setdefault(k, {}).setdefault(cn, []).append(v)
+    lines = ["# VALU issue of the frame kernels (1M-triangle frame, 1920x1080x64spp, depth 8)", "",
+             "Commands (from the repository root, then this summary):", "",
+             "    rocprofv3 --pmc " + " ".join(names) + " -d gpurun_out/pmc_valu -o run -- python3 bench.py "
+             "--steps 1 --warmup 0 --no-cpu",
+             "    python3 bench.py --valu-summary gpurun_out/pmc_valu", "",
+             "The rocprofv3 run: cd /tmp && export TMPDIR=/tmp, bench.py by its path in the repository; the counting",
+             "pass's k_trace<true, true>/k_shadow<true, true> are not counted here (its k_shade dispatches are).",
+             f"busy = 4 SQ_ACTIVE_INST_VALU / ({MI355X_SIMDS} SIMDs x GRBM_GUI_ACTIVE / {MI355X_XCDS}), summed over the "
+             "kernel's dispatches; lanes = SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU); wait = SQ_WAIT_ANY / "
+             "SQ_WAVE_CYCLES.", "",
+             "| kernel | dispatches | VALU instructions | SIMD VALU busy | active lanes per VALU instruction | wave cycles waiting |",
+             "|---|---|---|---|---|---|"]
+    out = {}
+    for k in KERNEL_KEYS:
+        c = per.get(k)
+        if not c or any(n not in c for n in names):
+            continue
+        tot = {n: sum(c[n]) for n in names}
+        busy = 4.0 * tot["SQ_ACTIVE_INST_VALU"] / (MI355X_SIMDS * tot["GRBM_GUI_ACTIVE"] / MI355X_XCDS)
+        lanes = tot["SQ_THREAD_CYCLES_VALU"] / (64.0 * tot["SQ_ACTIVE_INST_VALU"])
+        wait = tot["SQ_WAIT_ANY"] / tot["SQ_WAVE_CYCLES"]
+        out[k] = (busy, lanes)
+        lines.append(f"| {k} | {len(c['SQ_INSTS_VALU'])} | {tot['SQ_INSTS_VALU']:.4g} | {busy:.3f} | {lanes:.3f} | "
+                     f"{wait:.3f} |")
+    lines.append("")
+    for k, (busy, lanes) in out.items():
+        lines.append(f"valu_issue {k} {busy:.4f} {lanes:.4f}")
+    text = "\n".join(lines) + "\n"
+    with open(VALU_FILE, "w") as fh:
+        fh.write(text)
+    print(text)
+
+
+def measured_valu(kernel):
+    """(SIMD VALU busy fraction, active lanes per VALU instruction) of `kernel`
+    from the committed counter summary."""
+    if not os.path.exists(VALU_FILE):
+        return None
+    with open(VALU_FILE) as fh:
+        for line in fh:
+            p = line.split()
+            if len(p) == 4 and p[0] == "valu_issue" and p[1] == kernel:
+                return float(p[2]), float(p[3])
+    return None
 
 
 KSTATS_FILE = os.path.join(ROOT, "profiles", "frame_spheres1m_kernel_stats.md")
@@ -533,6 +596,7 @@ def main():
     ap.add_argument("--serve-seed", type=int, default=20261202)
     ap.add_argument("--pmc-summary", nargs=4, metavar="DIR")
     ap.add_argument("--kstats-summary", metavar="DIR")
+    ap.add_argument("--valu-summary", metavar="DIR")
     ap.add_argument("--mlp-summary", nargs=2, metavar="DIR")
     args = ap.parse_args()
     if args.mlp_summary:
@@ -543,6 +607,9 @@ def main():
         return
     if args.kstats_summary:
         kstats_summary(args.kstats_summary)
+        return
+    if args.valu_summary:
+        valu_summary(args.valu_summary)
         return
     if args.mode == "copy":
         copy_mode()
@@ -636,6 +703,10 @@ def main():
         roofline["hbm_share_of_algorithmic"] = round(traffic / bytes_launch, 3)
         roofline["note"] = ("algorithmic bytes count every node/triangle fetch; most are L2 hits, so HBM carries "
                             "hbm_share_of_algorithmic of them (traffic, measured); traversal is bound by VALU issue")
+    valu = measured_valu(kern)
+    if valu:
+        roofline["valu_issue"] = {"simd_busy_frac": round(valu[0], 3), "active_lanes_frac": round(valu[1], 3),
+                                  "source": os.path.relpath(VALU_FILE, ROOT)}
 
     cpu = None
     if not args.no_cpu:
