@@ -631,33 +631,40 @@ SRT_D DBox node_union(const BvhNode* nd) {
     grow(l, r);
     return l;
 }
-__global__ void __launch_bounds__(256) k_refit(int num_nodes, BvhNode* nodes, const int32_t* valid, const int32_t* parent,
-                                               const float4* tris, int32_t* flags) {
+// Level of every binary-tree node (root = 0; 0xff for unused slots), keyed for
+// the sort that orders the nodes by level
+__global__ void __launch_bounds__(256) k_node_level(int num_nodes, const int32_t* valid, const int32_t* parent,
+                                                    uint32_t* key, uint32_t* idx) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= num_nodes || !valid[i]) return;
-    BvhNode* nd = nodes + i;
-    int4 d = nd->d;
-    int arrivals = 0;
-    if (d.x < 0) { set_child(nd, 0, leaf_box(tris, d.x), d.x); arrivals++; }
-    if (d.y < 0) { set_child(nd, 1, leaf_box(tris, d.y), d.y); arrivals++; }
-    if (arrivals == 0) return;
-    __threadfence();
-    int old = atomicAdd(&flags[i], arrivals);
-    if (old + arrivals < 2) return;
-    int x = i;
-    for (;;) {
-        __threadfence();
-        DBox u = node_union(nodes + x);
-        int p = parent[x];
-        if (p < 0) return;
-        int pn = p >> 1, side = p & 1;
-        BvhNode* pd = nodes + pn;
-        set_child(pd, side, u, side ? pd->d.y : pd->d.x);
-        __threadfence();
-        old = atomicAdd(&flags[pn], 1);
-        if (old + 1 < 2) return;
-        x = pn;
+    if (i >= num_nodes) return;
+    uint32_t level = 0xffu;
+    if (valid[i]) {
+        level = 0;
+        for (int p = parent[i]; p >= 0; p = parent[p >> 1]) level++;
     }
+    key[i] = level;
+    idx[i] = (uint32_t)i;
+}
+// first position of every key in the sorted keys
+__global__ void __launch_bounds__(256) k_level_first(int n, const uint32_t* key, int32_t* first) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0 || key[i] != key[i - 1]) first[key[i]] = i;
+}
+// One level of the bottom-up refit (deepest level first): each node takes its
+// leaf children's boxes from the current triangles and its inner children's
+// boxes as the union of their two child boxes, written by the previous launch.
+// Kernel boundaries order the levels, so no device-wide fences are needed.
+__global__ void __launch_bounds__(256) k_refit_level(const uint32_t* ids, int count, BvhNode* nodes,
+                                                     const float4* tris) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    BvhNode* nd = nodes + ids[k];
+    const int4 d = nd->d;
+    const DBox l = d.x < 0 ? leaf_box(tris, d.x) : node_union(nodes + d.x);
+    const DBox r = d.y < 0 ? leaf_box(tris, d.y) : node_union(nodes + d.y);
+    set_child(nd, 0, l, d.x);
+    set_child(nd, 1, r, d.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -906,10 +913,10 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     const int num_nodes = std::max(top + std::max(n - 1, 0), 1);
     dev_free(c, s->nodes);
     dev_free(c, s->node_parent);
-    dev_free(c, s->refit_counter);
+    dev_free(c, s->level_nodes);
     dev_free(c, s->node_valid);
     if (dev_alloc(c, s->nodes, (size_t)num_nodes * sizeof(BvhNode)) || dev_alloc(c, s->node_parent, (size_t)num_nodes * 4) ||
-        dev_alloc(c, s->refit_counter, (size_t)num_nodes * 4) || dev_alloc(c, s->node_valid, (size_t)num_nodes * 4))
+        dev_alloc(c, s->level_nodes, (size_t)num_nodes * 4) || dev_alloc(c, s->node_valid, (size_t)num_nodes * 4))
         return SRT_E_OOM;
     BvhNode* nodes = (BvhNode*)s->nodes.p;
     int32_t* valid = (int32_t*)s->node_valid.p;
@@ -975,6 +982,39 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     if (int r = hip_check(c, hipStreamSynchronize(st), "bvh build")) return r;
     if (root < 0) depth = 0;
     if (depth > kMaxDepth) return set_err(c, SRT_E_LIMIT, "BVH depth %d exceeds the traversal limit %d", depth, kMaxDepth);
+    // nodes by level for the refit: sort (level, node) pairs, invalid nodes last
+    s->level_first.assign(1, 0);
+    if (root >= 0) {
+        uint32_t* lkey = tmp.get<uint32_t>(num_nodes);
+        uint32_t* lkey2 = tmp.get<uint32_t>(num_nodes);
+        uint32_t* lidx = tmp.get<uint32_t>(num_nodes);
+        int32_t* first = tmp.get<int32_t>(256);
+        if (!lkey || !lkey2 || !lidx || !first) return set_err(c, SRT_E_OOM, "level scratch");
+        k_node_level<<<nblk(num_nodes), 256, 0, st>>>(num_nodes, valid, parent, lkey, lidx);
+        size_t tbytes = 0;
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, lkey, lkey2, lidx, (uint32_t*)s->level_nodes.p, num_nodes,
+                                                 0, 8, st);
+        void* tstore = tmp.get<char>(tbytes);
+        if (!tstore) return set_err(c, SRT_E_OOM, "sort scratch");
+        if (int r = hip_check(c, hipcub::DeviceRadixSort::SortPairs(tstore, tbytes, lkey, lkey2, lidx,
+                                                                    (uint32_t*)s->level_nodes.p, num_nodes, 0, 8, st),
+                              "level sort"))
+            return r;
+        k_fill_i32<<<1, 256, 0, st>>>(first, 256, -1);
+        k_level_first<<<nblk(num_nodes), 256, 0, st>>>(num_nodes, lkey2, first);
+        std::vector<int32_t> hf(256);
+        if (int r = hip_check(c, hipMemcpyAsync(hf.data(), first, 256 * 4, hipMemcpyDeviceToHost, st), "copy")) return r;
+        if (int r = hip_check(c, hipStreamSynchronize(st), "level order")) return r;
+        // levels 0 .. depth-1 hold the inner nodes; 0xff marks unused node slots
+        s->level_first.assign((size_t)depth + 1, 0);
+        for (int L = 0; L < depth; L++) {
+            if (hf[L] < 0) return set_err(c, SRT_E_STATE, "BVH level %d is empty", L);
+            s->level_first[L] = hf[L];
+        }
+        for (int L = depth; L < 255; L++)
+            if (hf[L] >= 0) return set_err(c, SRT_E_STATE, "BVH node below the deepest level");
+        s->level_first[depth] = hf[255] >= 0 ? hf[255] : num_nodes;
+    }
     s->root = root;
     s->num_nodes = num_nodes;
     s->max_depth = depth;
@@ -1039,10 +1079,12 @@ int bvh_refit(srt_ctx* c, SceneRec* s) {
     const int n = s->nf;
     launch_tris(s, st);
     if (s->root >= 0) {
-        k_fill_i32<<<nblk(s->num_nodes), 256, 0, st>>>((int32_t*)s->refit_counter.p, s->num_nodes, 0);
-        k_refit<<<nblk(s->num_nodes), 256, 0, st>>>(s->num_nodes, (BvhNode*)s->nodes.p, (const int32_t*)s->node_valid.p,
-                                                    (const int32_t*)s->node_parent.p, (const float4*)s->tris.p,
-                                                    (int32_t*)s->refit_counter.p);
+        const int levels = (int)s->level_first.size() - 1;
+        for (int L = levels - 1; L >= 0; L--) {
+            const int first = s->level_first[L], count = s->level_first[L + 1] - first;
+            k_refit_level<<<nblk(count), 256, 0, st>>>((const uint32_t*)s->level_nodes.p + first, count,
+                                                       (BvhNode*)s->nodes.p, (const float4*)s->tris.p);
+        }
         if (s->num_nodes4 > 0)
             k_refit4<<<nblk(s->num_nodes4), 256, 0, st>>>(s->num_nodes4, (const BvhNode*)s->nodes.p,
                                                           (Bvh4Node*)s->nodes4.p);

@@ -115,7 +115,7 @@ static void free_scene(srt_ctx* c, SceneRec* s) {
     DevBuf* all[] = {&s->pos, &s->nrm, &s->tan, &s->idx, &s->mesh_mat, &s->mesh_first, &s->area_cdf, &s->mats,
                      &s->light_cdf, &s->light_mesh, &s->tri_mesh, &s->orig_pos, &s->orig_nrm, &s->orig_tan,
                      &s->bone_ids, &s->bone_w, &s->bones, &s->nodes, &s->tris, &s->order, &s->node_parent,
-                     &s->refit_counter, &s->node_valid, &s->nodes4, &s->tri_rec};
+                     &s->level_nodes, &s->node_valid, &s->nodes4, &s->tri_rec};
     for (DevBuf* b : all) dev_free(c, *b);
     delete s;
 }

@@ -233,7 +233,10 @@ struct SceneRec {
     // BVH
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};   // scene box (MLP input normalisation)
     bool has_bvh = false;
-    DevBuf nodes, tris, order, node_parent, refit_counter, node_valid;
+    DevBuf nodes, tris, order, node_parent, level_nodes, node_valid;
+    // refit order: the binary-tree nodes of level L (root = 0) are
+    // level_nodes[level_first[L] .. level_first[L + 1])
+    std::vector<int32_t> level_first;
     DevBuf tri_rec;              // per sorted triangle: 128-byte shading record (positions, material, normals, tangents)
     int32_t root = 0, num_nodes = 0, max_depth = 0, num_clusters = 0, num_top = 0, cluster_size = 0;
     DevBuf nodes4;
