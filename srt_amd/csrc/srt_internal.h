@@ -139,6 +139,9 @@ struct MlpIO {
     float* q_in9;          // mode 1 queue: features
     float* q_beta;         // mode 1 queue: throughput (3)
     uint32_t* q_slot;      // mode 1 queue: path slot
+    float* qs_in9;         // the same, as appended: partitioned like the path queues, then
+    float* qs_beta;        // compacted into q_* (k_mlp_compact) for the MLP
+    uint32_t* qs_slot;
     float* rec_in9;        // mode 2, per slot
     float* rec_beta;       // mode 2, per slot
     uint32_t* rec_valid;   // mode 2, per slot
@@ -192,7 +195,8 @@ enum {
     CNT_FETCH_T = CNT_SHADOW + kParts * kCntLine,      // kParts lines: next queue entry to trace (cleared by raygen and shade)
     CNT_FETCH_S = CNT_FETCH_T + kParts * kCntLine,     // kParts lines: next shadow entry to trace (cleared by trace)
     CNT_FETCH_Q = CNT_FETCH_S + kParts * kCntLine,     // kParts lines: next standalone ray query
-    CNT_MLPQ = CNT_FETCH_Q + kParts * kCntLine,        // one line: MLP query queue (dense)
+    CNT_MLPQ_PARTS = CNT_FETCH_Q + kParts * kCntLine,  // kParts lines: MLP query appends per partition
+    CNT_MLPQ = CNT_MLPQ_PARTS + kParts * kCntLine,     // one line: MLP queries after compaction (dense)
     CNT_WORDS = CNT_MLPQ + kCntLine
 };
 // statistics (unsigned 64-bit) after the counters, then the end of the buffer
@@ -282,7 +286,7 @@ struct srt_ctx {
     int64_t last_visits[4] = {0, 0, 0, 0};   // node visits / triangle tests: closest, shadow
     bool count_visits = false;
     std::map<void*, size_t> user_bufs;    // srt_device_alloc allocations
-    srt::DevBuf mlpbuf[10], scan_tmp, train_in9, train_t;   // MLP hooks of the integrator
+    srt::DevBuf mlpbuf[13], scan_tmp, train_in9, train_t;   // MLP hooks of the integrator
     srt::DevBuf segbuf;                   // device copy of the chunk's segment table
     int64_t mlp_pool_cap = 0, train_cap = 0;
     void* serve = nullptr;                // serving loop state; non-null while a dispatcher runs

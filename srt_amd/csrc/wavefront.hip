@@ -597,6 +597,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, 
         clear_parts(counts + CNT_FETCH_S);
         clear_parts(counts + cnt_q(qi ^ 1));
         clear_parts(counts + CNT_SHADOW);
+        clear_parts(counts + CNT_MLPQ_PARTS);
         if (threadIdx.x == 0) counts[CNT_MLPQ] = 0u;
     }
     if (blockIdx.x < (unsigned)kParts && threadIdx.x == 0) atomicAdd(&totals[0], (unsigned long long)(hi - lo));
@@ -808,8 +809,8 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
         float* q9;
         if (P.mlp_mode == 1) {
             // the path stops here; the prediction is added after the bounce's shadow rays
-            const int jq = wave_append(true, counts + CNT_MLPQ);
-            q9 = mio.q_in9 + (size_t)jq * 9;
+            const int ja = wave_append(true, counts + CNT_MLPQ_PARTS + part * kCntLine);
+            q9 = mio.qs_in9 + (size_t)(part * cap + (uint32_t)ja) * 9;
         } else {
             q9 = mio.rec_in9 + (size_t)slot * 9;
         }
@@ -819,11 +820,11 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
         q9[3] = (normal.x + 1.0f) * 0.5f; q9[4] = (normal.y + 1.0f) * 0.5f; q9[5] = (normal.z + 1.0f) * 0.5f;
         q9[6] = (-rd.x + 1.0f) * 0.5f; q9[7] = (-rd.y + 1.0f) * 0.5f; q9[8] = (-rd.z + 1.0f) * 0.5f;
         if (P.mlp_mode == 1) {
-            const int jq = (int)((q9 - mio.q_in9) / 9);
-            mio.q_beta[(size_t)jq * 3] = beta.x;
-            mio.q_beta[(size_t)jq * 3 + 1] = beta.y;
-            mio.q_beta[(size_t)jq * 3 + 2] = beta.z;
-            mio.q_slot[jq] = slot;
+            const size_t jq = (size_t)(q9 - mio.qs_in9) / 9;
+            mio.qs_beta[jq * 3] = beta.x;
+            mio.qs_beta[jq * 3 + 1] = beta.y;
+            mio.qs_beta[jq * 3 + 2] = beta.z;
+            mio.qs_slot[jq] = slot;
             return;
         }
         mio.rec_beta[(size_t)slot * 3] = beta.x;
@@ -942,6 +943,22 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue 
     if (COUNT) {
         atomicAdd(&totals[4], (unsigned long long)nv);
         atomicAdd(&totals[5], (unsigned long long)nt);
+    }
+}
+
+// MLP mode 1: the queries appended to the partitions, moved to [0, total) in
+// the order of the partitions' concatenation; counts[CNT_MLPQ] = total
+__global__ void __launch_bounds__(256) k_mlp_compact(uint32_t* counts, uint32_t cap, MlpIO mio) {
+    __shared__ uint32_t prefix[kParts + 1];
+    uint32_t lo, hi;
+    const QueueView view = queue_piece(counts, CNT_MLPQ_PARTS, cap, prefix, lo, hi);
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_MLPQ] = prefix[kParts];
+    const uint32_t step = (gridDim.x / (unsigned)kParts) * blockDim.x;
+    for (uint32_t r = (blockIdx.x / (unsigned)kParts) * blockDim.x + threadIdx.x; r < hi - lo; r += step) {
+        const size_t j = lo + r, e = view.entry(lo + r);
+        for (int k = 0; k < 9; k++) mio.q_in9[j * 9 + k] = mio.qs_in9[e * 9 + k];
+        for (int k = 0; k < 3; k++) mio.q_beta[j * 3 + k] = mio.qs_beta[e * 3 + k];
+        mio.q_slot[j] = mio.qs_slot[e];
     }
 }
 
@@ -1567,9 +1584,12 @@ static int ensure_mlp_bufs(srt_ctx* c, int64_t slots) {
     if (c->mlp_pool_cap >= slots) return SRT_OK;
     for (auto& b : c->mlpbuf) dev_free(c, b);
     const size_t n = (size_t)slots;
-    const size_t sizes[10] = {36 * n, 12 * n, 4 * n, 12 * n, 36 * n, 12 * n, 4 * n, 12 * n, 4 * n, 4 * n};
-    // 0 q_in9, 1 q_beta, 2 q_slot, 3 pred, 4 rec_in9, 5 rec_beta, 6 rec_valid, 7 snap/target, 8 flag, 9 scan
-    for (int j = 0; j < 10; j++)
+    const size_t qn = (size_t)kParts * part_cap((uint32_t)slots);   // partitioned appends (>= n)
+    const size_t sizes[13] = {36 * n, 12 * n, 4 * n, 12 * n, 36 * n, 12 * n, 4 * n, 12 * n, 4 * n, 4 * n,
+                              36 * qn, 12 * qn, 4 * qn};
+    // 0 q_in9, 1 q_beta, 2 q_slot, 3 pred, 4 rec_in9, 5 rec_beta, 6 rec_valid, 7 snap/target, 8 flag, 9 scan,
+    // 10..12 the mode-1 queue as appended (q_in9, q_beta, q_slot per partition)
+    for (int j = 0; j < 13; j++)
         if (int r = dev_alloc(c, c->mlpbuf[j], sizes[j])) return r;
     c->mlp_pool_cap = slots;
     return SRT_OK;
@@ -1609,6 +1629,9 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
         mio.q_in9 = (float*)c->mlpbuf[0].p;
         mio.q_beta = (float*)c->mlpbuf[1].p;
         mio.q_slot = (uint32_t*)c->mlpbuf[2].p;
+        mio.qs_in9 = (float*)c->mlpbuf[10].p;
+        mio.qs_beta = (float*)c->mlpbuf[11].p;
+        mio.qs_slot = (uint32_t*)c->mlpbuf[12].p;
         mio.rec_in9 = (float*)c->mlpbuf[4].p;
         mio.rec_beta = (float*)c->mlpbuf[5].p;
         mio.rec_valid = (uint32_t*)c->mlpbuf[6].p;
@@ -1671,6 +1694,7 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
         if (depth == 1 && mlp_mode == 1) {
             // queued vertex-2 paths: prediction after the bounce's shadow rays (NEE first, as in the plain order)
             float* pred = (float*)c->mlpbuf[3].p;
+            k_mlp_compact<<<shade_grid, 256, 0, st>>>(counts, cap, mio);
             if (int r = mlp_query_dev(c, mio.q_in9, counts + CNT_MLPQ, (int)n, pred)) return r;
             k_mlp_apply<<<shade_grid, 256, 0, st>>>(counts, pred, mio.q_beta, mio.q_slot, Lr, Lg, Lb);
         }
