@@ -10,14 +10,18 @@
 //   1. triangle boxes, box centroids, centroid bounds (two-level reduction)
 //   2. 63-bit Morton codes of centroids, stable radix sort of (code, id)
 //   3. Karras (HPG 2012) binary radix tree, index tie-break for equal codes
-//   4. bottom-up pass with per-node arrival counters: boxes, primitive counts,
-//      SAH costs, collapse-to-leaf decisions (leaves of <= max_leaf triangles)
+//   4. bottom-up pass, one launch per radix-tree depth (deepest first, nodes
+//      sorted by depth): boxes, primitive counts, SAH costs, collapse-to-leaf
+//      decisions (leaves of <= max_leaf triangles)
 //   5. frontier of the highest subtrees with <= T primitives ("clusters")
 //   6. top of the tree rebuilt with a 16-bin SAH sweep over the clusters, in one
 //      workgroup, breadth-first node numbering
 //   7. emit 64-byte nodes (two child boxes per node) and the triangles in
 //      Morton order (48 bytes each, primitive and mesh id in the w lanes)
-// Refit keeps the topology and recomputes child boxes bottom-up.
+// Refit keeps the topology and recomputes child boxes bottom-up, one launch per
+// tree level.  Kernel boundaries order the levels: per-node device-scope fences
+// (an L2 write-back and invalidate each on gfx950) made the arrival-counter
+// form of these passes several times slower.
 #include "srt_internal.h"
 #include <hipcub/hipcub.hpp>
 #include <cstdio>
@@ -186,44 +190,51 @@ __global__ void __launch_bounds__(256) k_karras(const uint64_t* sc, int n, int32
 // ---------------------------------------------------------------------------
 // 4. bottom-up boxes / counts / SAH costs / collapse
 // ---------------------------------------------------------------------------
+constexpr int kMaxRadixDepth = 4096;   // depth of the per-triangle radix tree (16-bit sort keys)
 struct KArr {
     BoxArr box;
     int32_t* count;
     float* cost;
     int32_t* collapse;
 };
-__global__ void __launch_bounds__(256) k_bottom_up(int n, const uint32_t* order, BoxArr tb, const int32_t* kl, const int32_t* kr,
-                                                   const int32_t* par_int, const int32_t* par_leaf, KArr K, int32_t* flags,
-                                                   int max_leaf, float c_trav, float c_isect) {
+// Depth of radix-tree node i below the root (node 0), keyed for the sort that
+// orders the nodes by depth
+__global__ void __launch_bounds__(256) k_radix_depth(int m, const int32_t* par_int, uint32_t* key, uint32_t* idx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    uint32_t depth = 0;
+    for (int p = par_int[i]; p >= 0; p = par_int[p]) depth++;
+    key[i] = depth;
+    idx[i] = (uint32_t)i;
+}
+// One depth of the bottom-up pass (deepest first): box, triangle count, SAH
+// cost and collapse decision of each node from its two children, which the
+// previous launch (or the triangle boxes) provides.
+__global__ void __launch_bounds__(256) k_bottom_up_level(const uint32_t* ids, int count, const uint32_t* order, BoxArr tb,
+                                                         const int32_t* kl, const int32_t* kr, KArr K, int max_leaf,
+                                                         float c_trav, float c_isect) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int p = par_leaf[k];
-    while (p >= 0) {
-        __threadfence();
-        int old = atomicAdd(&flags[p], 1);
-        if (old == 0) return;
-        __threadfence();
-        DBox bl, br;
-        int cl, cr;
-        float costl, costr;
-        int32_t L = kl[p], R = kr[p];
-        if (L < 0) { int q = -L - 1; bl = ldbox(tb, order[q]); cl = 1; costl = c_isect * 1.0f * box_area(bl); }
-        else { bl = ldbox(K.box, L); cl = K.count[L]; costl = K.cost[L]; }
-        if (R < 0) { int q = -R - 1; br = ldbox(tb, order[q]); cr = 1; costr = c_isect * 1.0f * box_area(br); }
-        else { br = ldbox(K.box, R); cr = K.count[R]; costr = K.cost[R]; }
-        DBox b = bl;
-        grow(b, br);
-        int cnt = cl + cr;
-        float sa = box_area(b);
-        float ci = c_trav * sa + costl + costr;
-        float cf = c_isect * (float)cnt * sa;
-        int col = (cnt <= max_leaf) && (cf <= ci);
-        stbox(K.box, p, b);
-        K.count[p] = cnt;
-        K.cost[p] = col ? cf : ci;
-        K.collapse[p] = col;
-        p = par_int[p];
-    }
+    if (k >= count) return;
+    const int p = (int)ids[k];
+    DBox bl, br;
+    int cl, cr;
+    float costl, costr;
+    int32_t L = kl[p], R = kr[p];
+    if (L < 0) { int q = -L - 1; bl = ldbox(tb, order[q]); cl = 1; costl = c_isect * 1.0f * box_area(bl); }
+    else { bl = ldbox(K.box, L); cl = K.count[L]; costl = K.cost[L]; }
+    if (R < 0) { int q = -R - 1; br = ldbox(tb, order[q]); cr = 1; costr = c_isect * 1.0f * box_area(br); }
+    else { br = ldbox(K.box, R); cr = K.count[R]; costr = K.cost[R]; }
+    DBox b = bl;
+    grow(b, br);
+    int cnt = cl + cr;
+    float sa = box_area(b);
+    float ci = c_trav * sa + costl + costr;
+    float cf = c_isect * (float)cnt * sa;
+    int col = (cnt <= max_leaf) && (cf <= ci);
+    stbox(K.box, p, b);
+    K.count[p] = cnt;
+    K.cost[p] = col ? cf : ci;
+    K.collapse[p] = col;
 }
 
 // ---------------------------------------------------------------------------
@@ -847,7 +858,6 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     int32_t* kfirst = tmp.get<int32_t>(n);
     int32_t* par_int = tmp.get<int32_t>(n);
     int32_t* par_leaf = tmp.get<int32_t>(n);
-    int32_t* flags = tmp.get<int32_t>(n);
     KArr K;
     K.box.lox = tmp.get<float>(n); K.box.loy = tmp.get<float>(n); K.box.loz = tmp.get<float>(n);
     K.box.hix = tmp.get<float>(n); K.box.hiy = tmp.get<float>(n); K.box.hiz = tmp.get<float>(n);
@@ -873,10 +883,42 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
             return r;
     }
     k_fill_i32<<<nblk(n), 256, 0, st>>>(par_int, n, -1);
-    k_fill_i32<<<nblk(n), 256, 0, st>>>(flags, n, 0);
     if (n > 1) {
         k_karras<<<nblk(n - 1), 256, 0, st>>>(keys2, n, kl, kr, kfirst, par_int, par_leaf);
-        k_bottom_up<<<nblk(n), 256, 0, st>>>(n, order, tb, kl, kr, par_int, par_leaf, K, flags, max_leaf, c_trav, c_isect);
+        // bottom-up by depth: sort the n - 1 radix-tree nodes by depth, then one launch per depth
+        const int m = n - 1;
+        uint32_t* dkey = tmp.get<uint32_t>(m);
+        uint32_t* dkey2 = tmp.get<uint32_t>(m);
+        uint32_t* didx = tmp.get<uint32_t>(m);
+        uint32_t* dids = tmp.get<uint32_t>(m);
+        int32_t* dfirst = tmp.get<int32_t>(kMaxRadixDepth + 1);
+        if (!dkey || !dkey2 || !didx || !dids || !dfirst) return set_err(c, SRT_E_OOM, "depth scratch");
+        k_radix_depth<<<nblk(m), 256, 0, st>>>(m, par_int, dkey, didx);
+        size_t tbytes = 0;
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, dkey, dkey2, didx, dids, m, 0, 16, st);
+        void* tstore = tmp.get<char>(tbytes);
+        if (!tstore) return set_err(c, SRT_E_OOM, "sort scratch");
+        if (int r = hip_check(c, hipcub::DeviceRadixSort::SortPairs(tstore, tbytes, dkey, dkey2, didx, dids, m, 0, 16, st),
+                              "depth sort"))
+            return r;
+        uint32_t dmax = 0;
+        if (int r = hip_check(c, hipMemcpyAsync(&dmax, dkey2 + (m - 1), 4, hipMemcpyDeviceToHost, st), "copy")) return r;
+        if (int r = hip_check(c, hipStreamSynchronize(st), "depth sort")) return r;
+        if (dmax >= (uint32_t)kMaxRadixDepth)
+            return set_err(c, SRT_E_LIMIT, "radix tree depth %u exceeds %d", dmax, kMaxRadixDepth);
+        k_fill_i32<<<nblk(dmax + 1), 256, 0, st>>>(dfirst, (int64_t)dmax + 1, -1);
+        k_level_first<<<nblk(m), 256, 0, st>>>(m, dkey2, dfirst);
+        std::vector<int32_t> df(dmax + 2);
+        if (int r = hip_check(c, hipMemcpyAsync(df.data(), dfirst, 4 * (size_t)(dmax + 1), hipMemcpyDeviceToHost, st), "copy"))
+            return r;
+        if (int r = hip_check(c, hipStreamSynchronize(st), "depth order")) return r;
+        df[dmax + 1] = m;
+        for (int D = (int)dmax; D >= 0; D--) {
+            if (df[D] < 0) return set_err(c, SRT_E_STATE, "radix tree depth %d is empty", D);
+            const int count = df[D + 1] - df[D];
+            k_bottom_up_level<<<nblk(count), 256, 0, st>>>(dids + df[D], count, order, tb, kl, kr, K, max_leaf, c_trav,
+                                                           c_isect);
+        }
     }
     // host needs a few scalars: root count/collapse
     int32_t root_count = 1, root_collapse = 0, root_first = 0;
