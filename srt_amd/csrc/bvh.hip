@@ -819,15 +819,33 @@ __global__ void __launch_bounds__(256) k_skin(int nv, const float* op, const flo
 // host side
 // ---------------------------------------------------------------------------
 namespace {
+// Build scratch: 256-byte aligned slices bumped out of 16 MiB (or larger)
+// device chunks, all freed when the build ends.  A hipMalloc/hipFree per
+// scratch array cost more than the build's kernels.
 struct Tmp {
+    static constexpr size_t kChunk = (size_t)16 << 20;
     srt_ctx* c;
-    std::vector<DevBuf> bufs;
+    std::vector<DevBuf> chunks;
+    size_t used = 0, cap = 0;      // of the last chunk
+    bool failed = false;           // some get() returned nullptr
     explicit Tmp(srt_ctx* cc) : c(cc) {}
-    ~Tmp() { for (auto& b : bufs) dev_free(c, b); }
+    ~Tmp() { for (auto& b : chunks) dev_free(c, b); }
     template <class T> T* get(size_t count) {
-        bufs.emplace_back();
-        if (dev_alloc(c, bufs.back(), std::max<size_t>(count, 1) * sizeof(T))) return nullptr;
-        return (T*)bufs.back().p;
+        const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+        if (chunks.empty() || used + bytes > cap) {
+            chunks.emplace_back();
+            const size_t size = std::max(bytes, kChunk);
+            if (dev_alloc(c, chunks.back(), size)) {
+                chunks.pop_back();
+                failed = true;
+                return nullptr;
+            }
+            used = 0;
+            cap = size;
+        }
+        T* p = (T*)((char*)chunks.back().p + used);
+        used += bytes;
+        return p;
     }
 };
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
@@ -863,7 +881,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     K.box.hix = tmp.get<float>(n); K.box.hiy = tmp.get<float>(n); K.box.hiz = tmp.get<float>(n);
     K.count = tmp.get<int32_t>(n); K.cost = tmp.get<float>(n); K.collapse = tmp.get<int32_t>(n);
     int32_t* tri_mesh = (int32_t*)s->tri_mesh.p;
-    if (!cen || !K.collapse) return set_err(c, SRT_E_OOM, "BVH build scratch exceeds the HBM budget");
+    if (tmp.failed || !cen || !K.collapse) return set_err(c, SRT_E_OOM, "BVH build scratch exceeds the HBM budget");
 
     // final outputs
     dev_free(c, s->order);
@@ -878,7 +896,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
         size_t tbytes = 0;
         (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, keys, keys2, vals, order, n, 0, 63, st);
         void* tstore = tmp.get<char>(tbytes);
-        if (!tstore) return set_err(c, SRT_E_OOM, "sort scratch");
+        if (tmp.failed || !tstore) return set_err(c, SRT_E_OOM, "sort scratch");
         if (int r = hip_check(c, hipcub::DeviceRadixSort::SortPairs(tstore, tbytes, keys, keys2, vals, order, n, 0, 63, st), "radix sort"))
             return r;
     }
@@ -892,12 +910,12 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
         uint32_t* didx = tmp.get<uint32_t>(m);
         uint32_t* dids = tmp.get<uint32_t>(m);
         int32_t* dfirst = tmp.get<int32_t>(kMaxRadixDepth + 1);
-        if (!dkey || !dkey2 || !didx || !dids || !dfirst) return set_err(c, SRT_E_OOM, "depth scratch");
+        if (tmp.failed || !dkey || !dkey2 || !didx || !dids || !dfirst) return set_err(c, SRT_E_OOM, "depth scratch");
         k_radix_depth<<<nblk(m), 256, 0, st>>>(m, par_int, dkey, didx);
         size_t tbytes = 0;
         (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, dkey, dkey2, didx, dids, m, 0, 16, st);
         void* tstore = tmp.get<char>(tbytes);
-        if (!tstore) return set_err(c, SRT_E_OOM, "sort scratch");
+        if (tmp.failed || !tstore) return set_err(c, SRT_E_OOM, "sort scratch");
         if (int r = hip_check(c, hipcub::DeviceRadixSort::SortPairs(tstore, tbytes, dkey, dkey2, didx, dids, m, 0, 16, st),
                               "depth sort"))
             return r;
@@ -936,7 +954,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
         int32_t* mark = tmp.get<int32_t>(n);
         clusters = tmp.get<int32_t>(n);
         int32_t* nsel = tmp.get<int32_t>(1);
-        if (!nsel) return set_err(c, SRT_E_OOM, "cluster scratch");
+        if (tmp.failed || !nsel) return set_err(c, SRT_E_OOM, "cluster scratch");
         k_fill_i32<<<nblk(n), 256, 0, st>>>(mark, n, 0x7fffffff);
         k_frontier<<<nblk(2 * (int64_t)n - 1), 256, 0, st>>>(n, T, kfirst, K.count, par_int, par_leaf, mark);
         struct NotSentinel {
@@ -945,7 +963,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
         size_t tbytes = 0;
         (void)hipcub::DeviceSelect::If(nullptr, tbytes, mark, clusters, nsel, n, NotSentinel(), st);
         void* tstore = tmp.get<char>(tbytes);
-        if (!tstore) return set_err(c, SRT_E_OOM, "select scratch");
+        if (tmp.failed || !tstore) return set_err(c, SRT_E_OOM, "select scratch");
         if (int r = hip_check(c, hipcub::DeviceSelect::If(tstore, tbytes, mark, clusters, nsel, n, NotSentinel(), st), "select"))
             return r;
         hip_check(c, hipMemcpyAsync(&m, nsel, 4, hipMemcpyDeviceToHost, st), "copy");
@@ -981,7 +999,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
         int32_t* ln[2] = {tmp.get<int32_t>(m), tmp.get<int32_t>(m)};
         TopSplit* sp = tmp.get<TopSplit>(m);
         int32_t* ctr = tmp.get<int32_t>(2);     // next free node id, next level size
-        if (!ctr) return set_err(c, SRT_E_OOM, "top-level scratch");
+        if (tmp.failed || !ctr) return set_err(c, SRT_E_OOM, "top-level scratch");
         k_items<<<nblk(m), 256, 0, st>>>(m, clusters, order, tb, K, kfirst, top, it);
         // level 0: the root (node 0) over all items; ids 1.. follow breadth-first
         k_fill_i32<<<1, 64, 0, st>>>(lb[0], 1, 0);
@@ -1031,13 +1049,13 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
         uint32_t* lkey2 = tmp.get<uint32_t>(num_nodes);
         uint32_t* lidx = tmp.get<uint32_t>(num_nodes);
         int32_t* first = tmp.get<int32_t>(256);
-        if (!lkey || !lkey2 || !lidx || !first) return set_err(c, SRT_E_OOM, "level scratch");
+        if (tmp.failed || !lkey || !lkey2 || !lidx || !first) return set_err(c, SRT_E_OOM, "level scratch");
         k_node_level<<<nblk(num_nodes), 256, 0, st>>>(num_nodes, valid, parent, lkey, lidx);
         size_t tbytes = 0;
         (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, lkey, lkey2, lidx, (uint32_t*)s->level_nodes.p, num_nodes,
                                                  0, 8, st);
         void* tstore = tmp.get<char>(tbytes);
-        if (!tstore) return set_err(c, SRT_E_OOM, "sort scratch");
+        if (tmp.failed || !tstore) return set_err(c, SRT_E_OOM, "sort scratch");
         if (int r = hip_check(c, hipcub::DeviceRadixSort::SortPairs(tstore, tbytes, lkey, lkey2, lidx,
                                                                     (uint32_t*)s->level_nodes.p, num_nodes, 0, 8, st),
                               "level sort"))
