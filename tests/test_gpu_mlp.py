@@ -127,3 +127,21 @@ def test_integrator_hooks_train_and_query(ctx):
         ctx.mlp_train_frame(h, s.camera, 128, 128, 4, 5, 4, seed=1000 + f, max_samples=1 << 15, steps=4)
     assert np.array_equal(ctx.mlp_params(), p_a)
     ctx.release(h)
+
+
+def test_query_paths_independent_of_chunking(ctx):
+    """Mode-1 paths queue their vertex-2 queries per queue partition and the
+    queries are compacted before inference: the image is the same for one
+    chunk and for chunks of 1000 paths (ragged partitions, other query order)."""
+    from srt_amd import scenes
+    s = scenes.cornell()
+    h = ctx.upload(s)
+    ctx.mlp_create(seed=3, lr=1e-3)
+    ctx.mlp_train_frame(h, s.camera, 64, 64, 2, 5, 4, seed=5, max_samples=1 << 12, steps=2)
+    a = ctx.render(h, s.camera, 48, 40, 8, 5, 4, 1, 1, 9, mlp_mode=1)
+    ctx.set_option("path_pool", 1000)
+    b = ctx.render(h, s.camera, 48, 40, 8, 5, 4, 1, 1, 9, mlp_mode=1)
+    ctx.set_option("path_pool", 1 << 27)
+    ctx.release(h)
+    assert np.isfinite(a).all()
+    assert np.array_equal(a, b)
