@@ -23,7 +23,8 @@ the counter reads for a copy kernel of known size (--mode copy), condensed by
 reads when present.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
-       python bench.py --mode skin|mlp|serve (skinned animation; radiance MLP at equal time; serving)
+       python bench.py --mode skin|mlp|serve|passes (skinned animation; radiance MLP at equal time; serving;
+                                    Simple / LightTracer / Denoise pass throughput)
        python bench.py --mode copy          (calibration kernel for the counters)
        python bench.py --pmc-summary FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
        python bench.py --kstats-summary DIR  (rocprofv3 --kernel-trace --stats output -> profiles/)
@@ -222,9 +223,13 @@ def measured_valu(kernel):
 KSTATS_FILE = os.path.join(ROOT, "profiles", "frame_spheres1m_kernel_stats.md")
 
 
-def kstats_summary(d):
+PASSES_KSTATS_FILE = os.path.join(ROOT, "profiles", "passes_spheres1m_kernel_stats.md")
+
+
+def kstats_summary(d, mode="frame"):
     """Per-kernel time summary of a `rocprofv3 --kernel-trace --stats` run of
-    this script (rocpd database or kernel_stats.csv)."""
+    this script (rocpd database or kernel_stats.csv); mode "frame" (the default
+    bench) or "passes" (--mode passes)."""
     import csv
     import glob
     import sqlite3
@@ -243,21 +248,34 @@ def kstats_summary(d):
         raise SystemExit(f"no kernel statistics under {d}")
     rows.sort(key=lambda r: -r[2])
     total = sum(r[2] for r in rows)
-    lines = ["# Kernel times: 1M-triangle frame (1920x1080x64spp, depth 8)", "",
-             "Command (from the repository root; TMPDIR=/tmp):", "",
-             "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_ks -o run -- python3 bench.py --steps 2 --warmup 1 --no-cpu",
-             "    python3 bench.py --kstats-summary gpurun_out/prof_ks", "",
-             "The run renders 6 frames: the untimed counting pass (k_trace<true, true>, k_shadow<true, true>), 1 warmup,",
-             "2 timed frames and 2 frames copied to host memory; the BVH build (k_top_split/k_top_link, k_bottom_up,",
-             "sorts, 4-wide collapse) runs once.", "",
-             "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
+    if mode == "passes":
+        lines = ["# Kernel times: Simple, LightTracer and Denoise passes on the 1M-triangle scene (1920x1080)", "",
+                 "Command (from the repository root; TMPDIR=/tmp):", "",
+                 "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_passes -o run -- python3 bench.py --mode passes "
+                 "--steps 3 --warmup 1",
+                 "    python3 bench.py --kstats-summary gpurun_out/prof_passes --kstats-mode passes", "",
+                 "Each pass runs 1 warmup + 3 timed calls: Simple Depth and Normal at 4 samples per pixel",
+                 "(k_simple_trace, k_simple_accum), LightTracer at 1 particle per pixel and depth 8 (k_lt_raygen, k_trace,",
+                 "k_lt_shade, k_lt_shadow), Denoise (k_denoise_guide, 5 x k_atrous); the BVH build runs once.", "",
+                 "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
+        out_file = PASSES_KSTATS_FILE
+    else:
+        lines = ["# Kernel times: 1M-triangle frame (1920x1080x64spp, depth 8)", "",
+                 "Command (from the repository root; TMPDIR=/tmp):", "",
+                 "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_ks -o run -- python3 bench.py --steps 2 --warmup 1 --no-cpu",
+                 "    python3 bench.py --kstats-summary gpurun_out/prof_ks", "",
+                 "The run renders 6 frames: the untimed counting pass (k_trace<true, true>, k_shadow<true, true>), 1 warmup,",
+                 "2 timed frames and 2 frames copied to host memory; the BVH build (k_top_split/k_top_link, k_bottom_up_level,",
+                 "sorts, 4-wide collapse) runs once.", "",
+                 "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
+        out_file = KSTATS_FILE
     for n, c, t, a in rows[:16]:
         short = n.split("(")[0].replace("void ", "")
         if len(short) > 70:
             short = short[:67] + "..."
         lines.append(f"| `{short}` | {c} | {t / 1e6:.3f} | {a / 1e3:.1f} | {100 * t / total:.1f}% |")
     text = "\n".join(lines) + "\n"
-    with open(KSTATS_FILE, "w") as fh:
+    with open(out_file, "w") as fh:
         fh.write(text)
     print(text)
 
@@ -311,6 +329,69 @@ def mlp_summary(ks_dir, pmc_dir):
 
 def _stats_ms(st, name, steps):
     return round(st[name]["ms"] / steps, 3) if name in st else None
+
+
+DENOISE_BYTES_PER_PIXEL = 32 + 5 * 48   # guide: 16 in + 16 out; each of 5 passes: colour + guide in, colour out
+
+
+def passes_mode(args):
+    """The other passes on the 1M-triangle scene at 1920x1080, output resident
+    in device memory: Simple (Depth and Normal, 4 samples per pixel = camera
+    rays traced), LightTracer (1 particle per pixel, depth 8) and Denoise
+    (five a-trous passes over the float4 image)."""
+    from srt_amd import abi, scenes
+    s = scenes.spheres()
+    R = s.render
+    W, H = R["width"], R["height"]
+    npix = W * H
+    ctx = abi.Context(0)
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    img = ctx.device_alloc(16 * npix)
+    reps = max(1, args.steps)
+    out = {"metric": "throughput of the Simple, LightTracer and Denoise passes", "n_gpus": 1, "dtype": "f32",
+           "data": "synthetic", "steps": reps,
+           "config": {"workload": "spheres-1M: 3125 frequency-4 icospheres (1,000,000 triangles) + 4 quad lights, "
+                                  "1920x1080", "width": W, "height": H}}
+
+    def timed(fn, key):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        ctx.set_option("profile", 1)
+        ctx.stats_reset()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        wall = (time.perf_counter() - t0) / reps
+        ctx.set_option("profile", 0)
+        st = ctx.stats()
+        return wall, st[key]["ms"] / st[key]["launches"] if key in st and st[key]["launches"] else None
+
+    spp = 4
+    for t in ("Depth", "Normal"):
+        wall, kms = timed(lambda: ctx.render_simple(h, s.camera, W, H, type=t, spp=spp, seed=1, device_ptr=img),
+                          "simple")
+        out[f"simple_{t.lower()}"] = {"spp": spp, "ms": round(wall * 1e3, 3), "gpu_ms": round(kms, 3),
+                                      "Mrays_s": round(npix * spp / (kms * 1e-3) / 1e6, 1)}
+    wall, kms = timed(lambda: ctx.render_light(h, s.camera, W, H, spp=1, max_depth=8, rr_depth=4, seed=2,
+                                               device_ptr=img), "light_tracer")
+    out["light_tracer"] = {"spp": 1, "max_depth": 8, "ms": round(wall * 1e3, 3), "gpu_ms": round(kms, 3),
+                           "Mpaths_s": round(npix / (kms * 1e-3) / 1e6, 1)}
+    rng = np.random.default_rng(3)
+    noisy = np.ones((npix, 4), np.float32)
+    noisy[:, :3] = rng.gamma(2.0, 0.5, size=(npix, 3)).astype(np.float32)
+    ctx.upload_bytes(img, noisy)
+    wall, kms = timed(lambda: ctx.denoise_device(img, W, H), "denoise")
+    achieved = DENOISE_BYTES_PER_PIXEL * npix / (kms * 1e-3) / 1e9
+    out["denoise"] = {"ms": round(wall * 1e3, 3), "gpu_ms": round(kms, 3),
+                      "Mpixels_s": round(npix / (kms * 1e-3) / 1e6, 1),
+                      "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                   "bytes_per_pixel": DENOISE_BYTES_PER_PIXEL}}
+    ctx.device_free(img)
+    ctx.release(h)
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def skin_mode(args):
@@ -587,7 +668,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-row-step", type=int, default=8, help="CPU baseline renders every k-th row")
     ap.add_argument("--path-pool", type=int, default=0)
-    ap.add_argument("--mode", choices=["frame", "copy", "skin", "mlp", "serve"], default="frame")
+    ap.add_argument("--mode", choices=["frame", "copy", "skin", "mlp", "serve", "passes"], default="frame")
     ap.add_argument("--mlp-frames", type=int, default=16)
     ap.add_argument("--mlp-steps", type=int, default=8)
     ap.add_argument("--mlp-scene", choices=["cornell", "spheres"], default="cornell")
@@ -597,6 +678,7 @@ def main():
     ap.add_argument("--pmc-summary", nargs=4, metavar="DIR")
     ap.add_argument("--kstats-summary", metavar="DIR")
     ap.add_argument("--valu-summary", metavar="DIR")
+    ap.add_argument("--kstats-mode", choices=["frame", "passes"], default="frame")
     ap.add_argument("--mlp-summary", nargs=2, metavar="DIR")
     args = ap.parse_args()
     if args.mlp_summary:
@@ -606,7 +688,7 @@ def main():
         pmc_summary(*args.pmc_summary)
         return
     if args.kstats_summary:
-        kstats_summary(args.kstats_summary)
+        kstats_summary(args.kstats_summary, args.kstats_mode)
         return
     if args.valu_summary:
         valu_summary(args.valu_summary)
@@ -620,7 +702,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.mode != "frame":
-        {"skin": skin_mode, "mlp": mlp_mode, "serve": serve_mode}[args.mode](args)
+        {"skin": skin_mode, "mlp": mlp_mode, "serve": serve_mode, "passes": passes_mode}[args.mode](args)
         return
 
     from srt_amd import abi, scenes

@@ -354,6 +354,27 @@ class Context:
         self.check(self.lib.srt_tonemap(self.h, _ptr(a), a.shape[0], t, C.c_float(exposure), 1 if use_gamma else 0, 0))
         return a
 
+    # ---- device buffers of the context ----
+    def device_alloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.lib.srt_device_alloc(self.h, nbytes, C.byref(p)))
+        return p.value
+
+    def device_free(self, ptr):
+        self.check(self.lib.srt_device_free(self.h, C.c_void_p(ptr)))
+
+    def upload_bytes(self, ptr, array):
+        a = np.ascontiguousarray(array)
+        self.check(self.lib.srt_memcpy(self.h, C.c_void_p(ptr), a.ctypes.data_as(C.c_void_p), a.nbytes, 0))
+
+    def download_bytes(self, array, ptr):
+        assert array.flags["C_CONTIGUOUS"]
+        self.check(self.lib.srt_memcpy(self.h, array.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), array.nbytes, 1))
+
+    def denoise_device(self, ptr, width, height):
+        """Denoise pass in place on a (H*W) float4 image in device memory."""
+        self.check(self.lib.srt_denoise(self.h, C.c_void_p(ptr), width, height, 1))
+
     def denoise(self, rgba, width, height):
         a = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4).copy()
         self.check(self.lib.srt_denoise(self.h, _ptr(a), width, height, 0))

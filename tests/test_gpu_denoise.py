@@ -44,4 +44,15 @@ def test_odd_sizes_and_device_film(ctx):
     W, H = 17, 5
     a = np.ones((W * H, 4), np.float32)
     a[:, :3] = rng.gamma(2.0, 0.3, size=(W * H, 3))
-    assert np.array_equal(ctx.denoise(a, W, H), O.denoise(a, W, H))
+    ref = O.denoise(a, W, H)
+    assert np.array_equal(ctx.denoise(a, W, H), ref)
+    # the same pass in place on a context-owned device buffer
+    ptr = ctx.device_alloc(a.nbytes)
+    try:
+        ctx.upload_bytes(ptr, a)
+        ctx.denoise_device(ptr, W, H)
+        back = np.zeros_like(a)
+        ctx.download_bytes(back, ptr)
+    finally:
+        ctx.device_free(ptr)
+    assert np.array_equal(back, ref)
