@@ -75,6 +75,15 @@ def test_queue_partition_shapes(ctx, pool):
     assert ca == cb
 
 
+@pytest.mark.parametrize("W,H,spp", [(1, 1, 1), (3, 2, 5), (7, 9, 1)])
+def test_tiny_images(ctx, W, H, spp):
+    """Fewer paths than one 64-entry turn of a partition: every other partition
+    is empty in every queue."""
+    s = scenes.cornell()
+    img, ref = _render_pair(ctx, s, W, H, spp, 5, seed=21)
+    assert np.array_equal(img, ref)
+
+
 def test_spheres_small(ctx):
     s = scenes.spheres()
     img, ref = _render_pair(ctx, s, 192, 108, 2, 8)
