@@ -503,6 +503,13 @@ SRT_D void camera_ray(const DevCamera& cam, int width, int height, uint32_t pixe
     rd = dir.x * X + dir.y * Y + dir.z * Z;
 }
 
+// Sample index of path id pid (>= 0) in planes of wh > 0 pixels: a 32-bit
+// division when pid fits in 32 bits (every frame below 2^32 paths), else 64-bit.
+SRT_D uint32_t path_sample(int64_t pid, int32_t wh) {
+    if (((uint64_t)pid >> 32) == 0u) return (uint32_t)pid / (uint32_t)wh;
+    return (uint32_t)(pid / wh);
+}
+
 // One camera ray per slot, at queue entry = slot.  Slot i belongs to
 // segment k: path id pid = path_begin + (i - slot_offset), sample = pid / (W H),
 // pixel = tile_pixel(pid mod (W H)).
@@ -518,7 +525,7 @@ __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* L
     const int k = find_segment(segs, nseg, (uint32_t)i);
     const Segment& sg = segs[k];
     int64_t pid = sg.path_begin + (i - sg.slot_offset);
-    uint32_t sample = (uint32_t)(pid / sg.wh);
+    uint32_t sample = path_sample(pid, sg.wh);
     uint32_t pixel = tile_pixel((uint32_t)(pid - (int64_t)sample * sg.wh), (uint32_t)sg.width, (uint32_t)sg.height);
     f3 ro, rd;
     Pcg32 rng;
@@ -768,7 +775,7 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     if (dot(normal, -rd) < 0.0f) normal = -normal;
     const bool last = (depth + 1 >= P.max_depth);
     const int64_t pid = P.path_begin + (int64_t)(slot - (uint32_t)P.slot_offset);
-    const uint32_t sample = (uint32_t)(pid / P.wh);
+    const uint32_t sample = path_sample(pid, P.wh);
     Pcg32 rng = rng_resume(qin.rng[i], sample);
     // next-event estimation (path_tracer.cu:113-142); the shadow ray and its
     // would-be contribution go to the shadow queue
