@@ -43,6 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TLANE_OPS = 256 * 4 * 16 * 2.4e9 / 1e12   # 256 CUs x 4 SIMDs x 16 lanes per clock x 2.4 GHz
 MFMA_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA, MI355X
 MLP_TRAIN_FLOP = 99072           # forward + backward + weight gradients per training sample (16,512 MACs x 6)
 MLP_FWD_FLOP = 33024             # forward per query (16,512 MACs x 2)
@@ -205,6 +206,26 @@ def valu_summary(d):
     with open(VALU_FILE, "w") as fh:
         fh.write(text)
     print(text)
+
+
+def measured_valu_counts(kernel):
+    """(wave64 VALU instructions per launch, active lanes per instruction, SIMD
+    busy fraction, source file) of `kernel` from the committed counter summary."""
+    if not os.path.exists(VALU_FILE):
+        return None
+    insts = None
+    with open(VALU_FILE) as fh:
+        for line in fh:
+            c = [x.strip() for x in line.strip().strip("|").split("|")]
+            if len(c) == 6 and c[0] == kernel:
+                try:
+                    insts = float(c[2]) / float(c[1])
+                except ValueError:
+                    pass
+    v = measured_valu(kernel)
+    if insts is None or v is None:
+        return None
+    return insts, v[1], v[0], os.path.relpath(VALU_FILE, ROOT)
 
 
 def measured_valu(kernel):
@@ -648,6 +669,85 @@ def serve_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def frame_roofline(kern, st, steps, bytes_step, segments):
+    """Roofline of the dominant frame kernel against the resource that binds it.
+
+    The traversal kernels are bound by vector-ALU issue, not by HBM: their
+    node and triangle fetches mostly hit in L2/MALL.  So `achieved` is the
+    kernel's active VALU lane-operations per second (wave64 VALU instructions
+    per launch x 64 x active-lane fraction, from the committed SQ counter
+    summary, over the live HIP-event launch time) against the peak of 256 CUs x
+    4 SIMDs x 16 lanes per clock at 2.4 GHz; `frac` = achieved / peak.
+    `traffic` is the kernel's calibrated HBM bytes per launch (FETCH_SIZE +
+    WRITE_SIZE counters), and `hbm` gives, for each frame kernel, those bytes
+    over its live launch time against the 8 TB/s HBM peak.  `algorithmic`
+    keeps the byte model (ray and hit records, 112 B per 4-wide node visit,
+    36 B per triangle test): it counts cache hits, so its rate is not an HBM
+    figure."""
+    avg_ms = {k: st[k]["ms"] / st[k]["launches"] for k in bytes_step if k in st and st[k]["launches"]}
+    launches = {k: st[k]["launches"] / steps for k in avg_ms}
+    hbm = {}
+    for k in avg_ms:
+        t = measured_traffic(k)
+        if t:
+            gbs = t / (avg_ms[k] * 1e-3) / 1e9
+            hbm[k] = {"bytes_per_launch": t, "avg_launch_ms": round(avg_ms[k], 4), "GBs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    alg = {k: {"bytes_per_launch": int(bytes_step[k] / launches[k]),
+               "GBs": round(bytes_step[k] / launches[k] / (avg_ms[k] * 1e-3) / 1e9, 1)} for k in avg_ms}
+    r = {"kernel": kern, "avg_launch_ms": round(avg_ms[kern], 4), "traffic": hbm.get(kern, {}).get("bytes_per_launch"),
+         "hbm": hbm, "algorithmic": alg}
+    vc = measured_valu_counts(kern)
+    if vc:
+        insts, lanes, busy, source = vc
+        lane_ops = insts * 64.0 * lanes
+        achieved = lane_ops / (avg_ms[kern] * 1e-3) / 1e12
+        r.update({"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TLANE_OPS,
+                  "unit": "Tlane-op/s", "frac": round(achieved / VALU_PEAK_TLANE_OPS, 4),
+                  "valu_issue": {"wave_instructions_per_launch": insts, "active_lanes_frac": round(lanes, 3),
+                                 "simd_busy_frac": round(busy, 3),
+                                 "lane_ops_per_segment": round(lane_ops * launches[kern] / segments, 1),
+                                 "source": source}})
+    elif kern in hbm:
+        r.update({"bound": "hbm", "achieved": hbm[kern]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": hbm[kern]["frac"]})
+    else:
+        r.update({"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                  "note": "no counter summaries under profiles/"})
+    return r
+
+
+def host_cpu():
+    """Usable host cores and their description: the scheduler affinity set,
+    capped by a cgroup CPU quota when one is set (cgroup v2 cpu.max or v1
+    cfs_quota_us), plus os.cpu_count() and the /proc/cpuinfo model name."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        try:
+            q = int(open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us").read())
+            per = int(open("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read())
+            if q > 0:
+                quota = q / per
+        except (OSError, ValueError):
+            pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"usable": usable, "affinity": affinity, "cgroup_quota": quota, "cpu_count": os.cpu_count(),
+            "model": model}
+
+
 def measured_traffic(kernel):
     """Calibrated HBM bytes per launch of `kernel` from the committed counter summary."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -769,32 +869,14 @@ def main():
                  + SHADOW_IN * shadow_rays,
     }
     kern = max((k for k in st if k in bytes_step), key=lambda k: st[k]["ms"])
-    launches_per_step = st[kern]["launches"] / args.steps
-    avg_ms = st[kern]["ms"] / st[kern]["launches"]
-    bytes_launch = bytes_step[kern] / launches_per_step
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    traffic = measured_traffic(kern)
-    roofline = {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_launch": int(bytes_launch), "avg_launch_ms": round(avg_ms, 4),
-                "per_kernel_GBs": {k: round(bytes_step[k] / (st[k]["ms"] / args.steps * 1e-3) / 1e9, 1)
-                                   for k in bytes_step if k in st and st[k]["ms"] > 0}}
-    if traffic:
-        # node and triangle fetches of the traversal mostly hit in L2: the
-        # algorithmic rate can exceed the HBM peak while HBM carries a fraction
-        roofline["hbm_share_of_algorithmic"] = round(traffic / bytes_launch, 3)
-        roofline["note"] = ("algorithmic bytes count every node/triangle fetch; most are L2 hits, so HBM carries "
-                            "hbm_share_of_algorithmic of them (traffic, measured); traversal is bound by VALU issue")
-    valu = measured_valu(kern)
-    if valu:
-        roofline["valu_issue"] = {"simd_busy_frac": round(valu[0], 3), "active_lanes_frac": round(valu[1], 3),
-                                  "source": os.path.relpath(VALU_FILE, ROOT)}
+    roofline = frame_roofline(kern, st, args.steps, bytes_step, segments)
 
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as O   # CPU baseline leg only
-        threads = min(os.cpu_count() or 1, 16)
-        log(f"CPU baseline: oracle on every {args.cpu_row_step}th row, {threads} threads")
+        hc = host_cpu()
+        threads = hc["usable"]
+        log(f"CPU baseline: oracle on every {args.cpu_row_step}th row, {threads} threads on {hc}")
         osc = O.Scene(s)
         ob = O.Bvh(osc)
         img, secs = O.render(osc, s.camera, W, H, spp, depth, R["rr_depth"], 1, 1, R["seed"], bvh=ob,
@@ -805,9 +887,14 @@ def main():
         b = img.reshape(H, W, 4)[rows, :, :3]
         rel = np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), 1e-6)
         cpu = {"value": round(cpu_paths / secs / 1e6, 3), "unit": "Mpaths/s", "cores": threads, "kind": "port",
+               "host": {"usable_cores": hc["usable"], "affinity_cores": hc["affinity"],
+                        "cgroup_cpu_quota": hc["cgroup_quota"], "os_cpu_count": hc["cpu_count"],
+                        "model": hc["model"]},
+               "gpu_vs_cpu": None,
                "sample": f"{len(rows)} of {H} rows (every {args.cpu_row_step}th) of the same 1920x1080x64spp frame, "
                          f"{cpu_paths} paths, {secs:.1f} s",
                "gpu_vs_cpu_max_rel_diff": float(rel.max())}
+        cpu["gpu_vs_cpu"] = round(value / cpu["value"], 1)
         log(f"CPU {cpu['value']} Mpaths/s, GPU/CPU {value / cpu['value']:.1f}x, max rel diff {rel.max():.2e}")
 
     lat_sorted = sorted(lat)

@@ -112,7 +112,12 @@ const char* srt_last_error(const srt_ctx* ctx);
  * (0: as many as fit), "trace_bvh_width" (4: traverse the 4-wide tree collapsed
  * from the binary one, default; 2: the binary tree; results are identical),
  * "trace_inner_exit" (a traversal wave leaves its inner-node loop once at most
- * this many lanes still search for a leaf, 0..63, default 8) */
+ * this many lanes still search for a leaf, 0..63, default 8), "max_wide_nodes"
+ * (a BVH built while this is set keeps its 4-wide form only up to this many
+ * 4-wide nodes, else it is traversed in binary form with the same results;
+ * 0..16777215, the default: 4-wide nodes are addressed with 32-bit byte
+ * offsets).  Limits: 2^28 - 1 triangles per scene; "path_pool" and the pool
+ * of srt_serve_start in [64, 2^31 - 1]. */
 int srt_set_option(srt_ctx* ctx, const char* key, int64_t value);
 size_t srt_hbm_used(const srt_ctx* ctx);
 

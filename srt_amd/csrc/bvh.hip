@@ -855,7 +855,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     if (max_leaf < 1 || max_leaf > kMaxLeaf) return set_err(c, SRT_E_LIMIT, "max_leaf must be in [1, %d]", kMaxLeaf);
     const int n = s->nf;
     if (n < 1) return set_err(c, SRT_E_INVALID, "scene has no triangles");
-    if ((int64_t)n >= (1ll << 28)) return set_err(c, SRT_E_LIMIT, "too many triangles for the leaf encoding");
+    if ((int64_t)n > kMaxTriangles) return set_err(c, SRT_E_LIMIT, "too many triangles for the leaf encoding");
     hipStream_t st = c->stream;
     const int T = cluster_size > 0 ? cluster_size : std::max(64, n / 16384);
     const float c_trav = 1.0f, c_isect = 1.0f;
@@ -1113,6 +1113,14 @@ int bvh4_build(srt_ctx* c, SceneRec* s) {
     int base = 0, nf = 1, cur = 0, levels = 0;
     while (nf > 0) {
         if (base + nf > n2) return set_err(c, SRT_E_LIMIT, "4-wide collapse exceeded the binary node count");
+        if (!nodes4_fit((int64_t)base + nf, c->max_wide_nodes)) {
+            // beyond 32-bit buffer offsets: keep the binary tree only
+            dev_free(c, s->nodes4);
+            s->root4 = s->root;
+            s->num_nodes4 = 0;
+            s->depth4 = 0;
+            return SRT_OK;
+        }
         k_c4_count<<<nblk(nf), 256, 0, st>>>(nf, lev[cur], (const BvhNode*)s->nodes.p, ninner);
         if (int r = hip_check(c, hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, ninner, off, nf, st), "scan"))
             return r;

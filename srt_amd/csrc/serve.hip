@@ -231,7 +231,9 @@ extern "C" {
 int srt_serve_start(srt_ctx* c, int64_t path_pool) {
     if (!c) return SRT_E_INVALID;
     if (c->serve) return set_err(c, SRT_E_STATE, "serving already started");
-    if (path_pool < 0) return set_err(c, SRT_E_INVALID, "path_pool must be >= 0");
+    if (path_pool != 0 && !path_pool_ok(path_pool))
+        return set_err(c, SRT_E_INVALID, "path_pool must be 0 (the context's) or in [%lld, %lld]",
+                       (long long)kMinPathPool, (long long)kMaxPathPool);
     ServeState* S = new ServeState();
     S->pool = path_pool > 0 ? path_pool : c->pool_request;
     S->t0 = std::chrono::steady_clock::now();

@@ -183,7 +183,9 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
     SRT_IDLE(c);
     if (!c || !key) return SRT_E_INVALID;
     if (!strcmp(key, "path_pool")) {
-        if (value < 64 || value > (int64_t)0x7fffffff) return set_err(c, SRT_E_INVALID, "path_pool out of range");
+        if (!path_pool_ok(value))
+            return set_err(c, SRT_E_INVALID, "path_pool must be in [%lld, %lld]", (long long)kMinPathPool,
+                           (long long)kMaxPathPool);
         c->pool_request = value;
         return SRT_OK;
     }
@@ -212,6 +214,12 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
     if (!strcmp(key, "trace_bvh_width")) {
         if (value != 2 && value != 4) return set_err(c, SRT_E_INVALID, "trace_bvh_width must be 2 or 4");
         c->trace_bvh_width = (int32_t)value;
+        return SRT_OK;
+    }
+    if (!strcmp(key, "max_wide_nodes")) {
+        if (value < 0 || value > kMaxNodes4)
+            return set_err(c, SRT_E_INVALID, "max_wide_nodes must be in [0, %lld]", (long long)kMaxNodes4);
+        c->max_wide_nodes = value;
         return SRT_OK;
     }
     if (!strcmp(key, "trace_blocks_per_cu")) {

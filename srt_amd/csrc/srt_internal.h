@@ -9,6 +9,7 @@
 #include <mutex>
 #include <atomic>
 #include "dev_math.h"
+#include "limits.h"
 #include "../../include/srt_hip.h"
 
 namespace srt {
@@ -276,6 +277,7 @@ struct srt_ctx {
     int32_t trace_lds_depth = 16, trace_refill = 56, shadow_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
     int32_t trace_bvh_width = 4;          // traversal over the 4-wide (4) or binary (2) tree
     int32_t trace_inner_exit = 8;         // stragglers resume after the leaf step
+    int64_t max_wide_nodes = srt::kMaxNodes4;   // 4-wide trees above this many nodes are traversed in binary form
     // profiling
     bool profile = false;
     std::map<std::string, srt::KernelStat> stats;

@@ -1464,11 +1464,17 @@ static int num_cus(srt_ctx* c) {
     return cached;
 }
 
-static TraceCfg trace_cfg(srt_ctx* c) {
+// The 4-wide tree is traversed when the context asks for it and the scene
+// has one (a tree beyond 32-bit node offsets keeps only its binary form).
+static bool use_wide(const srt_ctx* c, const SceneRec* s) {
+    return c->trace_bvh_width == 4 && (s == nullptr || s->num_nodes4 > 0 || s->root < 0);
+}
+
+static TraceCfg trace_cfg(srt_ctx* c, const SceneRec* s) {
     TraceCfg cfg;
     cfg.lds_depth = c->trace_lds_depth;
     cfg.refill = c->trace_refill;
-    cfg.wide = c->trace_bvh_width == 4 ? 1 : 0;
+    cfg.wide = use_wide(c, s) ? 1 : 0;
     cfg.inner_exit = c->trace_inner_exit;
     return cfg;
 }
@@ -1478,7 +1484,7 @@ static size_t trace_lds_bytes(const TraceCfg& cfg) { return (size_t)cfg.lds_dept
 // Persistent traversal grid: as many workgroups as are resident at once
 // (occupancy of the traversal kernel with its LDS stack), capped by the work.
 static int trace_grid(srt_ctx* c, int64_t n) {
-    const TraceCfg cfg = trace_cfg(c);
+    const TraceCfg cfg = trace_cfg(c, nullptr);
     int per_cu = c->trace_blocks_per_cu;
     if (per_cu <= 0) {
         int nb = 0;
@@ -1498,7 +1504,7 @@ static int trace_grid(srt_ctx* c, int64_t n) {
 // LDS entries.  A binary tree pushes at most one entry per level (depth + 1
 // entries); the 4-wide tree at most three per level.
 static int ensure_gstack(srt_ctx* c, const SceneRec* s, int64_t threads) {
-    const int64_t deepest = (c->trace_bvh_width == 4) ? 3 * (int64_t)s->depth4 + 1 : (int64_t)s->max_depth + 1;
+    const int64_t deepest = use_wide(c, s) ? 3 * (int64_t)s->depth4 + 1 : (int64_t)s->max_depth + 1;
     const int64_t per = std::max<int64_t>(1, deepest - c->trace_lds_depth);
     const size_t need = (size_t)(threads * per);
     if (c->gstack_entries >= need) return SRT_OK;
@@ -1620,7 +1626,7 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     const int tgrid = part_grid(trace_grid(c, n));
     const uint32_t cap = part_cap((uint32_t)n);
     if (int r = ensure_gstack(c, s, (int64_t)tgrid * kTraceBlock)) return r;
-    const TraceCfg tcfg = trace_cfg(c);
+    const TraceCfg tcfg = trace_cfg(c, s);
     TraceCfg scfg = tcfg;
     scfg.refill = c->shadow_refill;
     const size_t tlds = trace_lds_bytes(tcfg);
@@ -1793,7 +1799,7 @@ int render_simple(srt_ctx* c, SceneRec* s, const DevCamera& cam, int type, int s
     DevScene S = dev_scene(s);
     const int grid = trace_grid(c, n);
     if (int r = ensure_gstack(c, s, (int64_t)grid * kTraceBlock)) return r;
-    const TraceCfg cfg = trace_cfg(c);
+    const TraceCfg cfg = trace_cfg(c, s);
     float box_far[3], box_near[3];
     bool inside = true;
     for (int a = 0; a < 3; a++) {
@@ -1841,7 +1847,7 @@ int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max
     const int tgrid = part_grid(trace_grid(c, pool));
     DevScene S = dev_scene(s);
     if (int r = ensure_gstack(c, s, (int64_t)tgrid * kTraceBlock)) return r;
-    const TraceCfg tcfg = trace_cfg(c);
+    const TraceCfg tcfg = trace_cfg(c, s);
     TraceCfg scfg = tcfg;
     scfg.refill = c->shadow_refill;
     const size_t tlds = trace_lds_bytes(tcfg);
@@ -1892,7 +1898,7 @@ int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* 
     DevScene S = dev_scene(s);
     int grid = trace_grid(c, n);
     if (int r = ensure_gstack(c, s, (int64_t)grid * kTraceBlock)) return r;
-    const TraceCfg cfg = trace_cfg(c);
+    const TraceCfg cfg = trace_cfg(c, s);
     RaysIO io;
     io.o3 = o_dev; io.d3 = d_dev; io.tmax = tmax_dev; io.tmin = tmin;
     io.t = t; io.u = u; io.v = v; io.prim = prim; io.tris = S.tris; io.any = any;

@@ -78,3 +78,40 @@ def test_skinned_refit_keeps_trees_in_step(ctx):
     _assert_same(out)
     assert (out[4][0][3] >= 0).mean() > 0.2
     ctx.release(h)
+
+
+def test_wide_node_limit_falls_back_to_binary(ctx):
+    """A tree with more 4-wide nodes than 32-bit byte offsets can address is
+    traversed in its binary form.  The limit is forced small from the host
+    ("max_wide_nodes"): the render is the same image, and the node visits are
+    the binary tree's."""
+    s = scenes.spheres()
+    ref = {}
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    ctx.set_option("count_visits", 1)
+    for w in (2, 4):
+        ctx.set_option("trace_bvh_width", w)
+        ref[w] = (ctx.render(h, s.camera, 160, 90, 2, 8, 4, 1, 1, 5), ctx.last_visits()["closest_nodes"])
+    ctx.set_option("trace_bvh_width", 4)
+    ctx.release(h)
+    ctx.set_option("max_wide_nodes", 1000)
+    try:
+        h = ctx.upload(s)
+        ctx.build_bvh(h)
+        img = ctx.render(h, s.camera, 160, 90, 2, 8, 4, 1, 1, 5)
+        nodes = ctx.last_visits()["closest_nodes"]
+        o, d = O.primary_rays(s.camera, 160, 90, 5)
+        t, u, v, p = ctx.trace(h, o, d)
+        ctx.release(h)
+    finally:
+        ctx.set_option("max_wide_nodes", 16777215)
+        ctx.set_option("count_visits", 0)
+    assert np.array_equal(img, ref[4][0]) and np.array_equal(img, ref[2][0])
+    assert nodes == ref[2][1] and nodes != ref[4][1]
+    osc = O.Scene(s)
+    ob = O.Bvh(osc)
+    t_o, u_o, v_o, p_o = O.trace(osc, o, d, bvh=ob)
+    np.testing.assert_array_equal(p, p_o)
+    with pytest.raises(Exception):
+        ctx.set_option("max_wide_nodes", 16777216)

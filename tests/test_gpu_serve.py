@@ -121,3 +121,21 @@ def test_stop_without_drain_cancels_queued(served):
     used = ctx.lib.srt_hbm_used(ctx.h)
     assert np.array_equal(ctx.render(*reqs[0][:10]), alone[0])
     assert ctx.lib.srt_hbm_used(ctx.h) == used
+
+
+def test_pool_outside_32bit_slots_is_refused(served):
+    """Path slots are 32-bit: a serving pool outside [64, 2^31 - 1] is an
+    error at srt_serve_start (0 takes the context's pool), and the context
+    stays idle and usable."""
+    ctx, reqs, alone = served
+    for bad in (1 << 31, (1 << 31) + 5, 63, -1):
+        with pytest.raises(abi.SrtError):
+            ctx.serve_start(bad)
+        assert ctx.lib.srt_set_option(ctx.h, b"profile", 0) == 0      # not serving
+    ctx.serve_start((1 << 31) - 1)
+    try:
+        t, img = ctx.submit(*reqs[0][:10])
+        assert ctx.wait(t)["status"] == 2
+        assert np.array_equal(img, alone[0])
+    finally:
+        ctx.serve_stop(drain=True)
