@@ -149,6 +149,20 @@ int srt_scene_download_vertices(srt_ctx* ctx, srt_scene scene, float* positions,
 int srt_trace_rays(srt_ctx* ctx, srt_scene scene, int32_t n, const float* origins, const float* dirs, float tmin,
                    const float* tmax, int32_t any, float* t, float* u, float* v, int32_t* prim);
 
+/* ---- ray queries that allocate nothing ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The same queries and results as srt_trace_rays, but
+ * origins, dirs, tmax and t, u, v, prim are DEVICE arrays, and the call makes
+ * no allocation: the traversal's fetch counters and stack spill area live in
+ * a file-scope device array of 32 MiB per GPU (held until the process ends),
+ * looked up once per device by srt_create.  The persistent grid is sized to
+ * fit its stack spill in that array; a tree whose one-workgroup spill exceeds
+ * it returns SRT_E_LIMIT.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (from several contexts or threads) race. */
+int srt_trace_rays_noalloc(srt_ctx* ctx, srt_scene scene, int32_t n, const float* origins, const float* dirs,
+                           float tmin, const float* tmax, int32_t any, float* t, float* u, float* v, int32_t* prim);
+
 /* ---- render (replaces PathTracer::render + the megakernel, path_tracer.cu:13-33) ----
  * Writes float4 RGBA (alpha 1) for width*height pixels.  out_on_device = 1:
  * rgba_out is device memory; 0: host memory; rgba_out == NULL keeps the film in

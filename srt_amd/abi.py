@@ -25,7 +25,7 @@ EXPORTED = [
     "srt_scene_upload", "srt_scene_release", "srt_scene_lights",
     "srt_scene_build_bvh", "srt_scene_bvh_info", "srt_scene_bvh_download",
     "srt_scene_set_skin", "srt_scene_skin_refit", "srt_scene_download_vertices",
-    "srt_trace_rays", "srt_render", "srt_render_simple", "srt_render_light", "srt_film_download", "srt_tonemap", "srt_accumulate",
+    "srt_trace_rays", "srt_trace_rays_noalloc", "srt_render", "srt_render_simple", "srt_render_light", "srt_film_download", "srt_tonemap", "srt_accumulate",
     "srt_denoise",
     "srt_stats", "srt_stats_reset", "srt_last_counts", "srt_last_visits",
     "srt_device_alloc", "srt_device_free", "srt_memcpy",
@@ -108,6 +108,7 @@ def load_library(path: str = LIB_PATH):
         "srt_scene_skin_refit": (C.c_int, [P, C.c_uint64, P, C.c_int32]),
         "srt_scene_download_vertices": (C.c_int, [P, C.c_uint64, P, P, P]),
         "srt_trace_rays": (C.c_int, [P, C.c_uint64, C.c_int32, P, P, C.c_float, P, C.c_int32, P, P, P, P]),
+        "srt_trace_rays_noalloc": (C.c_int, [P, C.c_uint64, C.c_int32, P, P, C.c_float, P, C.c_int32, P, P, P, P]),
         "srt_render": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32, C.c_int32, P,
                                  C.c_int32]),
         "srt_render_light": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32, C.c_int32, P,
@@ -286,6 +287,14 @@ class Context:
         self.check(self.lib.srt_trace_rays(self.h, scene, n, _ptr(o), _ptr(d), C.c_float(tmin), _ptr(tm),
                                            1 if any_hit else 0, _ptr(t), _ptr(u), _ptr(v), _ptr(prim)))
         return t, u, v, prim
+
+    def trace_noalloc(self, scene, n, origins_dev, dirs_dev, tmax_dev, t_dev, u_dev, v_dev, prim_dev, tmin=1e-3,
+                      any_hit=False):
+        """srt_trace_rays_noalloc on device arrays (addresses as ints) of n rays."""
+        self.check(self.lib.srt_trace_rays_noalloc(self.h, scene, n, C.c_void_p(origins_dev), C.c_void_p(dirs_dev),
+                                                   C.c_float(tmin), C.c_void_p(tmax_dev), 1 if any_hit else 0,
+                                                   C.c_void_p(t_dev), C.c_void_p(u_dev), C.c_void_p(v_dev),
+                                                   C.c_void_p(prim_dev)))
 
     # ---- render ----
     @staticmethod

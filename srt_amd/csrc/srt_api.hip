@@ -151,6 +151,12 @@ int srt_create(int device, size_t hbm_budget_bytes, srt_ctx** out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SRT_E_HIP; }
     if (dev_alloc(c, c->counters, kCountersBytes)) { (void)hipStreamDestroy(c->stream); delete c; return SRT_E_OOM; }
     (void)hipMemset(c->counters.p, 0, kCountersBytes);
+    if (int r = noalloc_init(c)) {
+        dev_free(c, c->counters);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return r;
+    }
     *out = c;
     return SRT_OK;
 }
@@ -468,6 +474,17 @@ int srt_trace_rays(srt_ctx* c, srt_scene h, int32_t n, const float* o, const flo
     DevBuf* all[] = {&bo, &bd, &bt, &tt, &tu, &tv, &tp};
     for (DevBuf* b : all) dev_free(c, *b);
     return r;
+}
+
+int srt_trace_rays_noalloc(srt_ctx* c, srt_scene h, int32_t n, const float* o, const float* d, float tmin,
+                           const float* tmax, int32_t any, float* t, float* u, float* v, int32_t* prim) {
+    SRT_IDLE(c);
+    if (!c || n < 0 || !o || !d || !tmax || !prim || (!any && (!t || !u || !v))) return SRT_E_INVALID;
+    SceneRec* s = find_scene(c, h);
+    if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (!s->has_bvh) return set_err(c, SRT_E_STATE, "no BVH built");
+    if (n == 0) return SRT_OK;
+    return trace_rays_noalloc(c, s, n, o, d, tmin, tmax, any, t, u, v, prim);
 }
 
 int srt_render(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_params* p, int32_t W, int32_t H, float* rgba_out,

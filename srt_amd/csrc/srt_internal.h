@@ -342,6 +342,9 @@ int mlp_train_dev(srt_ctx* c, const float* in9, const float* target, int n, floa
 bool mlp_ready(srt_ctx* c);
 int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin, const float* tmax_dev,
                int any, float* t, float* u, float* v, int32_t* prim);
+int noalloc_init(srt_ctx* c);
+int trace_rays_noalloc(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin,
+                       const float* tmax_dev, int any, float* t, float* u, float* v, int32_t* prim);
 int tonemap_dev(srt_ctx* c, float* rgba, int n, int type, float exposure, int use_gamma);
 int accumulate_dev(srt_ctx* c, float* acc, float* rgba, int n, int frame_count);
 int denoise_dev(srt_ctx* c, float* rgba, float* tmp, int W, int H);

@@ -17,6 +17,8 @@
 #include <hipcub/hipcub.hpp>
 #include <cstdio>
 #include <cmath>
+#include <mutex>
+#include <algorithm>
 
 namespace srt {
 
@@ -1906,6 +1908,70 @@ int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* 
     k_clear_parts<<<1, kParts, 0, c->stream>>>(fetch);
     k_trace_rays<<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch, (int32_t*)c->gstack.p,
                                                                          cfg);
+    if (int r = hip_check(c, hipGetLastError(), "trace launch")) return r;
+    return hip_check(c, hipStreamSynchronize(c->stream), "trace");
+}
+
+// ---------------------------------------------------------------------------
+// Ray queries that allocate nothing (srt_trace_rays_noalloc): the traversal's
+// fetch counters and stack spill area live in `scratch`, a file-scope device
+// array of kNoallocScratchBytes on every device the library runs on, instead
+// of context allocations.  Host code never uses the array's name as an
+// address: noalloc_init looks its device address up once per device
+// (hipGetSymbolAddress, with that device current) and every launch passes
+// that address.  The array keeps the last call's contents; the counters are
+// cleared by a fill kernel before each traversal, and calls that overlap on
+// one device race on it.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoallocScratchBytes = 32ull << 20;               // 32 MiB per device
+constexpr size_t kNoallocCounterBytes = (size_t)kParts * kCntLine * 4;   // one fetch counter per piece, 8 KiB
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+constexpr int kNoallocMaxDevices = 64;
+static void* g_scratch_ptr[kNoallocMaxDevices];
+static std::mutex g_scratch_mu;
+
+int noalloc_init(srt_ctx* c) {
+    if (c->device < 0 || c->device >= kNoallocMaxDevices)
+        return set_err(c, SRT_E_LIMIT, "device %d beyond the %d devices of the scratch table", c->device,
+                       kNoallocMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[c->device]) return SRT_OK;
+    void* ptr = nullptr;
+    if (int r = hip_check(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "scratch symbol address")) return r;
+    if (!ptr) return set_err(c, SRT_E_HIP, "scratch symbol has no device address");
+    g_scratch_ptr[c->device] = ptr;
+    return SRT_OK;
+}
+
+// Closest or any hit of n rays given in device memory, into device memory;
+// the same kernels and results as trace_rays.  The persistent grid is sized
+// so that its stack spill fits the scratch array after the counters.
+int trace_rays_noalloc(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin,
+                       const float* tmax_dev, int any, float* t, float* u, float* v, int32_t* prim) {
+    void* base;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = (c->device >= 0 && c->device < kNoallocMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+    }
+    if (!base) return set_err(c, SRT_E_STATE, "scratch not set up on device %d", c->device);
+    DevScene S = dev_scene(s);
+    const TraceCfg cfg = trace_cfg(c, s);
+    const int64_t deepest = use_wide(c, s) ? 3 * (int64_t)s->depth4 + 1 : (int64_t)s->max_depth + 1;
+    const int64_t per_thread = std::max<int64_t>(1, deepest - cfg.lds_depth);     // int32 spill entries
+    const int64_t spill_entries = (int64_t)((kNoallocScratchBytes - kNoallocCounterBytes) / sizeof(int32_t));
+    const int64_t max_blocks = spill_entries / (per_thread * kTraceBlock);
+    if (max_blocks < 1)
+        return set_err(c, SRT_E_LIMIT, "a traversal stack of %lld entries per lane exceeds the %zu-byte scratch",
+                       (long long)deepest, kNoallocScratchBytes);
+    int grid = trace_grid(c, n);
+    if (grid > max_blocks) grid = (int)max_blocks;
+    RaysIO io;
+    io.o3 = o_dev; io.d3 = d_dev; io.tmax = tmax_dev; io.tmin = tmin;
+    io.t = t; io.u = u; io.v = v; io.prim = prim; io.tris = S.tris; io.any = any;
+    uint32_t* fetch = (uint32_t*)base;
+    int32_t* gstack = (int32_t*)((unsigned char*)base + kNoallocCounterBytes);
+    k_clear_parts<<<1, kParts, 0, c->stream>>>(fetch);
+    k_trace_rays<<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch, gstack, cfg);
     if (int r = hip_check(c, hipGetLastError(), "trace launch")) return r;
     return hip_check(c, hipStreamSynchronize(c->stream), "trace");
 }
