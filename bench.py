@@ -578,26 +578,30 @@ def mlp_mode(args):
     print(json.dumps(out), flush=True)
 
 
-def serve_mode(args):
-    """Continuous-batching serving: Poisson arrivals over 8 resident scenes
-    (3 Cornell variants, 3 1M-triangle sphere scenes with different generator
-    seeds, 2 posed skinned tubes); resolution uniform from {256^2, 512^2,
-    1280x720, 1920x1080}, spp uniform from {4, 16, 64}; the radiance MLP
-    (trained on the first Cornell scene) on with probability 1/2 for that
-    scene's requests.  Run 1: the offered rate (default 200/s) for a window,
-    then queued requests are cancelled; reports served Mpaths/s.  Run 2: a
-    rate at about 70 % of the measured capacity; reports p50/p99 latency."""
+def serve_load(rate_sat=200.0, window_sat=30.0, window_sub=30.0, seed=20261202, path_pool=0):
+    """Continuous-batching serving under Poisson arrivals over 8 resident
+    scenes (3 Cornell variants, 3 1M-triangle sphere scenes with different
+    generator seeds, 2 posed skinned tubes); resolution uniform from {256^2,
+    512^2, 1280x720, 1920x1080}, spp uniform from {4, 16, 64}; every scene has
+    its own radiance MLP, trained online from its own tracer samples, and the
+    MLP form is on for each request with probability 1/2.  Run 1: `rate_sat`
+    requests/s (200: the offered load of the serving configuration) for
+    `window_sat` s, then queued requests are cancelled; reports served
+    Mpaths/s.  Run 2: Poisson arrivals at about 70 % of the measured capacity
+    for `window_sub` s, drained; reports p50/p99 request latency (submit to
+    image in host memory)."""
     from srt_amd import abi, scenes
-    rng = np.random.default_rng(args.serve_seed)
+    rng = np.random.default_rng(seed)
     ctx = abi.Context(0)
     sc = []
     for fov in (40.0, 35.0, 45.0):
         sc.append(scenes.cornell(fov=fov))
-    for seed in (20261202, 20261203, 20261204):
-        sc.append(scenes.spheres(seed=seed))
+    for sd in (20261202, 20261203, 20261204):
+        sc.append(scenes.spheres(seed=sd))
     sc.append(scenes.tube())
     sc.append(scenes.tube(seed=12))
     handles = []
+    t0 = time.perf_counter()
     for k, s in enumerate(sc):
         h = ctx.upload(s)
         ctx.build_bvh(h)
@@ -605,10 +609,18 @@ def serve_mode(args):
             ctx.set_skin(h, s.skin["bone_ids"], s.skin["bone_weights"])
             ctx.skin_refit(h, s.skin["matrices"](0 if k == 6 else 30))
         handles.append(h)
-    ctx.mlp_create(seed=1, lr=1e-3)
-    s0 = sc[0]
-    for f in range(16):
-        ctx.mlp_train_frame(handles[0], s0.camera, 256, 256, 8, 5, 4, seed=7000 + f, max_samples=1 << 18, steps=16)
+    t_scenes = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k, (s, h) in enumerate(zip(sc, handles)):
+        ctx.mlp_select(h)
+        ctx.mlp_create(seed=1 + k, lr=1e-3)
+        R = s.render
+        for f in range(8):
+            ctx.mlp_train_frame(h, s.camera, 256, 256, 8, max(3, R["max_depth"]), R["rr_depth"], seed=7000 + f,
+                                max_samples=1 << 18, steps=16)
+    ctx.mlp_select(0)
+    t_mlp = time.perf_counter() - t0
+    log(f"serving: 8 scenes resident in {t_scenes:.2f} s, 8 MLPs trained in {t_mlp:.2f} s")
     res_choices = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
     spp_choices = [4, 16, 64]
 
@@ -616,11 +628,11 @@ def serve_mode(args):
         k = int(rng.integers(len(sc)))
         W, H = res_choices[int(rng.integers(4))]
         spp = spp_choices[int(rng.integers(3))]
-        m = 1 if (k == 0 and rng.random() < 0.5) else 0
+        m = 1 if rng.random() < 0.5 else 0
         return k, W, H, spp, m
 
     def run(rate, window, keep_images, drain):
-        ctx.serve_start(0)
+        ctx.serve_start(path_pool)
         reqs = []
         t0 = time.perf_counter()
         t_next = t0
@@ -646,106 +658,38 @@ def serve_mode(args):
         ctx.forget_tickets()
         lat = [r["done_ms"] - r["submit_ms"] for r in results if r["status"] == 2]
         return dict(requests=len(reqs), completed=len(lat), offered_Mpaths_s=round(offered / window / 1e6, 1),
-                    served_Mpaths_s=round(st_window["served_paths"] / t_window / 1e6, 1),
+                    served_Mpaths_s=round(st_window["served_paths"] / t_window / 1e6, 1), window_s=window,
                     p50_ms=round(float(np.percentile(lat, 50)), 1) if lat else None,
                     p99_ms=round(float(np.percentile(lat, 99)), 1) if lat else None)
 
-    sat = run(args.serve_rate, args.serve_window, False, False)
-    log(f"saturation run: {sat}")
+    sat = run(rate_sat, window_sat, False, False)
+    sat["rate_per_s"] = rate_sat
+    log(f"serving at {rate_sat}/s: {sat}")
     mean_paths = np.mean([W * H for W, H in res_choices]) * np.mean(spp_choices)
     sub_rate = max(1.0, 0.7 * sat["served_Mpaths_s"] * 1e6 / mean_paths)
-    sub = run(sub_rate, args.serve_window, True, True)
+    sub = run(sub_rate, window_sub, True, True)
     sub["rate_per_s"] = round(sub_rate, 1)
-    out = {"metric": "served Mpaths/s at saturation; p50/p99 request latency below saturation",
-           "value": sat["served_Mpaths_s"], "unit": "Mpaths/s", "n_gpus": 1, "higher_is_better": True,
-           "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "serving: Poisson arrivals over 8 resident scenes (3 Cornell variants, 3 1M-triangle "
-                                  "sphere scenes, 2 posed 200k-triangle tubes); res U{256^2,512^2,1280x720,1920x1080}, "
-                                  "spp U{4,16,64}, MLP on p=1/2 for the MLP-trained scene",
-                      "rate_per_s": args.serve_rate, "window_s": args.serve_window},
-           "saturation": sat, "below_saturation": sub,
-           "latency_ms": {"p50": sub["p50_ms"], "p99": sub["p99_ms"], "unit": "ms per request"}}
+    log(f"serving at {sub['rate_per_s']}/s (70 % of capacity): {sub}")
     ctx.close()
-    print(json.dumps(out), flush=True)
+    return {"workload": "Poisson arrivals over 8 resident scenes (3 Cornell variants, 3 1M-triangle sphere scenes, "
+                        "2 posed 200k-triangle tubes); res U{256^2,512^2,1280x720,1920x1080}, spp U{4,16,64}; "
+                        "one radiance MLP per scene, MLP form on p=1/2",
+            "saturation": sat, "below_saturation": sub,
+            "latency_ms": {"p50": sub["p50_ms"], "p99": sub["p99_ms"], "unit": "ms per request, submit to host image",
+                           "at_rate_per_s": sub["rate_per_s"]}}
 
 
-def frame_roofline(kern, st, steps, bytes_step, segments):
-    """Roofline of the dominant frame kernel against the resource that binds it.
-
-    The traversal kernels are bound by vector-ALU issue, not by HBM: their
-    node and triangle fetches mostly hit in L2/MALL.  So `achieved` is the
-    kernel's active VALU lane-operations per second (wave64 VALU instructions
-    per launch x 64 x active-lane fraction, from the committed SQ counter
-    summary, over the live HIP-event launch time) against the peak of 256 CUs x
-    4 SIMDs x 16 lanes per clock at 2.4 GHz; `frac` = achieved / peak.
-    `traffic` is the kernel's calibrated HBM bytes per launch (FETCH_SIZE +
-    WRITE_SIZE counters), and `hbm` gives, for each frame kernel, those bytes
-    over its live launch time against the 8 TB/s HBM peak.  `algorithmic`
-    keeps the byte model (ray and hit records, 112 B per 4-wide node visit,
-    36 B per triangle test): it counts cache hits, so its rate is not an HBM
-    figure."""
-    avg_ms = {k: st[k]["ms"] / st[k]["launches"] for k in bytes_step if k in st and st[k]["launches"]}
-    launches = {k: st[k]["launches"] / steps for k in avg_ms}
-    hbm = {}
-    for k in avg_ms:
-        t = measured_traffic(k)
-        if t:
-            gbs = t / (avg_ms[k] * 1e-3) / 1e9
-            hbm[k] = {"bytes_per_launch": t, "avg_launch_ms": round(avg_ms[k], 4), "GBs": round(gbs, 1),
-                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    alg = {k: {"bytes_per_launch": int(bytes_step[k] / launches[k]),
-               "GBs": round(bytes_step[k] / launches[k] / (avg_ms[k] * 1e-3) / 1e9, 1)} for k in avg_ms}
-    r = {"kernel": kern, "avg_launch_ms": round(avg_ms[kern], 4), "traffic": hbm.get(kern, {}).get("bytes_per_launch"),
-         "hbm": hbm, "algorithmic": alg}
-    vc = measured_valu_counts(kern)
-    if vc:
-        insts, lanes, busy, source = vc
-        lane_ops = insts * 64.0 * lanes
-        achieved = lane_ops / (avg_ms[kern] * 1e-3) / 1e12
-        r.update({"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TLANE_OPS,
-                  "unit": "Tlane-op/s", "frac": round(achieved / VALU_PEAK_TLANE_OPS, 4),
-                  "valu_issue": {"wave_instructions_per_launch": insts, "active_lanes_frac": round(lanes, 3),
-                                 "simd_busy_frac": round(busy, 3),
-                                 "lane_ops_per_segment": round(lane_ops * launches[kern] / segments, 1),
-                                 "source": source}})
-    elif kern in hbm:
-        r.update({"bound": "hbm", "achieved": hbm[kern]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": hbm[kern]["frac"]})
-    else:
-        r.update({"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-                  "note": "no counter summaries under profiles/"})
-    return r
-
-
-def host_cpu():
-    """Usable host cores and their description: the scheduler affinity set,
-    capped by a cgroup CPU quota when one is set (cgroup v2 cpu.max or v1
-    cfs_quota_us), plus os.cpu_count() and the /proc/cpuinfo model name."""
-    affinity = len(os.sched_getaffinity(0))
-    quota = None
-    try:
-        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
-        if q != "max":
-            quota = float(q) / float(per)
-    except (OSError, ValueError):
-        try:
-            q = int(open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us").read())
-            per = int(open("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read())
-            if q > 0:
-                quota = q / per
-        except (OSError, ValueError):
-            pass
-    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
-    model = None
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return {"usable": usable, "affinity": affinity, "cgroup_quota": quota, "cpu_count": os.cpu_count(),
-            "model": model}
+def serve_mode(args):
+    """The serving workload alone, with long windows (default 30 s each)."""
+    out = serve_load(args.serve_rate, args.serve_window, args.serve_window, args.serve_seed)
+    line = {"metric": "served Mpaths/s at saturation; p50/p99 request latency below saturation",
+            "value": out["saturation"]["served_Mpaths_s"], "unit": "Mpaths/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "serving: " + out["workload"], "rate_per_s": args.serve_rate,
+                       "window_s": args.serve_window},
+            "saturation": out["saturation"], "below_saturation": out["below_saturation"],
+            "latency_ms": out["latency_ms"]}
+    print(json.dumps(line), flush=True)
 
 
 def measured_traffic(kernel):
@@ -775,6 +719,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-window", type=float, default=30.0)
     ap.add_argument("--serve-seed", type=int, default=20261202)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default line")
+    ap.add_argument("--serve-leg-window", type=float, default=5.0,
+                    help="seconds of each Poisson window of the default line's serving leg")
     ap.add_argument("--pmc-summary", nargs=4, metavar="DIR")
     ap.add_argument("--kstats-summary", metavar="DIR")
     ap.add_argument("--valu-summary", metavar="DIR")
@@ -897,6 +844,11 @@ def main():
         cpu["gpu_vs_cpu"] = round(value / cpu["value"], 1)
         log(f"CPU {cpu['value']} Mpaths/s, GPU/CPU {value / cpu['value']:.1f}x, max rel diff {rel.max():.2e}")
 
+    ctx.close()
+    serving = None
+    if not args.no_serve:
+        serving = serve_load(args.serve_rate, args.serve_leg_window, args.serve_leg_window, args.serve_seed)
+
     lat_sorted = sorted(lat)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
@@ -906,15 +858,18 @@ def main():
                                "Disney, 1920x1080, 64 spp, max depth 8, NEE+MIS",
                    "width": W, "height": H, "spp": spp, "max_depth": depth, "triangles": s.num_triangles,
                    "paths_per_step": paths, "parallelism": "single-gpu"},
-        "latency_ms": {"p50": round(float(np.percentile(lat_sorted, 50)), 2),
-                       "p99": round(float(np.percentile(lat_sorted, 99)), 2), "unit": "ms per frame request"},
+        "latency_ms": (serving["latency_ms"] if serving else
+                       {"p50": round(float(np.percentile(lat_sorted, 50)), 2),
+                        "p99": round(float(np.percentile(lat_sorted, 99)), 2), "unit": "ms per frame request"}),
+        "frame_latency_ms": {"p50": round(float(np.percentile(lat_sorted, 50)), 2),
+                             "p99": round(float(np.percentile(lat_sorted, 99)), 2)},
         "segments_per_path": round(segments / paths, 3), "shadow_rays_per_path": round(shadow_rays / paths, 3),
         "bvh_build_ms": round(bvh_ms, 1),
         "pcie_inclusive": pcie,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 

@@ -217,6 +217,14 @@ int srt_denoise(srt_ctx* ctx, float* rgba, int32_t width, int32_t height, int32_
  * Parameters in logical order: W0[64][63] b0[64] W1..W3[64][64]+b W4[3][64] b4[3]. */
 int srt_mlp_num_params(void);
 int srt_mlp_create(srt_ctx* ctx, uint32_t seed, float lr);
+/* Selects the MLP that the other srt_mlp_* calls act on: the one bound to
+ * `scene`, or with scene 0 the context's own (the default).  srt_mlp_create
+ * creates (or replaces) the selected one.  Renders with mlp_mode 1, training
+ * frames and serving requests of a scene use the scene's own MLP when it has
+ * one, else the context's own.  Releasing a scene frees its MLP.  (The
+ * reference's NeuralRadiosity pass holds one network per pass,
+ * neural_radiosity.cu:76-82.) */
+int srt_mlp_select(srt_ctx* ctx, srt_scene scene);
 int srt_mlp_set_params(srt_ctx* ctx, const float* params, int32_t n);
 int srt_mlp_get_params(srt_ctx* ctx, float* params, int32_t n);
 int srt_mlp_set_hparams(srt_ctx* ctx, float lr, float beta1, float beta2, float eps);
@@ -254,9 +262,13 @@ int srt_memcpy(srt_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t k
  * srt_serve_start starts a dispatcher thread that owns the context's stream and
  * path pool until srt_serve_stop.  Requests are admitted first-come first-served
  * while their device film (28 bytes per pixel) fits in the HBM budget (a request
- * that can never fit ends with status SRT_E_OOM), and the requests admitted on
- * one scene fill each chunk of `path_pool` paths in arrival order (0: the
- * "path_pool" option).
+ * that can never fit ends with status SRT_E_OOM).  Each chunk of at most
+ * `path_pool` paths (0: the smaller of the "path_pool" option and 2^24) runs
+ * the scene of the admitted request with the fewest paths left, filled with
+ * that scene's admitted requests, fewest paths left first; a larger request
+ * is cut into chunks and yields the device to newer small ones between them.
+ * Requests with mlp_mode 1 query their scene's MLP (srt_mlp_select), else the
+ * context's own.
  * A request's image is bit-identical to srt_render with the same arguments.
  * While serving, only srt_submit, srt_poll, srt_wait, srt_serve_stats,
  * srt_serve_stop, srt_last_error and srt_hbm_used may be called; every other

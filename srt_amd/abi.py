@@ -29,7 +29,7 @@ EXPORTED = [
     "srt_denoise",
     "srt_stats", "srt_stats_reset", "srt_last_counts", "srt_last_visits",
     "srt_device_alloc", "srt_device_free", "srt_memcpy",
-    "srt_mlp_num_params", "srt_mlp_create", "srt_mlp_set_params", "srt_mlp_get_params", "srt_mlp_set_hparams",
+    "srt_mlp_num_params", "srt_mlp_create", "srt_mlp_select", "srt_mlp_set_params", "srt_mlp_get_params", "srt_mlp_set_hparams",
     "srt_mlp_set_bounds", "srt_mlp_infer", "srt_mlp_train_step", "srt_mlp_save", "srt_mlp_load",
     "srt_mlp_train_frame",
     "srt_serve_start", "srt_serve_stop", "srt_submit", "srt_poll", "srt_wait", "srt_serve_stats",
@@ -128,6 +128,7 @@ def load_library(path: str = LIB_PATH):
         "srt_memcpy": (C.c_int, [P, P, P, C.c_size_t, C.c_int32]),
         "srt_mlp_num_params": (C.c_int, []),
         "srt_mlp_create": (C.c_int, [P, C.c_uint32, C.c_float]),
+        "srt_mlp_select": (C.c_int, [P, C.c_uint64]),
         "srt_mlp_set_params": (C.c_int, [P, P, C.c_int32]),
         "srt_mlp_get_params": (C.c_int, [P, P, C.c_int32]),
         "srt_mlp_set_hparams": (C.c_int, [P, C.c_float, C.c_float, C.c_float, C.c_float]),
@@ -398,6 +399,10 @@ class Context:
     # ---- radiance MLP ----
     def mlp_create(self, seed=0, lr=1e-3):
         self.check(self.lib.srt_mlp_create(self.h, seed, C.c_float(lr)))
+
+    def mlp_select(self, scene=0):
+        """srt_mlp_* calls act on scene's own MLP from now on (0: the context's own)."""
+        self.check(self.lib.srt_mlp_select(self.h, scene))
 
     def mlp_num_params(self):
         return int(self.lib.srt_mlp_num_params())

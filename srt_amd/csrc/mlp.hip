@@ -603,13 +603,36 @@ struct MlpState {
 
 static MlpState* mlp_of(srt_ctx* c) { return (MlpState*)c->mlp; }
 
-static void mlp_free_all(srt_ctx* c) {
-    MlpState* m = mlp_of(c);
+static void mlp_free_one(srt_ctx* c, MlpState* m) {
     if (!m) return;
     DevBuf* all[] = {&m->W, &m->B, &m->mW, &m->vW, &m->mB, &m->vB, &m->wf, &m->wt, &m->bias_dev, &m->wpart, &m->bpart, &m->loss_part, &m->loss, &m->grad, &m->in_buf, &m->tgt_buf, &m->out_buf};
     for (DevBuf* b : all) dev_free(c, *b);
     delete m;
+}
+
+static void mlp_free_all(srt_ctx* c) {
+    for (auto& kv : c->mlps) mlp_free_one(c, (MlpState*)kv.second);
+    c->mlps.clear();
     c->mlp = nullptr;
+}
+
+void* mlp_for_scene(srt_ctx* c, uint64_t h) {
+    auto it = c->mlps.find(h);
+    if (it == c->mlps.end()) it = c->mlps.find(0);
+    return it == c->mlps.end() ? nullptr : it->second;
+}
+
+void mlp_release_scene(srt_ctx* c, uint64_t h) {
+    auto it = c->mlps.find(h);
+    if (h == 0 || it == c->mlps.end()) return;
+    if (c->mlp == it->second) c->mlp = nullptr;
+    mlp_free_one(c, (MlpState*)it->second);
+    c->mlps.erase(it);
+    if (c->mlp_sel == h) {
+        c->mlp_sel = 0;
+        auto d = c->mlps.find(0);
+        c->mlp = d == c->mlps.end() ? nullptr : d->second;
+    }
 }
 
 static MlpDev mlp_dev(MlpState* m) {
@@ -794,8 +817,15 @@ int srt_mlp_create(srt_ctx* c, uint32_t seed, float lr) {
     SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
     if (!(lr > 0.0f)) return set_err(c, SRT_E_INVALID, "learning rate must be positive");
-    if (c->mlp) mlp_free_all(c);
+    {
+        auto it = c->mlps.find(c->mlp_sel);
+        if (it != c->mlps.end()) {
+            mlp_free_one(c, (MlpState*)it->second);
+            c->mlps.erase(it);
+        }
+    }
     MlpState* m = new MlpState();
+    c->mlps[c->mlp_sel] = m;
     c->mlp = m;
     c->mlp_free = mlp_free_all;
     m->lr = lr;
@@ -806,7 +836,12 @@ int srt_mlp_create(srt_ctx* c, uint32_t seed, float lr) {
     for (DevBuf* b : f32b) if (!r) r = dev_alloc(c, *b, (size_t)B_TOTAL * 4);
     if (!r) r = dev_alloc(c, m->wf, (size_t)W_TOTAL * 2);
     if (!r) r = dev_alloc(c, m->wt, (size_t)W_TOTAL * 2);
-    if (r) { mlp_free_all(c); return r; }
+    if (r) {
+        mlp_free_one(c, m);
+        c->mlps.erase(c->mlp_sel);
+        c->mlp = nullptr;
+        return r;
+    }
     // He-uniform init from a PCG32 stream; padded entries stay zero
     std::vector<float> W(W_TOTAL, 0.0f), B(B_TOTAL, 0.0f), z(W_TOTAL, 0.0f);
     uint64_t state = 0, inc = ((uint64_t)seed << 1u) | 1u;
@@ -836,6 +871,19 @@ int srt_mlp_create(srt_ctx* c, uint32_t seed, float lr) {
     if (!r) r = refresh_copies(c, m);
     if (!r) r = hip_check(c, hipStreamSynchronize(c->stream), "mlp create");
     return r;
+}
+
+// Selects the MLP the other srt_mlp_* calls act on: scene h's own (0: the
+// context's own).  Renders with mlp_mode 1 and training frames of a scene use
+// its own MLP when it has one, else the context's own.
+int srt_mlp_select(srt_ctx* c, srt_scene h) {
+    SRT_IDLE(c);
+    if (!c) return SRT_E_INVALID;
+    if (h != 0 && !c->scenes.count(h)) return set_err(c, SRT_E_NOT_FOUND, "unknown scene %llu", (unsigned long long)h);
+    c->mlp_sel = h;
+    auto it = c->mlps.find(h);
+    c->mlp = it == c->mlps.end() ? nullptr : it->second;
+    return SRT_OK;
 }
 
 int srt_mlp_set_params(srt_ctx* c, const float* params, int32_t n) {

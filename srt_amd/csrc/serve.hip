@@ -7,11 +7,12 @@
 //   1. admission: queued requests (FIFO) are admitted while their device film
 //      (12 B accumulator + 16 B output per pixel) fits in the context's HBM
 //      budget; a request that can never fit fails with SRT_E_OOM;
-//   2. batching: each chunk takes the scene of the oldest admitted request and
-//      fills the path pool with the remaining paths of the admitted requests on
-//      that scene in arrival order (first come, first served; small requests
-//      share a chunk), one segment per request; requests admitted while a
-//      chunk runs join the next chunk;
+//   2. batching: each chunk takes the scene of the admitted request with the
+//      fewest paths left and fills the chunk (2^24 paths by default) with the
+//      remaining paths of the admitted requests on that scene, fewest left
+//      first (small requests share a chunk; a large one is cut into chunks and
+//      yields the device between them), one segment per request; requests
+//      admitted while a chunk runs join the next chunk;
 //   3. completion: a request whose last paths were in the chunk has its film
 //      copied to the caller's buffer and its device memory released.
 // A request's paths run in increasing path-id order with its own PCG streams,
@@ -36,6 +37,10 @@ struct Request {
     int64_t chunks = 0;
     bool inflight = false;            // part of the chunk the dispatcher is running
 };
+
+// default paths per serving chunk: about 9 ms of GPU time at 1.9 Gpaths/s,
+// the longest a newly admitted request waits for the device
+constexpr int64_t kServeChunk = 1ll << 24;
 
 struct ServeState {
     std::thread th;
@@ -113,16 +118,27 @@ static void dispatcher(srt_ctx* c) {
                 S->cv.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
             }
-            // chunk: the oldest request's scene; the pool is filled with the
-            // remaining paths of that scene's requests in arrival order
-            uint64_t sh = S->active.front()->req.scene;
+            // chunk: the scene of the admitted request with the fewest paths
+            // left (shortest remaining work first); the chunk takes that
+            // scene's requests with the fewest paths left first, up to
+            // S->pool paths, so a large request yields the device to newer
+            // small ones between its chunks (ties: arrival order)
+            Request* first = nullptr;
+            for (Request* q : S->active)
+                if (!first || q->total - q->next < first->total - first->next) first = q;
+            uint64_t sh = first->req.scene;
             auto it = c->scenes.find(sh);
             scene = it == c->scenes.end() ? nullptr : it->second;
+            std::vector<Request*> cand;
+            for (Request* q : S->active)
+                if (q->req.scene == sh) cand.push_back(q);
+            std::stable_sort(cand.begin(), cand.end(), [](const Request* a, const Request* b) {
+                return a->total - a->next < b->total - b->next;
+            });
             int64_t left = S->pool;
             take.clear();
-            for (Request* q : S->active) {
+            for (Request* q : cand) {
                 if (left == 0 || (int)batch.size() == kMaxSegments) break;
-                if (q->req.scene != sh) continue;
                 q->inflight = true;
                 batch.push_back(q);
                 take.push_back(std::min(q->total - q->next, left));
@@ -235,7 +251,7 @@ int srt_serve_start(srt_ctx* c, int64_t path_pool) {
         return set_err(c, SRT_E_INVALID, "path_pool must be 0 (the context's) or in [%lld, %lld]",
                        (long long)kMinPathPool, (long long)kMaxPathPool);
     ServeState* S = new ServeState();
-    S->pool = path_pool > 0 ? path_pool : c->pool_request;
+    S->pool = path_pool > 0 ? path_pool : std::min<int64_t>(c->pool_request, kServeChunk);
     S->t0 = std::chrono::steady_clock::now();
     c->serve = S;
     c->serve_free = serve_free;
@@ -282,7 +298,7 @@ int srt_submit(srt_ctx* c, const srt_request* r, uint64_t* ticket) {
         p.max_depth < 0 || p.rr_depth < 0 || p.mlp_mode < 0 || p.mlp_mode > 1 ||
         !(r->camera.fov_deg > 0.0f && r->camera.fov_deg < 180.0f))
         return SRT_E_INVALID;
-    if (p.mlp_mode == 1 && !mlp_ready(c)) return SRT_E_STATE;
+    if (p.mlp_mode == 1 && !mlp_for_scene(c, r->scene)) return SRT_E_STATE;
     std::lock_guard<std::mutex> lk(S->mu);
     auto it = c->scenes.find(r->scene);
     if (it == c->scenes.end() || !it->second->has_bvh) return SRT_E_NOT_FOUND;

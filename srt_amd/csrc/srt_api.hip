@@ -327,6 +327,7 @@ int srt_scene_upload(srt_ctx* c, const srt_scene_desc* d, srt_scene* out) {
     if (!r && s->nlights > 0) r = upload(c, s->light_mesh, light_mesh.data(), 4 * light_mesh.size());
     if (r) { free_scene(c, s); return r; }
     uint64_t h = c->next_scene++;
+    s->handle = h;
     c->scenes[h] = s;
     *out = h;
     return SRT_OK;
@@ -338,6 +339,7 @@ int srt_scene_release(srt_ctx* c, srt_scene h) {
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene %llu", (unsigned long long)h);
     (void)hipStreamSynchronize(c->stream);
+    mlp_release_scene(c, h);
     free_scene(c, s);
     c->scenes.erase(h);
     return SRT_OK;
@@ -500,7 +502,7 @@ int srt_render(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_para
     if ((int64_t)W * H * p->samples_per_pixel > (1ll << 40)) return set_err(c, SRT_E_LIMIT, "too many paths");
     if (!(cam->fov_deg > 0.0f && cam->fov_deg < 180.0f)) return set_err(c, SRT_E_INVALID, "fov must be in (0, 180)");
     if (p->mlp_mode < 0 || p->mlp_mode > 1) return set_err(c, SRT_E_INVALID, "mlp_mode must be 0 or 1");
-    if (p->mlp_mode == 1 && !mlp_ready(c)) return set_err(c, SRT_E_STATE, "mlp_mode 1 needs srt_mlp_create");
+    if (p->mlp_mode == 1 && !mlp_for_scene(c, h)) return set_err(c, SRT_E_STATE, "mlp_mode 1 needs srt_mlp_create");
     if (!s->has_bvh) {
         if (int r = bvh_build(c, s, 4, 0)) return r;
     }
@@ -598,9 +600,10 @@ int srt_mlp_train_frame(srt_ctx* c, srt_scene h, const srt_camera* cam, const sr
                         int64_t max_samples, int32_t steps, float* loss_out, int64_t* samples_out) {
     SRT_IDLE(c);
     if (!c || !cam || !p || W <= 0 || H <= 0 || max_samples <= 0 || steps <= 0) return SRT_E_INVALID;
-    if (!mlp_ready(c)) return set_err(c, SRT_E_STATE, "no MLP: call srt_mlp_create first");
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (!mlp_for_scene(c, h)) return set_err(c, SRT_E_STATE, "no MLP: call srt_mlp_create first");
+    const MlpBind bind(c, h);
     if (p->samples_per_pixel <= 0 || p->max_depth < 3) return set_err(c, SRT_E_INVALID, "training paths need max_depth >= 3");
     if (!s->has_bvh) if (int r = bvh_build(c, s, 4, 0)) return r;
     if (c->train_cap < max_samples) {

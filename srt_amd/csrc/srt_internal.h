@@ -227,6 +227,7 @@ struct DevBuf {
 };
 
 struct SceneRec {
+    uint64_t handle = 0;
     int32_t nv = 0, nf = 0, nmesh = 0, nmat = 0, nlights = 0;
     bool has_nrm = false, has_tan = false;
     float env[3] = {0, 0, 0};
@@ -292,7 +293,9 @@ struct srt_ctx {
     srt::DevBuf segbuf;                   // device copy of the chunk's segment table
     int64_t mlp_pool_cap = 0, train_cap = 0;
     void* serve = nullptr;                // serving loop state; non-null while a dispatcher runs
-    void* mlp = nullptr;                  // radiance MLP state
+    void* mlp = nullptr;                  // the radiance MLP in use (srt_mlp_* calls; the integrator binds a scene's)
+    std::map<uint64_t, void*> mlps;       // radiance MLPs by scene handle; 0: the context's own
+    uint64_t mlp_sel = 0;                 // the MLP srt_mlp_* calls act on (srt_mlp_select)
     void (*serve_free)(srt_ctx*) = nullptr;
     void (*mlp_free)(srt_ctx*) = nullptr;
 };
@@ -340,6 +343,16 @@ int mlp_infer_dev(srt_ctx* c, const float* in9, int n, float* out3);
 int mlp_query_dev(srt_ctx* c, const float* in9, const uint32_t* count_dev, int max_n, float* out3);
 int mlp_train_dev(srt_ctx* c, const float* in9, const float* target, int n, float* loss_host);
 bool mlp_ready(srt_ctx* c);
+// the MLP that renders of scene h query: h's own, else the context's own (or null)
+void* mlp_for_scene(srt_ctx* c, uint64_t h);
+void mlp_release_scene(srt_ctx* c, uint64_t h);
+// makes scene h's MLP the one in use for a render or a training frame
+struct MlpBind {
+    srt_ctx* c;
+    void* prev;
+    MlpBind(srt_ctx* ctx, uint64_t h) : c(ctx), prev(ctx->mlp) { c->mlp = mlp_for_scene(c, h); }
+    ~MlpBind() { c->mlp = prev; }
+};
 int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin, const float* tmax_dev,
                int any, float* t, float* u, float* v, int32_t* prim);
 int noalloc_init(srt_ctx* c);

@@ -139,3 +139,70 @@ def test_pool_outside_32bit_slots_is_refused(served):
         assert np.array_equal(img, alone[0])
     finally:
         ctx.serve_stop(drain=True)
+
+
+def test_request_arriving_mid_chunk_joins_the_next_chunk(served):
+    """A large request is cut into chunks; a small request submitted while it
+    runs is admitted between two of its chunks, runs in one chunk ahead of the
+    large one's remaining paths (fewest paths left first), finishes first, and
+    both images equal their solo renders."""
+    ctx, reqs, alone = served
+    hs = reqs[2][0]                                  # the 1M-triangle scene
+    cam = reqs[2][1]
+    big = (hs, cam, 1024, 1024, 64, 4, 4, 1, 1, 21)   # 67.1 M paths
+    small = reqs[0]
+    big_alone = ctx.render(*big)
+    pool = 1 << 22
+    ctx.serve_start(pool)
+    try:
+        tb, img_b = ctx.submit(*big)
+        # wait until the large request has run at least one chunk and is still running
+        import time
+        t_end = time.time() + 30
+        while ctx.poll(tb)["chunks"] < 1 and time.time() < t_end:
+            time.sleep(0.0002)
+        assert ctx.poll(tb)["status"] == 1
+        ts, img_s = ctx.submit(*small[:10])
+        rs = ctx.wait(ts)
+        rb = ctx.wait(tb)
+    finally:
+        ctx.serve_stop()
+    assert rs["status"] == 2 and rb["status"] == 2
+    assert rs["chunks"] == 1
+    assert rb["chunks"] == (64 * 1024 * 1024 + pool - 1) // pool
+    assert rs["done_ms"] < rb["done_ms"]
+    assert rs["admit_ms"] > rb["admit_ms"]
+    assert np.array_equal(img_s, alone[0])
+    assert np.array_equal(img_b, big_alone)
+
+
+def test_each_scene_queries_its_own_mlp(served):
+    """Radiance MLPs bound to scenes (srt_mlp_select): a scene's mlp_mode-1
+    render uses its own MLP, a scene without one the context's own, alone and
+    in the serving mix; releasing a scene frees its MLP."""
+    ctx, reqs, alone = served
+    hc = reqs[0][0]
+    cb = scenes.cornell()
+    h2 = ctx.upload(cb)
+    ctx.build_bvh(h2)
+    try:
+        ctx.mlp_select(h2)
+        ctx.mlp_create(seed=99)
+        own = ctx.render(h2, cb.camera, 32, 24, 4, 5, 4, 1, 1, 16, mlp_mode=1)
+        ctx.mlp_select(0)
+        shared = ctx.render(hc, cb.camera, 32, 24, 4, 5, 4, 1, 1, 16, mlp_mode=1)
+        assert np.array_equal(shared, alone[5])         # the context's own MLP (seed 7)
+        assert not np.array_equal(own, shared)          # same scene data, different MLP
+        ctx.serve_start(0)
+        try:
+            t1, i1 = ctx.submit(h2, cb.camera, 32, 24, 4, 5, 4, 1, 1, 16, mlp_mode=1)
+            t2, i2 = ctx.submit(hc, cb.camera, 32, 24, 4, 5, 4, 1, 1, 16, mlp_mode=1)
+            assert ctx.wait(t1)["status"] == 2 and ctx.wait(t2)["status"] == 2
+        finally:
+            ctx.serve_stop()
+        assert np.array_equal(i1, own) and np.array_equal(i2, shared)
+        used = ctx.lib.srt_hbm_used(ctx.h)
+    finally:
+        ctx.release(h2)
+    assert ctx.lib.srt_hbm_used(ctx.h) < used          # the scene and its MLP are freed
+    assert np.array_equal(ctx.render(hc, cb.camera, 32, 24, 4, 5, 4, 1, 1, 16, mlp_mode=1), shared)
