@@ -692,6 +692,85 @@ def serve_mode(args):
     print(json.dumps(line), flush=True)
 
 
+def frame_roofline(kern, st, steps, bytes_step, segments):
+    """Roofline of the dominant frame kernel against the resource that binds it.
+
+    The traversal kernels are bound by vector-ALU issue, not by HBM: their
+    node and triangle fetches mostly hit in L2/MALL.  So `achieved` is the
+    kernel's active VALU lane-operations per second (wave64 VALU instructions
+    per launch x 64 x active-lane fraction, from the committed SQ counter
+    summary, over the live HIP-event launch time) against the peak of 256 CUs x
+    4 SIMDs x 16 lanes per clock at 2.4 GHz; `frac` = achieved / peak.
+    `traffic` is the kernel's calibrated HBM bytes per launch (FETCH_SIZE +
+    WRITE_SIZE counters), and `hbm` gives, for each frame kernel, those bytes
+    over its live launch time against the 8 TB/s HBM peak.  `algorithmic`
+    keeps the byte model (ray and hit records, 112 B per 4-wide node visit,
+    36 B per triangle test): it counts cache hits, so its rate is not an HBM
+    figure."""
+    avg_ms = {k: st[k]["ms"] / st[k]["launches"] for k in bytes_step if k in st and st[k]["launches"]}
+    launches = {k: st[k]["launches"] / steps for k in avg_ms}
+    hbm = {}
+    for k in avg_ms:
+        t = measured_traffic(k)
+        if t:
+            gbs = t / (avg_ms[k] * 1e-3) / 1e9
+            hbm[k] = {"bytes_per_launch": t, "avg_launch_ms": round(avg_ms[k], 4), "GBs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    alg = {k: {"bytes_per_launch": int(bytes_step[k] / launches[k]),
+               "GBs": round(bytes_step[k] / launches[k] / (avg_ms[k] * 1e-3) / 1e9, 1)} for k in avg_ms}
+    r = {"kernel": kern, "avg_launch_ms": round(avg_ms[kern], 4), "traffic": hbm.get(kern, {}).get("bytes_per_launch"),
+         "hbm": hbm, "algorithmic": alg}
+    vc = measured_valu_counts(kern)
+    if vc:
+        insts, lanes, busy, source = vc
+        lane_ops = insts * 64.0 * lanes
+        achieved = lane_ops / (avg_ms[kern] * 1e-3) / 1e12
+        r.update({"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TLANE_OPS,
+                  "unit": "Tlane-op/s", "frac": round(achieved / VALU_PEAK_TLANE_OPS, 4),
+                  "valu_issue": {"wave_instructions_per_launch": insts, "active_lanes_frac": round(lanes, 3),
+                                 "simd_busy_frac": round(busy, 3),
+                                 "lane_ops_per_segment": round(lane_ops * launches[kern] / segments, 1),
+                                 "source": source}})
+    elif kern in hbm:
+        r.update({"bound": "hbm", "achieved": hbm[kern]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": hbm[kern]["frac"]})
+    else:
+        r.update({"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                  "note": "no counter summaries under profiles/"})
+    return r
+
+
+def host_cpu():
+    """Usable host cores and their description: the scheduler affinity set,
+    capped by a cgroup CPU quota when one is set (cgroup v2 cpu.max or v1
+    cfs_quota_us), plus os.cpu_count() and the /proc/cpuinfo model name."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        try:
+            q = int(open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us").read())
+            per = int(open("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read())
+            if q > 0:
+                quota = q / per
+        except (OSError, ValueError):
+            pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"usable": usable, "affinity": affinity, "cgroup_quota": quota, "cpu_count": os.cpu_count(),
+            "model": model}
+
+
 def measured_traffic(kernel):
     """Calibrated HBM bytes per launch of `kernel` from the committed counter summary."""
     if not os.path.exists(TRAFFIC_FILE):
