@@ -85,7 +85,18 @@ SRT_D bool intersect_tri(const RayPre& r, f3 o, f3 v0, f3 v1, f3 v2, float tmin,
     return true;
 }
 
+// Reciprocal of the ray direction for the slab test: 0 = correctly rounded
+// division; 1 = the hardware reciprocal (error at most 1 ulp, so the robust
+// factor covers four roundings instead of three).  The slab test only prunes:
+// either form gives the same hits.
+#ifndef SRT_FAST_INV
+#define SRT_FAST_INV 0
+#endif
+#if SRT_FAST_INV
+constexpr float kRobust = 1.0000005f;   // 1 + 2 gamma_4 (Ize 2013) with a 1-ulp reciprocal
+#else
 constexpr float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013): conservative slab test
+#endif
 
 // ---------------------------------------------------------------------------
 // wave-level queue append: one atomic per wave
@@ -112,20 +123,9 @@ SRT_D void clear_parts(uint32_t* first) {
 // Partition of this workgroup in a partitioned queue (grid: a multiple of kParts)
 SRT_D uint32_t my_part() { return blockIdx.x % (uint32_t)kParts; }
 
-// Slab test form of the 4-wide traversal: 1 = one fma per plane with the
-// ray's -origin/direction precomputed and an absolute slack that covers its
-// rounding; 0 = (plane - origin) * (1/direction).
-#ifndef SRT_SLAB_FMA
-#define SRT_SLAB_FMA 0
-#endif
-
 // One ray's traversal state, kept in registers while its lane walks the BVH.
 struct TravState {
     f3 o, inv;
-#if SRT_SLAB_FMA
-    f3 c;              // -o * inv per axis: plane distance = fma(plane, inv, c)
-    float slack;       // absolute bound on the rounding of c, added to the far distance
-#endif
     RayPre rp;
     float tmin, best_t, bu, bv;
     int32_t best_prim, best_pos;
@@ -140,24 +140,14 @@ struct TravState {
 SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root) {
     st.o = o;
     st.rp = ray_pre(d);
+#if SRT_FAST_INV
+    st.inv = mk(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
+#else
     st.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+#endif
     // a ray lying in a slab plane with a zero direction component yields NaN
     // slab distances, which fmaxf/fminf ignore
     st.sx = st.inv.x < 0.0f; st.sy = st.inv.y < 0.0f; st.sz = st.inv.z < 0.0f;
-#if SRT_SLAB_FMA
-    // c rounds -o * inv once: a plane distance fma(b, inv, c) is off from
-    // (b - o) * inv by at most |c| 2^-24 besides its own relative rounding
-    // (which kRobust covers as in the two-step form), so adding 16 times the
-    // sum of |c| over the axes to the far distance keeps the test
-    // conservative.  A zero direction component gives an infinite inv: its
-    // distances are then inf or NaN (ignored by fmaxf/fminf, so that axis only
-    // ever widens the box) and its c stays out of the slack.
-    st.c = mk(-(o.x * st.inv.x), -(o.y * st.inv.y), -(o.z * st.inv.z));
-    const float ax = __builtin_isfinite(st.c.x) ? fabsf(st.c.x) : 0.0f;
-    const float ay = __builtin_isfinite(st.c.y) ? fabsf(st.c.y) : 0.0f;
-    const float az = __builtin_isfinite(st.c.z) ? fabsf(st.c.z) : 0.0f;
-    st.slack = ((ax + ay) + az) * 0x1p-20f;
-#endif
     st.onx = st.sx ? 16u : 0u;    // lox 0, hix 16, loy 32, hiy 48, loz 64, hiz 80
     st.ony = st.sy ? 48u : 32u;
     st.onz = st.sz ? 80u : 64u;
@@ -253,19 +243,8 @@ SRT_D void trav_inner(const DevScene& S, TravState& st, const TravStack& k, uint
     }
 }
 
-// Slab test of one box given its near and far planes.
-#if SRT_SLAB_FMA
-SRT_D bool slab(const TravState& st, float bnx, float bfx, float bny, float bfy, float bnz, float bfz, float* tn) {
-    const f3 inv = st.inv, c = st.c;
-    const float nx = fmaf(bnx, inv.x, c.x), fx = fmaf(bfx, inv.x, c.x);
-    const float ny = fmaf(bny, inv.y, c.y), fy = fmaf(bfy, inv.y, c.y);
-    const float nz = fmaf(bnz, inv.z, c.z), fz = fmaf(bfz, inv.z, c.z);
-    const float tnear = fmaxf(fmaxf(fmaxf(st.tmin, nx), ny), nz);
-    const float tfar = fminf(fminf(fminf(st.best_t, fx), fy), fz);
-    *tn = tnear;
-    return tnear <= fmaf(tfar, kRobust, st.slack);
-}
-#else
+// Slab test of one box given its near and far planes (same arithmetic as the
+// binary step).
 SRT_D bool slab(const TravState& st, float bnx, float bfx, float bny, float bfy, float bnz, float bfz, float* tn) {
     const f3 o = st.o, inv = st.inv;
     const float nx = (bnx - o.x) * inv.x, fx = (bfx - o.x) * inv.x;
@@ -276,7 +255,6 @@ SRT_D bool slab(const TravState& st, float bnx, float bfx, float bny, float bfy,
     *tn = tnear;
     return tnear <= tfar;
 }
-#endif
 SRT_D void cswap(float& ka, int32_t& ra, float& kb, int32_t& rb) {
     const bool s = kb < ka;
     const float k = s ? kb : ka;
