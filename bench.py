@@ -245,6 +245,7 @@ KSTATS_FILE = os.path.join(ROOT, "profiles", "frame_spheres1m_kernel_stats.md")
 
 
 PASSES_KSTATS_FILE = os.path.join(ROOT, "profiles", "passes_spheres1m_kernel_stats.md")
+SKIN_KSTATS_FILE = os.path.join(ROOT, "profiles", "skin_tube200k_kernel_stats.md")
 
 
 def kstats_summary(d, mode="frame"):
@@ -269,7 +270,18 @@ def kstats_summary(d, mode="frame"):
         raise SystemExit(f"no kernel statistics under {d}")
     rows.sort(key=lambda r: -r[2])
     total = sum(r[2] for r in rows)
-    if mode == "passes":
+    if mode == "skin":
+        lines = ["# Kernel times: skinned 200k-triangle tube, 60 frames of skin + refit + 1280x720x16spp depth 6", "",
+                 "Command (from the repository root; TMPDIR=/tmp):", "",
+                 "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_skin -o run -- python3 bench.py --mode skin "
+                 "--warmup 1",
+                 "    python3 bench.py --kstats-summary gpurun_out/prof_skin --kstats-mode skin", "",
+                 "The run builds the BVH once, then renders 1 warmup frame and 60 timed frames; every frame",
+                 "skins the tube (k_skin), refits the binary tree level by level (k_refit_level) and the 4-wide",
+                 "tree (k_refit4), then renders (k_raygen, k_trace, k_shade, k_shadow, k_finalize).", "",
+                 "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
+        out_file = SKIN_KSTATS_FILE
+    elif mode == "passes":
         lines = ["# Kernel times: Simple, LightTracer and Denoise passes on the 1M-triangle scene (1920x1080)", "",
                  "Command (from the repository root; TMPDIR=/tmp):", "",
                  "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_passes -o run -- python3 bench.py --mode passes "
@@ -804,7 +816,7 @@ def main():
     ap.add_argument("--pmc-summary", nargs=4, metavar="DIR")
     ap.add_argument("--kstats-summary", metavar="DIR")
     ap.add_argument("--valu-summary", metavar="DIR")
-    ap.add_argument("--kstats-mode", choices=["frame", "passes"], default="frame")
+    ap.add_argument("--kstats-mode", choices=["frame", "passes", "skin"], default="frame")
     ap.add_argument("--mlp-summary", nargs=2, metavar="DIR")
     args = ap.parse_args()
     if args.mlp_summary:

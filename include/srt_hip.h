@@ -264,9 +264,10 @@ int srt_memcpy(srt_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t k
  * while their device film (28 bytes per pixel) fits in the HBM budget (a request
  * that can never fit ends with status SRT_E_OOM).  Each chunk of at most
  * `path_pool` paths (0: the smaller of the "path_pool" option and 2^24) runs
- * the scene of the admitted request with the fewest paths left, filled with
- * that scene's admitted requests, fewest paths left first; a larger request
- * is cut into chunks and yields the device to newer small ones between them.
+ * the scene of the admitted request with the earliest due time (submit time +
+ * remaining paths at 2 Gpaths/s), filled with that scene's admitted requests
+ * in due-time order; a large request is cut into chunks, and newer small ones
+ * overtake it between its chunks for as long as it has waited.
  * Requests with mlp_mode 1 query their scene's MLP (srt_mlp_select), else the
  * context's own.
  * A request's image is bit-identical to srt_render with the same arguments.

@@ -8,11 +8,12 @@
 //      (12 B accumulator + 16 B output per pixel) fits in the context's HBM
 //      budget; a request that can never fit fails with SRT_E_OOM;
 //   2. batching: each chunk takes the scene of the admitted request with the
-//      fewest paths left and fills the chunk (2^24 paths by default) with the
-//      remaining paths of the admitted requests on that scene, fewest left
-//      first (small requests share a chunk; a large one is cut into chunks and
-//      yields the device between them), one segment per request; requests
-//      admitted while a chunk runs join the next chunk;
+//      earliest due time (submit time + remaining paths at 2 Gpaths/s) and
+//      fills the chunk (2^24 paths by default) with the remaining paths of the
+//      admitted requests on that scene in due-time order (small requests
+//      share a chunk; a large one is cut into chunks, and newer small
+//      requests overtake it for as long as it has waited), one segment per
+//      request; requests admitted while a chunk runs join the next chunk;
 //   3. completion: a request whose last paths were in the chunk has its film
 //      copied to the caller's buffer and its device memory released.
 // A request's paths run in increasing path-id order with its own PCG streams,
@@ -41,6 +42,8 @@ struct Request {
 // default paths per serving chunk: about 9 ms of GPU time at 1.9 Gpaths/s,
 // the longest a newly admitted request waits for the device
 constexpr int64_t kServeChunk = 1ll << 24;
+// paths per millisecond that the scheduling due times assume (2 Gpaths/s)
+constexpr double kServeRate = 2.0e6;
 
 struct ServeState {
     std::thread th;
@@ -118,23 +121,24 @@ static void dispatcher(srt_ctx* c) {
                 S->cv.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
             }
-            // chunk: the scene of the admitted request with the fewest paths
-            // left (shortest remaining work first); the chunk takes that
-            // scene's requests with the fewest paths left first, up to
-            // S->pool paths, so a large request yields the device to newer
-            // small ones between its chunks (ties: arrival order)
+            // chunk: the scene of the admitted request with the earliest
+            // due time, submit time + remaining paths / kServeRate (the
+            // time its remaining work would take on an idle device); the
+            // chunk takes that scene's requests in due-time order, up to
+            // S->pool paths.  Small requests thus overtake a large one
+            // between its chunks, but only for as long as the large one
+            // has waited (ties: arrival order).
+            auto due = [](const Request* q) { return q->t_submit + (double)(q->total - q->next) / kServeRate; };
             Request* first = nullptr;
             for (Request* q : S->active)
-                if (!first || q->total - q->next < first->total - first->next) first = q;
+                if (!first || due(q) < due(first)) first = q;
             uint64_t sh = first->req.scene;
             auto it = c->scenes.find(sh);
             scene = it == c->scenes.end() ? nullptr : it->second;
             std::vector<Request*> cand;
             for (Request* q : S->active)
                 if (q->req.scene == sh) cand.push_back(q);
-            std::stable_sort(cand.begin(), cand.end(), [](const Request* a, const Request* b) {
-                return a->total - a->next < b->total - b->next;
-            });
+            std::stable_sort(cand.begin(), cand.end(), [&](const Request* a, const Request* b) { return due(a) < due(b); });
             int64_t left = S->pool;
             take.clear();
             for (Request* q : cand) {

@@ -85,18 +85,7 @@ SRT_D bool intersect_tri(const RayPre& r, f3 o, f3 v0, f3 v1, f3 v2, float tmin,
     return true;
 }
 
-// Reciprocal of the ray direction for the slab test: 0 = correctly rounded
-// division; 1 = the hardware reciprocal (error at most 1 ulp, so the robust
-// factor covers four roundings instead of three).  The slab test only prunes:
-// either form gives the same hits.
-#ifndef SRT_FAST_INV
-#define SRT_FAST_INV 0
-#endif
-#if SRT_FAST_INV
-constexpr float kRobust = 1.0000005f;   // 1 + 2 gamma_4 (Ize 2013) with a 1-ulp reciprocal
-#else
 constexpr float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013): conservative slab test
-#endif
 
 // ---------------------------------------------------------------------------
 // wave-level queue append: one atomic per wave
@@ -140,11 +129,7 @@ struct TravState {
 SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root) {
     st.o = o;
     st.rp = ray_pre(d);
-#if SRT_FAST_INV
-    st.inv = mk(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
-#else
     st.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-#endif
     // a ray lying in a slab plane with a zero direction component yields NaN
     // slab distances, which fmaxf/fminf ignore
     st.sx = st.inv.x < 0.0f; st.sy = st.inv.y < 0.0f; st.sz = st.inv.z < 0.0f;

@@ -173,3 +173,32 @@ def test_queue_counts_equal_megakernel_traces(ctx, name, W, H, spp, depth):
     assert seg == traces
     assert sh == shadows
     assert seg >= W * H * spp
+
+
+@pytest.mark.parametrize("frame", [17, 42])
+def test_skinned_tube_posed_frame(ctx, frame):
+    """Config 2 on a posed frame: linear blend skinning of the 200k-triangle
+    tube with frame `frame`'s bone matrices, BVH refit, then a render at
+    reduced resolution (depth 6, 2 spp), per pixel within 1e-4 of the oracle's
+    render of the oracle-skinned vertices (scene.cu:336-395, then
+    optix_ray_tracer.cu:443-463, then path_tracer.cu:50-169)."""
+    s = scenes.tube()
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    ctx.set_skin(h, s.skin["bone_ids"], s.skin["bone_weights"])
+    M = s.skin["matrices"](frame)
+    ctx.skin_refit(h, M)
+    W, H = 160, 90
+    img = ctx.render(h, s.camera, W, H, 2, 6, 4, 1, 1, 100 + frame)
+    ctx.release(h)
+    p, n, t = O.skin(s.positions, s.normals, s.tangents, s.skin["bone_ids"], s.skin["bone_weights"], M)
+    posed = scenes.SceneData(positions=p, indices=s.indices, mesh_first=s.mesh_first, mesh_material=s.mesh_material,
+                             materials=s.materials, normals=n, tangents=t, env=s.env, camera=s.camera)
+    osc = O.Scene(posed)
+    ob = O.Bvh(osc)
+    ref, _ = O.render(osc, s.camera, W, H, 2, 6, 4, 1, 1, 100 + frame, bvh=ob)
+    r = _rel(img[:, :3], ref[:, :3])
+    assert r.max() <= TOL, f"max rel {r.max()} at {np.unravel_index(r.argmax(), r.shape)}"
+    # the pose moved the tube: the image differs from the rest pose's
+    rest, _ = O.render(O.Scene(s), s.camera, W, H, 2, 6, 4, 1, 1, 100 + frame)
+    assert np.mean(np.any(np.abs(rest[:, :3] - ref[:, :3]) > 1e-3, axis=1)) > 0.02

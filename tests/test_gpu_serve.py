@@ -144,8 +144,9 @@ def test_pool_outside_32bit_slots_is_refused(served):
 def test_request_arriving_mid_chunk_joins_the_next_chunk(served):
     """A large request is cut into chunks; a small request submitted while it
     runs is admitted between two of its chunks, runs in one chunk ahead of the
-    large one's remaining paths (fewest paths left first), finishes first, and
-    both images equal their solo renders."""
+    large one's remaining paths (its due time, submit time + remaining paths
+    at 2 Gpaths/s, is earlier), finishes first, and both images equal their
+    solo renders."""
     ctx, reqs, alone = served
     hs = reqs[2][0]                                  # the 1M-triangle scene
     cam = reqs[2][1]
