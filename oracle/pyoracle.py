@@ -58,6 +58,10 @@ def lib():
             "orc_tonemap": (None, [C.c_int, C.c_float, C.c_int, C.c_int, P]),
             "orc_accumulate": (None, [C.c_int, C.c_int, P, P]),
             "orc_skin": (None, [C.c_int, P, P, P, P, P, C.c_int, P, P, P, P]),
+            "orc_scene_set_instances": (C.c_int, [P, C.c_int, P, P]),
+            "orc_scene_instance_inverses": (None, [P, P]),
+            "orc_scene_light_instances": (None, [P, P]),
+            "orc_trace_inst": (None, [P, P, C.c_int, C.c_int, P, P, C.c_float, P, P, P, P, P, P]),
         }
         for k, (r, a) in sigs.items():
             f = getattr(L, k)
@@ -139,6 +143,23 @@ class Scene:
         self.nv = a["pos"].shape[0]
         self.nf = a["idx"].shape[0]
         self.nmesh = a["mm"].shape[0]
+        self.ninst = 0
+        if getattr(s, "inst_mesh", None) is not None:
+            self.set_instances(s.inst_mesh, s.inst_xf)
+
+    def set_instances(self, inst_mesh, inst_xf):
+        """Meshes become object space, placed by instances (mesh index, 3x4 transform)."""
+        im = np.ascontiguousarray(inst_mesh, np.int32)
+        xf = np.ascontiguousarray(inst_xf, np.float32).reshape(-1, 12)
+        self._arrs["im"], self._arrs["xf"] = im, xf
+        if lib().orc_scene_set_instances(self.h, im.shape[0], _p(im), _p(xf)) != 0:
+            raise ValueError("bad instance mesh index or singular transform")
+        self.ninst = im.shape[0]
+
+    def instance_inverses(self):
+        out = np.zeros((self.ninst, 3, 4), np.float32)
+        lib().orc_scene_instance_inverses(self.h, _p(out))
+        return out
 
     def __del__(self):
         try:
@@ -227,6 +248,26 @@ def trace(scene: Scene, origins, dirs, tmin=1e-3, tmax=None, mode="closest", bvh
     if counters:
         return t, u, v, prim, vis, tst
     return t, u, v, prim
+
+
+def trace_inst(scene: Scene, origins, dirs, tmin=1e-3, tmax=None, mode="closest", bvh: Bvh | None = None):
+    """Instanced scene: (t, u, v, prim, inst) for closest hits, or 0/1 per ray for any hits."""
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    tm = np.full(n, 1e16, np.float32) if tmax is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(tmax, np.float32), (n,)))
+    m = {"closest": 0, "any": 1}[mode] + (0 if bvh is not None else 2)
+    t = np.zeros(n, np.float32)
+    u = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    prim = np.zeros(n, np.int32)
+    inst = np.zeros(n, np.int32)
+    lib().orc_trace_inst(scene.h, None if bvh is None else bvh.h, m, n, _p(o), _p(d), C.c_float(tmin), _p(tm), _p(t),
+                         _p(u), _p(v), _p(prim), _p(inst))
+    if mode == "any":
+        return prim
+    return t, u, v, prim, inst
 
 
 def camera_basis(cam, W, H):

@@ -341,6 +341,32 @@ void bvh_refit(const Scene& s, Bvh& bvh) {
     }
 }
 
+// One BVH per mesh of an instanced scene, over the mesh's object-space
+// triangles; sorted triangles and the sorted order carry scene primitive ids.
+void blas_build(const Scene& s, const BvhParams& prm, std::vector<Bvh>& out) {
+    out.assign(s.nmesh, Bvh());
+    for (int m = 0; m < s.nmesh; m++) {
+        const int f0 = s.mesh_first[m], n = s.mesh_first[m + 1] - f0;
+        Scene sub;
+        sub.nv = s.nv;
+        sub.pos = s.pos;
+        sub.nf = n;
+        sub.idx.assign(s.idx.begin() + 3 * (size_t)f0, s.idx.begin() + 3 * (size_t)(f0 + n));
+        sub.nmesh = 1;
+        sub.mesh_first = {0, n};
+        sub.mesh_mat = {s.mesh_mat[m]};
+        sub.mats = s.mats;
+        sub.tri_mesh.assign(n, 0);
+        bvh_build(sub, prm, out[m]);
+        for (auto& o : out[m].order) o += (uint32_t)f0;
+        for (int k = 0; k < n; k++) {
+            float* tr = &out[m].tris[12 * (size_t)k];
+            tr[3] = u2f(f2u(tr[3]) + (uint32_t)f0);
+            tr[7] = u2f((uint32_t)m);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Traversal
 // ---------------------------------------------------------------------------
@@ -444,6 +470,56 @@ bool brute_any(const Scene& s, const float o[3], const float d[3], float tmin, f
     return false;
 }
 
+// Instanced scenes: every instance's ray in object space (inverse transform,
+// apply_point / apply_vector in float), the instance's mesh BVH (or all its
+// triangles), closest hit = lexicographic minimum of (t, primitive id,
+// instance) over all instances, so no top-level tree is needed here.
+static bool trace_inst(const Scene& s, const std::vector<Bvh>* blas, const float o[3], const float d[3], float tmin,
+                       float tmax, bool any, Hit* hit) {
+    bool found = false;
+    Hit best;
+    best.t = tmax;
+    for (int i = 0; i < s.ninst(); i++) {
+        const float* Mi = &s.inst_inv[12 * (size_t)i];
+        float oo[3], dd[3];
+        xf_point(Mi, o, oo);
+        xf_vector(Mi, d, dd);
+        const int m = s.inst_mesh[i];
+        const float tm = found ? best.t : tmax;
+        Hit h;
+        bool ok = false;
+        if (blas) {
+            ok = any ? trace_any((*blas)[m], oo, dd, tmin, tmax) : trace_closest((*blas)[m], oo, dd, tmin, tm, &h);
+        } else {
+            RayPre rp = ray_pre(oo, dd);
+            float bt = tm;
+            int32_t bp = -1;
+            float bu = 0, bv = 0;
+            for (int f = s.mesh_first[m]; f < s.mesh_first[m + 1]; f++) {
+                float t, u, v;
+                if (!intersect_tri(rp, &s.pos[3 * (size_t)s.idx[3 * (size_t)f]], &s.pos[3 * (size_t)s.idx[3 * (size_t)f + 1]],
+                                   &s.pos[3 * (size_t)s.idx[3 * (size_t)f + 2]], tmin, any ? tmax : bt, &t, &u, &v))
+                    continue;
+                if (any) { ok = true; break; }
+                if (bp < 0 || t < bt || (t == bt && f < bp)) { bt = t; bp = f; bu = u; bv = v; }
+            }
+            if (!any && bp >= 0) { ok = true; h.t = bt; h.prim = bp; h.u = bu; h.v = bv; h.pos = -1; }
+        }
+        if (any) {
+            if (ok) return true;
+            continue;
+        }
+        if (ok && (!found || h.t < best.t || (h.t == best.t && (h.prim < best.prim || (h.prim == best.prim && i < best.inst))))) {
+            best = h;
+            best.inst = i;
+            found = true;
+        }
+    }
+    if (any || !found) return false;
+    *hit = best;
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // Integrator (path_tracer.cu:50-169)
 // ---------------------------------------------------------------------------
@@ -454,9 +530,12 @@ template <class R> struct HitInfo {
     int mesh;
 };
 
-// hit_info.h:27-65 (identity instance transform)
+// hit_info.h:27-65: object-space reconstruction, then the instance transform
+// (position by apply_point, geometric normal by apply_vector without
+// renormalising, the shading frame axis by axis, transform.h:37-52, 72-75);
+// inst < 0: world-space mesh (identity instance)
 template <class R>
-static void get_hitinfo(const Scene& s, int prim, R u, R v, HitInfo<R>& info) {
+static void get_hitinfo(const Scene& s, int prim, R u, R v, HitInfo<R>& info, int inst = -1) {
     const uint32_t* ix = &s.idx[3 * (size_t)prim];
     V3<R> v0 = cvt<R>(&s.pos[3 * (size_t)ix[0]]), v1 = cvt<R>(&s.pos[3 * (size_t)ix[1]]), v2 = cvt<R>(&s.pos[3 * (size_t)ix[2]]);
     R w = R(1) - u - v;
@@ -481,6 +560,12 @@ static void get_hitinfo(const Scene& s, int prim, R u, R v, HitInfo<R>& info) {
     const MaterialRec& m = s.mats[info.mat];
     info.color = V3<R>((R)m.base_color[0], (R)m.base_color[1], (R)m.base_color[2]);
     info.onb = Onb<R>(sn, tg);
+    if (inst >= 0) {
+        const float* T = &s.inst_xf[12 * (size_t)inst];
+        info.pos = xf_point<R>(T, info.pos);
+        info.normal = xf_vector<R>(T, info.normal);
+        info.onb = Onb<R>(xf_vector<R>(T, info.onb.z), xf_vector<R>(T, info.onb.x), xf_vector<R>(T, info.onb.y));
+    }
 }
 
 template <class R> struct LightSample { V3<R> pos, normal, emission; R pdf; };
@@ -525,6 +610,12 @@ static LightSample<R> light_sample(const Scene& s, V2<R> rnd) {
         if (dot(normal, sn) < R(0)) normal = -normal;
     }
     ls.normal = normal;
+    if (s.instanced()) {
+        // GInstance::sample (gmesh.h:77-78): apply_point / apply_vector of the instance transform
+        const float* T = &s.inst_xf[12 * (size_t)s.light_inst[idx]];
+        ls.pos = xf_point<R>(T, ls.pos);
+        ls.normal = xf_vector<R>(T, ls.normal);
+    }
     const MaterialRec& m = s.mats[s.mesh_mat[mesh]];
     ls.emission = V3<R>((R)m.emission_color[0], (R)m.emission_color[1], (R)m.emission_color[2]) * (R)m.intensity;
     ls.pdf = (R)m.intensity / (R)s.light_weight_sum;
@@ -536,10 +627,13 @@ template <class R> static inline bool valid(R x) { return !std::isnan(x) && !std
 struct Tracer {
     const Scene* s;
     const Bvh* bvh;    // null: brute force
+    const std::vector<Bvh>* blas = nullptr;   // instanced scenes: per-mesh trees (null: brute force)
     bool closest(const float o[3], const float d[3], float tmin, float tmax, Hit* h) const {
+        if (s->instanced()) return trace_inst(*s, blas, o, d, tmin, tmax, false, h);
         return bvh ? trace_closest(*bvh, o, d, tmin, tmax, h) : brute_closest(*s, o, d, tmin, tmax, h);
     }
     bool any(const float o[3], const float d[3], float tmin, float tmax) const {
+        if (s->instanced()) return trace_inst(*s, blas, o, d, tmin, tmax, true, nullptr);
         return bvh ? trace_any(*bvh, o, d, tmin, tmax) : brute_any(*s, o, d, tmin, tmax);
     }
 };
@@ -580,7 +674,7 @@ static V3<R> render_pixel(const Scene& s, const Tracer& tr, const Camera& cam, c
             bool hit = tr.closest(of, df, 1e-3f, 1e16f, &h);
             if (!hit) { L = L + beta * env; break; }
             HitInfo<R> info;
-            get_hitinfo<R>(s, h.prim, (R)h.u, (R)h.v, info);
+            get_hitinfo<R>(s, h.prim, (R)h.u, (R)h.v, info, h.inst);
             const MaterialRec& m = s.mats[info.mat];
             BxDF<R> bx(m);
             if (mat_is_emissive(m)) {
@@ -660,7 +754,7 @@ using namespace orc;
 
 struct OrcScene { Scene s; };
 static int64_t g_last_counts[2] = {0, 0};
-struct OrcBvh { Bvh b; };
+struct OrcBvh { Bvh b; std::vector<Bvh> blas; };   // instanced scenes: blas, one per mesh
 
 extern "C" {
 
@@ -773,8 +867,55 @@ void* orc_bvh_new(void* scene, int max_leaf, int cluster_size) {
     BvhParams prm;
     prm.max_leaf = max_leaf;
     prm.cluster_size = cluster_size;
-    bvh_build(((OrcScene*)scene)->s, prm, b->b);
+    const Scene& s = ((OrcScene*)scene)->s;
+    if (s.instanced()) blas_build(s, prm, b->blas);
+    else bvh_build(s, prm, b->b);
     return b;
+}
+
+// Instances: mesh per instance and 3x4 row-major transforms (12 floats each);
+// the meshes become object space.  Returns 0, or -1 for a bad mesh index or
+// a singular transform.
+int orc_scene_set_instances(void* p, int n, const int32_t* inst_mesh, const float* xf) {
+    Scene& s = ((OrcScene*)p)->s;
+    std::vector<float> inv(12 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        if (inst_mesh[i] < 0 || inst_mesh[i] >= s.nmesh) return -1;
+        if (!affine_inverse(xf + 12 * (size_t)i, &inv[12 * (size_t)i])) return -1;
+    }
+    s.inst_mesh.assign(inst_mesh, inst_mesh + n);
+    s.inst_xf.assign(xf, xf + 12 * (size_t)n);
+    s.inst_inv = inv;
+    scene_finalize(s);
+    return 0;
+}
+// the inverse transforms as computed (12 floats per instance)
+void orc_scene_instance_inverses(void* p, float* out) {
+    Scene& s = ((OrcScene*)p)->s;
+    std::memcpy(out, s.inst_inv.data(), sizeof(float) * s.inst_inv.size());
+}
+// lights of an instanced scene: the instance of each light
+void orc_scene_light_instances(void* p, int32_t* out) {
+    Scene& s = ((OrcScene*)p)->s;
+    std::memcpy(out, s.light_inst.data(), sizeof(int32_t) * s.light_inst.size());
+}
+
+// closest or any hit in an instanced scene: mode 0 closest over the per-mesh
+// BVHs, 1 any over them, 2 closest brute force, 3 any brute force; prim -1
+// (closest, miss) or 0/1 (any); inst: the instance hit (closest), else -1
+void orc_trace_inst(void* scene, void* bvh, int mode, int n, const float* o, const float* d, float tmin, const float* tmax,
+                    float* t, float* u, float* v, int32_t* prim, int32_t* inst) {
+    Scene& s = ((OrcScene*)scene)->s;
+    const std::vector<Bvh>* blas = (mode < 2 && bvh) ? &((OrcBvh*)bvh)->blas : nullptr;
+    for (int i = 0; i < n; i++) {
+        Hit h;
+        const bool any = (mode & 1) != 0;
+        bool r = trace_inst(s, blas, o + 3 * (size_t)i, d + 3 * (size_t)i, tmin, tmax[i], any, &h);
+        if (any) { prim[i] = r ? 1 : 0; if (inst) inst[i] = -1; continue; }
+        t[i] = r ? h.t : 0.0f; u[i] = r ? h.u : 0.0f; v[i] = r ? h.v : 0.0f;
+        prim[i] = r ? h.prim : -1;
+        if (inst) inst[i] = r ? h.inst : -1;
+    }
 }
 void orc_bvh_free(void* p) { delete (OrcBvh*)p; }
 // info[0]=num_nodes, [1]=root, [2]=max_depth, [3]=num_clusters, [4]=top nodes, [5]=cluster size
@@ -852,7 +993,7 @@ double orc_render(void* scene, void* bvh, const float* cam_pos, const float* cam
                   int W, int H, int spp, int max_depth, int rr_depth, int use_nee, int use_mis, uint32_t seed, int fp64,
                   int threads, int row_begin, int row_step, int sample_begin, int sample_end, float* out_f, double* out_d) {
     Scene& s = ((OrcScene*)scene)->s;
-    Tracer tr{&s, bvh ? &((OrcBvh*)bvh)->b : nullptr};
+    Tracer tr{&s, bvh ? &((OrcBvh*)bvh)->b : nullptr, bvh ? &((OrcBvh*)bvh)->blas : nullptr};
     Camera cam = make_camera(cam_pos, cam_target, cam_up, fov, W, H);
     PtParams p;
     p.spp = spp; p.max_depth = max_depth; p.rr_depth = rr_depth; p.use_nee = use_nee; p.use_mis = use_mis; p.seed = seed;
@@ -903,7 +1044,7 @@ double orc_render(void* scene, void* bvh, const float* cam_pos, const float* cam
 void orc_render_simple(void* scene, void* bvh, const float* cam_pos, const float* cam_target, const float* cam_up,
                        float fov, int W, int H, int spp, uint32_t seed, int type, float* out, float* range2) {
     Scene& s = ((OrcScene*)scene)->s;
-    Tracer tr{&s, bvh ? &((OrcBvh*)bvh)->b : nullptr};
+    Tracer tr{&s, bvh ? &((OrcBvh*)bvh)->b : nullptr, bvh ? &((OrcBvh*)bvh)->blas : nullptr};
     Camera cam = make_camera(cam_pos, cam_target, cam_up, fov, W, H);
     float lo[3], hi[3];
     for (int a = 0; a < 3; a++) { lo[a] = s.pos[a]; hi[a] = s.pos[a]; }
@@ -1045,7 +1186,7 @@ void orc_camera_pixel(const float* pos, const float* target, const float* up, fl
 void orc_render_light(void* scene, void* bvh, const float* cam_pos, const float* cam_target, const float* cam_up,
                       float fov, int W, int H, int spp, int max_depth, int rr_depth, uint32_t seed, float* out) {
     Scene& s = ((OrcScene*)scene)->s;
-    Tracer tr{&s, bvh ? &((OrcBvh*)bvh)->b : nullptr};
+    Tracer tr{&s, bvh ? &((OrcBvh*)bvh)->b : nullptr, bvh ? &((OrcBvh*)bvh)->blas : nullptr};
     Camera cam = make_camera(cam_pos, cam_target, cam_up, fov, W, H);
     const V3<float> CP = cvt<float>(cam.pos);
     const float aspect = (float)W / (float)H;

@@ -398,6 +398,8 @@ template <class R> struct Onb {
     V3<R> z, x, y;
     Onb() {}
     Onb(V3<R> n, V3<R> t) : z(n) { x = normalize(t - z * dot(t, z)); y = cross(z, x); }
+    // onb.h:31: from three given axes (Transform::apply_onb, transform.h:72-75)
+    Onb(V3<R> n, V3<R> t, V3<R> b) : z(n), x(t), y(b) {}
     V3<R> to_world(V3<R> p) const { return p.x * x + p.y * y + p.z * z; }
     V3<R> to_local(V3<R> p) const { return V3<R>(dot(p, x), dot(p, y), dot(p, z)); }
 };
@@ -654,8 +656,13 @@ template <class R> inline R mat_pdf(const BxDF<R>& b, V3<R> wi, V3<R> wo, const 
 }
 
 // ---------------------------------------------------------------------------
-// Scene: flat world-space triangle soup grouped into meshes (one instance per
-// mesh with identity transform; every config uses identity instances).
+// Scene: triangle meshes.  Without instances every mesh is one identity
+// instance in world space.  With instances (scene.cu:225-288, the IAS of
+// optix_ray_tracer.cu:337-359) the meshes are in object space and instance i
+// places mesh inst_mesh[i] with the 3x4 row-major transform inst_xf[12 i..]
+// (Transform::apply_point / apply_vector, transform.h:37-52); its inverse
+// inst_inv is computed in double from the 3x4 (cofactors / determinant,
+// translation -A^-1 t) and rounded to float, as the product library does.
 // ---------------------------------------------------------------------------
 struct Scene {
     int nv = 0, nf = 0, nmesh = 0, nmat = 0;
@@ -673,8 +680,50 @@ struct Scene {
     std::vector<int> light_mesh;               // nlights
     std::vector<float> light_cdf;              // nlights
     float light_weight_sum = 0.0f;
+    // instances (empty: one identity instance per mesh, world-space meshes)
+    std::vector<int> inst_mesh;                // ninst
+    std::vector<float> inst_xf, inst_inv;      // 12 per instance
+    std::vector<int> light_inst;               // nlights: the light's instance (instanced scenes)
     int nlights() const { return (int)light_mesh.size(); }
+    bool instanced() const { return !inst_mesh.empty(); }
+    int ninst() const { return (int)inst_mesh.size(); }
 };
+
+// 3x4 affine inverse: double cofactors / determinant, translation -A^-1 t,
+// rounded to float.  Returns false for a singular transform.
+inline bool affine_inverse(const float m[12], float out[12]) {
+    const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], k = m[10];
+    const double A = e * k - f * h, B = f * g - d * k, Cc = d * h - e * g;
+    const double det = a * A + b * B + c * Cc;
+    if (!(det != 0.0) || !std::isfinite(det)) return false;
+    const double inv[9] = {A / det, (c * h - b * k) / det, (b * f - c * e) / det,
+                           B / det, (a * k - c * g) / det, (c * d - a * f) / det,
+                           Cc / det, (b * g - a * h) / det, (a * e - b * d) / det};
+    const double t[3] = {m[3], m[7], m[11]};
+    for (int r = 0; r < 3; r++) {
+        out[4 * r + 0] = (float)inv[3 * r + 0];
+        out[4 * r + 1] = (float)inv[3 * r + 1];
+        out[4 * r + 2] = (float)inv[3 * r + 2];
+        out[4 * r + 3] = (float)(-(inv[3 * r + 0] * t[0] + inv[3 * r + 1] * t[1] + inv[3 * r + 2] * t[2]));
+    }
+    return true;
+}
+// transform.h:37-52, in float in the reference's operation order
+inline void xf_point(const float* m, const float p[3], float out[3]) {
+    for (int r = 0; r < 3; r++) out[r] = m[4 * r] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3];
+}
+inline void xf_vector(const float* m, const float v[3], float out[3]) {
+    for (int r = 0; r < 3; r++) out[r] = m[4 * r] * v[0] + m[4 * r + 1] * v[1] + m[4 * r + 2] * v[2];
+}
+template <class R> inline V3<R> xf_point(const float* m, V3<R> p) {
+    return V3<R>((R)m[0] * p.x + (R)m[1] * p.y + (R)m[2] * p.z + (R)m[3],
+                 (R)m[4] * p.x + (R)m[5] * p.y + (R)m[6] * p.z + (R)m[7],
+                 (R)m[8] * p.x + (R)m[9] * p.y + (R)m[10] * p.z + (R)m[11]);
+}
+template <class R> inline V3<R> xf_vector(const float* m, V3<R> v) {
+    return V3<R>((R)m[0] * v.x + (R)m[1] * v.y + (R)m[2] * v.z, (R)m[4] * v.x + (R)m[5] * v.y + (R)m[6] * v.z,
+                 (R)m[8] * v.x + (R)m[9] * v.y + (R)m[10] * v.z);
+}
 
 // mesh.cpp:3-18 and scene.cu:240-264 (light weight = intensity * mesh area)
 inline void scene_finalize(Scene& s) {
@@ -697,13 +746,19 @@ inline void scene_finalize(Scene& s) {
     }
     s.mesh_light.assign(s.nmesh, -1);
     s.light_mesh.clear();
+    s.light_inst.clear();
     s.light_cdf.clear();
     float wsum = 0.0f;
-    for (int m = 0; m < s.nmesh; m++) {
+    // lights: emissive instances (scene.cu:240-264), weight = intensity x the
+    // mesh's object-space area; without instances, emissive meshes
+    const int nunits = s.instanced() ? s.ninst() : s.nmesh;
+    for (int u = 0; u < nunits; u++) {
+        const int m = s.instanced() ? s.inst_mesh[u] : u;
         const MaterialRec& mat = s.mats[s.mesh_mat[m]];
         if (mat.intensity <= 0.0f) continue;
-        s.mesh_light[m] = (int)s.light_mesh.size();
+        if (!s.instanced()) s.mesh_light[m] = (int)s.light_mesh.size();
         s.light_mesh.push_back(m);
+        if (s.instanced()) s.light_inst.push_back(u);
         float w = mat.intensity * s.mesh_area[m];
         wsum += w;
         s.light_cdf.push_back(wsum);
@@ -893,10 +948,14 @@ inline int kdelta(const std::vector<uint64_t>& c, int n, int i, int j) {
 }
 
 void bvh_build(const Scene& s, const BvhParams& prm, Bvh& out);
+// instanced scenes: one BVH per mesh over the mesh's object-space triangles
+// (sorted triangles carry the scene's primitive ids)
+void blas_build(const Scene& s, const BvhParams& prm, std::vector<Bvh>& out);
 void bvh_refit(const Scene& s, Bvh& bvh);   // same topology, boxes from current vertex positions
 
-// closest hit over the BVH: returns sorted position (or -1)
-struct Hit { float t, u, v; int32_t prim; int32_t pos; };
+// closest hit over the BVH: returns sorted position (or -1); inst: the
+// instance hit (instanced scenes; -1 otherwise)
+struct Hit { float t, u, v; int32_t prim; int32_t pos; int32_t inst = -1; };
 bool trace_closest(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax, Hit* hit, int* nodes_visited = nullptr, int* tris_tested = nullptr);
 bool trace_any(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax);
 // brute force over the scene (reference semantics of D3)

@@ -80,6 +80,11 @@ class SceneData:
     render: dict = dataclasses.field(default_factory=dict)
     skin: dict | None = None       # bone_ids (V,4) int32, bone_weights (V,4) float32, frames callable
     name: str = ""
+    # instances (None: every mesh is one identity instance in world space):
+    # mesh per instance (I,) int32 and 3x4 row-major transforms (I, 3, 4) float32;
+    # the meshes are then in object space
+    inst_mesh: np.ndarray | None = None
+    inst_xf: np.ndarray | None = None
 
     @property
     def num_triangles(self):
@@ -382,3 +387,89 @@ def save_obj(s: SceneData, path: str):
             f.write(f"g mesh{k}\nusemtl m{int(s.mesh_material[k])}\n")
             for t in s.indices[s.mesh_first[k]:s.mesh_first[k + 1]]:
                 f.write(f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")
+
+
+def _rotation(axis, angle):
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1.0 - math.cos(angle)) * (K @ K)
+
+
+def affine(R=None, t=(0.0, 0.0, 0.0), scale=1.0):
+    """3x4 row-major transform [scale R | t] in float32."""
+    M = np.zeros((3, 4), np.float64)
+    M[:, :3] = (np.eye(3) if R is None else np.asarray(R, np.float64)) * scale
+    M[:, 3] = t
+    return M.astype(np.float32)
+
+
+def identity_instanced(s: SceneData) -> SceneData:
+    """The same scene with every mesh as one identity instance (two-level form)."""
+    out = dataclasses.replace(s)
+    n = s.mesh_material.shape[0]
+    out.inst_mesh = np.arange(n, dtype=np.int32)
+    out.inst_xf = np.tile(np.eye(4, dtype=np.float32)[:3], (n, 1, 1))
+    return out
+
+
+def cornell_instanced(fov=40.0, box_angles=(0.3, -0.25)) -> SceneData:
+    """Cornell box whose two boxes are instances of one unit cube mesh (scaled,
+    rotated about y, translated) and whose walls and light are identity
+    instances of their own meshes: the reference's instance form
+    (one mesh, several transformed instances)."""
+    red = material(base_color=(0.63, 0.065, 0.05), roughness=1.0)
+    green = material(base_color=(0.14, 0.45, 0.091), roughness=1.0)
+    grey = material(base_color=(0.75, 0.75, 0.75), roughness=1.0)
+    light = material(base_color=(0.75, 0.75, 0.75), roughness=1.0, emission=(17.0, 12.0, 4.0))
+    b = _Builder()
+    b.quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1), 2, (0, 1, 0))
+    b.quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), 2, (0, -1, 0))
+    b.quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1), 2, (0, 0, 1))
+    b.quad((-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1), 0, (1, 0, 0))
+    b.quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), 1, (-1, 0, 0))
+    b.quad((-0.24, 1.98, -0.22), (0.23, 1.98, -0.22), (0.23, 1.98, 0.16), (-0.24, 1.98, 0.16), 3, (0, -1, 0))
+    b.box((-0.5, 0.0, -0.5), (0.5, 1.0, 0.5), 2)           # mesh 6: unit cube on the floor plane
+    s = b.finish([red, green, grey, light], with_normals=False, env=(0.0, 0.0, 0.0), name="cornell_instanced")
+    xf = [affine() for _ in range(6)]
+    xf.append(affine(_rotation((0, 1, 0), box_angles[0]) @ np.diag([0.6, 0.6, 0.6]), (0.32, 0.0, 0.37)))
+    xf.append(affine(_rotation((0, 1, 0), box_angles[1]) @ np.diag([0.6, 1.2, 0.6]), (-0.33, 0.0, -0.29)))
+    s.inst_mesh = np.array([0, 1, 2, 3, 4, 5, 6, 6], np.int32)
+    s.inst_xf = np.stack(xf)
+    s.camera = dict(position=(0.0, 1.0, 5.5), target=(0.0, 1.0, 0.5), up=(0.0, 1.0, 0.0), fov=fov)
+    s.render = dict(width=256, height=256, spp=16, max_depth=5, rr_depth=4, seed=1, use_nee=1, use_mis=1)
+    return s
+
+
+def spheres_instanced(num_spheres=3125, num_meshes=16, seed=20261202) -> SceneData:
+    """The 1M-triangle sphere field in instance form: `num_meshes` unit
+    frequency-4 icospheres (one Disney material each) instanced num_spheres
+    times with a random rotation, the log-uniform radius as uniform scale and
+    the uniform centre as translation; the four quad lights are identity
+    instances.  Same sizes and camera as spheres()."""
+    rng = np.random.default_rng(seed)
+    uv, ut = icosphere_f4()
+    b = _Builder()
+    mats = []
+    for k in range(num_meshes):
+        mats.append(material(base_color=rng.uniform(0.0, 1.0, 3), metallic=float(rng.uniform() < 0.3),
+                             roughness=float(rng.uniform(0.05, 1.0)), specular=0.5,
+                             clearcoat=float(rng.uniform() < 0.2)))
+        b.add_mesh(uv, ut, k, normals=uv)
+    mats.append(material(base_color=(0.8, 0.8, 0.8), emission=(8.0, 8.0, 8.0)))
+    for (cx, cz) in [(-25.0, -25.0), (25.0, -25.0), (-25.0, 25.0), (25.0, 25.0)]:
+        b.quad((cx - 10, 70, cz - 10), (cx + 10, 70, cz - 10), (cx + 10, 70, cz + 10), (cx - 10, 70, cz + 10),
+               num_meshes, (0, -1, 0))
+    s = b.finish(mats, with_normals=True, env=(0.05, 0.05, 0.07), name="spheres_instanced")
+    centres = rng.uniform(-50.0, 50.0, size=(num_spheres, 3))
+    radii = np.exp(rng.uniform(math.log(0.2), math.log(3.0), size=num_spheres))
+    axes = rng.normal(size=(num_spheres, 3))
+    angles = rng.uniform(0.0, 2.0 * math.pi, size=num_spheres)
+    which = rng.integers(0, num_meshes, size=num_spheres)
+    xf = [affine(_rotation(axes[i], angles[i]), centres[i], radii[i]) for i in range(num_spheres)]
+    xf += [affine() for _ in range(4)]
+    s.inst_mesh = np.concatenate([which, num_meshes + np.arange(4)]).astype(np.int32)
+    s.inst_xf = np.stack(xf)
+    s.camera = dict(position=(0.0, 0.0, 160.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=40.0)
+    s.render = dict(width=1920, height=1080, spp=64, max_depth=8, rr_depth=4, seed=1, use_nee=1, use_mis=1)
+    return s
