@@ -129,6 +129,27 @@ int srt_scene_release(srt_ctx* ctx, srt_scene scene);
 int srt_scene_lights(srt_ctx* ctx, srt_scene scene, float* light_cdf, int32_t* light_mesh, float* area_cdf,
                      float* mesh_area, float* weight_sum);
 
+/* ---- instances (replace the IAS over per-mesh GAS with 3x4 instance
+ *      transforms, optix_ray_tracer.cu:337-359, and the instance data of
+ *      Scene::build_gscene_instances, scene.cu:225-288) ----
+ * After srt_scene_upload: the scene's meshes become object space and
+ * instance i places mesh inst_mesh[i] with the row-major 3x4 transform
+ * xforms_3x4[12 i .. 12 i + 11] (a mesh may be placed several times).  The
+ * lights become the emissive instances, weighted by intensity x the mesh's
+ * object-space area (scene.cu:240-264).  srt_scene_build_bvh then builds one
+ * tree per mesh plus a top-level tree over the instances; hits apply the
+ * transform as hit_info.h:58-62 does (position by apply_point, normal and
+ * shading frame by apply_vector, not renormalised), and so do light samples
+ * (gmesh.h:77-78).  Closest hit: (t, primitive id, instance) lexicographic.
+ * Instanced scenes take the path tracer, serving and the radiance MLP; not
+ * skinning, the Simple or LightTracer passes, nor srt_scene_bvh_download.
+ * Limits: fewer than 2^27 triangles, fewer than 2^30 - 1 instances. */
+int srt_scene_set_instances(srt_ctx* ctx, srt_scene scene, int32_t n, const int32_t* inst_mesh, const float* xforms_3x4);
+/* node animation (Scene::update_gscene's instance transforms, scene.cu:309-334,
+ * then update_as, optix_ray_tracer.cu:443-463): new transforms for all
+ * instances, then the top-level tree is refit (the mesh trees are unchanged) */
+int srt_scene_update_instances(srt_ctx* ctx, srt_scene scene, const float* xforms_3x4);
+
 /* ---- acceleration structure (replaces OptixRayTracer::build_gas/build_ias,
  *      optix_ray_tracer.cu:213-359, and update_as, :443-463) ---- */
 int srt_scene_build_bvh(srt_ctx* ctx, srt_scene scene, const srt_bvh_opts* opts);
@@ -148,6 +169,11 @@ int srt_scene_download_vertices(srt_ctx* ctx, srt_scene scene, float* positions,
  * anything is hit in [tmin, tmax[i]], else 0. */
 int srt_trace_rays(srt_ctx* ctx, srt_scene scene, int32_t n, const float* origins, const float* dirs, float tmin,
                    const float* tmax, int32_t any, float* t, float* u, float* v, int32_t* prim);
+
+/* As srt_trace_rays, with the instance hit (closest hits of instanced
+ * scenes; -1 on a miss and for scenes without instances). */
+int srt_trace_rays_inst(srt_ctx* ctx, srt_scene scene, int32_t n, const float* origins, const float* dirs, float tmin,
+                        const float* tmax, int32_t any, float* t, float* u, float* v, int32_t* prim, int32_t* inst);
 
 /* ---- ray queries that allocate nothing ----
  * No counterpart in the reference.  For callers that cannot allocate device

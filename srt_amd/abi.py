@@ -25,7 +25,8 @@ EXPORTED = [
     "srt_scene_upload", "srt_scene_release", "srt_scene_lights",
     "srt_scene_build_bvh", "srt_scene_bvh_info", "srt_scene_bvh_download",
     "srt_scene_set_skin", "srt_scene_skin_refit", "srt_scene_download_vertices",
-    "srt_trace_rays", "srt_trace_rays_noalloc", "srt_render", "srt_render_simple", "srt_render_light", "srt_film_download", "srt_tonemap", "srt_accumulate",
+    "srt_trace_rays", "srt_trace_rays_noalloc", "srt_trace_rays_inst", "srt_scene_set_instances",
+    "srt_scene_update_instances", "srt_render", "srt_render_simple", "srt_render_light", "srt_film_download", "srt_tonemap", "srt_accumulate",
     "srt_denoise",
     "srt_stats", "srt_stats_reset", "srt_last_counts", "srt_last_visits",
     "srt_device_alloc", "srt_device_free", "srt_memcpy",
@@ -109,6 +110,9 @@ def load_library(path: str = LIB_PATH):
         "srt_scene_download_vertices": (C.c_int, [P, C.c_uint64, P, P, P]),
         "srt_trace_rays": (C.c_int, [P, C.c_uint64, C.c_int32, P, P, C.c_float, P, C.c_int32, P, P, P, P]),
         "srt_trace_rays_noalloc": (C.c_int, [P, C.c_uint64, C.c_int32, P, P, C.c_float, P, C.c_int32, P, P, P, P]),
+        "srt_trace_rays_inst": (C.c_int, [P, C.c_uint64, C.c_int32, P, P, C.c_float, P, C.c_int32, P, P, P, P, P]),
+        "srt_scene_set_instances": (C.c_int, [P, C.c_uint64, C.c_int32, P, P]),
+        "srt_scene_update_instances": (C.c_int, [P, C.c_uint64, P]),
         "srt_render": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32, C.c_int32, P,
                                  C.c_int32]),
         "srt_render_light": (C.c_int, [P, C.c_uint64, C.POINTER(Camera), C.POINTER(PtParams), C.c_int32, C.c_int32, P,
@@ -225,7 +229,24 @@ class Context:
             d.env_color[i] = s.env[i]
         out = C.c_uint64()
         self.check(self.lib.srt_scene_upload(self.h, C.byref(d), C.byref(out)))
+        if getattr(s, "inst_mesh", None) is not None:
+            try:
+                self.set_instances(out.value, s.inst_mesh, s.inst_xf)
+            except SrtError:
+                self.lib.srt_scene_release(self.h, out.value)
+                raise
         return out.value
+
+    def set_instances(self, scene, inst_mesh, inst_xf):
+        """srt_scene_set_instances: meshes become object space, placed by (mesh, 3x4 transform) instances."""
+        im = np.ascontiguousarray(inst_mesh, np.int32)
+        xf = np.ascontiguousarray(inst_xf, np.float32).reshape(-1, 12)
+        self.check(self.lib.srt_scene_set_instances(self.h, scene, im.shape[0], _ptr(im), _ptr(xf)))
+
+    def update_instances(self, scene, inst_xf):
+        """srt_scene_update_instances: new transforms for every instance, then the top-level refit."""
+        xf = np.ascontiguousarray(inst_xf, np.float32).reshape(-1, 12)
+        self.check(self.lib.srt_scene_update_instances(self.h, scene, _ptr(xf)))
 
     def release(self, scene):
         self.check(self.lib.srt_scene_release(self.h, scene))
@@ -288,6 +309,24 @@ class Context:
         self.check(self.lib.srt_trace_rays(self.h, scene, n, _ptr(o), _ptr(d), C.c_float(tmin), _ptr(tm),
                                            1 if any_hit else 0, _ptr(t), _ptr(u), _ptr(v), _ptr(prim)))
         return t, u, v, prim
+
+    def trace_inst(self, scene, origins, dirs, tmin=1e-3, tmax=None, any_hit=False):
+        """srt_trace_rays_inst: (t, u, v, prim, instance) for closest hits, or 0/1 per ray for any hits."""
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = o.shape[0]
+        tm = np.full(n, 1e16, np.float32) if tmax is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(tmax, np.float32), (n,)))
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.int32)
+        inst = np.zeros(n, np.int32)
+        self.check(self.lib.srt_trace_rays_inst(self.h, scene, n, _ptr(o), _ptr(d), C.c_float(tmin), _ptr(tm),
+                                                1 if any_hit else 0, _ptr(t), _ptr(u), _ptr(v), _ptr(prim), _ptr(inst)))
+        if any_hit:
+            return prim
+        return t, u, v, prim, inst
 
     def trace_noalloc(self, scene, n, origins_dev, dirs_dev, tmax_dev, t_dev, u_dev, v_dev, prim_dev, tmin=1e-3,
                       any_hit=False):

@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libsrt_hip.so")
 HOSTLIB = os.path.join(HERE, "libsrt_host.so")
 SRTRACER = os.path.join(HERE, "srtracer")
 
-HIP_SOURCES = ["srt_api.hip", "wavefront.hip", "bvh.hip", "mlp.hip", "serve.hip"]
+HIP_SOURCES = ["srt_api.hip", "wavefront.hip", "bvh.hip", "mlp.hip", "serve.hip", "instances.hip"]
 # per-source code-generation options: the MLP kernels keep MFMA accumulators in
 # ordinary VGPRs (their epilogues read every element; the AGPR form would add a
 # v_accvgpr_read per element)

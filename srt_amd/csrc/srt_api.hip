@@ -108,6 +108,11 @@ DevScene dev_scene(const SceneRec* s) {
     S.num_lights = s->nlights;
     S.light_wsum = s->light_wsum;
     S.env[0] = s->env[0]; S.env[1] = s->env[1]; S.env[2] = s->env[2];
+    S.num_inst = s->ninst;
+    S.inst_xf = (const float4*)s->inst_xf.p;
+    S.inst_inv = (const float4*)s->inst_inv.p;
+    S.inst_root = (const int32_t*)s->inst_root.p;
+    S.light_inst = (const int32_t*)s->light_inst.p;
     return S;
 }
 
@@ -115,7 +120,8 @@ static void free_scene(srt_ctx* c, SceneRec* s) {
     DevBuf* all[] = {&s->pos, &s->nrm, &s->tan, &s->idx, &s->mesh_mat, &s->mesh_first, &s->area_cdf, &s->mats,
                      &s->light_cdf, &s->light_mesh, &s->tri_mesh, &s->orig_pos, &s->orig_nrm, &s->orig_tan,
                      &s->bone_ids, &s->bone_w, &s->bones, &s->nodes, &s->tris, &s->order, &s->node_parent,
-                     &s->level_nodes, &s->node_valid, &s->nodes4, &s->tri_rec};
+                     &s->level_nodes, &s->node_valid, &s->nodes4, &s->tri_rec, &s->inst_xf, &s->inst_inv,
+                     &s->inst_root, &s->light_inst};
     for (DevBuf* b : all) dev_free(c, *b);
     delete s;
 }
@@ -313,6 +319,16 @@ int srt_scene_upload(srt_ctx* c, const srt_scene_desc* d, srt_scene* out) {
     for (int i = 0; i < 3; i++) s->env[i] = d->env_color[i];
     s->light_wsum = wsum;
     s->h_mesh_area = mesh_area;
+    s->h_mesh_box.assign(6 * (size_t)nm, 0.0f);
+    for (int m = 0; m < nm; m++) {
+        float* b = &s->h_mesh_box[6 * (size_t)m];
+        for (int a = 0; a < 3; a++) { b[a] = INFINITY; b[3 + a] = -INFINITY; }
+        for (int f = d->mesh_first_triangle[m]; f < d->mesh_first_triangle[m + 1]; f++)
+            for (int k = 0; k < 3; k++) {
+                const float* v = d->positions + 3 * (size_t)d->indices[3 * (size_t)f + k];
+                for (int a = 0; a < 3; a++) { b[a] = std::min(b[a], v[a]); b[3 + a] = std::max(b[3 + a], v[a]); }
+            }
+    }
     int r = SRT_OK;
     if (!r) r = upload(c, s->pos, d->positions, 12 * (size_t)nv);
     if (!r && s->has_nrm) r = upload(c, s->nrm, d->normals, 12 * (size_t)nv);
@@ -359,6 +375,13 @@ int srt_scene_lights(srt_ctx* c, srt_scene h, float* light_cdf, int32_t* light_m
     return s->nlights;
 }
 
+// Render entries build a default tree on first use: the two-level form for
+// instanced scenes, the single tree otherwise (leaves of up to 4 triangles).
+static int ensure_bvh(srt_ctx* c, SceneRec* s) {
+    if (s->has_bvh) return SRT_OK;
+    return s->ninst > 0 ? inst_build(c, s, 4, 0) : bvh_build(c, s, 4, 0);
+}
+
 int srt_scene_build_bvh(srt_ctx* c, srt_scene h, const srt_bvh_opts* o) {
     SRT_IDLE(c);
     if (!c) return SRT_E_INVALID;
@@ -369,8 +392,86 @@ int srt_scene_build_bvh(srt_ctx* c, srt_scene h, const srt_bvh_opts* o) {
     if (max_leaf <= 0) max_leaf = 4;
     hipEvent_t ev;
     prof_begin(c, "bvh_build", &ev);
-    int r = bvh_build(c, s, max_leaf, cluster);
+    int r = s->ninst > 0 ? inst_build(c, s, max_leaf, cluster) : bvh_build(c, s, max_leaf, cluster);
     prof_end(c, "bvh_build", ev, s->nf);
+    prof_collect(c);
+    return r;
+}
+
+// Instances (the IAS over per-mesh GAS of optix_ray_tracer.cu:337-359 and the
+// instance transforms of scene.cu:225-288): the meshes become object space;
+// lights become the emissive instances (weight intensity x the mesh's
+// object-space area, scene.cu:240-264).  The BVH must be built afterwards.
+int srt_scene_set_instances(srt_ctx* c, srt_scene h, int32_t n, const int32_t* inst_mesh, const float* xf) {
+    SRT_IDLE(c);
+    if (!c || n <= 0 || !inst_mesh || !xf) return SRT_E_INVALID;
+    SceneRec* s = find_scene(c, h);
+    if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (s->nbones > 0 || s->bone_ids.p) return set_err(c, SRT_E_STATE, "a skinned scene cannot also have instances");
+    if (n >= 0x3fffffff) return set_err(c, SRT_E_LIMIT, "too many instances");
+    if ((int64_t)s->nf > kMaxInstTriangles) return set_err(c, SRT_E_LIMIT, "an instanced scene holds < 2^27 triangles");
+    std::vector<float> inv(12 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        if (inst_mesh[i] < 0 || inst_mesh[i] >= s->nmesh)
+            return set_err(c, SRT_E_INVALID, "instance %d references mesh %d", i, inst_mesh[i]);
+        for (int k = 0; k < 12; k++)
+            if (!std::isfinite(xf[12 * (size_t)i + k])) return set_err(c, SRT_E_INVALID, "instance %d transform is not finite", i);
+        if (!affine_inverse(xf + 12 * (size_t)i, &inv[12 * (size_t)i]))
+            return set_err(c, SRT_E_INVALID, "instance %d transform is singular", i);
+    }
+    // lights over instances, host float arithmetic in instance order
+    std::vector<float> light_cdf;
+    std::vector<int32_t> light_mesh, light_inst;
+    float wsum = 0.0f;
+    std::vector<srt_material> mats(s->nmat);
+    std::vector<int32_t> mesh_mat(s->nmesh);
+    int r = hip_check(c, hipMemcpy(mats.data(), s->mats.p, sizeof(srt_material) * mats.size(), hipMemcpyDeviceToHost), "materials");
+    if (!r) r = hip_check(c, hipMemcpy(mesh_mat.data(), s->mesh_mat.p, 4 * mesh_mat.size(), hipMemcpyDeviceToHost), "materials");
+    if (r) return r;
+    for (int i = 0; i < n; i++) {
+        const int m = inst_mesh[i];
+        const srt_material& mat = mats[mesh_mat[m]];
+        if (mat.intensity <= 0.0f) continue;
+        light_mesh.push_back(m);
+        light_inst.push_back(i);
+        float w = mat.intensity * s->h_mesh_area[m];
+        wsum += w;
+        light_cdf.push_back(wsum);
+    }
+    for (auto& x : light_cdf) x /= wsum;
+    // device copies: transforms as three float4 rows each
+    DevBuf* bufs[] = {&s->inst_xf, &s->inst_inv, &s->inst_root, &s->light_inst, &s->light_cdf, &s->light_mesh};
+    for (DevBuf* b : bufs) dev_free(c, *b);
+    r = upload(c, s->inst_xf, xf, 48 * (size_t)n);
+    if (!r) r = upload(c, s->inst_inv, inv.data(), 48 * (size_t)n);
+    if (!r && !light_cdf.empty()) r = upload(c, s->light_cdf, light_cdf.data(), 4 * light_cdf.size());
+    if (!r && !light_cdf.empty()) r = upload(c, s->light_mesh, light_mesh.data(), 4 * light_mesh.size());
+    if (!r && !light_cdf.empty()) r = upload(c, s->light_inst, light_inst.data(), 4 * light_inst.size());
+    if (r) return r;
+    s->ninst = n;
+    s->h_inst_mesh.assign(inst_mesh, inst_mesh + n);
+    s->h_inst_xf.assign(xf, xf + 12 * (size_t)n);
+    s->h_inst_inv = inv;
+    s->nlights = (int32_t)light_cdf.size();
+    s->light_wsum = wsum;
+    s->has_bvh = false;
+    return SRT_OK;
+}
+
+// Node animation (scene.cu:309-334 updates instance transforms; update_as,
+// optix_ray_tracer.cu:443-463, refits): new transforms for every instance,
+// then the top-level tree is refit.
+int srt_scene_update_instances(srt_ctx* c, srt_scene h, const float* xf) {
+    SRT_IDLE(c);
+    if (!c || !xf) return SRT_E_INVALID;
+    SceneRec* s = find_scene(c, h);
+    if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (s->ninst <= 0) return set_err(c, SRT_E_STATE, "the scene has no instances");
+    if (!s->has_bvh) return set_err(c, SRT_E_STATE, "no BVH built");
+    hipEvent_t ev;
+    prof_begin(c, "inst_refit", &ev);
+    int r = inst_update(c, s, xf);
+    prof_end(c, "inst_refit", ev, s->ninst);
     prof_collect(c);
     return r;
 }
@@ -391,6 +492,7 @@ int srt_scene_bvh_download(srt_ctx* c, srt_scene h, void* nodes, uint32_t* order
     if (!c) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (s->ninst > 0) return set_err(c, SRT_E_INVALID, "an instanced scene holds no single binary tree");
     if (!s->has_bvh) return set_err(c, SRT_E_STATE, "no BVH built");
     if (int r = hip_check(c, hipStreamSynchronize(c->stream), "sync")) return r;
     if (nodes) if (int r = hip_check(c, hipMemcpy(nodes, s->nodes.p, (size_t)s->num_nodes * 64, hipMemcpyDeviceToHost), "copy")) return r;
@@ -404,6 +506,7 @@ int srt_scene_set_skin(srt_ctx* c, srt_scene h, const int32_t* bone_ids, const f
     if (!c || !bone_ids || !bone_weights) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (s->ninst > 0) return set_err(c, SRT_E_INVALID, "instanced scenes are animated by instance transforms (srt_scene_update_instances)");
     for (int64_t i = 0; i < 4 * (int64_t)s->nv; i++)
         if (bone_ids[i] < -1) return set_err(c, SRT_E_INVALID, "bone id %d invalid", bone_ids[i]);
     dev_free(c, s->bone_ids); dev_free(c, s->bone_w);
@@ -428,6 +531,7 @@ int srt_scene_skin_refit(srt_ctx* c, srt_scene h, const float* bones_3x4, int32_
     if (!c || (n_bones > 0 && !bones_3x4)) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (s->ninst > 0) return set_err(c, SRT_E_INVALID, "instanced scenes are animated by instance transforms (srt_scene_update_instances)");
     if (int r = skin_vertices(c, s, bones_3x4, n_bones)) return r;
     hipEvent_t ev;
     prof_begin(c, "refit", &ev);
@@ -478,6 +582,40 @@ int srt_trace_rays(srt_ctx* c, srt_scene h, int32_t n, const float* o, const flo
     return r;
 }
 
+int srt_trace_rays_inst(srt_ctx* c, srt_scene h, int32_t n, const float* o, const float* d, float tmin, const float* tmax,
+                        int32_t any, float* t, float* u, float* v, int32_t* prim, int32_t* inst) {
+    SRT_IDLE(c);
+    if (!c || n < 0 || !o || !d || !tmax || !prim || (!any && (!t || !u || !v || !inst))) return SRT_E_INVALID;
+    SceneRec* s = find_scene(c, h);
+    if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (!s->has_bvh) return set_err(c, SRT_E_STATE, "no BVH built");
+    if (n == 0) return SRT_OK;
+    DevBuf bo, bd, bt, tt, tu, tv, tp, ti;
+    int r = upload(c, bo, o, 12 * (size_t)n);
+    if (!r) r = upload(c, bd, d, 12 * (size_t)n);
+    if (!r) r = upload(c, bt, tmax, 4 * (size_t)n);
+    if (!r) r = dev_alloc(c, tt, 4 * (size_t)n);
+    if (!r) r = dev_alloc(c, tu, 4 * (size_t)n);
+    if (!r) r = dev_alloc(c, tv, 4 * (size_t)n);
+    if (!r) r = dev_alloc(c, tp, 4 * (size_t)n);
+    if (!r) r = dev_alloc(c, ti, 4 * (size_t)n);
+    if (!r) r = trace_rays(c, s, n, (const float*)bo.p, (const float*)bd.p, tmin, (const float*)bt.p, any, (float*)tt.p,
+                           (float*)tu.p, (float*)tv.p, (int32_t*)tp.p, any ? nullptr : (int32_t*)ti.p);
+    if (!r) r = hip_check(c, hipMemcpy(prim, tp.p, 4 * (size_t)n, hipMemcpyDeviceToHost), "copy");
+    if (!r && !any) {
+        r = hip_check(c, hipMemcpy(t, tt.p, 4 * (size_t)n, hipMemcpyDeviceToHost), "copy");
+        if (!r) r = hip_check(c, hipMemcpy(u, tu.p, 4 * (size_t)n, hipMemcpyDeviceToHost), "copy");
+        if (!r) r = hip_check(c, hipMemcpy(v, tv.p, 4 * (size_t)n, hipMemcpyDeviceToHost), "copy");
+        if (!r) {
+            if (s->ninst > 0) r = hip_check(c, hipMemcpy(inst, ti.p, 4 * (size_t)n, hipMemcpyDeviceToHost), "copy");
+            else for (int32_t i = 0; i < n; i++) inst[i] = -1;
+        }
+    }
+    DevBuf* all[] = {&bo, &bd, &bt, &tt, &tu, &tv, &tp, &ti};
+    for (DevBuf* b : all) dev_free(c, *b);
+    return r;
+}
+
 int srt_trace_rays_noalloc(srt_ctx* c, srt_scene h, int32_t n, const float* o, const float* d, float tmin,
                            const float* tmax, int32_t any, float* t, float* u, float* v, int32_t* prim) {
     SRT_IDLE(c);
@@ -503,9 +641,7 @@ int srt_render(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_pt_para
     if (!(cam->fov_deg > 0.0f && cam->fov_deg < 180.0f)) return set_err(c, SRT_E_INVALID, "fov must be in (0, 180)");
     if (p->mlp_mode < 0 || p->mlp_mode > 1) return set_err(c, SRT_E_INVALID, "mlp_mode must be 0 or 1");
     if (p->mlp_mode == 1 && !mlp_for_scene(c, h)) return set_err(c, SRT_E_STATE, "mlp_mode 1 needs srt_mlp_create");
-    if (!s->has_bvh) {
-        if (int r = bvh_build(c, s, 4, 0)) return r;
-    }
+    if (int r = ensure_bvh(c, s)) return r;
     if (p->use_nee && s->nlights == 0) {
         // no area light: next-event estimation never fires (light.h sample on
         // an empty CDF is undefined in the reference); render without it
@@ -538,14 +674,13 @@ int srt_render_simple(srt_ctx* c, srt_scene h, const srt_camera* cam, int32_t ty
     if (!c || !cam || !rgba_out) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (s->ninst > 0) return set_err(c, SRT_E_INVALID, "the Simple pass does not take instanced scenes");
     if (W <= 0 || H <= 0) return set_err(c, SRT_E_INVALID, "resolution must be positive");
     if ((int64_t)W * H > (1ll << 28)) return set_err(c, SRT_E_LIMIT, "resolution too large");
     if (spp <= 0) return set_err(c, SRT_E_INVALID, "samples_per_pixel must be positive");
     if (type < 0 || type > 4) return set_err(c, SRT_E_INVALID, "type must be 0..4 (Depth, Normal, BaseColor, Ambient, FaceOrientation)");
     if (!(cam->fov_deg > 0.0f && cam->fov_deg < 180.0f)) return set_err(c, SRT_E_INVALID, "fov must be in (0, 180)");
-    if (!s->has_bvh) {
-        if (int r = bvh_build(c, s, 4, 0)) return r;
-    }
+    if (int r = ensure_bvh(c, s)) return r;
     DevCamera dc = make_camera(cam->position, cam->target, cam->up, cam->fov_deg, W, H);
     const int64_t wh = (int64_t)W * H;
     float* film = nullptr;
@@ -569,6 +704,7 @@ int srt_render_light(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_p
     if (!c || !cam || !p || !rgba_out) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
+    if (s->ninst > 0) return set_err(c, SRT_E_INVALID, "the LightTracer pass does not take instanced scenes");
     if (W <= 0 || H <= 0) return set_err(c, SRT_E_INVALID, "resolution must be positive");
     if ((int64_t)W * H > (1ll << 28)) return set_err(c, SRT_E_LIMIT, "resolution too large");
     if (p->samples_per_pixel <= 0) return set_err(c, SRT_E_INVALID, "samples_per_pixel must be positive");
@@ -576,9 +712,7 @@ int srt_render_light(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_p
     if (p->max_depth < 0 || p->rr_depth < 0) return set_err(c, SRT_E_INVALID, "depths must be >= 0");
     if (!(cam->fov_deg > 0.0f && cam->fov_deg < 180.0f)) return set_err(c, SRT_E_INVALID, "fov must be in (0, 180)");
     if (s->nlights == 0) return set_err(c, SRT_E_INVALID, "the light tracer needs an area light in the scene");
-    if (!s->has_bvh) {
-        if (int r = bvh_build(c, s, 4, 0)) return r;
-    }
+    if (int r = ensure_bvh(c, s)) return r;
     DevCamera dc = make_camera(cam->position, cam->target, cam->up, cam->fov_deg, W, H);
     const int64_t wh = (int64_t)W * H;
     float* film = nullptr;
@@ -605,7 +739,7 @@ int srt_mlp_train_frame(srt_ctx* c, srt_scene h, const srt_camera* cam, const sr
     if (!mlp_for_scene(c, h)) return set_err(c, SRT_E_STATE, "no MLP: call srt_mlp_create first");
     const MlpBind bind(c, h);
     if (p->samples_per_pixel <= 0 || p->max_depth < 3) return set_err(c, SRT_E_INVALID, "training paths need max_depth >= 3");
-    if (!s->has_bvh) if (int r = bvh_build(c, s, 4, 0)) return r;
+    if (int r = ensure_bvh(c, s)) return r;
     if (c->train_cap < max_samples) {
         dev_free(c, c->train_in9);
         dev_free(c, c->train_t);

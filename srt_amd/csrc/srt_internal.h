@@ -95,7 +95,24 @@ struct DevScene {
     int32_t num_lights;
     float light_wsum;
     float env[3];
+    // instanced scenes (num_inst > 0): meshes in object space, the 4-wide
+    // nodes hold every mesh's tree and the top-level tree (root4) over the
+    // instances; transforms as three float4 rows (3x4 row-major)
+    int32_t num_inst;
+    const float4* inst_xf;     // 3 per instance: object -> world
+    const float4* inst_inv;    // 3 per instance: world -> object
+    const int32_t* inst_root;  // per instance: root ref of its mesh's tree in nodes4
+    const int32_t* light_inst; // per light: its instance
 };
+
+// Node refs of the top-level tree of an instanced scene: an instance leaf is
+// kInstLeaf | instance, and the return marker pushed when a ray enters an
+// instance is kInstReturn (triangle leaves of instanced scenes keep bit 30
+// clear: fewer than 2^27 triangles).
+constexpr int32_t kInstLeaf = (int32_t)0xC0000000u;
+constexpr int32_t kInstReturn = -1;
+constexpr int64_t kMaxInstTriangles = (1ll << 27) - 1;
+__host__ __device__ inline bool is_inst_ref(int32_t r) { return r < 0 && (r & 0x40000000); }
 
 // camera in the form the kernels use (computed on the host in float)
 struct DevCamera {
@@ -164,6 +181,7 @@ struct PathQueue {
 struct HitBuf {
     float *t, *u, *v;
     int32_t* tri;                         // sorted triangle position or -1
+    int32_t* inst = nullptr;              // instanced scenes: the instance hit
 };
 struct ShadowQueue {
     float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
@@ -250,6 +268,18 @@ struct SceneRec {
     int32_t max_leaf = 4;
     // host copies needed for light CDF construction
     std::vector<float> h_mesh_area;
+    std::vector<float> h_mesh_box;   // 6 per mesh: object-space lo xyz, hi xyz (at upload)
+    // instances (srt_scene_set_instances): meshes in object space
+    int32_t ninst = 0;
+    std::vector<int32_t> h_inst_mesh;
+    std::vector<float> h_inst_xf, h_inst_inv;   // 12 per instance
+    DevBuf inst_xf, inst_inv, inst_root, light_inst;
+    std::vector<int32_t> h_mesh_root4;          // per mesh: root ref of its tree in nodes4
+    // top-level tree (host-built, refit on the host after transform updates):
+    // nodes4[tlas_first ..] ; child c of top node k is top node or instance ref
+    int32_t tlas_first = 0;
+    std::vector<Bvh4Node> h_tlas;
+    std::vector<int32_t> h_tlas_order;          // top nodes in post order (children first)
 };
 
 struct KernelStat {
@@ -272,7 +302,7 @@ struct srt_ctx {
     // render pool
     int64_t pool_cap = 0;                 // path slots allocated
     int64_t pool_request = 1ll << 27;      // ~184 B per slot: 24.7 GB covers a 1080p x 64 spp frame in one chunk
-    srt::DevBuf q[2][14], hits[4], shq[11], Lbuf[3], counters, film_acc, film_out, gstack;
+    srt::DevBuf q[2][14], hits[5], shq[11], Lbuf[3], counters, film_acc, film_out, gstack;
     int64_t film_pixels = 0;
     size_t gstack_entries = 0;            // int32 entries allocated in gstack
     int32_t trace_lds_depth = 16, trace_refill = 56, shadow_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
@@ -322,6 +352,10 @@ void prof_collect(srt_ctx* c);
 int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size);
 int bvh_refit(srt_ctx* c, SceneRec* s);
 int bvh4_build(srt_ctx* c, SceneRec* s);
+// instanced scenes (instances.hip)
+int inst_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size);
+int inst_update(srt_ctx* c, SceneRec* s, const float* xf);
+bool affine_inverse(const float m[12], float out[12]);
 int skin_vertices(srt_ctx* c, SceneRec* s, const float* bones_3x4, int nb);
 // wavefront integrator (wavefront.hip)
 // training samples produced by a mode-2 render
@@ -354,7 +388,7 @@ struct MlpBind {
     ~MlpBind() { c->mlp = prev; }
 };
 int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin, const float* tmax_dev,
-               int any, float* t, float* u, float* v, int32_t* prim);
+               int any, float* t, float* u, float* v, int32_t* prim, int32_t* inst = nullptr);
 int noalloc_init(srt_ctx* c);
 int trace_rays_noalloc(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin,
                        const float* tmax_dev, int any, float* t, float* u, float* v, int32_t* prim);

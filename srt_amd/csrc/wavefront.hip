@@ -118,6 +118,7 @@ struct TravState {
     RayPre rp;
     float tmin, best_t, bu, bv;
     int32_t best_prim, best_pos;
+    int32_t inst, best_inst;   // instanced scenes: the instance the ray is in (-1: world), the one hit
     int32_t cur;       // next node (>= 0) or leaf (< 0) to visit; valid while cur_ok
     int32_t pend;      // postponed leaf; valid while has_pend
     int32_t sp;        // stack depth
@@ -126,7 +127,9 @@ struct TravState {
     uint32_t onx, ony, onz;   // byte offsets of the near planes in a Bvh4Node (far plane: offset ^ 16)
 };
 
-SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root) {
+// The ray a lane traverses with: origin, reciprocal direction, the
+// watertight test's shear, the direction signs that pick near/far planes.
+SRT_D void set_ray(TravState& st, f3 o, f3 d) {
     st.o = o;
     st.rp = ray_pre(d);
     st.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -136,10 +139,34 @@ SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t
     st.onx = st.sx ? 16u : 0u;    // lox 0, hix 16, loy 32, hiy 48, loz 64, hiz 80
     st.ony = st.sy ? 48u : 32u;
     st.onz = st.sz ? 80u : 64u;
+}
+
+// Traversal kernels of instanced scenes keep each lane's world ray in LDS
+// (ahead of the stack: 6 floats per lane) so a ray leaving an instance gets
+// its world ray back exactly.
+extern __shared__ int32_t trav_lds[];    // dynamic LDS of the traversal kernels
+constexpr int kRayStash = 6 * kTraceBlock;   // int32 entries of the stash
+SRT_D void stash_ray(f3 o, f3 d) {
+    float* r = reinterpret_cast<float*>(trav_lds) + threadIdx.x;
+    r[0] = o.x; r[kTraceBlock] = o.y; r[2 * kTraceBlock] = o.z;
+    r[3 * kTraceBlock] = d.x; r[4 * kTraceBlock] = d.y; r[5 * kTraceBlock] = d.z;
+}
+SRT_D void unstash_ray(f3& o, f3& d) {
+    const float* r = reinterpret_cast<const float*>(trav_lds) + threadIdx.x;
+    o = mk(r[0], r[kTraceBlock], r[2 * kTraceBlock]);
+    d = mk(r[3 * kTraceBlock], r[4 * kTraceBlock], r[5 * kTraceBlock]);
+}
+
+template <bool INST = false>
+SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root) {
+    set_ray(st, o, d);
+    if (INST) stash_ray(o, d);
     st.tmin = tmin;
     st.best_t = tmax;
     st.best_prim = 0x7fffffff;
     st.best_pos = -1;
+    st.inst = -1;
+    st.best_inst = -1;
     st.bu = 0.0f; st.bv = 0.0f;
     st.cur = root;
     st.pend = 0;
@@ -152,7 +179,6 @@ SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t
 // (lane-interleaved: entry e of thread t at lds[e * blockDim + t]); deeper
 // entries go to a per-thread global area (entry e at gst[(e - lds_depth) *
 // gstride]).  The build guarantees a tree depth <= kMaxDepth.
-extern __shared__ int32_t trav_lds[];    // dynamic LDS of the traversal kernels: lds_depth * blockDim entries
 // The traversal kernels run kTraceBlock threads per workgroup, so entry sp of a
 // lane sits at trav_lds[sp * kTraceBlock + threadIdx.x] (a shift, no multiply).
 struct TravStack {
@@ -272,11 +298,44 @@ SRT_D int4 ld4i(__amdgpu_buffer_rsrc_t r, uint32_t off) {
 // loop once at most `inner_exit` of its lanes still search for a first leaf:
 // those lanes keep their state and continue after the leaf step, instead of
 // every other lane waiting for them.
-template <bool COUNT>
+// Instanced scenes (INST): a top-level leaf kInstLeaf | i moves the lane's ray
+// into instance i's object space (inverse transform, applied as
+// apply_point / apply_vector in float) and pushes kInstReturn before the
+// mesh tree's root; popping kInstReturn restores the world ray from the LDS
+// stash, after the lane's postponed leaf (which lies in that instance) has
+// been tested.
+SRT_D f3 xf_point(float4 r0, float4 r1, float4 r2, f3 p) {
+    return mk(r0.x * p.x + r0.y * p.y + r0.z * p.z + r0.w, r1.x * p.x + r1.y * p.y + r1.z * p.z + r1.w,
+              r2.x * p.x + r2.y * p.y + r2.z * p.z + r2.w);
+}
+SRT_D f3 xf_vector(float4 r0, float4 r1, float4 r2, f3 v) {
+    return mk(r0.x * v.x + r0.y * v.y + r0.z * v.z, r1.x * v.x + r1.y * v.y + r1.z * v.z,
+              r2.x * v.x + r2.y * v.y + r2.z * v.z);
+}
+template <bool COUNT, bool INST = false>
 SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nv, int inner_exit) {
     const __amdgpu_buffer_rsrc_t rsrc = node4_rsrc(S);
     while (st.cur_ok) {
-        if (st.cur < 0) {
+        if (INST && is_inst_ref(st.cur)) {
+            f3 o, d;
+            if (st.cur == kInstReturn) {
+                if (st.has_pend) break;           // test the postponed leaf in this instance's space first
+                unstash_ray(o, d);
+                set_ray(st, o, d);
+                st.inst = -1;
+                stk_pop(k, st);
+            } else {
+                const int32_t i = st.cur & 0x3fffffff;
+                unstash_ray(o, d);
+                const float4* M = S.inst_inv + 3 * (size_t)i;
+                const float4 r0 = M[0], r1 = M[1], r2 = M[2];
+                set_ray(st, xf_point(r0, r1, r2, o), xf_vector(r0, r1, r2, d));
+                st.inst = i;
+                stk_push(k, st.sp, kInstReturn);
+                st.sp++;
+                st.cur = S.inst_root[i];
+            }
+        } else if (st.cur < 0) {
             if (st.has_pend) break;
             st.pend = st.cur;
             st.has_pend = true;
@@ -319,7 +378,7 @@ SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uin
 // The triangles of the postponed leaf.  Closest hit: (t, primitive id)
 // lexicographic minimum, so the result does not depend on the visiting order.
 // Any hit: the first hit ends the traversal; returns true.
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, bool INST = false>
 SRT_D bool trav_leaf(const DevScene& S, TravState& st, uint32_t& nt) {
     st.has_pend = false;
     const uint32_t f = leaf_first(st.pend), c = leaf_count(st.pend);
@@ -336,8 +395,11 @@ SRT_D bool trav_leaf(const DevScene& S, TravState& st, uint32_t& nt) {
             return true;
         }
         const int32_t prim = __float_as_int(A.w);
-        if (t < st.best_t || (t == st.best_t && prim < st.best_prim)) {
+        // (t, primitive, instance) lexicographic: instances of one mesh share primitive ids
+        if (t < st.best_t ||
+            (t == st.best_t && (prim < st.best_prim || (INST && prim == st.best_prim && st.inst < st.best_inst)))) {
             st.best_t = t; st.best_prim = prim; st.best_pos = (int32_t)j; st.bu = u; st.bv = v;
+            if (INST) st.best_inst = st.inst;
         }
     }
     return false;
@@ -357,9 +419,10 @@ SRT_D bool trav_leaf(const DevScene& S, TravState& st, uint32_t& nt) {
 //   void store(uint32_t i, const TravState&, bool occluded)   (on the storing lanes)
 // Every wave leaves the loop once `fetch` has passed hi - lo and its lanes
 // have finished and stored.
-template <bool ANY, bool COUNT, bool WIDE, class IO, class View>
+template <bool ANY, bool COUNT, bool WIDE, class IO, class View, bool INST = false>
 SRT_D void trace_persistent(const DevScene& S, const View& view, uint32_t lo, uint32_t hi, uint32_t* fetch, int refill,
                             const TravStack& k, IO& io, uint32_t& nv, uint32_t& nt) {
+    static_assert(!INST || WIDE, "instanced scenes are traversed in the 4-wide form");
     const int32_t root = WIDE ? S.root4 : S.root;
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -387,7 +450,7 @@ SRT_D void trace_persistent(const DevScene& S, const View& view, uint32_t lo, ui
                     ri = view.entry(idx);
                     has = true;
                     occl = false;
-                    io.load(ri, st, root);
+                    io.template load<INST>(ri, st, root);
                 } else {
                     exhausted = true;
                 }
@@ -396,9 +459,9 @@ SRT_D void trace_persistent(const DevScene& S, const View& view, uint32_t lo, ui
         }
         if (act == 0ull) break;
         if (has) {
-            if (WIDE) trav_inner4<COUNT>(S, st, k, nv, k.inner_exit);
+            if (WIDE) trav_inner4<COUNT, INST>(S, st, k, nv, k.inner_exit);
             else trav_inner<COUNT>(S, st, k, nv);
-            if (st.has_pend) occl = trav_leaf<ANY, COUNT>(S, st, nt);
+            if (st.has_pend) occl = trav_leaf<ANY, COUNT, INST>(S, st, nt);
             if (occl || (!st.cur_ok && !st.has_pend)) {
                 has = false;
                 done = true;
@@ -547,8 +610,9 @@ __global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* L
 struct QueueIO {       // closest hit of queue entries -> hit buffer (same index)
     PathQueue q;
     HitBuf h;
+    template <bool INST>
     SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
-        trav_begin(st, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), 1e-3f, 1e16f, root);
+        trav_begin<INST>(st, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), 1e-3f, 1e16f, root);
     }
     SRT_D void store(uint32_t i, const TravState& st, bool) const {
         const bool hit = st.best_pos >= 0;
@@ -556,6 +620,7 @@ struct QueueIO {       // closest hit of queue entries -> hit buffer (same index
         h.u[i] = hit ? st.bu : 0.0f;
         h.v[i] = hit ? st.bv : 0.0f;
         h.tri[i] = hit ? st.best_pos : -1;
+        if (h.inst) h.inst[i] = hit ? st.best_inst : -1;
     }
 };
 // L[slot] += c for the three channels; the three loads go out before the stores
@@ -569,8 +634,9 @@ SRT_D void add_rgb(float* Lr, float* Lg, float* Lb, uint32_t slot, float r, floa
 struct ShadowIO {      // any hit in [1e-3, tmax]; unoccluded rays add their contribution
     ShadowQueue sq;
     float *Lr, *Lg, *Lb;
+    template <bool INST>
     SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
-        trav_begin(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root);
+        trav_begin<INST>(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root);
     }
     SRT_D void store(uint32_t i, const TravState&, bool occluded) const {
         if (occluded) return;
@@ -579,10 +645,12 @@ struct ShadowIO {      // any hit in [1e-3, tmax]; unoccluded rays add their con
     }
 };
 
+// INST: the LDS stack sits after the world-ray stash
+template <bool INST = false>
 SRT_D TravStack make_stack(int32_t* gstack, TraceCfg cfg) {
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     TravStack k;
-    k.lane0 = (int)threadIdx.x;
+    k.lane0 = (int)threadIdx.x + (INST ? kRayStash : 0);
     k.gst = gstack + gtid;
     k.gstride = (size_t)gridDim.x * blockDim.x;
     k.lds_depth = cfg.lds_depth;
@@ -594,7 +662,7 @@ SRT_D TravStack make_stack(int32_t* gstack, TraceCfg cfg) {
 // following shade and shadow stages append to or fetch from.  COUNT: also
 // total node visits and triangle tests (an untimed pass used for the
 // algorithmic byte count).
-template <bool COUNT, bool WIDE>
+template <bool COUNT, bool WIDE, bool INST = false>
 __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
                                                       uint32_t cap, int32_t* gstack, unsigned long long* totals,
                                                       TraceCfg cfg) {
@@ -610,10 +678,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, 
         if (threadIdx.x == 0) counts[CNT_MLPQ] = 0u;
     }
     if (blockIdx.x < (unsigned)kParts && threadIdx.x == 0) atomicAdd(&totals[0], (unsigned long long)(hi - lo));
-    const TravStack k = make_stack(gstack, cfg);
+    const TravStack k = make_stack<INST>(gstack, cfg);
     QueueIO io{q, h};
     uint32_t nv = 0, nt = 0;
-    trace_persistent<false, COUNT, WIDE>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
+    trace_persistent<false, COUNT, WIDE, QueueIO, QueueView, INST>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
     if (COUNT) {
         atomicAdd(&totals[2], (unsigned long long)nv);
         atomicAdd(&totals[3], (unsigned long long)nt);
@@ -647,6 +715,13 @@ SRT_D LightSample light_sample(const DevScene& S, f2 rnd) {
         if (dot(normal, sn) < 0.0f) normal = -normal;
     }
     ls.normal = normal;
+    if (S.num_inst > 0) {
+        // GInstance::sample (gmesh.h:77-78): apply_point / apply_vector, no renormalising
+        const float4* T = S.inst_xf + 3 * (size_t)S.light_inst[idx];
+        const float4 r0 = T[0], r1 = T[1], r2 = T[2];
+        ls.pos = xf_point(r0, r1, r2, ls.pos);
+        ls.normal = xf_vector(r0, r1, r2, ls.normal);
+    }
     const MatRec& m = S.mats[S.mesh_mat[mesh]];
     ls.emission = mk(m.emission_color[0], m.emission_color[1], m.emission_color[2]) * m.intensity;
     ls.pdf = m.intensity / S.light_wsum;
@@ -745,7 +820,16 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     const uint32_t segk = flags >> 8;
     const SegHead P = load_seg_head(segs, segk);
     const float u = h.u[i], v = h.v[i];
-    const HitGeom hg = hit_geometry(S, tri, u, v);
+    HitGeom hg = hit_geometry(S, tri, u, v);
+    float4 X0, X1, X2;         // instance transform (instanced scenes)
+    if (S.num_inst > 0) {
+        // hit_info.h:58-62: position by apply_point, geometric normal by
+        // apply_vector (not renormalised); the shading frame below axis by axis
+        const float4* T = S.inst_xf + 3 * (size_t)h.inst[i];
+        X0 = T[0]; X1 = T[1]; X2 = T[2];
+        hg.pos = xf_point(X0, X1, X2, hg.pos);
+        hg.normal = xf_vector(X0, X1, X2, hg.normal);
+    }
     const float w = hg.w;
     const f3 pos = hg.pos, sn = hg.sn;
     f3 normal = hg.normal;
@@ -773,7 +857,14 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
         return;
     }
     const f3 tg = hit_tangent(S, tri, hg, u, v);
-    const ShadePoint spt = shade_point(m, -rd, make_onb(sn, tg), color);
+    Onb onb = make_onb(sn, tg);
+    if (S.num_inst > 0) {
+        // Transform::apply_onb (transform.h:72-75)
+        onb.z = xf_vector(X0, X1, X2, onb.z);
+        onb.x = xf_vector(X0, X1, X2, onb.x);
+        onb.y = xf_vector(X0, X1, X2, onb.y);
+    }
+    const ShadePoint spt = shade_point(m, -rd, onb, color);
     if (dot(normal, -rd) < 0.0f) normal = -normal;
     const bool last = (depth + 1 >= P.max_depth);
     const int64_t pid = P.path_begin + (int64_t)(slot - (uint32_t)P.slot_offset);
@@ -936,7 +1027,7 @@ __global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segm
 }
 
 // Shadow rays of the bounce (fetch counter cleared by the trace stage).
-template <bool COUNT, bool WIDE>
+template <bool COUNT, bool WIDE, bool INST = false>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue sq, float* Lr, float* Lg, float* Lb,
                                                        uint32_t* counts, uint32_t cap, int32_t* gstack,
                                                        unsigned long long* totals, TraceCfg cfg) {
@@ -945,10 +1036,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, ShadowQueue 
     const QueueView view = queue_piece(counts, CNT_SHADOW, cap, prefix, lo, hi);
     uint32_t* fetch = counts + CNT_FETCH_S + my_part() * kCntLine;
     if (blockIdx.x < (unsigned)kParts && threadIdx.x == 0) atomicAdd(&totals[1], (unsigned long long)(hi - lo));
-    const TravStack k = make_stack(gstack, cfg);
+    const TravStack k = make_stack<INST>(gstack, cfg);
     ShadowIO io{sq, Lr, Lg, Lb};
     uint32_t nv = 0, nt = 0;
-    trace_persistent<true, COUNT, WIDE>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
+    trace_persistent<true, COUNT, WIDE, ShadowIO, QueueView, INST>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
     if (COUNT) {
         atomicAdd(&totals[4], (unsigned long long)nv);
         atomicAdd(&totals[5], (unsigned long long)nt);
@@ -1049,10 +1140,12 @@ struct RaysIO {        // standalone queries on flat arrays
     float tmin;
     float *t, *u, *v;
     int32_t* prim;
+    int32_t* inst = nullptr;   // instanced scenes: the instance hit (may be null)
     const float4* tris;
     int any;
+    template <bool INST>
     SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
-        trav_begin(st, ld3(o3 + 3 * (size_t)i), ld3(d3 + 3 * (size_t)i), tmin, tmax[i], root);
+        trav_begin<INST>(st, ld3(o3 + 3 * (size_t)i), ld3(d3 + 3 * (size_t)i), tmin, tmax[i], root);
     }
     SRT_D void store(uint32_t i, const TravState& st, bool occluded) const {
         if (any) {
@@ -1064,16 +1157,21 @@ struct RaysIO {        // standalone queries on flat arrays
         u[i] = hit ? st.bu : 0.0f;
         v[i] = hit ? st.bv : 0.0f;
         prim[i] = hit ? __float_as_int(tris[3 * (size_t)st.best_pos].w) : -1;
+        if (inst) inst[i] = hit ? st.best_inst : -1;
     }
 };
 
+template <bool INST = false>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, uint32_t n, RaysIO io, uint32_t* fetch,
                                                            int32_t* gstack, TraceCfg cfg) {
-    const TravStack k = make_stack(gstack, cfg);
+    const TravStack k = make_stack<INST>(gstack, cfg);
     uint32_t nv = 0, nt = 0, lo, hi;
     fetch = dense_range(n, fetch, lo, hi);
     const IdentityEntry view;
-    if (cfg.wide) {
+    if (INST) {
+        if (io.any) trace_persistent<true, false, true, RaysIO, IdentityEntry, true>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
+        else trace_persistent<false, false, true, RaysIO, IdentityEntry, true>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
+    } else if (cfg.wide) {
         if (io.any) trace_persistent<true, false, true>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
         else trace_persistent<false, false, true>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
     } else {
@@ -1088,11 +1186,12 @@ struct SimpleIO {
     int width, height;
     uint32_t seed, sample;
     HitBuf h;
+    template <bool INST>
     SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
         f3 ro, rd;
         Pcg32 rng;
         camera_ray(cam, width, height, i, seed, sample, ro, rd, rng);
-        trav_begin(st, ro, rd, 1e-3f, 1e16f, root);
+        trav_begin<INST>(st, ro, rd, 1e-3f, 1e16f, root);
     }
     SRT_D void store(uint32_t i, const TravState& st, bool) const {
         const bool hit = st.best_pos >= 0;
@@ -1328,8 +1427,9 @@ __global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int de
 struct LtShadowIO {    // camera connections: unoccluded ones splat their contribution
     ShadowQueue sq;
     float4* film;
+    template <bool INST>
     SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
-        trav_begin(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root);
+        trav_begin<INST>(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root);
     }
     SRT_D void store(uint32_t i, const TravState&, bool occluded) const {
         if (!occluded) splat(film, (int)sq.slot[i], mk(sq.cr[i], sq.cg[i], sq.cb[i]));
@@ -1469,6 +1569,7 @@ static int num_cus(srt_ctx* c) {
 // The 4-wide tree is traversed when the context asks for it and the scene
 // has one (a tree beyond 32-bit node offsets keeps only its binary form).
 static bool use_wide(const srt_ctx* c, const SceneRec* s) {
+    if (s && s->ninst > 0) return true;      // instanced scenes hold only the 4-wide form
     return c->trace_bvh_width == 4 && (s == nullptr || s->num_nodes4 > 0 || s->root < 0);
 }
 
@@ -1481,7 +1582,9 @@ static TraceCfg trace_cfg(srt_ctx* c, const SceneRec* s) {
     return cfg;
 }
 
-static size_t trace_lds_bytes(const TraceCfg& cfg) { return (size_t)cfg.lds_depth * kTraceBlock * sizeof(int32_t); }
+static size_t trace_lds_bytes(const TraceCfg& cfg, bool inst = false) {
+    return ((size_t)cfg.lds_depth * kTraceBlock + (inst ? kRayStash : 0)) * sizeof(int32_t);
+}
 
 // Persistent traversal grid: as many workgroups as are resident at once
 // (occupancy of the traversal kernel with its LDS stack), capped by the work.
@@ -1541,7 +1644,7 @@ static int ensure_pool(srt_ctx* c, int64_t slots) {
             if (bytes && dev_alloc(c, c->q[k][j], bytes)) return SRT_E_OOM;
         }
     }
-    for (int j = 0; j < 4; j++) if (dev_alloc(c, c->hits[j], 4 * qn)) return SRT_E_OOM;
+    for (int j = 0; j < 5; j++) if (dev_alloc(c, c->hits[j], 4 * qn)) return SRT_E_OOM;   // 4: instance hit
     for (int j = 0; j < 11; j++) if (dev_alloc(c, c->shq[j], 4 * qn)) return SRT_E_OOM;
     for (int j = 0; j < 3; j++) if (dev_alloc(c, c->Lbuf[j], 4 * n)) return SRT_E_OOM;
     c->pool_cap = slots;
@@ -1662,8 +1765,11 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     uint32_t* counts = (uint32_t*)c->counters.p;
     unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + kTotalsOffset);
     PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
+    const bool inst = s->ninst > 0;
     HitBuf H;
     H.t = (float*)c->hits[0].p; H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
+    H.inst = inst ? (int32_t*)c->hits[4].p : nullptr;
+    const size_t ilds = trace_lds_bytes(tcfg, true);
     ShadowQueue SQ;
     {
         auto f = [&](int j) { return (float*)c->shq[j].p; };
@@ -1682,7 +1788,11 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     for (int depth = 0; depth < max_depth; depth++) {
         int qi = depth & 1;
         prof_begin(c, "trace", &ev);
-        if (tcfg.wide) {
+        if (inst) {
+            if (c->count_visits)
+                k_trace<true, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+            else k_trace<false, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+        } else if (tcfg.wide) {
             if (c->count_visits)
                 k_trace<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
             else k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
@@ -1697,7 +1807,11 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
         prof_end(c, "shade", ev, 0);
         if (any_nee) {
             prof_begin(c, "shadow", &ev);
-            if (tcfg.wide) {
+            if (inst) {
+                if (c->count_visits)
+                    k_shadow<true, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, SQ, Lr, Lg, Lb, counts, cap, gst, totals, scfg);
+                else k_shadow<false, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, SQ, Lr, Lg, Lb, counts, cap, gst, totals, scfg);
+            } else if (tcfg.wide) {
                 if (c->count_visits)
                     k_shadow<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, cap, gst, totals, scfg);
                 else k_shadow<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, Lr, Lg, Lb, counts, cap, gst, totals, scfg);
@@ -1898,7 +2012,7 @@ int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max
 }
 
 int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* d_dev, float tmin, const float* tmax_dev,
-               int any, float* t, float* u, float* v, int32_t* prim) {
+               int any, float* t, float* u, float* v, int32_t* prim, int32_t* inst) {
     DevScene S = dev_scene(s);
     int grid = trace_grid(c, n);
     if (int r = ensure_gstack(c, s, (int64_t)grid * kTraceBlock)) return r;
@@ -1906,10 +2020,15 @@ int trace_rays(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const float* 
     RaysIO io;
     io.o3 = o_dev; io.d3 = d_dev; io.tmax = tmax_dev; io.tmin = tmin;
     io.t = t; io.u = u; io.v = v; io.prim = prim; io.tris = S.tris; io.any = any;
+    io.inst = inst;
     uint32_t* fetch = (uint32_t*)c->counters.p + CNT_FETCH_Q;
     k_clear_parts<<<1, kParts, 0, c->stream>>>(fetch);
-    k_trace_rays<<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch, (int32_t*)c->gstack.p,
-                                                                         cfg);
+    if (s->ninst > 0)
+        k_trace_rays<true><<<grid, kTraceBlock, trace_lds_bytes(cfg, true), c->stream>>>(S, (uint32_t)n, io, fetch,
+                                                                                       (int32_t*)c->gstack.p, cfg);
+    else
+        k_trace_rays<false><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch,
+                                                                                  (int32_t*)c->gstack.p, cfg);
     if (int r = hip_check(c, hipGetLastError(), "trace launch")) return r;
     return hip_check(c, hipStreamSynchronize(c->stream), "trace");
 }
@@ -1973,7 +2092,10 @@ int trace_rays_noalloc(srt_ctx* c, SceneRec* s, int n, const float* o_dev, const
     uint32_t* fetch = (uint32_t*)base;
     int32_t* gstack = (int32_t*)((unsigned char*)base + kNoallocCounterBytes);
     k_clear_parts<<<1, kParts, 0, c->stream>>>(fetch);
-    k_trace_rays<<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch, gstack, cfg);
+    if (s->ninst > 0)
+        k_trace_rays<true><<<grid, kTraceBlock, trace_lds_bytes(cfg, true), c->stream>>>(S, (uint32_t)n, io, fetch, gstack, cfg);
+    else
+        k_trace_rays<false><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch, gstack, cfg);
     if (int r = hip_check(c, hipGetLastError(), "trace launch")) return r;
     return hip_check(c, hipStreamSynchronize(c->stream), "trace");
 }
