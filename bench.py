@@ -590,7 +590,7 @@ def mlp_mode(args):
     print(json.dumps(out), flush=True)
 
 
-def serve_load(rate_sat=200.0, window_sat=30.0, window_sub=30.0, seed=20261202, path_pool=0):
+def serve_load(rate_sat=200.0, window_sat=30.0, window_sub=30.0, seed=20261202, path_pool=0, serve_age=None):
     """Continuous-batching serving under Poisson arrivals over 8 resident
     scenes (3 Cornell variants, 3 1M-triangle sphere scenes with different
     generator seeds, 2 posed skinned tubes); resolution uniform from {256^2,
@@ -605,6 +605,8 @@ def serve_load(rate_sat=200.0, window_sat=30.0, window_sub=30.0, seed=20261202, 
     from srt_amd import abi, scenes
     rng = np.random.default_rng(seed)
     ctx = abi.Context(0)
+    if serve_age is not None:
+        ctx.set_option("serve_age", serve_age)
     sc = []
     for fov in (40.0, 35.0, 45.0):
         sc.append(scenes.cornell(fov=fov))
@@ -693,7 +695,7 @@ def serve_load(rate_sat=200.0, window_sat=30.0, window_sub=30.0, seed=20261202, 
 
 def serve_mode(args):
     """The serving workload alone, with long windows (default 30 s each)."""
-    out = serve_load(args.serve_rate, args.serve_window, args.serve_window, args.serve_seed)
+    out = serve_load(args.serve_rate, args.serve_window, args.serve_window, args.serve_seed, serve_age=args.serve_age)
     line = {"metric": "served Mpaths/s at saturation; p50/p99 request latency below saturation",
             "value": out["saturation"]["served_Mpaths_s"], "unit": "Mpaths/s", "n_gpus": 1,
             "higher_is_better": True, "dtype": "f32", "data": "synthetic",
@@ -810,6 +812,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-window", type=float, default=30.0)
     ap.add_argument("--serve-seed", type=int, default=20261202)
+    ap.add_argument("--serve-age", type=int, default=None,
+                    help="serving order weight of waiting time, per mille (library default when absent)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default line")
     ap.add_argument("--serve-leg-window", type=float, default=5.0,
                     help="seconds of each Poisson window of the default line's serving leg")
@@ -818,6 +822,9 @@ def main():
     ap.add_argument("--valu-summary", metavar="DIR")
     ap.add_argument("--kstats-mode", choices=["frame", "passes", "skin"], default="frame")
     ap.add_argument("--mlp-summary", nargs=2, metavar="DIR")
+    ap.add_argument("--scene", choices=["spheres", "spheres_instanced"], default="spheres",
+                    help="frame mode: the flat 1M-triangle field, or the same field as 16 meshes under 3129 "
+                         "instances (two-level tree)")
     args = ap.parse_args()
     if args.mlp_summary:
         mlp_summary(*args.mlp_summary)
@@ -845,8 +852,8 @@ def main():
 
     from srt_amd import abi, scenes
 
-    log("generating the 1M-triangle scene")
-    s = scenes.spheres()
+    log(f"generating the 1M-triangle scene ({args.scene})")
+    s = scenes.spheres() if args.scene == "spheres" else scenes.spheres_instanced()
     R = s.render
     W, H, spp, depth = R["width"], R["height"], R["spp"], R["max_depth"]
     paths = W * H * spp
@@ -907,7 +914,8 @@ def main():
                  + SHADOW_IN * shadow_rays,
     }
     kern = max((k for k in st if k in bytes_step), key=lambda k: st[k]["ms"])
-    roofline = frame_roofline(kern, st, args.steps, bytes_step, segments)
+    # the VALU and traffic counters behind the roofline were collected on the flat field
+    roofline = frame_roofline(kern, st, args.steps, bytes_step, segments) if args.scene == "spheres" else None
 
     cpu = None
     if not args.no_cpu:
@@ -938,16 +946,21 @@ def main():
     ctx.close()
     serving = None
     if not args.no_serve:
-        serving = serve_load(args.serve_rate, args.serve_leg_window, args.serve_leg_window, args.serve_seed)
+        serving = serve_load(args.serve_rate, args.serve_leg_window, args.serve_leg_window, args.serve_seed,
+                             serve_age=args.serve_age)
 
     lat_sorted = sorted(lat)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "spheres-1M: 3125 frequency-4 icospheres (1,000,000 triangles) + 4 quad lights, "
-                               "Disney, 1920x1080, 64 spp, max depth 8, NEE+MIS",
-                   "width": W, "height": H, "spp": spp, "max_depth": depth, "triangles": s.num_triangles,
+        "config": {"workload": ("spheres-1M: 3125 frequency-4 icospheres (1,000,000 triangles) + 4 quad lights, "
+                                "Disney, 1920x1080, 64 spp, max depth 8, NEE+MIS") if args.scene == "spheres" else
+                               ("spheres-1M instanced: 16 frequency-4 icosphere meshes under 3125 instances "
+                                "(1,000,000 world triangles) + 4 quad-light instances, Disney, 1920x1080, 64 spp, "
+                                "max depth 8, NEE+MIS"),
+                   "width": W, "height": H, "spp": spp, "max_depth": depth,
+                   "triangles": s.num_triangles if args.scene == "spheres" else 3125 * 320 + 8,
                    "paths_per_step": paths, "parallelism": "single-gpu"},
         "latency_ms": (serving["latency_ms"] if serving else
                        {"p50": round(float(np.percentile(lat_sorted, 50)), 2),

@@ -1034,6 +1034,40 @@ double orc_render(void* scene, void* bvh, const float* cam_pos, const float* cam
     return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// The scene box (aabb.h): the vertex box of a world-space scene; for an
+// instanced one, Scene::compute_aabb (scene.cu:299-307): the union of each
+// instance's mesh vertex box, its eight corners through apply_point
+// (Transform::apply_aabb, transform.h:58-70), in float.
+static void scene_box(const Scene& s, float lo[3], float hi[3]) {
+    if (!s.instanced()) {
+        for (int a = 0; a < 3; a++) { lo[a] = s.pos[a]; hi[a] = s.pos[a]; }
+        for (int i = 0; i < s.nv; i++)
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], s.pos[3 * (size_t)i + a]);
+                hi[a] = std::max(hi[a], s.pos[3 * (size_t)i + a]);
+            }
+        return;
+    }
+    for (int a = 0; a < 3; a++) { lo[a] = INFINITY; hi[a] = -INFINITY; }
+    for (int i = 0; i < s.ninst(); i++) {
+        const int m = s.inst_mesh[i];
+        float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int f = s.mesh_first[m]; f < s.mesh_first[m + 1]; f++)
+            for (int k = 0; k < 3; k++) {
+                const float* v = &s.pos[3 * (size_t)s.idx[3 * (size_t)f + k]];
+                for (int a = 0; a < 3; a++) { bl[a] = std::min(bl[a], v[a]); bh[a] = std::max(bh[a], v[a]); }
+            }
+        if (!(bl[0] <= bh[0])) continue;                  // a mesh without triangles
+        const float* T = &s.inst_xf[12 * (size_t)i];
+        for (int c = 0; c < 8; c++) {
+            const float p[3] = {(c & 4) ? bh[0] : bl[0], (c & 2) ? bh[1] : bl[1], (c & 1) ? bh[2] : bl[2]};
+            float w[3];
+            xf_point(T, p, w);
+            for (int a = 0; a < 3; a++) { lo[a] = std::fmin(lo[a], w[a]); hi[a] = std::fmax(hi[a], w[a]); }
+        }
+    }
+}
+
 // Simple NPR pass, float (shader/npr/simple.cu:42-97, passes/npr/simple.cu:14-33):
 // samples 0..spp-1 of each pixel take the path tracer's camera ray (same
 // stream), the closest hit, and one first-hit colouring (type 0 Depth, 1
@@ -1047,12 +1081,7 @@ void orc_render_simple(void* scene, void* bvh, const float* cam_pos, const float
     Tracer tr{&s, bvh ? &((OrcBvh*)bvh)->b : nullptr, bvh ? &((OrcBvh*)bvh)->blas : nullptr};
     Camera cam = make_camera(cam_pos, cam_target, cam_up, fov, W, H);
     float lo[3], hi[3];
-    for (int a = 0; a < 3; a++) { lo[a] = s.pos[a]; hi[a] = s.pos[a]; }
-    for (int i = 0; i < s.nv; i++)
-        for (int a = 0; a < 3; a++) {
-            lo[a] = std::min(lo[a], s.pos[3 * (size_t)i + a]);
-            hi[a] = std::max(hi[a], s.pos[3 * (size_t)i + a]);
-        }
+    scene_box(s, lo, hi);
     bool inside = true;
     V3<float> far3, near3;
     float fa[3], na[3];
@@ -1087,7 +1116,7 @@ void orc_render_simple(void* scene, void* bvh, const float* cam_pos, const float
             V3<float> L(1.0f);
             if (tr.closest(of, df, 1e-3f, 1e16f, &h)) {
                 HitInfo<float> info;
-                get_hitinfo<float>(s, h.prim, h.u, h.v, info);
+                get_hitinfo<float>(s, h.prim, h.u, h.v, info, h.inst);
                 if (type == 0) {
                     const float d = length(info.pos - CP);
                     const float ratio = (d - min_depth) / (max_depth - min_depth);
@@ -1218,7 +1247,7 @@ void orc_render_light(void* scene, void* bvh, const float* cam_pos, const float*
                 Hit h;
                 bool hit = tr.closest(of, df, 1e-3f, 1e16f, &h);
                 HitInfo<float> info;
-                if (hit) get_hitinfo<float>(s, h.prim, h.u, h.v, info);
+                if (hit) get_hitinfo<float>(s, h.prim, h.u, h.v, info, h.inst);
                 if (specular) {
                     V3<float> to_camera = CP - ro;
                     const float t = length(to_camera);

@@ -181,7 +181,6 @@ double scene_scale(const SceneRec* s) {
 std::vector<TItem> inst_items(SceneRec* s) {
     const double scale = scene_scale(s);
     std::vector<TItem> it(s->ninst);
-    for (int i = 0; i < 3; i++) { s->bounds_lo[i] = INFINITY; s->bounds_hi[i] = -INFINITY; }
     for (int i = 0; i < s->ninst; i++) {
         TItem& t = it[i];
         t.ref = kInstLeaf | i;
@@ -191,11 +190,7 @@ std::vector<TItem> inst_items(SceneRec* s) {
             continue;
         }
         inst_box(&s->h_inst_xf[12 * (size_t)i], ob, scale, t.lo, t.hi);
-        for (int a = 0; a < 3; a++) {
-            t.c[a] = 0.5f * (t.lo[a] + t.hi[a]);
-            s->bounds_lo[a] = std::min(s->bounds_lo[a], t.lo[a]);
-            s->bounds_hi[a] = std::max(s->bounds_hi[a], t.hi[a]);
-        }
+        for (int a = 0; a < 3; a++) t.c[a] = 0.5f * (t.lo[a] + t.hi[a]);
     }
     return it;
 }
@@ -229,6 +224,26 @@ void tlas_refit(SceneRec* s, const std::vector<TItem>& it) {
 }
 
 }  // namespace
+
+// Scene box of an instanced scene (Scene::compute_aabb, scene.cu:299-307, with
+// Transform::apply_aabb, transform.h:58-70): the union of each instance's
+// mesh-box corners through apply_point, in float.
+void inst_scene_box(SceneRec* s) {
+    for (int a = 0; a < 3; a++) { s->bounds_lo[a] = INFINITY; s->bounds_hi[a] = -INFINITY; }
+    for (int i = 0; i < s->ninst; i++) {
+        const float* M = &s->h_inst_xf[12 * (size_t)i];
+        const float* ob = &s->h_mesh_box[6 * (size_t)s->h_inst_mesh[i]];
+        if (box_empty(ob)) continue;
+        for (int k = 0; k < 8; k++) {
+            const float x = (k & 4) ? ob[3] : ob[0], y = (k & 2) ? ob[4] : ob[1], z = (k & 1) ? ob[5] : ob[2];
+            for (int r = 0; r < 3; r++) {
+                const float w = M[4 * r] * x + M[4 * r + 1] * y + M[4 * r + 2] * z + M[4 * r + 3];
+                s->bounds_lo[r] = std::fmin(s->bounds_lo[r], w);
+                s->bounds_hi[r] = std::fmax(s->bounds_hi[r], w);
+            }
+        }
+    }
+}
 
 int inst_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     hipStream_t st = c->stream;
@@ -373,6 +388,8 @@ int inst_update(srt_ctx* c, SceneRec* s, const float* xf) {
     }
     s->h_inst_xf.assign(xf, xf + 12 * (size_t)s->ninst);
     s->h_inst_inv = inv;
+    // the scene box stays the one of the transforms given to
+    // srt_scene_set_instances (scene.h:72: computed once, without animation)
     std::vector<TItem> it = inst_items(s);
     tlas_refit(s, it);
     hipStream_t st = c->stream;

@@ -128,7 +128,10 @@ static void dispatcher(srt_ctx* c) {
             // S->pool paths.  Small requests thus overtake a large one
             // between its chunks, but only for as long as the large one
             // has waited (ties: arrival order).
-            auto due = [](const Request* q) { return q->t_submit + (double)(q->total - q->next) / kServeRate; };
+            const double age_w = c->serve_age_pm * 1e-3;
+            auto due = [age_w](const Request* q) {
+                return age_w * q->t_submit + (double)(q->total - q->next) / kServeRate;
+            };
             Request* first = nullptr;
             for (Request* q : S->active)
                 if (!first || due(q) < due(first)) first = q;

@@ -223,6 +223,11 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
         c->trace_inner_exit = (int32_t)value;
         return SRT_OK;
     }
+    if (!strcmp(key, "serve_age")) {
+        if (value < 0 || value > 1000) return set_err(c, SRT_E_INVALID, "serve_age must be in [0, 1000] (per mille)");
+        c->serve_age_pm = (int32_t)value;
+        return SRT_OK;
+    }
     if (!strcmp(key, "trace_bvh_width")) {
         if (value != 2 && value != 4) return set_err(c, SRT_E_INVALID, "trace_bvh_width must be 2 or 4");
         c->trace_bvh_width = (int32_t)value;
@@ -452,6 +457,7 @@ int srt_scene_set_instances(srt_ctx* c, srt_scene h, int32_t n, const int32_t* i
     s->h_inst_mesh.assign(inst_mesh, inst_mesh + n);
     s->h_inst_xf.assign(xf, xf + 12 * (size_t)n);
     s->h_inst_inv = inv;
+    inst_scene_box(s);
     s->nlights = (int32_t)light_cdf.size();
     s->light_wsum = wsum;
     s->has_bvh = false;
@@ -674,7 +680,6 @@ int srt_render_simple(srt_ctx* c, srt_scene h, const srt_camera* cam, int32_t ty
     if (!c || !cam || !rgba_out) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
-    if (s->ninst > 0) return set_err(c, SRT_E_INVALID, "the Simple pass does not take instanced scenes");
     if (W <= 0 || H <= 0) return set_err(c, SRT_E_INVALID, "resolution must be positive");
     if ((int64_t)W * H > (1ll << 28)) return set_err(c, SRT_E_LIMIT, "resolution too large");
     if (spp <= 0) return set_err(c, SRT_E_INVALID, "samples_per_pixel must be positive");
@@ -704,7 +709,6 @@ int srt_render_light(srt_ctx* c, srt_scene h, const srt_camera* cam, const srt_p
     if (!c || !cam || !p || !rgba_out) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
-    if (s->ninst > 0) return set_err(c, SRT_E_INVALID, "the LightTracer pass does not take instanced scenes");
     if (W <= 0 || H <= 0) return set_err(c, SRT_E_INVALID, "resolution must be positive");
     if ((int64_t)W * H > (1ll << 28)) return set_err(c, SRT_E_LIMIT, "resolution too large");
     if (p->samples_per_pixel <= 0) return set_err(c, SRT_E_INVALID, "samples_per_pixel must be positive");

@@ -309,6 +309,9 @@ struct srt_ctx {
     int32_t trace_bvh_width = 4;          // traversal over the 4-wide (4) or binary (2) tree
     int32_t trace_inner_exit = 8;         // stragglers resume after the leaf step
     int64_t max_wide_nodes = srt::kMaxNodes4;   // 4-wide trees above this many nodes are traversed in binary form
+    // serving order: remaining paths at 2 Gpaths/s minus serve_age/1000 of the
+    // time waited (0: shortest remaining first; 1000: earliest due time)
+    int32_t serve_age_pm = 1000;
     // profiling
     bool profile = false;
     std::map<std::string, srt::KernelStat> stats;
@@ -355,6 +358,7 @@ int bvh4_build(srt_ctx* c, SceneRec* s);
 // instanced scenes (instances.hip)
 int inst_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size);
 int inst_update(srt_ctx* c, SceneRec* s, const float* xf);
+void inst_scene_box(SceneRec* s);
 bool affine_inverse(const float m[12], float out[12]);
 int skin_vertices(srt_ctx* c, SceneRec* s, const float* bones_3x4, int nb);
 // wavefront integrator (wavefront.hip)

@@ -765,6 +765,23 @@ SRT_D HitGeom hit_geometry(const DevScene& S, int32_t tri, float u, float v) {
     return g;
 }
 
+// Instanced scenes (hit_info.h:58-62): the hit position by apply_point and the
+// geometric normal by apply_vector of the instance transform (no
+// renormalising); returns the transform for the shading frame.
+SRT_D void inst_hit(const DevScene& S, int32_t inst, HitGeom& hg, float4* X = nullptr) {
+    const float4* T = S.inst_xf + 3 * (size_t)inst;
+    const float4 X0 = T[0], X1 = T[1], X2 = T[2];
+    hg.pos = xf_point(X0, X1, X2, hg.pos);
+    hg.normal = xf_vector(X0, X1, X2, hg.normal);
+    if (X) { X[0] = X0; X[1] = X1; X[2] = X2; }
+}
+// Transform::apply_onb (transform.h:72-75): the frame axis by axis
+SRT_D void inst_onb(const float4* X, Onb& onb) {
+    onb.z = xf_vector(X[0], X[1], X[2], onb.z);
+    onb.x = xf_vector(X[0], X[1], X[2], onb.x);
+    onb.y = xf_vector(X[0], X[1], X[2], onb.y);
+}
+
 // The leading 64 bytes of a segment (path range and depth settings), read
 // together as eight 8-byte parts.
 struct SegHead {
@@ -821,15 +838,8 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     const SegHead P = load_seg_head(segs, segk);
     const float u = h.u[i], v = h.v[i];
     HitGeom hg = hit_geometry(S, tri, u, v);
-    float4 X0, X1, X2;         // instance transform (instanced scenes)
-    if (S.num_inst > 0) {
-        // hit_info.h:58-62: position by apply_point, geometric normal by
-        // apply_vector (not renormalised); the shading frame below axis by axis
-        const float4* T = S.inst_xf + 3 * (size_t)h.inst[i];
-        X0 = T[0]; X1 = T[1]; X2 = T[2];
-        hg.pos = xf_point(X0, X1, X2, hg.pos);
-        hg.normal = xf_vector(X0, X1, X2, hg.normal);
-    }
+    float4 X[3];               // instance transform (instanced scenes)
+    if (S.num_inst > 0) inst_hit(S, h.inst[i], hg, X);
     const float w = hg.w;
     const f3 pos = hg.pos, sn = hg.sn;
     f3 normal = hg.normal;
@@ -858,12 +868,7 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     }
     const f3 tg = hit_tangent(S, tri, hg, u, v);
     Onb onb = make_onb(sn, tg);
-    if (S.num_inst > 0) {
-        // Transform::apply_onb (transform.h:72-75)
-        onb.z = xf_vector(X0, X1, X2, onb.z);
-        onb.x = xf_vector(X0, X1, X2, onb.x);
-        onb.y = xf_vector(X0, X1, X2, onb.y);
-    }
+    if (S.num_inst > 0) inst_onb(X, onb);
     const ShadePoint spt = shade_point(m, -rd, onb, color);
     if (dot(normal, -rd) < 0.0f) normal = -normal;
     const bool last = (depth + 1 >= P.max_depth);
@@ -1199,16 +1204,18 @@ struct SimpleIO {
         h.u[i] = hit ? st.bu : 0.0f;
         h.v[i] = hit ? st.bv : 0.0f;
         h.tri[i] = hit ? st.best_pos : -1;
+        if (h.inst) h.inst[i] = hit ? st.best_inst : -1;
     }
 };
 
-template <bool WIDE>
+template <bool WIDE, bool INST>
 __global__ void __launch_bounds__(kTraceBlock) k_simple_trace(DevScene S, uint32_t n, SimpleIO io, uint32_t* fetch,
                                                              int32_t* gstack, TraceCfg cfg) {
-    const TravStack k = make_stack(gstack, cfg);
+    const TravStack k = make_stack<INST>(gstack, cfg);
     uint32_t nv = 0, nt = 0, lo, hi;
     fetch = dense_range(n, fetch, lo, hi);
-    trace_persistent<false, false, WIDE>(S, IdentityEntry{}, lo, hi, fetch, cfg.refill, k, io, nv, nt);
+    trace_persistent<false, false, WIDE, SimpleIO, IdentityEntry, INST>(S, IdentityEntry{}, lo, hi, fetch, cfg.refill, k,
+                                                                        io, nv, nt);
 }
 
 // the Simple pass types (SimpleParams::Type, params/npr_params.h)
@@ -1226,7 +1233,8 @@ __global__ void __launch_bounds__(256) k_simple_accum(DevScene S, SimpleIO io, i
     f3 L = mk(1.0f);
     const int32_t tri = io.h.tri[i];
     if (tri >= 0) {
-        const HitGeom hg = hit_geometry(S, tri, io.h.u[i], io.h.v[i]);
+        HitGeom hg = hit_geometry(S, tri, io.h.u[i], io.h.v[i]);
+        if (S.num_inst > 0) inst_hit(S, io.h.inst[i], hg);
         const MatRec& m = S.mats[hg.matid];
         const f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
         if (type == kSimpleDepth) {
@@ -1348,7 +1356,11 @@ __global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int de
         const int32_t tri = h.tri[i];
         const bool hit = tri >= 0;
         HitGeom hg;
-        if (hit) hg = hit_geometry(S, tri, h.u[i], h.v[i]);
+        float4 X[3];
+        if (hit) {
+            hg = hit_geometry(S, tri, h.u[i], h.v[i]);
+            if (S.num_inst > 0) inst_hit(S, h.inst[i], hg, X);
+        }
         // a specular particle through the camera position (light_tracer.cu:89-110)
         if (qin.flags[i] & 1u) {
             f3 to_camera = cam_pos - ro;
@@ -1366,7 +1378,9 @@ __global__ void __launch_bounds__(256) k_lt_shade(DevScene S, LtParams P, int de
         if (dot(-rd, normal) < 0.0f) normal = -normal;
         const f3 color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
         const f3 tg = hit_tangent(S, tri, hg, h.u[i], h.v[i]);
-        const ShadePoint spt = shade_point(m, -rd, make_onb(hg.sn, tg), color);
+        Onb onb = make_onb(hg.sn, tg);
+        if (S.num_inst > 0) inst_onb(X, onb);
+        const ShadePoint spt = shade_point(m, -rd, onb, color);
         Pcg32 rng = rng_resume(qin.rng[i], sample);
         // camera connection (light_tracer.cu:124-142): the contribution waits in the shadow queue
         bool emit = false;
@@ -1436,17 +1450,17 @@ struct LtShadowIO {    // camera connections: unoccluded ones splat their contri
     }
 };
 
-template <bool WIDE>
+template <bool WIDE, bool INST>
 __global__ void __launch_bounds__(kTraceBlock) k_lt_shadow(DevScene S, ShadowQueue sq, float4* film, uint32_t* counts,
                                                           uint32_t cap, int32_t* gstack, TraceCfg cfg) {
     __shared__ uint32_t prefix[kParts + 1];
     uint32_t lo, hi;
     const QueueView view = queue_piece(counts, CNT_SHADOW, cap, prefix, lo, hi);
     uint32_t* fetch = counts + CNT_FETCH_S + my_part() * kCntLine;
-    const TravStack k = make_stack(gstack, cfg);
+    const TravStack k = make_stack<INST>(gstack, cfg);
     LtShadowIO io{sq, film};
     uint32_t nv = 0, nt = 0;
-    trace_persistent<true, false, WIDE>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
+    trace_persistent<true, false, WIDE, LtShadowIO, QueueView, INST>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
 }
 
 __global__ void __launch_bounds__(256) k_film_clear(float4* film, int64_t n) {
@@ -1936,16 +1950,23 @@ int render_simple(srt_ctx* c, SceneRec* s, const DevCamera& cam, int type, int s
     io.seed = seed;
     io.h.t = (float*)c->hits[0].p; io.h.u = (float*)c->hits[1].p; io.h.v = (float*)c->hits[2].p;
     io.h.tri = (int32_t*)c->hits[3].p;
+    const bool inst = s->ninst > 0;
+    io.h.inst = inst ? (int32_t*)c->hits[4].p : nullptr;
     uint32_t* fetch = (uint32_t*)c->counters.p + CNT_FETCH_Q;
     hipEvent_t ev;
     prof_begin(c, "simple", &ev);
     for (int smp = 0; smp < spp; smp++) {
         io.sample = (uint32_t)smp;
         k_clear_parts<<<1, kParts, 0, c->stream>>>(fetch);
-        if (cfg.wide) k_simple_trace<true><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch,
-                                                                                                  (int32_t*)c->gstack.p, cfg);
-        else k_simple_trace<false><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(S, (uint32_t)n, io, fetch,
-                                                                                          (int32_t*)c->gstack.p, cfg);
+        if (inst)
+            k_simple_trace<true, true><<<grid, kTraceBlock, trace_lds_bytes(cfg, true), c->stream>>>(
+                S, (uint32_t)n, io, fetch, (int32_t*)c->gstack.p, cfg);
+        else if (cfg.wide)
+            k_simple_trace<true, false><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(
+                S, (uint32_t)n, io, fetch, (int32_t*)c->gstack.p, cfg);
+        else
+            k_simple_trace<false, false><<<grid, kTraceBlock, trace_lds_bytes(cfg), c->stream>>>(
+                S, (uint32_t)n, io, fetch, (int32_t*)c->gstack.p, cfg);
         k_simple_accum<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(S, io, spp, type, min_depth, max_depth,
                                                                           (float4*)out_dev);
     }
@@ -1968,13 +1989,15 @@ int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max
     const TraceCfg tcfg = trace_cfg(c, s);
     TraceCfg scfg = tcfg;
     scfg.refill = c->shadow_refill;
-    const size_t tlds = trace_lds_bytes(tcfg);
+    const bool inst = s->ninst > 0;
+    const size_t tlds = trace_lds_bytes(tcfg, inst);
     hipStream_t st = c->stream;
     uint32_t* counts = (uint32_t*)c->counters.p;
     unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + kTotalsOffset);
     PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
     HitBuf H;
     H.t = (float*)c->hits[0].p; H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
+    H.inst = inst ? (int32_t*)c->hits[4].p : nullptr;
     ShadowQueue SQ;
     {
         auto f = [&](int j) { return (float*)c->shq[j].p; };
@@ -1998,12 +2021,15 @@ int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max
         k_lt_raygen<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(S, P, Q[0], n, cap, counts);
         for (int depth = 0; depth < max_depth; depth++) {
             const int qi = depth & 1;
-            if (tcfg.wide)
+            if (inst)
+                k_trace<false, true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+            else if (tcfg.wide)
                 k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
             else k_trace<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
             k_lt_shade<<<shade_grid, 256, 0, st>>>(S, P, depth, Q[qi], H, Q[qi ^ 1], SQ, film, counts, qi, cap);
-            if (tcfg.wide) k_lt_shadow<true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, cap, gst, scfg);
-            else k_lt_shadow<false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, cap, gst, scfg);
+            if (inst) k_lt_shadow<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, cap, gst, scfg);
+            else if (tcfg.wide) k_lt_shadow<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, cap, gst, scfg);
+            else k_lt_shadow<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, SQ, film, counts, cap, gst, scfg);
         }
     }
     prof_end(c, "light_tracer", ev, total);

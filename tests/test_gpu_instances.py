@@ -10,7 +10,10 @@ hit_info.h:58-62, gmesh.h:77-78, optix_ray_tracer.cu:337-359 and 443-463):
   for bit;
 - new instance transforms (a top-level refit, no rebuild) render what the
   oracle renders with the moved instances, and what a fresh build renders;
-- the passes that take no instanced scene refuse one with an error.
+- the Simple pass (every colouring, the depth range from the instanced scene
+  box of scene.cu:299-307) equals the oracle's bit for bit, and the
+  LightTracer's splats equal the oracle's within 1e-4 (float atomic adds);
+- skinning and the binary-tree download refuse an instanced scene.
 """
 import sys
 
@@ -159,15 +162,38 @@ def test_update_instances_refit(ctx):
     assert np.array_equal(img, fresh)
 
 
+@pytest.mark.parametrize("kind", ["cornell", "spheres"])
+@pytest.mark.parametrize("type", list(O.SIMPLE))
+def test_simple_pass_equals_oracle(ctx, kind, type):
+    _say(f"[inst] simple {kind} {type}")
+    s = _scene(kind)
+    h = ctx.upload(s)
+    img = ctx.render_simple(h, s.camera, 48, 40, type=type, spp=2, seed=4)
+    ctx.release(h)
+    osc = O.Scene(s)
+    ref, _ = O.render_simple(osc, s.camera, 48, 40, type=type, spp=2, seed=4, bvh=O.Bvh(osc))
+    assert np.array_equal(img, ref), np.abs(img - ref).max()
+
+
+def test_light_tracer_equals_oracle(ctx):
+    _say("[inst] light tracer")
+    s = scenes.cornell_instanced()
+    h = ctx.upload(s)
+    img = ctx.render_light(h, s.camera, 48, 36, spp=2, max_depth=5, rr_depth=2, seed=7)
+    ctx.release(h)
+    osc = O.Scene(s)
+    ref = O.render_light(osc, s.camera, 48, 36, spp=2, max_depth=5, rr_depth=2, seed=7, bvh=O.Bvh(osc))
+    floor = 1e-6 * max(np.abs(img[:, :3]).max(), np.abs(ref[:, :3]).max())
+    d = np.abs(img[:, :3] - ref[:, :3])
+    assert np.all(d <= 1e-4 * np.maximum(np.abs(img[:, :3]), np.abs(ref[:, :3])) + floor), d.max()
+    assert img[:, :3].max() > 0
+
+
 def test_instanced_scene_refusals(ctx):
     _say("[inst] refusals")
     s = scenes.cornell_instanced()
     h = ctx.upload(s)
     ctx.build_bvh(h)
-    with pytest.raises(abi.SrtError):
-        ctx.render_simple(h, s.camera, 16, 16, "Depth", 1, 0)
-    with pytest.raises(abi.SrtError):
-        ctx.render_light(h, s.camera, 16, 16, 1, 5, 4, 0)
     with pytest.raises(abi.SrtError):
         ctx.bvh_download(h)
     nv = s.positions.shape[0]

@@ -151,3 +151,47 @@ def test_singular_instance_rejected():
         O.Scene(scenes.SceneData(positions=s.positions, indices=s.indices, mesh_first=s.mesh_first,
                                  mesh_material=s.mesh_material, materials=s.materials,
                                  inst_mesh=np.arange(n, dtype=np.int32), inst_xf=xf))
+
+
+@pytest.mark.parametrize("type", ["Depth", "Normal", "BaseColor", "Ambient", "FaceOrientation"])
+def test_identity_instances_simple_pass_equals_flattened(type):
+    """The Simple pass over identity instances: the scene box (the union of
+    the instances' transformed mesh boxes) is the vertex box, and every hit
+    colour equals the flattened scene's."""
+    s = scenes.cornell()
+    si = scenes.identity_instanced(s)
+    osc, osi = O.Scene(s), O.Scene(si)
+    a, ra = O.render_simple(osc, s.camera, 32, 32, type, 2, 5, bvh=O.Bvh(osc))
+    b, rb = O.render_simple(osi, s.camera, 32, 32, type, 2, 5, bvh=O.Bvh(osi))
+    assert ra == rb
+    assert np.array_equal(a, b)
+
+
+def test_identity_instances_light_tracer_equals_flattened():
+    s = scenes.cornell()
+    si = scenes.identity_instanced(s)
+    osc, osi = O.Scene(s), O.Scene(si)
+    a = O.render_light(osc, s.camera, 32, 32, 2, 5, 4, 3, bvh=O.Bvh(osc))
+    b = O.render_light(osi, s.camera, 32, 32, 2, 5, 4, 3, bvh=O.Bvh(osi))
+    assert np.array_equal(a, b)
+    assert a[:, :3].sum() > 0
+
+
+def test_instanced_scene_box_and_depth_range():
+    """Depth range of an instanced scene: a unit cube scaled by 2 about the
+    origin and moved by (3, 0, 0), seen from (3, 0, 10): box [2, 4] x [-1, 1] x
+    [-1, 1], so the range is |(1, 1, 9)| .. |(1, 1, 11)| (dyadic transform:
+    every corner is exact)."""
+    b = scenes._Builder()
+    b.box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), 0)
+    s = b.finish([scenes.material()], with_normals=False)
+    s.inst_mesh = np.array([0], np.int32)
+    s.inst_xf = scenes.affine(None, (3.0, 0.0, 0.0), 2.0)[None]
+    cam = dict(position=(3.0, 0.0, 10.0), target=(3.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=30.0)
+    osc = O.Scene(s)
+    img, (lo, hi) = O.render_simple(osc, cam, 16, 16, "Depth", 1, 0, bvh=O.Bvh(osc))
+    assert lo == np.float32(np.sqrt(np.float32(83.0)))
+    assert hi == np.float32(np.sqrt(np.float32(123.0)))
+    # a pixel next to the centre sees the front face at distance 9 / cos(angle < 0.05)
+    d = (img.reshape(16, 16, 4)[8, 8, 0] - 0.2) / 0.8 * (hi - lo) + lo
+    assert 9.0 - 1e-4 <= d <= 9.0 / np.cos(0.05)
