@@ -33,6 +33,7 @@ Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -483,6 +484,98 @@ def skin_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def anim_mode(args):
+    """Node animation of the 1M-triangle sphere field: every sphere turns about
+    its own centre and bobs, per frame.  Flat form: each sphere's vertices are
+    one rigid bone (linear blend skinning with weight 1), then the whole BVH
+    is refit.  Two-level form: each sphere is its own mesh under one instance,
+    and a frame sets the 3129 instance transforms and refits the top-level
+    tree (the reference's node animation: instance transforms, then an IAS
+    update).  Each frame is then rendered at the full configuration."""
+    from srt_amd import abi, scenes
+    s = scenes.spheres()
+    R = s.render
+    W, H, spp, depth = R["width"], R["height"], R["spp"], R["max_depth"]
+    frames = max(1, args.steps)
+    nsph = 3125
+    rng = np.random.default_rng(7)
+    axes = rng.normal(size=(nsph, 3))
+    phase = rng.uniform(0.0, 2.0 * math.pi, size=nsph)
+    first = s.mesh_first
+    centres = np.stack([s.positions[s.indices[first[i]:first[i + 1]].ravel()].mean(0) for i in range(nsph)])
+
+    def motion(f):
+        xf = np.zeros((nsph + 4, 3, 4), np.float32)
+        xf[:, :, :3] = np.eye(3, dtype=np.float32)
+        for i in range(nsph):
+            Rm = scenes._rotation(axes[i], 0.05 * f)
+            t = centres[i] - Rm @ centres[i] + np.array([0.0, 0.5 * math.sin(0.2 * f + phase[i]), 0.0])
+            xf[i, :, :3] = Rm
+            xf[i, :, 3] = t
+        return xf
+
+    moves = [motion(f + 1) for f in range(frames)]
+    out = {"metric": "node animation: ms per frame (update + render), 1M-triangle sphere field, every sphere moving",
+           "frames": frames, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "spheres-1M (3125 spheres, 1,000,008 triangles) with every sphere rigidly animated; "
+                                  "1920x1080, 64 spp, depth 8 per frame", "width": W, "height": H, "spp": spp,
+                      "max_depth": depth}}
+    ctx = abi.Context(0)
+    # flat: one rigid bone per sphere (bone nsph + k: the identity, for the lights)
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    nv = s.positions.shape[0]
+    ids = np.full((nv, 4), -1, np.int32)
+    w = np.zeros((nv, 4), np.float32)
+    owner = np.full(nv, nsph, np.int32)
+    for i in range(nsph):
+        owner[s.indices[first[i]:first[i + 1]].ravel()] = i
+    ids[:, 0] = owner
+    w[:, 0] = 1.0
+    ctx.set_skin(h, ids, w)
+    upd, ren = [], []
+    for f, xf in enumerate(moves):
+        bones = np.concatenate([xf[:nsph], np.eye(4, dtype=np.float32)[None, :3]])
+        t1 = time.perf_counter()
+        ctx.skin_refit(h, bones)
+        t2 = time.perf_counter()
+        ctx.render_resident(h, s.camera, W, H, spp, depth, R["rr_depth"], 1, 1, R["seed"])
+        t3 = time.perf_counter()
+        if f > 0 or frames == 1:
+            upd.append((t2 - t1) * 1e3)
+            ren.append((t3 - t2) * 1e3)
+    flat_img = ctx.film(W * H)
+    ctx.release(h)
+    out["flat"] = {"update_ms": round(float(np.mean(upd)), 3), "render_ms": round(float(np.mean(ren)), 2),
+                   "frame_ms": round(float(np.mean(upd) + np.mean(ren)), 2),
+                   "update": "skin (3126 rigid bones over 506k vertices) + refit of the whole BVH"}
+    log(f"flat node animation: {out['flat']}")
+    # two-level: every sphere its own mesh under one instance
+    si = scenes.identity_instanced(s)
+    h = ctx.upload(si)
+    ctx.build_bvh(h)
+    upd, ren = [], []
+    for f, xf in enumerate(moves):
+        t1 = time.perf_counter()
+        ctx.update_instances(h, xf)
+        t2 = time.perf_counter()
+        ctx.render_resident(h, s.camera, W, H, spp, depth, R["rr_depth"], 1, 1, R["seed"])
+        t3 = time.perf_counter()
+        if f > 0 or frames == 1:
+            upd.append((t2 - t1) * 1e3)
+            ren.append((t3 - t2) * 1e3)
+    inst_img = ctx.film(W * H)
+    ctx.release(h)
+    ctx.close()
+    rel = np.abs(flat_img[:, :3] - inst_img[:, :3]).mean() / max(float(np.abs(flat_img[:, :3]).mean()), 1e-12)
+    out["instanced"] = {"update_ms": round(float(np.mean(upd)), 3), "render_ms": round(float(np.mean(ren)), 2),
+                        "frame_ms": round(float(np.mean(upd) + np.mean(ren)), 2),
+                        "update": "3129 instance transforms: inverses, instance boxes, top-level refit (host), upload"}
+    out["image_mean_rel_diff"] = round(float(rel), 5)
+    log(f"instanced node animation: {out['instanced']}")
+    print(json.dumps(out), flush=True)
+
+
 def mlp_mode(args):
     """Radiance MLP against the plain tracer at equal wall time on the Cornell
     box (256x256, depth 5; default) or the 1M-triangle sphere scene (192x108,
@@ -805,7 +898,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-row-step", type=int, default=8, help="CPU baseline renders every k-th row")
     ap.add_argument("--path-pool", type=int, default=0)
-    ap.add_argument("--mode", choices=["frame", "copy", "skin", "mlp", "serve", "passes"], default="frame")
+    ap.add_argument("--mode", choices=["frame", "copy", "skin", "mlp", "serve", "passes", "anim"], default="frame")
     ap.add_argument("--mlp-frames", type=int, default=16)
     ap.add_argument("--mlp-steps", type=int, default=8)
     ap.add_argument("--mlp-scene", choices=["cornell", "spheres"], default="cornell")
@@ -847,7 +940,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.mode != "frame":
-        {"skin": skin_mode, "mlp": mlp_mode, "serve": serve_mode, "passes": passes_mode}[args.mode](args)
+        {"skin": skin_mode, "mlp": mlp_mode, "serve": serve_mode, "passes": passes_mode,
+         "anim": anim_mode}[args.mode](args)
         return
 
     from srt_amd import abi, scenes

@@ -73,16 +73,19 @@ __global__ void __launch_bounds__(256) k_prim_offset(float4* tris, int n, uint32
 
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
 
-// World box of an instance: the corners of its mesh's object box through the
-// transform in double, padded by 2^-14 of (coordinate magnitude + extent +
-// the scene scale) and rounded outward.  The padding covers the difference
-// between a world ray and the object-space ray the traversal computes from it
-// (one float rounding per coordinate of M^-1 o and M^-1 d), so culling with
-// these boxes never drops an instance whose object-space test would hit.
-void inst_box(const float* M, const float* ob, double scene_scale, float lo[3], float hi[3]) {
+// World box of an instance: points of its mesh through the transform in
+// double, padded by 2^-14 of (coordinate magnitude + extent + the scene
+// scale) and rounded outward.  The padding covers the difference between a
+// world ray and the object-space ray the traversal computes from it (one
+// float rounding per coordinate of M^-1 o and M^-1 d), so culling with these
+// boxes never drops an instance whose object-space test would hit.  The
+// points are the mesh's vertices (the tight box of the placed mesh) or, for
+// meshes of more than kHullVerts vertices, the corners of its object box.
+constexpr int64_t kHullVerts = 4096;
+void inst_box(const float* M, const float* pts, int64_t npts, double scene_scale, float lo[3], float hi[3]) {
     double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int k = 0; k < 8; k++) {
-        const double p[3] = {(k & 1) ? ob[3] : ob[0], (k & 2) ? ob[4] : ob[1], (k & 4) ? ob[5] : ob[2]};
+    for (int64_t k = 0; k < npts; k++) {
+        const double p[3] = {pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]};
         for (int r = 0; r < 3; r++) {
             const double w = (double)M[4 * r] * p[0] + (double)M[4 * r + 1] * p[1] + (double)M[4 * r + 2] * p[2] +
                              (double)M[4 * r + 3];
@@ -189,7 +192,19 @@ std::vector<TItem> inst_items(SceneRec* s) {
             for (int a = 0; a < 3; a++) { t.lo[a] = INFINITY; t.hi[a] = -INFINITY; t.c[a] = 0.0f; }
             continue;
         }
-        inst_box(&s->h_inst_xf[12 * (size_t)i], ob, scale, t.lo, t.hi);
+        const int m = s->h_inst_mesh[i];
+        const int64_t p0 = s->h_mesh_pts_first[m], np = s->h_mesh_pts_first[m + 1] - p0;
+        if (np <= kHullVerts) {
+            inst_box(&s->h_inst_xf[12 * (size_t)i], &s->h_mesh_pts[3 * p0], np, scale, t.lo, t.hi);
+        } else {
+            float corners[24];
+            for (int k = 0; k < 8; k++) {
+                corners[3 * k] = (k & 1) ? ob[3] : ob[0];
+                corners[3 * k + 1] = (k & 2) ? ob[4] : ob[1];
+                corners[3 * k + 2] = (k & 4) ? ob[5] : ob[2];
+            }
+            inst_box(&s->h_inst_xf[12 * (size_t)i], corners, 8, scale, t.lo, t.hi);
+        }
         for (int a = 0; a < 3; a++) t.c[a] = 0.5f * (t.lo[a] + t.hi[a]);
     }
     return it;
@@ -245,12 +260,36 @@ void inst_scene_box(SceneRec* s) {
     }
 }
 
+// Each mesh's distinct vertices (object space), for the instance boxes.
+static int mesh_points(srt_ctx* c, SceneRec* s, const std::vector<int32_t>& mesh_first) {
+    if (!s->h_mesh_pts_first.empty()) return SRT_OK;
+    std::vector<float> pos(3 * (size_t)s->nv);
+    std::vector<uint32_t> idx(3 * (size_t)s->nf);
+    int r = hip_check(c, hipMemcpy(pos.data(), s->pos.p, 12 * (size_t)s->nv, hipMemcpyDeviceToHost), "vertices");
+    if (!r) r = hip_check(c, hipMemcpy(idx.data(), s->idx.p, 12 * (size_t)s->nf, hipMemcpyDeviceToHost), "indices");
+    if (r) return r;
+    std::vector<int32_t> seen(s->nv, -1);
+    s->h_mesh_pts.clear();
+    s->h_mesh_pts_first.assign(1, 0);
+    for (int m = 0; m < s->nmesh; m++) {
+        for (int64_t k = 3 * (int64_t)mesh_first[m]; k < 3 * (int64_t)mesh_first[m + 1]; k++) {
+            const uint32_t v = idx[k];
+            if (seen[v] == m) continue;
+            seen[v] = m;
+            s->h_mesh_pts.insert(s->h_mesh_pts.end(), &pos[3 * (size_t)v], &pos[3 * (size_t)v] + 3);
+        }
+        s->h_mesh_pts_first.push_back((int64_t)(s->h_mesh_pts.size() / 3));
+    }
+    return SRT_OK;
+}
+
 int inst_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     hipStream_t st = c->stream;
     const int nm = s->nmesh;
     std::vector<int32_t> mesh_first(nm + 1);
     if (int r = hip_check(c, hipMemcpy(mesh_first.data(), s->mesh_first.p, 4 * (size_t)(nm + 1), hipMemcpyDeviceToHost), "meshes"))
         return r;
+    if (int r = mesh_points(c, s, mesh_first)) return r;
     // 1. one tree per mesh: the bvh.hip build on a view of the mesh's triangles
     struct Blas {
         DevBuf nodes4, tris, tri_rec;

@@ -269,6 +269,10 @@ struct SceneRec {
     // host copies needed for light CDF construction
     std::vector<float> h_mesh_area;
     std::vector<float> h_mesh_box;   // 6 per mesh: object-space lo xyz, hi xyz (at upload)
+    // instanced scenes: each mesh's distinct vertex positions (object space),
+    // mesh m's at [h_mesh_pts_first[m], h_mesh_pts_first[m + 1]) (xyz triples)
+    std::vector<float> h_mesh_pts;
+    std::vector<int64_t> h_mesh_pts_first;
     // instances (srt_scene_set_instances): meshes in object space
     int32_t ninst = 0;
     std::vector<int32_t> h_inst_mesh;

@@ -129,10 +129,7 @@ struct TravState {
 
 // The ray a lane traverses with: origin, reciprocal direction, the
 // watertight test's shear, the direction signs that pick near/far planes.
-SRT_D void set_ray(TravState& st, f3 o, f3 d) {
-    st.o = o;
-    st.rp = ray_pre(d);
-    st.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+SRT_D void set_planes(TravState& st) {
     // a ray lying in a slab plane with a zero direction component yields NaN
     // slab distances, which fmaxf/fminf ignore
     st.sx = st.inv.x < 0.0f; st.sy = st.inv.y < 0.0f; st.sz = st.inv.z < 0.0f;
@@ -140,27 +137,47 @@ SRT_D void set_ray(TravState& st, f3 o, f3 d) {
     st.ony = st.sy ? 48u : 32u;
     st.onz = st.sz ? 80u : 64u;
 }
+SRT_D void set_ray(TravState& st, f3 o, f3 d) {
+    st.o = o;
+    st.rp = ray_pre(d);
+    st.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    set_planes(st);
+}
 
 // Traversal kernels of instanced scenes keep each lane's world ray in LDS
-// (ahead of the stack: 6 floats per lane) so a ray leaving an instance gets
-// its world ray back exactly.
+// (ahead of the stack, lane-interleaved): origin and direction, from which
+// an instance's object-space ray is computed on entry, and the reciprocal
+// direction and triangle-test shear, which a ray leaving an instance gets
+// back as they were (no divisions on the way out).
 extern __shared__ int32_t trav_lds[];    // dynamic LDS of the traversal kernels
-constexpr int kRayStash = 6 * kTraceBlock;   // int32 entries of the stash
-SRT_D void stash_ray(f3 o, f3 d) {
+constexpr int kStashWords = 13;
+constexpr int kRayStash = kStashWords * kTraceBlock;   // int32 entries of the stash
+SRT_D void stash_ray(const TravState& st, f3 d) {
     float* r = reinterpret_cast<float*>(trav_lds) + threadIdx.x;
-    r[0] = o.x; r[kTraceBlock] = o.y; r[2 * kTraceBlock] = o.z;
-    r[3 * kTraceBlock] = d.x; r[4 * kTraceBlock] = d.y; r[5 * kTraceBlock] = d.z;
+    const float w[kStashWords] = {st.o.x, st.o.y, st.o.z, d.x, d.y, d.z, st.inv.x, st.inv.y, st.inv.z,
+                                  st.rp.Sx, st.rp.Sy, st.rp.Sz,
+                                  __int_as_float(st.rp.kx | (st.rp.ky << 2) | (st.rp.kz << 4))};
+    for (int j = 0; j < kStashWords; j++) r[j * kTraceBlock] = w[j];
 }
 SRT_D void unstash_ray(f3& o, f3& d) {
     const float* r = reinterpret_cast<const float*>(trav_lds) + threadIdx.x;
     o = mk(r[0], r[kTraceBlock], r[2 * kTraceBlock]);
     d = mk(r[3 * kTraceBlock], r[4 * kTraceBlock], r[5 * kTraceBlock]);
 }
+SRT_D void restore_world_ray(TravState& st) {
+    const float* r = reinterpret_cast<const float*>(trav_lds) + threadIdx.x;
+    st.o = mk(r[0], r[kTraceBlock], r[2 * kTraceBlock]);
+    st.inv = mk(r[6 * kTraceBlock], r[7 * kTraceBlock], r[8 * kTraceBlock]);
+    st.rp.Sx = r[9 * kTraceBlock]; st.rp.Sy = r[10 * kTraceBlock]; st.rp.Sz = r[11 * kTraceBlock];
+    const int kp = __float_as_int(r[12 * kTraceBlock]);
+    st.rp.kx = kp & 3; st.rp.ky = (kp >> 2) & 3; st.rp.kz = kp >> 4;
+    set_planes(st);
+}
 
 template <bool INST = false>
 SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root) {
     set_ray(st, o, d);
-    if (INST) stash_ray(o, d);
+    if (INST) stash_ray(st, d);
     st.tmin = tmin;
     st.best_t = tmax;
     st.best_prim = 0x7fffffff;
@@ -298,12 +315,14 @@ SRT_D int4 ld4i(__amdgpu_buffer_rsrc_t r, uint32_t off) {
 // loop once at most `inner_exit` of its lanes still search for a first leaf:
 // those lanes keep their state and continue after the leaf step, instead of
 // every other lane waiting for them.
-// Instanced scenes (INST): a top-level leaf kInstLeaf | i moves the lane's ray
+// Instanced scenes (INST): a top-level leaf kInstLeaf | i is postponed like a
+// triangle leaf; the step after the loop (trav_pend) moves the lane's ray
 // into instance i's object space (inverse transform, applied as
-// apply_point / apply_vector in float) and pushes kInstReturn before the
-// mesh tree's root; popping kInstReturn restores the world ray from the LDS
-// stash, after the lane's postponed leaf (which lies in that instance) has
-// been tested.
+// apply_point / apply_vector in float, with the divisions of the new
+// reciprocal direction and shear), so the lanes of a wave enter their
+// instances together.  Popping kInstReturn restores the world ray from the
+// LDS stash, after the lane's postponed leaf (which lies in that instance)
+// has been tested.
 SRT_D f3 xf_point(float4 r0, float4 r1, float4 r2, f3 p) {
     return mk(r0.x * p.x + r0.y * p.y + r0.z * p.z + r0.w, r1.x * p.x + r1.y * p.y + r1.z * p.z + r1.w,
               r2.x * p.x + r2.y * p.y + r2.z * p.z + r2.w);
@@ -317,23 +336,16 @@ SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uin
     const __amdgpu_buffer_rsrc_t rsrc = node4_rsrc(S);
     while (st.cur_ok) {
         if (INST && is_inst_ref(st.cur)) {
-            f3 o, d;
             if (st.cur == kInstReturn) {
                 if (st.has_pend) break;           // test the postponed leaf in this instance's space first
-                unstash_ray(o, d);
-                set_ray(st, o, d);
+                restore_world_ray(st);
                 st.inst = -1;
                 stk_pop(k, st);
             } else {
-                const int32_t i = st.cur & 0x3fffffff;
-                unstash_ray(o, d);
-                const float4* M = S.inst_inv + 3 * (size_t)i;
-                const float4 r0 = M[0], r1 = M[1], r2 = M[2];
-                set_ray(st, xf_point(r0, r1, r2, o), xf_vector(r0, r1, r2, d));
-                st.inst = i;
-                stk_push(k, st.sp, kInstReturn);
-                st.sp++;
-                st.cur = S.inst_root[i];
+                if (st.has_pend) break;           // second postponed item: wait here
+                st.pend = st.cur;
+                st.has_pend = true;
+                stk_pop(k, st);
             }
         } else if (st.cur < 0) {
             if (st.has_pend) break;
@@ -405,6 +417,31 @@ SRT_D bool trav_leaf(const DevScene& S, TravState& st, uint32_t& nt) {
     return false;
 }
 
+// The postponed item of a lane: a triangle leaf (trav_leaf), or, in an
+// instanced scene, an instance to enter: the lane's current world-space
+// entry (if any) and kInstReturn are pushed, and the ray continues at the
+// instance's mesh tree in object space.
+template <bool ANY, bool COUNT, bool INST>
+SRT_D bool trav_pend(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nt) {
+    if (INST && is_inst_ref(st.pend)) {
+        st.has_pend = false;
+        const int32_t i = st.pend & 0x3fffffff;
+        f3 o, d;
+        unstash_ray(o, d);
+        const float4* M = S.inst_inv + 3 * (size_t)i;
+        const float4 r0 = M[0], r1 = M[1], r2 = M[2];
+        set_ray(st, xf_point(r0, r1, r2, o), xf_vector(r0, r1, r2, d));
+        st.inst = i;
+        if (st.cur_ok) { stk_push(k, st.sp, st.cur); st.sp++; }
+        stk_push(k, st.sp, kInstReturn);
+        st.sp++;
+        st.cur = S.inst_root[i];
+        st.cur_ok = true;
+        return false;
+    }
+    return trav_leaf<ANY, COUNT, INST>(S, st, nt);
+}
+
 // Persistent traversal with dynamic ray fetch.  Each lane holds one ray; when
 // at least `refill` lanes of the wave (or all of them) are idle, the wave
 // stores the finished lanes' results and takes that many new ray indices from
@@ -461,7 +498,7 @@ SRT_D void trace_persistent(const DevScene& S, const View& view, uint32_t lo, ui
         if (has) {
             if (WIDE) trav_inner4<COUNT, INST>(S, st, k, nv, k.inner_exit);
             else trav_inner<COUNT>(S, st, k, nv);
-            if (st.has_pend) occl = trav_leaf<ANY, COUNT, INST>(S, st, nt);
+            if (st.has_pend) occl = trav_pend<ANY, COUNT, INST>(S, st, k, nt);
             if (occl || (!st.cur_ok && !st.has_pend)) {
                 has = false;
                 done = true;
