@@ -116,7 +116,8 @@ const char* srt_last_error(const srt_ctx* ctx);
  * (a BVH built while this is set keeps its 4-wide form only up to this many
  * 4-wide nodes, else it is traversed in binary form with the same results;
  * 0..16777215, the default: 4-wide nodes are addressed with 32-bit byte
- * offsets).  Limits: 2^28 - 1 triangles per scene; "path_pool" and the pool
+ * offsets), "serve_age" (serving order, below: the weight of the time a request
+ * has waited, per mille, 0..1000, default 250).  Limits: 2^28 - 1 triangles per scene; "path_pool" and the pool
  * of srt_serve_start in [64, 2^31 - 1]. */
 int srt_set_option(srt_ctx* ctx, const char* key, int64_t value);
 size_t srt_hbm_used(const srt_ctx* ctx);
@@ -290,10 +291,11 @@ int srt_memcpy(srt_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t k
  * while their device film (28 bytes per pixel) fits in the HBM budget (a request
  * that can never fit ends with status SRT_E_OOM).  Each chunk of at most
  * `path_pool` paths (0: the smaller of the "path_pool" option and 2^24) runs
- * the scene of the admitted request with the earliest due time (submit time +
- * remaining paths at 2 Gpaths/s), filled with that scene's admitted requests
- * in due-time order; a large request is cut into chunks, and newer small ones
- * overtake it between its chunks for as long as it has waited.
+ * the scene of the admitted request with the smallest key, remaining paths at
+ * 2 Gpaths/s + serve_age / 1000 x submit time (0: shortest remaining first;
+ * 1000: earliest due time), filled with that scene's admitted requests in key
+ * order; a large request is cut into chunks, and newer small ones overtake it
+ * between its chunks until its waiting time outweighs their sizes.
  * Requests with mlp_mode 1 query their scene's MLP (srt_mlp_select), else the
  * context's own.
  * A request's image is bit-identical to srt_render with the same arguments.

@@ -8,12 +8,14 @@
 //      (12 B accumulator + 16 B output per pixel) fits in the context's HBM
 //      budget; a request that can never fit fails with SRT_E_OOM;
 //   2. batching: each chunk takes the scene of the admitted request with the
-//      earliest due time (submit time + remaining paths at 2 Gpaths/s) and
-//      fills the chunk (2^24 paths by default) with the remaining paths of the
-//      admitted requests on that scene in due-time order (small requests
+//      smallest key (remaining paths at 2 Gpaths/s + w x submit time, w =
+//      serve_age / 1000: 0 is shortest remaining first, 1 earliest due time)
+//      and fills the chunk (2^24 paths by default) with the remaining paths
+//      of the admitted requests on that scene in key order (small requests
 //      share a chunk; a large one is cut into chunks, and newer small
-//      requests overtake it for as long as it has waited), one segment per
-//      request; requests admitted while a chunk runs join the next chunk;
+//      requests overtake it until its waiting time outweighs their sizes),
+//      one segment per request; requests admitted while a chunk runs join
+//      the next chunk;
 //   3. completion: a request whose last paths were in the chunk has its film
 //      copied to the caller's buffer and its device memory released.
 // A request's paths run in increasing path-id order with its own PCG streams,
@@ -121,13 +123,13 @@ static void dispatcher(srt_ctx* c) {
                 S->cv.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
             }
-            // chunk: the scene of the admitted request with the earliest
-            // due time, submit time + remaining paths / kServeRate (the
-            // time its remaining work would take on an idle device); the
-            // chunk takes that scene's requests in due-time order, up to
+            // chunk: the scene of the admitted request with the smallest
+            // key, remaining paths / kServeRate (the time its remaining
+            // work would take on an idle device) + w x submit time; the
+            // chunk takes that scene's requests in key order, up to
             // S->pool paths.  Small requests thus overtake a large one
-            // between its chunks, but only for as long as the large one
-            // has waited (ties: arrival order).
+            // between its chunks until w x its waiting time outweighs
+            // their sizes (ties: arrival order).
             const double age_w = c->serve_age_pm * 1e-3;
             auto due = [age_w](const Request* q) {
                 return age_w * q->t_submit + (double)(q->total - q->next) / kServeRate;

@@ -315,7 +315,7 @@ struct srt_ctx {
     int64_t max_wide_nodes = srt::kMaxNodes4;   // 4-wide trees above this many nodes are traversed in binary form
     // serving order: remaining paths at 2 Gpaths/s minus serve_age/1000 of the
     // time waited (0: shortest remaining first; 1000: earliest due time)
-    int32_t serve_age_pm = 1000;
+    int32_t serve_age_pm = 250;
     // profiling
     bool profile = false;
     std::map<std::string, srt::KernelStat> stats;
