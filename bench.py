@@ -342,12 +342,14 @@ def mlp_summary(ks_dir, pmc_dir):
              "    rocprofv3 --pmc " + " ".join(MLP_PMC) + " -d gpurun_out/mlp_pmc -o run -- python3 bench.py --mode mlp "
              "--mlp-frames 4",
              "    python3 bench.py --mlp-summary gpurun_out/mlp_ks gpurun_out/mlp_pmc", "",
-             "The run trains on 6 frames (8 steps each, about 120k vertex-2 samples per step) and renders the",
-             "MLP form twice at 16 spp and three times at 256 spp (k_mlp_infer, one launch per render).",
-             "Counter GFLOP = SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 per dispatch, as read.  For k_mlp_infer it equals",
-             "the FLOP of the MFMA instructions the launch issues (36 per 32 queries); for k_mlp_train it reads",
-             "exactly one eighth of them (98 per 32 samples), so the algorithmic rates in `bench.py --mode mlp`",
-             "(HIP events, 99,072 FLOP per training sample, 33,024 per query) are the ones quoted.", "",
+             "The run trains on 6 frames of about 120k vertex-2 samples each; srt_mlp_train_frame splits a",
+             "frame's samples into 8 mini-batches, one Adam step each, so a k_mlp_train dispatch trains on about",
+             "15k samples.  It renders the MLP form twice at 16 spp and three times at 256 spp (k_mlp_infer, one",
+             "launch per render).  Counter GFLOP = SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 per dispatch, as read: for",
+             "k_mlp_infer the FLOP of its MFMA instructions (36 per 32 queries), for k_mlp_train those of its",
+             "15k samples (98 MFMA instructions per 32 samples: 1.59 GFLOP / 106k counter FLOP per sample).",
+             "The rates in `bench.py --mode mlp` are algorithmic (99,072 FLOP per training sample, 33,024 per",
+             "query) over HIP-event time.", "",
              "| kernel | calls | average us | counter GFLOP/dispatch |",
              "|---|---|---|---|"]
     for n in mlp_names:
@@ -603,11 +605,21 @@ def mlp_mode(args):
     def rmse(img):
         return float(np.sqrt(np.mean((img[:, :3].astype(np.float64) - gold) ** 2)))
 
+    reps = 4      # independent seeds per comparison point: RMSE = sqrt(mean MSE), low-spp images are firefly-noisy
+
     def timed_render(spp, seed, mode=0):
+        """(root mean MSE over `reps` seeds, the mean image, seconds of one render)"""
         ctx.render(h, s.camera, W, H, spp, depth, rr, 1, 1, seed, mlp_mode=mode)     # warm
         t0 = time.perf_counter()
         img = ctx.render(h, s.camera, W, H, spp, depth, rr, 1, 1, seed + 1, mlp_mode=mode)
-        return img, time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        mse = [rmse(img) ** 2]
+        acc = img[:, :3].astype(np.float64)
+        for r in range(1, reps):
+            im = ctx.render(h, s.camera, W, H, spp, depth, rr, 1, 1, seed + 1 + 7919 * r, mlp_mode=mode)
+            mse.append(rmse(im) ** 2)
+            acc += im[:, :3]
+        return float(np.sqrt(np.mean(mse))), acc / reps, dt
 
     # online training: frames of up to 2^18 samples harvested from W x H x train_spp tracer paths
     train_frames, steps = args.mlp_frames, args.mlp_steps
@@ -618,17 +630,24 @@ def mlp_mode(args):
         loss, n = ctx.mlp_train_frame(h, s.camera, W, H, train_spp, depth, rr, seed=5000 + f, max_samples=1 << 18,
                                       steps=steps)
         losses.append(loss)
+    if args.mlp_decay_frames > 0:        # a few more frames at a tenth of the learning rate
+        ctx.mlp_set_hparams(lr=1e-4)
+        for f in range(args.mlp_decay_frames):
+            loss, n = ctx.mlp_train_frame(h, s.camera, W, H, train_spp, depth, rr, seed=6000 + f,
+                                          max_samples=1 << 18, steps=steps)
+            losses.append(loss)
     t_train = time.perf_counter() - t0
 
     # MFMA roofline of the training step: two more training frames (not part of
-    # the comparison) with per-kernel HIP-event timing
+    # the comparison) with per-kernel HIP-event timing; a frame's n samples are
+    # split into `steps` mini-batches, so the steps together see each sample once
     ctx.set_option("profile", 1)
     ctx.stats_reset()
     n_prof = 0
     for f in range(2):
         _, n = ctx.mlp_train_frame(h, s.camera, W, H, train_spp, depth, rr, seed=9000 + f, max_samples=1 << 18,
                                    steps=steps)
-        n_prof += n * steps
+        n_prof += n
     ctx.set_option("profile", 0)
     st = ctx.stats()
     train_k = [k for k in ("mlp_train",) if k in st]
@@ -638,26 +657,34 @@ def mlp_mode(args):
     mlp_roofline = {"bound": "mfma", "kernel": "+".join(train_k), "achieved": round(achieved, 2),
                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4),
                     "flop_per_sample": MLP_TRAIN_FLOP, "samples": int(n_prof),
-                    "ms_per_step": round(k_ms / (2 * steps), 4),
+                    "ms_per_step": round(k_ms / (2 * steps), 4), "samples_per_step": int(n_prof // (2 * steps)),
                     "adam_ms_per_step": round(st.get("mlp_adam", {}).get("ms", 0.0) / (2 * steps), 4),
                     "per_kernel_ms": {k: round(st[k]["ms"] / (2 * steps), 4) for k in train_k}}
     log(f"MLP training: {achieved:.1f} TFLOP/s over {train_k}, {mlp_roofline['ms_per_step']} ms/step")
     budgets = {}
     for spp in (16, 256):
-        img_p, t_p = timed_render(spp, 100 + spp)                  # plain at this spp
-        img_m, t_m = timed_render(spp, 200 + spp, mode=1)          # MLP form at this spp
-        spp_eq = max(1, int(round(spp * t_m / t_p)))               # plain given the MLP render's time
-        img_e, t_e = timed_render(spp_eq, 300 + spp)
-        spp_tot = max(1, int(round(spp * (t_m + t_train) / t_p)))  # plain given render + all training time
-        img_t, t_t = timed_render(spp_tot, 400 + spp)
+        r_p, img_p, t_p = timed_render(spp, 100 + spp)                  # plain at this spp
+        r_m, img_m, t_m = timed_render(spp, 200 + spp, mode=1)          # MLP form at this spp
+        spp_eq = max(1, int(round(spp * t_m / t_p)))                    # plain given the MLP render's time
+        r_e, _, t_e = timed_render(spp_eq, 300 + spp)
+        spp_tot = max(1, int(round(spp * (t_m + t_train) / t_p)))       # plain given render + all training time
+        r_t, _, t_t = timed_render(spp_tot, 400 + spp)
+        # training amortised over a sequence of K frames of the scene (one training, K MLP renders)
+        amort = {}
+        for K in (8, 32, 128):
+            spp_k = max(1, int(round(spp * (t_m + t_train / K) / t_p)))
+            r_k, _, _ = timed_render(spp_k, 500 + spp + K)
+            amort[f"K{K}"] = {"plain_spp": spp_k, "plain_rmse": round(r_k, 5), "mlp_wins": bool(r_m < r_k)}
         budgets[f"{spp}spp"] = {
-            "mlp": {"rmse": round(rmse(img_m), 5), "ms": round(t_m * 1e3, 2)},
-            "plain_same_spp": {"rmse": round(rmse(img_p), 5), "ms": round(t_p * 1e3, 2)},
-            "plain_equal_render_time": {"spp": spp_eq, "rmse": round(rmse(img_e), 5), "ms": round(t_e * 1e3, 2)},
-            "plain_equal_time_with_training": {"spp": spp_tot, "rmse": round(rmse(img_t), 5), "ms": round(t_t * 1e3, 2)},
-            "mlp_wins_equal_render_time": bool(rmse(img_m) < rmse(img_e)),
-            "mlp_wins_counting_training": bool(rmse(img_m) < rmse(img_t)),
-            "mlp_mean_ratio": round(float(img_m[:, :3].mean() / img_p[:, :3].mean()), 4)}
+            "mlp": {"rmse": round(r_m, 5), "ms": round(t_m * 1e3, 2)},
+            "plain_same_spp": {"rmse": round(r_p, 5), "ms": round(t_p * 1e3, 2)},
+            "plain_equal_render_time": {"spp": spp_eq, "rmse": round(r_e, 5), "ms": round(t_e * 1e3, 2)},
+            "plain_equal_time_with_training": {"spp": spp_tot, "rmse": round(r_t, 5), "ms": round(t_t * 1e3, 2)},
+            "mlp_wins_equal_render_time": bool(r_m < r_e),
+            "mlp_wins_counting_training": bool(r_m < r_t),
+            "training_amortised_over_K_frames": amort,
+            "mlp_mean_ratio": round(float(img_m.mean() / img_p.mean()), 4),
+            "mlp_mean_vs_golden": round(float(img_m.mean() / gold.mean()), 4)}
     # MLP inference rate inside a 256-spp MLP-form render (per-kernel HIP events, query count read back)
     ctx.set_option("profile", 1)
     ctx.stats_reset()
@@ -674,11 +701,12 @@ def mlp_mode(args):
            "config": {"workload": workload + f", {train_frames} training frames x {steps} steps, up to 2^18 "
                                              "samples/frame",
                       "golden": os.path.relpath(gold_path, ROOT)},
-           "training": {"ms": round(t_train * 1e3, 1), "loss_first": round(float(losses[0]), 4),
+           "training": {"ms": round(t_train * 1e3, 1), "frames": train_frames, "decay_frames": args.mlp_decay_frames,
+                        "loss_first": round(float(losses[0]), 4),
                         "loss_last": round(float(losses[-1]), 4)},
            "roofline": mlp_roofline,
            "inference": inference,
-           "budgets": budgets}
+           "budgets": budgets, "rmse_seeds_per_point": reps}
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -901,6 +929,8 @@ def main():
     ap.add_argument("--mode", choices=["frame", "copy", "skin", "mlp", "serve", "passes", "anim"], default="frame")
     ap.add_argument("--mlp-frames", type=int, default=16)
     ap.add_argument("--mlp-steps", type=int, default=8)
+    ap.add_argument("--mlp-decay-frames", type=int, default=0,
+                    help="MLP mode: training frames at lr 1e-4 after the first ones")
     ap.add_argument("--mlp-scene", choices=["cornell", "spheres"], default="cornell")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-window", type=float, default=30.0)
