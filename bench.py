@@ -247,6 +247,7 @@ KSTATS_FILE = os.path.join(ROOT, "profiles", "frame_spheres1m_kernel_stats.md")
 
 PASSES_KSTATS_FILE = os.path.join(ROOT, "profiles", "passes_spheres1m_kernel_stats.md")
 SKIN_KSTATS_FILE = os.path.join(ROOT, "profiles", "skin_tube200k_kernel_stats.md")
+INST_KSTATS_FILE = os.path.join(ROOT, "profiles", "frame_spheres1m_instanced_kernel_stats.md")
 
 
 def kstats_summary(d, mode="frame"):
@@ -293,6 +294,18 @@ def kstats_summary(d, mode="frame"):
                  "k_lt_shade, k_lt_shadow), Denoise (k_denoise_guide, 5 x k_atrous); the BVH build runs once.", "",
                  "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
         out_file = PASSES_KSTATS_FILE
+    elif mode == "instanced":
+        lines = ["# Kernel times: 1M-triangle frame in two-level form (1920x1080x64spp, depth 8)", "",
+                 "Command (from the repository root; TMPDIR=/tmp):", "",
+                 "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_inst -o run -- python3 bench.py --steps 2 "
+                 "--warmup 1 --no-cpu --no-serve --scene spheres_instanced",
+                 "    python3 bench.py --kstats-summary gpurun_out/prof_inst --kstats-mode instanced", "",
+                 "16 icosphere meshes (320 triangles each) under 3125 instances plus 4 light instances; the run",
+                 "renders 6 frames (the counting pass, 1 warmup, 2 timed, 2 copied to host memory) with the",
+                 "instanced traversal kernels (k_trace<.., .., true>, k_shadow<.., .., true>); the per-mesh trees",
+                 "are built once by the same kernels as the flat build.", "",
+                 "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
+        out_file = INST_KSTATS_FILE
     else:
         lines = ["# Kernel times: 1M-triangle frame (1920x1080x64spp, depth 8)", "",
                  "Command (from the repository root; TMPDIR=/tmp):", "",
@@ -943,7 +956,7 @@ def main():
     ap.add_argument("--pmc-summary", nargs=4, metavar="DIR")
     ap.add_argument("--kstats-summary", metavar="DIR")
     ap.add_argument("--valu-summary", metavar="DIR")
-    ap.add_argument("--kstats-mode", choices=["frame", "passes", "skin"], default="frame")
+    ap.add_argument("--kstats-mode", choices=["frame", "passes", "skin", "instanced"], default="frame")
     ap.add_argument("--mlp-summary", nargs=2, metavar="DIR")
     ap.add_argument("--scene", choices=["spheres", "spheres_instanced"], default="spheres",
                     help="frame mode: the flat 1M-triangle field, or the same field as 16 meshes under 3129 "

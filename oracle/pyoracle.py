@@ -128,6 +128,9 @@ class Scene:
     """Oracle copy of a SceneData (srt_amd.scenes)."""
 
     def __init__(self, s):
+        mf = np.asarray(s.mesh_first)
+        if np.any(np.diff(mf) <= 0):     # as srt_scene_upload: every mesh holds triangles
+            raise ValueError("a mesh has no triangles")
         self.data = s
         self._arrs = dict(
             pos=np.ascontiguousarray(s.positions, np.float32), idx=np.ascontiguousarray(s.indices, np.uint32),

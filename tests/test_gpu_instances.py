@@ -202,3 +202,46 @@ def test_instanced_scene_refusals(ctx):
     with pytest.raises(abi.SrtError):
         ctx.update_instances(h, np.zeros((s.inst_xf.shape[0], 3, 4), np.float32))   # singular
     ctx.release(h)
+
+
+def test_upload_refuses_a_mesh_without_triangles(ctx):
+    _say("[inst] empty mesh")
+    b = scenes._Builder()
+    b.box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), 0)
+    s = b.finish([scenes.material()], with_normals=False)
+    s.mesh_first = np.array([0, 12, 12], np.int32)
+    s.mesh_material = np.array([0, 0], np.int32)
+    s.inst_mesh = np.array([0, 1], np.int32)
+    s.inst_xf = np.stack([scenes.affine(), scenes.affine(None, (3.0, 0.0, 0.0))])
+    with pytest.raises(abi.SrtError):
+        ctx.upload(s)
+
+
+def test_serving_instanced_scenes_mix_equals_alone():
+    """Requests on instanced and flattened scenes through the serving loop
+    render the same bits as alone."""
+    _say("[inst] serving")
+    c = abi.Context(0)
+    try:
+        ci = scenes.cornell_instanced()
+        cf = scenes.cornell()
+        si = _scene("spheres")
+        hi = c.upload(ci)
+        hf = c.upload(cf)
+        hs = c.upload(si)
+        for h in (hi, hf, hs):
+            c.build_bvh(h)
+        reqs = [(hi, ci.camera, 48, 40, 4, 5, 21), (hf, cf.camera, 32, 32, 8, 5, 22), (hs, si.camera, 40, 30, 3, 6, 23),
+                (hi, ci.camera, 24, 24, 16, 4, 24)]
+        alone = [c.render(h, cam, W, H, spp, d, 4, 1, 1, seed) for (h, cam, W, H, spp, d, seed) in reqs]
+        c.serve_start(5000)
+        try:
+            tickets = [c.submit(h, cam, W, H, spp, d, 4, 1, 1, seed) for (h, cam, W, H, spp, d, seed) in reqs]
+            for k, (t, img) in enumerate(tickets):
+                res = c.wait(t)
+                assert res["status"] == 2, (k, res)
+                assert np.array_equal(img, alone[k]), k
+        finally:
+            c.serve_stop()
+    finally:
+        c.close()

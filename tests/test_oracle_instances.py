@@ -195,3 +195,15 @@ def test_instanced_scene_box_and_depth_range():
     # a pixel next to the centre sees the front face at distance 9 / cos(angle < 0.05)
     d = (img.reshape(16, 16, 4)[8, 8, 0] - 0.2) / 0.8 * (hi - lo) + lo
     assert 9.0 - 1e-4 <= d <= 9.0 / np.cos(0.05)
+
+
+def test_mesh_without_triangles_is_refused():
+    """Every mesh holds triangles (srt_scene_upload refuses an empty one; so
+    does the oracle's scene)."""
+    b = scenes._Builder()
+    b.box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), 0)
+    s = b.finish([scenes.material()], with_normals=False)
+    s.mesh_first = np.array([0, 12, 12], np.int32)
+    s.mesh_material = np.array([0, 0], np.int32)
+    with pytest.raises(ValueError):
+        O.Scene(s)
