@@ -245,3 +245,34 @@ def test_serving_instanced_scenes_mix_equals_alone():
             c.serve_stop()
     finally:
         c.close()
+
+
+def test_mirrored_and_sheared_instances():
+    """Transforms with a reflection (negative determinant: the winding flips)
+    and with shear: first hits and renders equal the oracle's."""
+    _say("[inst] mirror and shear")
+    s = scenes.cornell_instanced()
+    xf = s.inst_xf.copy()
+    mirror = np.diag([-1.0, 1.0, 1.0]) @ (xf[6, :, :3].astype(np.float64))
+    xf[6, :, :3] = mirror.astype(np.float32)
+    shear = np.array([[1.0, 0.35, 0.0], [0.0, 1.0, 0.0], [0.2, 0.0, 1.0]]) @ xf[7, :, :3].astype(np.float64)
+    xf[7, :, :3] = shear.astype(np.float32)
+    s.inst_xf = xf
+    c = abi.Context(0)
+    try:
+        h = c.upload(s)
+        c.build_bvh(h)
+        osc = O.Scene(s)
+        ob = O.Bvh(osc)
+        o, d = _rays(s, 20000, 3)
+        t, u, v, p, i = c.trace_inst(h, o, d)
+        t0, u0, v0, p0, i0 = O.trace_inst(osc, o, d, bvh=ob)
+        assert np.array_equal(p, p0) and np.array_equal(i, i0)
+        hit = p0 >= 0
+        assert np.isin(i0[hit], [6, 7]).any()
+        assert np.array_equal(t[hit].view(np.uint32), t0[hit].view(np.uint32))
+        img = c.render(h, s.camera, 48, 48, 8, 5, 4, 1, 1, 5)
+        ref, _ = O.render(osc, s.camera, 48, 48, 8, 5, 4, 1, 1, 5, bvh=ob)
+        assert _rel(img[:, :3], ref[:, :3]).max() <= TOL
+    finally:
+        c.close()
