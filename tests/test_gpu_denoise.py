@@ -32,11 +32,19 @@ def test_lowers_error_against_the_fp64_golden(ctx):
     img = ctx.render(h, s.camera, W, H, 16, 5, 4, 1, 1, 21)
     ctx.release(h)
     gold = np.load(GOLD)
+    out = ctx.denoise(img, W, H)
     raw = np.sqrt(np.mean((img[:, :3].astype(np.float64) - gold) ** 2))
-    den = np.sqrt(np.mean((ctx.denoise(img, W, H)[:, :3].astype(np.float64) - gold) ** 2))
-    # the Cornell frame's error is mostly bright, isolated paths that an edge-avoiding filter keeps:
-    # the gain is modest (0.925 of the raw error on the oracle's 16-spp frame)
+    den = np.sqrt(np.mean((out[:, :3].astype(np.float64) - gold) ** 2))
+    # over the whole frame the gain is modest (0.95 of the raw error on the oracle's 16-spp frame):
+    # about 85 % of the raw squared error sits on the 0.5 % of pixels within two pixels of the
+    # light, whose antialiasing noise is a true edge that an edge-avoiding filter keeps
     assert den < 0.97 * raw, (raw, den)
+    from scipy.ndimage import binary_dilation
+    near_light = binary_dilation(gold.reshape(H, W, 3).max(-1) > 1.0, iterations=2).ravel()
+    er = ((img[:, :3].astype(np.float64) - gold) ** 2)[~near_light]
+    ed = ((out[:, :3].astype(np.float64) - gold) ** 2)[~near_light]
+    # elsewhere the filter removes most of the noise (0.37 of the raw error on the oracle's frame)
+    assert np.sqrt(ed.mean()) < 0.5 * np.sqrt(er.mean()), (np.sqrt(er.mean()), np.sqrt(ed.mean()))
 
 
 def test_odd_sizes_and_device_film(ctx):
