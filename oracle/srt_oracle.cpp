@@ -372,20 +372,26 @@ void blas_build(const Scene& s, const BvhParams& prm, std::vector<Bvh>& out) {
 // ---------------------------------------------------------------------------
 namespace {
 const float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013, "Robust BVH Ray Traversal")
+// Boxes are culled against the best distance so far times 1 + 2^-12, as the
+// kernels do: a triangle's computed t can lie below its own box's computed
+// entry distance by more than the slab rounding, and the margin keeps every
+// triangle with t <= best distance in the test, so the (t, primitive) minimum
+// does not depend on the visiting order (coplanar faces).
+const float kCullTol = 1.000244140625f;
 
 // Slab test with the near/far plane of each axis chosen by the sign of the
 // inverse direction (Williams et al. 2005).  A ray lying exactly in a slab
 // plane with a zero direction component gives 0 * inf = NaN, which fmax/fmin
 // ignore, so that axis adds no constraint (conservative).
 inline bool slab(const float lo[3], const float hi[3], const float o[3], const float inv[3], float tmin, float tmax, float* tnear) {
-    float tn = tmin, tf = tmax;
+    float tn = tmin, tf = INFINITY;
     for (int a = 0; a < 3; a++) {
         float nb = (inv[a] < 0.0f) ? hi[a] : lo[a];
         float fb = (inv[a] < 0.0f) ? lo[a] : hi[a];
         tn = std::fmax(tn, (nb - o[a]) * inv[a]);
         tf = std::fmin(tf, (fb - o[a]) * inv[a]);
     }
-    tf = tf * kRobust;
+    tf = std::fmin(tf * kRobust, tmax * kCullTol);
     *tnear = tn;
     return tn <= tf;
 }

@@ -86,6 +86,15 @@ SRT_D bool intersect_tri(const RayPre& r, f3 o, f3 v0, f3 v1, f3 v2, float tmin,
 }
 
 constexpr float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013): conservative slab test
+// Boxes are culled against the best hit distance so far times 1 + 2^-12: a
+// triangle's computed hit distance can lie below its own box's computed entry
+// distance by more than the slab test's rounding (the watertight test's t
+// carries the error of the shear and of T / det), so without the margin a
+// triangle hit at nearly the same distance as the best one (coplanar faces)
+// could be culled or kept depending on the visiting order.  With it, every
+// triangle whose computed t is at most the best distance is tested, and the
+// (t, primitive) minimum does not depend on the order.
+constexpr float kCullTol = 1.000244140625f;
 
 // ---------------------------------------------------------------------------
 // wave-level queue append: one atomic per wave
@@ -117,6 +126,7 @@ struct TravState {
     f3 o, inv;
     RayPre rp;
     float tmin, best_t, bu, bv;
+    float cull_t;      // boxes entered beyond this are skipped: best_t * kCullTol
     int32_t best_prim, best_pos;
     int32_t inst, best_inst;   // instanced scenes: the instance the ray is in (-1: world), the one hit
     int32_t cur;       // next node (>= 0) or leaf (< 0) to visit; valid while cur_ok
@@ -180,6 +190,7 @@ SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t
     if (INST) stash_ray(st, d);
     st.tmin = tmin;
     st.best_t = tmax;
+    st.cull_t = tmax * kCullTol;
     st.best_prim = 0x7fffffff;
     st.best_pos = -1;
     st.inst = -1;
@@ -254,9 +265,9 @@ SRT_D void trav_inner(const DevScene& S, TravState& st, const TravStack& k, uint
             float rny = ((sy ? nb.w : nb.z) - o.y) * inv.y, rfy = ((sy ? nb.z : nb.w) - o.y) * inv.y;
             float rnz = ((sz ? nc.w : nc.z) - o.z) * inv.z, rfz = ((sz ? nc.z : nc.w) - o.z) * inv.z;
             float tln = fmaxf(fmaxf(fmaxf(st.tmin, lnx), lny), lnz);
-            float tlf = fminf(fminf(fminf(st.best_t, lfx), lfy), lfz) * kRobust;
+            float tlf = fminf(fminf(fminf(lfx, lfy), lfz) * kRobust, st.cull_t);
             float trn = fmaxf(fmaxf(fmaxf(st.tmin, rnx), rny), rnz);
-            float trf = fminf(fminf(fminf(st.best_t, rfx), rfy), rfz) * kRobust;
+            float trf = fminf(fminf(fminf(rfx, rfy), rfz) * kRobust, st.cull_t);
             const bool hl = tln <= tlf, hr = trn <= trf;
             const bool swap = hr && (!hl || trn < tln);        // right child first
             const int32_t first = swap ? nd.y : nd.x, second = swap ? nd.x : nd.y;
@@ -279,7 +290,7 @@ SRT_D bool slab(const TravState& st, float bnx, float bfx, float bny, float bfy,
     const float ny = (bny - o.y) * inv.y, fy = (bfy - o.y) * inv.y;
     const float nz = (bnz - o.z) * inv.z, fz = (bfz - o.z) * inv.z;
     const float tnear = fmaxf(fmaxf(fmaxf(st.tmin, nx), ny), nz);
-    const float tfar = fminf(fminf(fminf(st.best_t, fx), fy), fz) * kRobust;
+    const float tfar = fminf(fminf(fminf(fx, fy), fz) * kRobust, st.cull_t);
     *tn = tnear;
     return tnear <= tfar;
 }
@@ -411,6 +422,7 @@ SRT_D bool trav_leaf(const DevScene& S, TravState& st, uint32_t& nt) {
         if (t < st.best_t ||
             (t == st.best_t && (prim < st.best_prim || (INST && prim == st.best_prim && st.inst < st.best_inst)))) {
             st.best_t = t; st.best_prim = prim; st.best_pos = (int32_t)j; st.bu = u; st.bv = v;
+            st.cull_t = t * kCullTol;
             if (INST) st.best_inst = st.inst;
         }
     }
