@@ -244,3 +244,30 @@ def test_watertight_closed_mesh():
     d2 = targets - o2
     d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
     assert np.all(O.trace(osc, o2, d2, bvh=b)[3] >= 0)
+
+
+def test_coplanar_faces_closest_equals_brute_force():
+    """Rays from inside the Cornell boxes toward the floor: each box's bottom
+    face is coplanar with the floor, so two triangles are hit at nearly the
+    same computed distance.  The tree traversal (culling at the best distance
+    x (1 + 2^-12)) returns brute force's (t, primitive) minimum, also against
+    the wall that a box touches."""
+    s = scenes.cornell()
+    osc = O.Scene(s)
+    ob = O.Bvh(osc)
+    rng = np.random.default_rng(12)
+    n = 20000
+    mf = s.mesh_first
+    o = []
+    for m in (5, 6):                              # the two boxes
+        P = s.positions[s.indices[mf[m]:mf[m + 1]].ravel()]
+        lo, hi = P.min(0), P.max(0)
+        o.append(rng.uniform(lo, hi, size=(n // 2, 3)))
+    o = np.concatenate(o).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d[:, 1] = -np.abs(d[:, 1]) * rng.uniform(0.02, 1.0, size=n)     # downward, many at grazing angles
+    d = d.astype(np.float32)
+    a = O.trace(osc, o, d, bvh=ob)
+    b = O.trace(osc, o, d, bvh=None)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
