@@ -372,18 +372,20 @@ void blas_build(const Scene& s, const BvhParams& prm, std::vector<Bvh>& out) {
 // ---------------------------------------------------------------------------
 namespace {
 const float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013, "Robust BVH Ray Traversal")
-// Boxes are culled against the best distance so far times 1 + 2^-12, as the
-// kernels do: a triangle's computed t can lie below its own box's computed
-// entry distance by more than the slab rounding, and the margin keeps every
-// triangle with t <= best distance in the test, so the (t, primitive) minimum
-// does not depend on the visiting order (coplanar faces).
+// Closest-hit traversals cull boxes at the best distance so far times
+// 1 + 2^-12, as the kernels do: a triangle's computed t can lie below its own
+// box's computed entry distance by more than the slab rounding, and the margin
+// keeps every triangle with t <= best distance in the test, so the
+// (t, primitive) minimum does not depend on the visiting order (coplanar
+// faces).  Any-hit traversals cull at tmax (their answer is order-independent).
 const float kCullTol = 1.000244140625f;
 
 // Slab test with the near/far plane of each axis chosen by the sign of the
 // inverse direction (Williams et al. 2005).  A ray lying exactly in a slab
 // plane with a zero direction component gives 0 * inf = NaN, which fmax/fmin
 // ignore, so that axis adds no constraint (conservative).
-inline bool slab(const float lo[3], const float hi[3], const float o[3], const float inv[3], float tmin, float tmax, float* tnear) {
+inline bool slab(const float lo[3], const float hi[3], const float o[3], const float inv[3], float tmin, float tmax,
+                 float* tnear, bool closest = true) {
     float tn = tmin, tf = INFINITY;
     for (int a = 0; a < 3; a++) {
         float nb = (inv[a] < 0.0f) ? hi[a] : lo[a];
@@ -391,7 +393,7 @@ inline bool slab(const float lo[3], const float hi[3], const float o[3], const f
         tn = std::fmax(tn, (nb - o[a]) * inv[a]);
         tf = std::fmin(tf, (fb - o[a]) * inv[a]);
     }
-    tf = std::fmin(tf * kRobust, tmax * kCullTol);
+    tf = std::fmin(tf * kRobust, tmax * (closest ? kCullTol : kRobust));
     *tnear = tn;
     return tn <= tf;
 }
@@ -427,8 +429,8 @@ static bool trace_impl(const Bvh& bvh, const float o[3], const float d[3], float
         float llo[3] = {nd.lx[0], nd.ly[0], nd.lz[0]}, lhi[3] = {nd.lx[1], nd.ly[1], nd.lz[1]};
         float rlo[3] = {nd.rx[0], nd.ry[0], nd.rz[0]}, rhi[3] = {nd.rx[1], nd.ry[1], nd.rz[1]};
         float tl, tr_;
-        bool hl = slab(llo, lhi, o, inv, tmin, best_t, &tl);
-        bool hr = slab(rlo, rhi, o, inv, tmin, best_t, &tr_);
+        bool hl = slab(llo, lhi, o, inv, tmin, best_t, &tl, !ANY);
+        bool hr = slab(rlo, rhi, o, inv, tmin, best_t, &tr_, !ANY);
         if (hl && hr) {
             if (tr_ < tl) { stack[sp++] = nd.left; stack[sp++] = nd.right; }
             else { stack[sp++] = nd.right; stack[sp++] = nd.left; }

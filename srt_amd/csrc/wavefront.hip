@@ -86,7 +86,8 @@ SRT_D bool intersect_tri(const RayPre& r, f3 o, f3 v0, f3 v1, f3 v2, float tmin,
 }
 
 constexpr float kRobust = 1.0000004f;   // 1 + 2 gamma_3 (Ize 2013): conservative slab test
-// Boxes are culled against the best hit distance so far times 1 + 2^-12: a
+// Closest-hit traversals cull boxes at the best hit distance so far times
+// 1 + 2^-12 (any-hit traversals at tmax, as the slab test's far plane): a
 // triangle's computed hit distance can lie below its own box's computed entry
 // distance by more than the slab test's rounding (the watertight test's t
 // carries the error of the shear and of T / det), so without the margin a
@@ -126,7 +127,7 @@ struct TravState {
     f3 o, inv;
     RayPre rp;
     float tmin, best_t, bu, bv;
-    float cull_t;      // boxes entered beyond this are skipped: best_t * kCullTol
+    float cull_t;      // boxes entered beyond this are skipped: best_t * kCullTol (any hit: tmax * kRobust)
     int32_t best_prim, best_pos;
     int32_t inst, best_inst;   // instanced scenes: the instance the ray is in (-1: world), the one hit
     int32_t cur;       // next node (>= 0) or leaf (< 0) to visit; valid while cur_ok
@@ -185,12 +186,13 @@ SRT_D void restore_world_ray(TravState& st) {
 }
 
 template <bool INST = false>
-SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root) {
+SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t root, bool closest = true) {
     set_ray(st, o, d);
     if (INST) stash_ray(st, d);
     st.tmin = tmin;
     st.best_t = tmax;
-    st.cull_t = tmax * kCullTol;
+    // any-hit rays cull at tmax (their answer does not depend on the order)
+    st.cull_t = tmax * (closest ? kCullTol : kRobust);
     st.best_prim = 0x7fffffff;
     st.best_pos = -1;
     st.inst = -1;
@@ -685,7 +687,8 @@ struct ShadowIO {      // any hit in [1e-3, tmax]; unoccluded rays add their con
     float *Lr, *Lg, *Lb;
     template <bool INST>
     SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
-        trav_begin<INST>(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root);
+        trav_begin<INST>(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root,
+                         false);
     }
     SRT_D void store(uint32_t i, const TravState&, bool occluded) const {
         if (occluded) return;
@@ -1199,7 +1202,7 @@ struct RaysIO {        // standalone queries on flat arrays
     int any;
     template <bool INST>
     SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
-        trav_begin<INST>(st, ld3(o3 + 3 * (size_t)i), ld3(d3 + 3 * (size_t)i), tmin, tmax[i], root);
+        trav_begin<INST>(st, ld3(o3 + 3 * (size_t)i), ld3(d3 + 3 * (size_t)i), tmin, tmax[i], root, !any);
     }
     SRT_D void store(uint32_t i, const TravState& st, bool occluded) const {
         if (any) {
@@ -1492,7 +1495,8 @@ struct LtShadowIO {    // camera connections: unoccluded ones splat their contri
     float4* film;
     template <bool INST>
     SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
-        trav_begin<INST>(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root);
+        trav_begin<INST>(st, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), 1e-3f, sq.tmax[i], root,
+                         false);
     }
     SRT_D void store(uint32_t i, const TravState&, bool occluded) const {
         if (!occluded) splat(film, (int)sq.slot[i], mk(sq.cr[i], sq.cg[i], sq.cb[i]));
