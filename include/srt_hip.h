@@ -95,8 +95,9 @@ typedef struct srt_pt_params {
 } srt_pt_params;
 
 typedef struct srt_bvh_opts {
-    int32_t max_leaf;      /* 1..8, default 4 */
-    int32_t cluster_size;  /* 0: automatic (max(64, n/16384)) */
+    int32_t max_leaf;      /* 1..8; 0: the default, 2 */
+    int32_t cluster_size;  /* 0: automatic (max(4, n/262144) triangles per cluster under the binned-SAH top);
+                            * at least max_leaf either way */
 } srt_bvh_opts;
 
 /* ---- context (replaces OptixRayTracer's context/stream, optix_ray_tracer.cu:37-56) ---- */

@@ -194,7 +194,7 @@ class Scene:
 
 
 class Bvh:
-    def __init__(self, scene: Scene, max_leaf=4, cluster_size=0):
+    def __init__(self, scene: Scene, max_leaf=2, cluster_size=0):
         self.scene = scene
         self.h = lib().orc_bvh_new(scene.h, max_leaf, cluster_size)
 

@@ -129,7 +129,8 @@ void bvh_build(const Scene& s, const BvhParams& prm, Bvh& out) {
     }
 
     // 5. cluster frontier: highest nodes with count <= T, left to right
-    const int T = prm.cluster_size > 0 ? prm.cluster_size : std::max(64, n / 16384);
+    // the library's automatic size; a cluster holds at least a leaf's triangles
+    const int T = std::max(prm.max_leaf, prm.cluster_size > 0 ? prm.cluster_size : std::max(4, n / 262144));
     out.cluster_size = T;
     std::vector<int32_t> clusters;     // Karras refs (>=0 internal, <0 leaf)
     if (n == 1) clusters.push_back(-1);

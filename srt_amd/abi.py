@@ -261,7 +261,7 @@ class Context:
         self.check(self.lib.srt_scene_lights(self.h, scene, _ptr(cdf), _ptr(mesh), _ptr(acdf), _ptr(marea), C.byref(ws)))
         return cdf[:n], mesh[:n], acdf, marea, ws.value
 
-    def build_bvh(self, scene, max_leaf=4, cluster_size=0):
+    def build_bvh(self, scene, max_leaf=0, cluster_size=0):
         o = BvhOpts(max_leaf, cluster_size)
         self.check(self.lib.srt_scene_build_bvh(self.h, scene, C.byref(o)))
 

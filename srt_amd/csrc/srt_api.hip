@@ -384,7 +384,7 @@ int srt_scene_lights(srt_ctx* c, srt_scene h, float* light_cdf, int32_t* light_m
 // instanced scenes, the single tree otherwise (leaves of up to 4 triangles).
 static int ensure_bvh(srt_ctx* c, SceneRec* s) {
     if (s->has_bvh) return SRT_OK;
-    return s->ninst > 0 ? inst_build(c, s, 4, 0) : bvh_build(c, s, 4, 0);
+    return s->ninst > 0 ? inst_build(c, s, kDefaultMaxLeaf, 0) : bvh_build(c, s, kDefaultMaxLeaf, 0);
 }
 
 int srt_scene_build_bvh(srt_ctx* c, srt_scene h, const srt_bvh_opts* o) {
@@ -392,9 +392,9 @@ int srt_scene_build_bvh(srt_ctx* c, srt_scene h, const srt_bvh_opts* o) {
     if (!c) return SRT_E_INVALID;
     SceneRec* s = find_scene(c, h);
     if (!s) return set_err(c, SRT_E_NOT_FOUND, "unknown scene");
-    int max_leaf = o ? o->max_leaf : 4;
+    int max_leaf = o ? o->max_leaf : kDefaultMaxLeaf;
     int cluster = o ? o->cluster_size : 0;
-    if (max_leaf <= 0) max_leaf = 4;
+    if (max_leaf <= 0) max_leaf = kDefaultMaxLeaf;
     hipEvent_t ev;
     prof_begin(c, "bvh_build", &ev);
     int r = s->ninst > 0 ? inst_build(c, s, max_leaf, cluster) : bvh_build(c, s, max_leaf, cluster);

@@ -49,6 +49,8 @@ __host__ __device__ inline uint32_t leaf_first(int32_t r) { return ((uint32_t)r 
 __host__ __device__ inline uint32_t leaf_count(int32_t r) { return ((uint32_t)r & 7u) + 1u; }
 
 constexpr int kMaxLeaf = 8;
+constexpr int kDefaultMaxLeaf = 2;     // triangles per leaf when the caller gives none
+constexpr int kAutoClusterMin = 4, kAutoClusterDiv = 262144;   // automatic top-level cluster size
 constexpr int kMaxDepth = 96;      // checked after every build
 constexpr int kTraceBlock = 128;   // traversal workgroup size (2 waves)
 

@@ -54,12 +54,17 @@ def test_leaf_sizes(ctx, leaf):
     _compare(ctx, _soup(3000, seed=4), max_leaf=leaf, cluster_size=64)
 
 
+def test_cluster_size_below_leaf_size_is_raised(ctx):
+    info = _compare(ctx, _soup(2000, seed=11), max_leaf=8, cluster_size=2)
+    assert info["cluster_size"] == 8
+
+
 def test_cornell(ctx):
     _compare(ctx, scenes.cornell())
 
 
 def test_spheres_1m(ctx):
-    info = _compare(ctx, scenes.spheres())
+    info = _compare(ctx, scenes.spheres(), max_leaf=2)     # the library's defaults
     assert info["triangles"] == 1000008
     assert info["clusters"] > 100
 

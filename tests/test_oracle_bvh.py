@@ -271,3 +271,15 @@ def test_coplanar_faces_closest_equals_brute_force():
     b = O.trace(osc, o, d, bvh=None)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_cluster_size_holds_at_least_a_leaf():
+    """A cluster size below the leaf size is raised to it (a collapsed leaf
+    never spans the cluster frontier), so the build is the clamped one."""
+    s = soup(2000, seed=11, size=1.5)
+    osc = O.Scene(s)
+    a = O.Bvh(osc, max_leaf=8, cluster_size=2)
+    b = O.Bvh(osc, max_leaf=8, cluster_size=8)
+    assert a.info() == b.info() and a.info()["cluster_size"] == 8
+    for x, y in zip(a.arrays(), b.arrays()):
+        assert np.array_equal(x, y)

@@ -18,7 +18,6 @@ import pytest
 from srt_amd import scenes
 from oracle import pyoracle as O
 
-RNG = np.random.default_rng(12345)
 
 
 def _hemi(n, rng):
@@ -41,10 +40,11 @@ DISNEY_REFLECT = [
 
 
 def test_lambert_closed_form():
+    rng_local = np.random.default_rng(101)      # its own stream, independent of the test order
     m = scenes.material(type=scenes.BXDF_DIFFUSE)
     col = (0.2, 0.5, 0.9)
-    wo = _hemi(100, RNG)
-    wi = _hemi(100, RNG)
+    wo = _hemi(100, rng_local)
+    wi = _hemi(100, rng_local)
     f = O.bxdf(m, "eval", wi=wi, wo=wo, color=col)
     assert np.allclose(f, np.asarray(col) / np.pi, rtol=1e-12)
     p = O.bxdf(m, "pdf", wi=wi, wo=wo, color=col)
@@ -55,10 +55,11 @@ def test_lambert_closed_form():
 
 @pytest.mark.parametrize("params", DISNEY_REFLECT)
 def test_disney_reciprocity(params):
+    rng_local = np.random.default_rng(102)      # its own stream, independent of the test order
     m = scenes.material(**params)
     col = (0.8, 0.3, 0.1)
-    wo = _hemi(2000, RNG)
-    wi = _hemi(2000, RNG)
+    wo = _hemi(2000, rng_local)
+    wi = _hemi(2000, rng_local)
     a = O.bxdf(m, "eval", wi=wi, wo=wo, color=col)
     b = O.bxdf(m, "eval", wi=wo, wo=wi, color=col)
     assert np.allclose(a, b, rtol=1e-10, atol=0)
@@ -66,11 +67,12 @@ def test_disney_reciprocity(params):
 
 @pytest.mark.parametrize("params", DISNEY_REFLECT + [dict(roughness=0.5, spec_trans=0.8)])
 def test_disney_pdf_of_sample(params):
+    rng_local = np.random.default_rng(103)      # its own stream, independent of the test order
     m = scenes.material(**params)
     col = (0.6, 0.6, 0.2)
     n = 5000
-    wo = _hemi(n, RNG)
-    rnd = RNG.uniform(size=(n, 2))
+    wo = _hemi(n, rng_local)
+    rnd = rng_local.uniform(size=(n, 2))
     s = O.bxdf(m, "sample", wo=wo, rnd=rnd, color=col)
     ok = s["pdf"] > 0
     assert ok.mean() > 0.5
@@ -97,10 +99,11 @@ def test_sampler_unbiased_cosine_integral(params):
 
 
 def test_float_path_tracks_fp64():
+    rng = np.random.default_rng(29)      # its own stream: the inputs do not depend on the test order
     for params in DISNEY_REFLECT:
         m = scenes.material(**params)
-        wo = _hemi(500, RNG)
-        wi = _hemi(500, RNG)
+        wo = _hemi(500, rng)
+        wi = _hemi(500, rng)
         a = O.bxdf(m, "eval", wi=wi, wo=wo, color=(0.7, 0.4, 0.2), fp64=True)
         b = O.bxdf(m, "eval", wi=wi, wo=wo, color=(0.7, 0.4, 0.2), fp64=False)
         big = a > 1e-3
