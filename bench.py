@@ -132,8 +132,8 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
     fetch, write = per_kernel(fetch_dir, "FETCH_SIZE"), per_kernel(write_dir, "WRITE_SIZE")
     lines = ["# HBM traffic per launch of the frame kernels (1M-triangle frame, 1920x1080x64spp, depth 8)", "",
              "Commands (from the repository root, then this summary):", "",
-             "    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_FETCH_SIZE -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu",
-             "    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_WRITE_SIZE -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu",
+             "    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_FETCH_SIZE -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu --no-serve",
+             "    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_WRITE_SIZE -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu --no-serve",
              "    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_copy_FETCH_SIZE -o run -- python3 bench.py --mode copy",
              "    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_copy_WRITE_SIZE -o run -- python3 bench.py --mode copy",
              "    python3 bench.py --pmc-summary gpurun_out/pmc_FETCH_SIZE gpurun_out/pmc_WRITE_SIZE "
@@ -179,7 +179,7 @@ def valu_summary(d):
     lines = ["# VALU issue of the frame kernels (1M-triangle frame, 1920x1080x64spp, depth 8)", "",
              "Commands (from the repository root, then this summary):", "",
              "    rocprofv3 --pmc " + " ".join(names) + " -d gpurun_out/pmc_valu -o run -- python3 bench.py "
-             "--steps 1 --warmup 0 --no-cpu",
+             "--steps 1 --warmup 0 --no-cpu --no-serve",
              "    python3 bench.py --valu-summary gpurun_out/pmc_valu", "",
              "The rocprofv3 run: cd /tmp && export TMPDIR=/tmp, bench.py by its path in the repository; the counting",
              "pass's k_trace<true, true>/k_shadow<true, true> are not counted here (its k_shade dispatches are).",
@@ -309,7 +309,7 @@ def kstats_summary(d, mode="frame"):
     else:
         lines = ["# Kernel times: 1M-triangle frame (1920x1080x64spp, depth 8)", "",
                  "Command (from the repository root; TMPDIR=/tmp):", "",
-                 "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_ks -o run -- python3 bench.py --steps 2 --warmup 1 --no-cpu",
+                 "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_ks -o run -- python3 bench.py --steps 2 --warmup 1 --no-cpu --no-serve",
                  "    python3 bench.py --kstats-summary gpurun_out/prof_ks", "",
                  "The run renders 6 frames: the untimed counting pass (k_trace<true, true>, k_shadow<true, true>), 1 warmup,",
                  "2 timed frames and 2 frames copied to host memory; the BVH build (k_top_split/k_top_link, k_bottom_up_level,",
