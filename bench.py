@@ -67,7 +67,8 @@ VALU_FILE = os.path.join(ROOT, "profiles", "trace_spheres1m_valu_issue.md")
 MI355X_SIMDS, MI355X_XCDS = 256 * 4, 8
 COPY_ELEMS = 1 << 28          # calibration copy: 1 GiB read + 1 GiB written (float32)
 # kernels of the frame, as rocprofv3 names them
-KERNEL_KEYS = {"trace": "k_trace<false, true>", "shadow": "k_shadow<false, true>", "shade": "k_shade("}
+# the timed (not counting) 4-wide kernels of a flat scene
+KERNEL_KEYS = {"trace": "k_trace<false, true, false>", "shadow": "k_shadow<false, true, false>", "shade": "k_shade("}
 
 
 def log(msg):
