@@ -326,29 +326,52 @@ struct TopSplit {
     float box[2][6];        // child boxes: lo xyz, hi xyz
 };
 constexpr int kSplitThreads = 256;
+static_assert(kSplitThreads == 256, "k_top_split ranks items four waves at a time");
+// min / max of ordered float keys over the 64 lanes of a wave (every lane gets them)
+SRT_D uint32_t wave_min_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+SRT_D uint32_t wave_max_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+constexpr int kSplitWaves = 4;     // kSplitThreads / 64
 __global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp, const int32_t* task_b, const int32_t* task_e,
                                                              TopSplit* out) {
     __shared__ uint32_t s_cb[6];
-    __shared__ uint32_t s_bin[3][kBins][6];
-    __shared__ int32_t s_cnt[3][kBins];
+    // bins per wave (fewer lanes contend for one LDS word), merged after binning
+    __shared__ uint32_t s_bin[kSplitWaves][3][kBins][6];
+    __shared__ int32_t s_cnt[kSplitWaves][3][kBins];
     __shared__ int32_t s_scan[kSplitThreads];
     __shared__ uint32_t s_box[2][6];
     __shared__ int32_t s_axis, s_split, s_run;
     __shared__ float s_lo[3], s_ext[3];
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     const int b = task_b[blockIdx.x], e = task_e[blockIdx.x];
-    if (tid < 6) s_cb[tid] = (tid < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
-    for (int j = tid; j < 3 * kBins * 6; j += kSplitThreads)
-        (&s_bin[0][0][0])[j] = ((j % 6) < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
-    for (int j = tid; j < 3 * kBins; j += kSplitThreads) (&s_cnt[0][0])[j] = 0;
-    if (tid < 12) (&s_box[0][0])[tid] = ((tid % 6) < 3) ? fkey(__builtin_huge_valf()) : fkey(-__builtin_huge_valf());
+    const uint32_t kinf = fkey(__builtin_huge_valf()), kninf = fkey(-__builtin_huge_valf());
+    if (tid < 6) s_cb[tid] = (tid < 3) ? kinf : kninf;
+    for (int j = tid; j < kSplitWaves * 3 * kBins * 6; j += kSplitThreads)
+        (&s_bin[0][0][0][0])[j] = ((j % 6) < 3) ? kinf : kninf;
+    for (int j = tid; j < kSplitWaves * 3 * kBins; j += kSplitThreads) (&s_cnt[0][0][0])[j] = 0;
+    if (tid < 12) (&s_box[0][0])[tid] = ((tid % 6) < 3) ? kinf : kninf;
     __syncthreads();
-    // centroid bounds
-    for (int i = b + tid; i < e; i += kSplitThreads) {
+    // centroid bounds: per-lane keys, a wave reduction, one LDS atomic per wave
+    {
+        uint32_t mn[3] = {kinf, kinf, kinf}, mx[3] = {kninf, kninf, kninf};
+        for (int i = b + tid; i < e; i += kSplitThreads)
+            for (int a = 0; a < 3; a++) {
+                const uint32_t k = fkey(item_c(it, a, i));
+                mn[a] = min(mn[a], k);
+                mx[a] = max(mx[a], k);
+            }
         for (int a = 0; a < 3; a++) {
-            uint32_t k = fkey(item_c(it, a, i));
-            atomicMin(&s_cb[a], k);
-            atomicMax(&s_cb[3 + a], k);
+            const uint32_t wmn = wave_min_u32(mn[a]), wmx = wave_max_u32(mx[a]);
+            if (lane == 0) {
+                atomicMin(&s_cb[a], wmn);
+                atomicMax(&s_cb[3 + a], wmx);
+            }
         }
     }
     __syncthreads();
@@ -368,10 +391,25 @@ __global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp
             float scale = (float)kBins / ex;
             int bi = bin_of(item_c(it, a, i), s_lo[a], scale);
             for (int q = 0; q < 3; q++) {
-                atomicMin(&s_bin[a][bi][q], fkey(ib.lo[q]));
-                atomicMax(&s_bin[a][bi][3 + q], fkey(ib.hi[q]));
+                atomicMin(&s_bin[wv][a][bi][q], fkey(ib.lo[q]));
+                atomicMax(&s_bin[wv][a][bi][3 + q], fkey(ib.hi[q]));
             }
-            atomicAdd(&s_cnt[a][bi], cnt);
+            atomicAdd(&s_cnt[wv][a][bi], cnt);
+        }
+    }
+    __syncthreads();
+    // merge the waves' bins into copy 0 (min / max of keys and integer sums: order-free)
+    for (int j = tid; j < 3 * kBins * 7; j += kSplitThreads) {
+        const int w = j % 7, ab = j / 7;
+        const int a = ab / kBins, k = ab % kBins;
+        if (w < 6) {
+            uint32_t v = s_bin[0][a][k][w];
+            for (int q = 1; q < kSplitWaves; q++) v = (w < 3) ? min(v, s_bin[q][a][k][w]) : max(v, s_bin[q][a][k][w]);
+            s_bin[0][a][k][w] = v;
+        } else {
+            int32_t v = s_cnt[0][a][k];
+            for (int q = 1; q < kSplitWaves; q++) v += s_cnt[q][a][k];
+            s_cnt[0][a][k] = v;
         }
     }
     __syncthreads();
@@ -387,9 +425,9 @@ __global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp
             int cnt = 0;
             for (int k = 0; k < kBins; k++) {
                 DBox bb;
-                for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[a][k][q]); bb.hi[q] = fromkey(s_bin[a][k][3 + q]); }
+                for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[0][a][k][q]); bb.hi[q] = fromkey(s_bin[0][a][k][3 + q]); }
                 grow(acc, bb);
-                cnt += s_cnt[a][k];
+                cnt += s_cnt[0][a][k];
                 la[k] = box_area(acc);
                 lc[k] = cnt;
             }
@@ -397,9 +435,9 @@ __global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp
             cnt = 0;
             for (int k = kBins - 1; k >= 0; k--) {
                 DBox bb;
-                for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[a][k][q]); bb.hi[q] = fromkey(s_bin[a][k][3 + q]); }
+                for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[0][a][k][q]); bb.hi[q] = fromkey(s_bin[0][a][k][3 + q]); }
                 grow(acc, bb);
-                cnt += s_cnt[a][k];
+                cnt += s_cnt[0][a][k];
                 ra[k] = box_area(acc);
                 rc[k] = cnt;
             }
@@ -421,27 +459,31 @@ __global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp
     } else {
         const float scale = (float)kBins / s_ext[axis];
         const float lo = s_lo[axis];
+        // stable partition, 256 items per step: each wave ranks its items with
+        // a ballot, the four wave counts go through LDS
+        const unsigned long long below = (1ull << lane) - 1ull;
         for (int pass = 0; pass < 2; pass++) {
             const int base = (pass == 0) ? b : b + s_run;   // s_run holds nleft after pass 0
             __syncthreads();
             int run = 0;
             for (int start = b; start < e; start += kSplitThreads) {
                 int i = start + tid;
-                int flag = 0;
+                bool flag = false;
                 if (i < e) {
                     bool left = bin_of(item_c(it, axis, i), lo, scale) <= split;
-                    flag = (pass == 0) ? (left ? 1 : 0) : (left ? 0 : 1);
+                    flag = (pass == 0) ? left : !left;
                 }
-                s_scan[tid] = flag;
+                const unsigned long long m = __ballot(flag);
+                if (lane == 0) s_scan[wv] = __popcll(m);
                 __syncthreads();
-                for (int off = 1; off < kSplitThreads; off <<= 1) {
-                    int v = (tid >= off) ? s_scan[tid - off] : 0;
-                    __syncthreads();
-                    s_scan[tid] += v;
-                    __syncthreads();
+                int wbase = 0, total = 0;
+                for (int q = 0; q < kSplitWaves; q++) {
+                    const int c = s_scan[q];
+                    wbase += (q < wv) ? c : 0;
+                    total += c;
                 }
-                if (flag) copy_item(it, i, tmp, base + run + s_scan[tid] - 1);
-                run += s_scan[kSplitThreads - 1];
+                if (flag) copy_item(it, i, tmp, base + run + wbase + __popcll(m & below));
+                run += total;
                 __syncthreads();
             }
             if (pass == 0 && tid == 0) s_run = run;
@@ -452,14 +494,27 @@ __global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp
         for (int i = b + tid; i < e; i += kSplitThreads) copy_item(tmp, i, it, i);
         __syncthreads();
     }
-    // child boxes
-    for (int i = b + tid; i < e; i += kSplitThreads) {
-        int side = (i < mid) ? 0 : 1;
-        DBox ib = ldbox(it.box, i);
-        for (int q = 0; q < 3; q++) {
-            atomicMin(&s_box[side][q], fkey(ib.lo[q]));
-            atomicMax(&s_box[side][3 + q], fkey(ib.hi[q]));
+    // child boxes: per-lane keys of each side, wave reductions, one LDS atomic per wave
+    {
+        uint32_t mn[2][3], mx[2][3];
+        for (int sd = 0; sd < 2; sd++)
+            for (int q = 0; q < 3; q++) { mn[sd][q] = kinf; mx[sd][q] = kninf; }
+        for (int i = b + tid; i < e; i += kSplitThreads) {
+            const int side = (i < mid) ? 0 : 1;
+            const DBox ib = ldbox(it.box, i);
+            for (int q = 0; q < 3; q++) {
+                mn[side][q] = min(mn[side][q], fkey(ib.lo[q]));
+                mx[side][q] = max(mx[side][q], fkey(ib.hi[q]));
+            }
         }
+        for (int sd = 0; sd < 2; sd++)
+            for (int q = 0; q < 3; q++) {
+                const uint32_t wmn = wave_min_u32(mn[sd][q]), wmx = wave_max_u32(mx[sd][q]);
+                if (lane == 0) {
+                    atomicMin(&s_box[sd][q], wmn);
+                    atomicMax(&s_box[sd][3 + q], wmx);
+                }
+            }
     }
     __syncthreads();
     if (tid < 12) out[blockIdx.x].box[tid / 6][tid % 6] = fromkey(s_box[tid / 6][tid % 6]);
