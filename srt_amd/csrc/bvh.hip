@@ -337,6 +337,66 @@ SRT_D uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 constexpr int kSplitWaves = 4;     // kSplitThreads / 64
+// SAH over the 15 bin boundaries of each axis (cost = area x count per side of
+// the boundary) from the bins' key boxes and counts: the first minimum in
+// (axis, boundary) order; axis -1 when no boundary separates items (every
+// extent zero or every item in one bin).
+// The same choice by all 64 lanes of a wave (every lane gets the result):
+// lane 16a + k holds bin k of axis a; inclusive prefix and suffix unions of
+// the bins' key boxes and counts come from shuffles within each group of 16
+// lanes (unions of keys are exact, so each prefix box equals the sequential
+// one); the cost of boundary k uses the suffix of lane k + 1; the minimum over
+// (cost, 16a + k) picks the first minimum in (axis, boundary) order, as the
+// sequential loop does with its strict comparison.
+SRT_D void sah_choose_wave(const uint32_t (*bin)[kBins][6], const int32_t (*cnt)[kBins], const float* ext, int32_t* axis,
+                           int32_t* split) {
+    static_assert(kBins == 16, "one group of 16 lanes per axis");
+    const int lane = __lane_id();
+    const int a = lane >> 4, k = lane & 15;
+    const bool on = a < 3;
+    const uint32_t kinf = fkey(__builtin_huge_valf()), kninf = fkey(-__builtin_huge_valf());
+    uint32_t pl[6], sl[6];
+    int32_t pc = 0, sc = 0;
+    for (int q = 0; q < 6; q++) pl[q] = on ? bin[a][k][q] : (q < 3 ? kinf : kninf);
+    if (on) pc = cnt[a][k];
+    for (int q = 0; q < 6; q++) sl[q] = pl[q];
+    sc = pc;
+    for (int d = 1; d < 16; d <<= 1) {
+        int32_t u = __shfl_up(pc, d, 16), w = __shfl_down(sc, d, 16);
+        if (k >= d) pc += u;
+        if (k + d < 16) sc += w;
+        for (int q = 0; q < 6; q++) {
+            const uint32_t x = (uint32_t)__shfl_up((int)pl[q], d, 16), y = (uint32_t)__shfl_down((int)sl[q], d, 16);
+            if (k >= d) pl[q] = (q < 3) ? min(pl[q], x) : max(pl[q], x);
+            if (k + d < 16) sl[q] = (q < 3) ? min(sl[q], y) : max(sl[q], y);
+        }
+    }
+    DBox lb, rb;
+    for (int q = 0; q < 3; q++) {
+        lb.lo[q] = fromkey(pl[q]); lb.hi[q] = fromkey(pl[3 + q]);
+        rb.lo[q] = fromkey(sl[q]); rb.hi[q] = fromkey(sl[3 + q]);
+    }
+    const float la = box_area(lb), ra = box_area(rb);
+    // the right side of boundary k: suffix of bin k + 1
+    const float ra1 = __shfl_down(ra, 1, 16);
+    const int32_t rc1 = __shfl_down(sc, 1, 16);
+    unsigned long long key = ~0ull;
+    if (on && k < kBins - 1 && ext[a] > 0.0f && pc != 0 && rc1 != 0) {
+        const float cost = la * (float)pc + ra1 * (float)rc1;
+        if (cost < __builtin_huge_valf()) key = ((unsigned long long)__float_as_uint(cost) << 32) | (unsigned)lane;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(key, off);
+        key = o < key ? o : key;
+    }
+    if (key == ~0ull) {
+        *axis = -1;
+        *split = -1;
+    } else {
+        *axis = (int32_t)((key & 0xffffffffu) >> 4);
+        *split = (int32_t)(key & 15u);
+    }
+}
 __global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp, const int32_t* task_b, const int32_t* task_e,
                                                              TopSplit* out) {
     __shared__ uint32_t s_cb[6];
@@ -413,43 +473,14 @@ __global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp
         }
     }
     __syncthreads();
-    if (tid == 0) {
-        // SAH over the 15 bin boundaries of each axis (cost = area x count per side)
-        float best = __builtin_huge_valf();
-        int best_axis = -1, best_split = -1;
-        for (int a = 0; a < 3; a++) {
-            if (!(s_ext[a] > 0.0f)) continue;
-            float la[kBins], ra[kBins];
-            int lc[kBins], rc[kBins];
-            DBox acc = empty_box();
-            int cnt = 0;
-            for (int k = 0; k < kBins; k++) {
-                DBox bb;
-                for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[0][a][k][q]); bb.hi[q] = fromkey(s_bin[0][a][k][3 + q]); }
-                grow(acc, bb);
-                cnt += s_cnt[0][a][k];
-                la[k] = box_area(acc);
-                lc[k] = cnt;
-            }
-            acc = empty_box();
-            cnt = 0;
-            for (int k = kBins - 1; k >= 0; k--) {
-                DBox bb;
-                for (int q = 0; q < 3; q++) { bb.lo[q] = fromkey(s_bin[0][a][k][q]); bb.hi[q] = fromkey(s_bin[0][a][k][3 + q]); }
-                grow(acc, bb);
-                cnt += s_cnt[0][a][k];
-                ra[k] = box_area(acc);
-                rc[k] = cnt;
-            }
-            for (int k = 0; k < kBins - 1; k++) {
-                if (lc[k] == 0 || rc[k + 1] == 0) continue;
-                float cost = la[k] * (float)lc[k] + ra[k + 1] * (float)rc[k + 1];
-                if (cost < best) { best = cost; best_axis = a; best_split = k; }
-            }
+    if (tid < 64) {
+        int32_t ax, sp;
+        sah_choose_wave(s_bin[0], s_cnt[0], s_ext, &ax, &sp);
+        if (tid == 0) {
+            s_axis = ax;
+            s_split = sp;
+            s_run = 0;
         }
-        s_axis = best_axis;
-        s_split = best_split;
-        s_run = 0;
     }
     __syncthreads();
     const int axis = s_axis, split = s_split;
@@ -519,6 +550,449 @@ __global__ void __launch_bounds__(kSplitThreads) k_top_split(Items it, Items tmp
     __syncthreads();
     if (tid < 12) out[blockIdx.x].box[tid / 6][tid % 6] = fromkey(s_box[tid / 6][tid % 6]);
     if (tid == 0) out[blockIdx.x].mid = mid;
+}
+
+
+// Wide levels: a level of at most kWideTasks nodes (the first levels, where
+// one workgroup per node would leave most CUs idle) is split by a grid of
+// workgroups over chunks of kWideChunk items; chunk g of the level belongs to
+// the node whose chunk range holds it.  Per-node accumulators live in global
+// memory: centroid bounds, bins and child boxes are min / max of ordered float
+// keys and integer sums (independent of the order of the atomics), the split
+// is chosen by sah_choose_wave as above, and the partition places each chunk's
+// items after the left / right items of the node's earlier chunks, ranked
+// within the chunk by wave ballots: the same split, the same stable partition
+// and the same child boxes as k_top_split.
+constexpr int kWideTasks = 64;
+constexpr int kWideChunk = 1024;      // 4 items per thread
+constexpr int kWideMaxGrid = 1024;
+struct WideAcc {
+    uint32_t cb[kWideTasks][6];                 // centroid bounds (keys): lo xyz, hi xyz
+    uint32_t bin[kWideTasks][3][kBins][6];      // bin boxes (keys)
+    int32_t cnt[kWideTasks][3][kBins];          // bin triangle counts
+    int32_t axis[kWideTasks], split[kWideTasks], nleft[kWideTasks];
+    float lo[kWideTasks][3], ext[kWideTasks][3];
+    uint32_t box[kWideTasks][2][6];             // child boxes (keys)
+};
+// Chunk ranges of the level's nodes (wave 0 scans the chunk counts; ntask <=
+// 64) into s_first[0..ntask]; returns the number of chunks.
+SRT_D int wide_chunks(int ntask, const int32_t* tb, const int32_t* te, int32_t* s_first) {
+    if (threadIdx.x < 64u) {
+        const int t = threadIdx.x;
+        int v = t < ntask ? (te[t] - tb[t] + kWideChunk - 1) / kWideChunk : 0;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(v, d);
+            if (t >= d) v += u;
+        }
+        s_first[t + 1] = v;
+        if (t == 0) s_first[0] = 0;
+    }
+    __syncthreads();
+    return s_first[ntask];
+}
+SRT_D int wide_task(const int32_t* s_first, int g) {
+    int t = 0;
+    while (g >= s_first[t + 1]) t++;
+    return t;
+}
+__global__ void __launch_bounds__(256) k_wide_init(int ntask, WideAcc* A) {
+    const uint32_t kinf = fkey(__builtin_huge_valf()), kninf = fkey(-__builtin_huge_valf());
+    const int j0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int j = j0; j < ntask * 6; j += stride) (&A->cb[0][0])[j] = (j % 6 < 3) ? kinf : kninf;
+    for (int j = j0; j < ntask * 3 * kBins * 6; j += stride) (&A->bin[0][0][0][0])[j] = (j % 6 < 3) ? kinf : kninf;
+    for (int j = j0; j < ntask * 3 * kBins; j += stride) (&A->cnt[0][0][0])[j] = 0;
+    for (int j = j0; j < ntask * 12; j += stride) (&A->box[0][0][0])[j] = (j % 6 < 3) ? kinf : kninf;
+}
+__global__ void __launch_bounds__(256) k_wide_bounds(int ntask, const int32_t* tb, const int32_t* te, Items it, WideAcc* A) {
+    __shared__ int32_t s_first[kWideTasks + 1];
+    __shared__ uint32_t s_cb[6];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t kinf = fkey(__builtin_huge_valf()), kninf = fkey(-__builtin_huge_valf());
+    const int total = wide_chunks(ntask, tb, te, s_first);
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int t = wide_task(s_first, g);
+        const int b = tb[t] + (g - s_first[t]) * kWideChunk, e = min(b + kWideChunk, te[t]);
+        if (tid < 6) s_cb[tid] = (tid < 3) ? kinf : kninf;
+        __syncthreads();
+        uint32_t mn[3] = {kinf, kinf, kinf}, mx[3] = {kninf, kninf, kninf};
+        for (int i = b + tid; i < e; i += 256)
+            for (int a = 0; a < 3; a++) {
+                const uint32_t k = fkey(item_c(it, a, i));
+                mn[a] = min(mn[a], k);
+                mx[a] = max(mx[a], k);
+            }
+        for (int a = 0; a < 3; a++) {
+            const uint32_t wmn = wave_min_u32(mn[a]), wmx = wave_max_u32(mx[a]);
+            if (lane == 0) {
+                atomicMin(&s_cb[a], wmn);
+                atomicMax(&s_cb[3 + a], wmx);
+            }
+        }
+        __syncthreads();
+        if (tid < 3) atomicMin(&A->cb[t][tid], s_cb[tid]);
+        else if (tid < 6) atomicMax(&A->cb[t][tid], s_cb[tid]);
+        __syncthreads();
+    }
+}
+__global__ void __launch_bounds__(256) k_wide_bins(int ntask, const int32_t* tb, const int32_t* te, Items it, WideAcc* A) {
+    __shared__ int32_t s_first[kWideTasks + 1];
+    __shared__ uint32_t s_bin[kSplitWaves][3][kBins][6];
+    __shared__ int32_t s_cnt[kSplitWaves][3][kBins];
+    __shared__ float s_lo[3], s_ext[3];
+    const int tid = threadIdx.x, wv = tid >> 6;
+    const uint32_t kinf = fkey(__builtin_huge_valf()), kninf = fkey(-__builtin_huge_valf());
+    const int total = wide_chunks(ntask, tb, te, s_first);
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int t = wide_task(s_first, g);
+        const int b = tb[t] + (g - s_first[t]) * kWideChunk, e = min(b + kWideChunk, te[t]);
+        for (int j = tid; j < kSplitWaves * 3 * kBins * 6; j += 256) (&s_bin[0][0][0][0])[j] = ((j % 6) < 3) ? kinf : kninf;
+        for (int j = tid; j < kSplitWaves * 3 * kBins; j += 256) (&s_cnt[0][0][0])[j] = 0;
+        if (tid < 3) {
+            const float lo = fromkey(A->cb[t][tid]), hi = fromkey(A->cb[t][3 + tid]);
+            s_lo[tid] = lo;
+            s_ext[tid] = hi - lo;
+        }
+        __syncthreads();
+        for (int i = b + tid; i < e; i += 256) {
+            const DBox ib = ldbox(it.box, i);
+            const int cnt = it.count[i];
+            for (int a = 0; a < 3; a++) {
+                const float ex = s_ext[a];
+                if (!(ex > 0.0f)) continue;
+                const float scale = (float)kBins / ex;
+                const int bi = bin_of(item_c(it, a, i), s_lo[a], scale);
+                for (int q = 0; q < 3; q++) {
+                    atomicMin(&s_bin[wv][a][bi][q], fkey(ib.lo[q]));
+                    atomicMax(&s_bin[wv][a][bi][3 + q], fkey(ib.hi[q]));
+                }
+                atomicAdd(&s_cnt[wv][a][bi], cnt);
+            }
+        }
+        __syncthreads();
+        // merge the waves' bins and add the non-empty ones to the node's bins
+        for (int j = tid; j < 3 * kBins * 7; j += 256) {
+            const int w = j % 7, ab = j / 7;
+            const int a = ab / kBins, k = ab % kBins;
+            int32_t n = 0;
+            for (int q = 0; q < kSplitWaves; q++) n += s_cnt[q][a][k];
+            if (n == 0) continue;
+            if (w < 6) {
+                uint32_t v = s_bin[0][a][k][w];
+                for (int q = 1; q < kSplitWaves; q++) v = (w < 3) ? min(v, s_bin[q][a][k][w]) : max(v, s_bin[q][a][k][w]);
+                if (w < 3) atomicMin(&A->bin[t][a][k][w], v);
+                else atomicMax(&A->bin[t][a][k][w], v);
+            } else {
+                atomicAdd(&A->cnt[t][a][k], n);
+            }
+        }
+        __syncthreads();
+    }
+}
+__global__ void __launch_bounds__(64) k_wide_sah(WideAcc* A) {
+    const int t = blockIdx.x;
+    float ext[3];
+    for (int a = 0; a < 3; a++) {
+        const float lo = fromkey(A->cb[t][a]), hi = fromkey(A->cb[t][3 + a]);
+        ext[a] = hi - lo;
+        if (threadIdx.x == 0) {
+            A->lo[t][a] = lo;
+            A->ext[t][a] = ext[a];
+        }
+    }
+    int32_t ax, sp;
+    sah_choose_wave(A->bin[t], A->cnt[t], ext, &ax, &sp);
+    if (threadIdx.x == 0) {
+        A->axis[t] = ax;
+        A->split[t] = sp;
+    }
+}
+// left items of each chunk (a node without a split keeps its order)
+__global__ void __launch_bounds__(256) k_wide_count(int ntask, const int32_t* tb, const int32_t* te, Items it,
+                                                    const WideAcc* A, int32_t* chunk_left) {
+    __shared__ int32_t s_first[kWideTasks + 1];
+    __shared__ int32_t s_w[kSplitWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int total = wide_chunks(ntask, tb, te, s_first);
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int t = wide_task(s_first, g);
+        const int axis = A->axis[t];
+        if (axis < 0) continue;          // uniform over the workgroup
+        const int b = tb[t] + (g - s_first[t]) * kWideChunk, e = min(b + kWideChunk, te[t]);
+        const float scale = (float)kBins / A->ext[t][axis], lo = A->lo[t][axis];
+        const int split = A->split[t];
+        int n = 0;
+        for (int i = b + tid; i < e; i += 256) n += (bin_of(item_c(it, axis, i), lo, scale) <= split) ? 1 : 0;
+        for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+        if (lane == 0) s_w[wv] = n;
+        __syncthreads();
+        if (tid == 0) chunk_left[g] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        __syncthreads();
+    }
+}
+// stable partition of each chunk into tmp: its left items after the left
+// items of the node's earlier chunks, its right items after all left items
+// and the right items of the earlier chunks
+__global__ void __launch_bounds__(256) k_wide_scatter(int ntask, const int32_t* tb, const int32_t* te, Items it, Items tmp,
+                                                      WideAcc* A, const int32_t* chunk_left) {
+    __shared__ int32_t s_first[kWideTasks + 1];
+    __shared__ int32_t s_w[2][kSplitWaves];
+    __shared__ int32_t s_sum[2][kSplitWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int total = wide_chunks(ntask, tb, te, s_first);
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int t = wide_task(s_first, g);
+        const int axis = A->axis[t];
+        if (axis < 0) continue;          // uniform over the workgroup
+        const int k = g - s_first[t];
+        const int tb0 = tb[t];
+        const int b = tb0 + k * kWideChunk, e = min(b + kWideChunk, te[t]);
+        // left items before this chunk and in the whole node
+        int before = 0, all = 0;
+        for (int q = s_first[t] + tid; q < s_first[t + 1]; q += 256) {
+            const int c = chunk_left[q];
+            all += c;
+            before += (q < g) ? c : 0;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            before += __shfl_xor(before, off);
+            all += __shfl_xor(all, off);
+        }
+        if (lane == 0) { s_sum[0][wv] = before; s_sum[1][wv] = all; }
+        __syncthreads();
+        const int lbefore = s_sum[0][0] + s_sum[0][1] + s_sum[0][2] + s_sum[0][3];
+        const int nleft = s_sum[1][0] + s_sum[1][1] + s_sum[1][2] + s_sum[1][3];
+        if (k == 0 && tid == 0) A->nleft[t] = nleft;
+        const float scale = (float)kBins / A->ext[t][axis], lo = A->lo[t][axis];
+        const int split = A->split[t];
+        int lrun = tb0 + lbefore;                          // next left slot
+        int rrun = tb0 + nleft + (b - tb0 - lbefore);      // next right slot
+        for (int start = b; start < e; start += 256) {
+            const int i = start + tid;
+            bool has = i < e, left = false;
+            if (has) left = bin_of(item_c(it, axis, i), lo, scale) <= split;
+            const unsigned long long ml = __ballot(has && left), mr = __ballot(has && !left);
+            if (lane == 0) { s_w[0][wv] = __popcll(ml); s_w[1][wv] = __popcll(mr); }
+            __syncthreads();
+            int lb = 0, rb = 0, lt = 0, rt = 0;
+            for (int q = 0; q < kSplitWaves; q++) {
+                lb += (q < wv) ? s_w[0][q] : 0;
+                rb += (q < wv) ? s_w[1][q] : 0;
+                lt += s_w[0][q];
+                rt += s_w[1][q];
+            }
+            if (has) {
+                const int dst = left ? lrun + lb + __popcll(ml & below) : rrun + rb + __popcll(mr & below);
+                copy_item(it, i, tmp, dst);
+            }
+            lrun += lt;
+            rrun += rt;
+            __syncthreads();
+        }
+    }
+}
+// copy the partitioned chunks back and reduce the child boxes
+__global__ void __launch_bounds__(256) k_wide_boxes(int ntask, const int32_t* tb, const int32_t* te, Items it, Items tmp,
+                                                    WideAcc* A) {
+    __shared__ int32_t s_first[kWideTasks + 1];
+    __shared__ uint32_t s_box[2][6];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t kinf = fkey(__builtin_huge_valf()), kninf = fkey(-__builtin_huge_valf());
+    const int total = wide_chunks(ntask, tb, te, s_first);
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int t = wide_task(s_first, g);
+        const int axis = A->axis[t];
+        const int b = tb[t] + (g - s_first[t]) * kWideChunk, e = min(b + kWideChunk, te[t]);
+        const int mid = axis < 0 ? (tb[t] + te[t]) / 2 : tb[t] + A->nleft[t];
+        if (tid < 12) (&s_box[0][0])[tid] = ((tid % 6) < 3) ? kinf : kninf;
+        __syncthreads();
+        uint32_t mn[2][3], mx[2][3];
+        for (int sd = 0; sd < 2; sd++)
+            for (int q = 0; q < 3; q++) { mn[sd][q] = kinf; mx[sd][q] = kninf; }
+        for (int i = b + tid; i < e; i += 256) {
+            DBox ib;
+            if (axis >= 0) {
+                ib = ldbox(tmp.box, i);
+                copy_item(tmp, i, it, i);
+            } else {
+                ib = ldbox(it.box, i);
+            }
+            const int side = (i < mid) ? 0 : 1;
+            for (int q = 0; q < 3; q++) {
+                mn[side][q] = min(mn[side][q], fkey(ib.lo[q]));
+                mx[side][q] = max(mx[side][q], fkey(ib.hi[q]));
+            }
+        }
+        for (int sd = 0; sd < 2; sd++)
+            for (int q = 0; q < 3; q++) {
+                const uint32_t wmn = wave_min_u32(mn[sd][q]), wmx = wave_max_u32(mx[sd][q]);
+                if (lane == 0) {
+                    atomicMin(&s_box[sd][q], wmn);
+                    atomicMax(&s_box[sd][3 + q], wmx);
+                }
+            }
+        __syncthreads();
+        if (tid < 12) {
+            const int sd = tid / 6, q = tid % 6;
+            if (q < 3) atomicMin(&A->box[t][sd][q], s_box[sd][q]);
+            else atomicMax(&A->box[t][sd][q], s_box[sd][q]);
+        }
+        __syncthreads();
+    }
+}
+__global__ void __launch_bounds__(64) k_wide_final(int ntask, const int32_t* tb, const int32_t* te, const WideAcc* A,
+                                                   TopSplit* out) {
+    const int t = blockIdx.x, tid = threadIdx.x;
+    if (t >= ntask) return;
+    if (tid < 12) out[t].box[tid / 6][tid % 6] = fromkey(A->box[t][tid / 6][tid % 6]);
+    if (tid == 0) out[t].mid = A->axis[t] < 0 ? (tb[t] + te[t]) / 2 : tb[t] + A->nleft[t];
+}
+
+// Levels of many small nodes: one wave per node (four per workgroup), each
+// with its own LDS bins; the same bounds, bins, split, stable partition and
+// child boxes as k_top_split, with wave reductions and ballots in place of the
+// workgroup's.  A node of any size is handled (in rounds of 64 items).
+constexpr int kWaveSplitMin = 1024;      // levels of more nodes than this use the wave form
+__global__ void __launch_bounds__(256) k_top_split_wave(int ntask, Items it, Items tmp, const int32_t* task_b,
+                                                        const int32_t* task_e, TopSplit* out) {
+    __shared__ uint32_t s_bin[kSplitWaves][3][kBins][6];
+    __shared__ int32_t s_cnt[kSplitWaves][3][kBins];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int t = blockIdx.x * kSplitWaves + wv;
+    const bool active = t < ntask;       // uniform over the wave
+    const int b = active ? task_b[t] : 0, e = active ? task_e[t] : 0;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t kinf = fkey(__builtin_huge_valf()), kninf = fkey(-__builtin_huge_valf());
+    for (int j = lane; j < 3 * kBins * 6; j += 64) (&s_bin[wv][0][0][0])[j] = ((j % 6) < 3) ? kinf : kninf;
+    for (int j = lane; j < 3 * kBins; j += 64) (&s_cnt[wv][0][0])[j] = 0;
+    // centroid bounds
+    float lo[3], ext[3];
+    {
+        uint32_t mn[3] = {kinf, kinf, kinf}, mx[3] = {kninf, kninf, kninf};
+        for (int i = b + lane; i < e; i += 64)
+            for (int a = 0; a < 3; a++) {
+                const uint32_t k = fkey(item_c(it, a, i));
+                mn[a] = min(mn[a], k);
+                mx[a] = max(mx[a], k);
+            }
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fromkey(wave_min_u32(mn[a]));
+            ext[a] = fromkey(wave_max_u32(mx[a])) - lo[a];
+        }
+    }
+    __syncthreads();
+    for (int i = b + lane; i < e; i += 64) {
+        const DBox ib = ldbox(it.box, i);
+        const int cnt = it.count[i];
+        for (int a = 0; a < 3; a++) {
+            if (!(ext[a] > 0.0f)) continue;
+            const float scale = (float)kBins / ext[a];
+            const int bi = bin_of(item_c(it, a, i), lo[a], scale);
+            for (int q = 0; q < 3; q++) {
+                atomicMin(&s_bin[wv][a][bi][q], fkey(ib.lo[q]));
+                atomicMax(&s_bin[wv][a][bi][3 + q], fkey(ib.hi[q]));
+            }
+            atomicAdd(&s_cnt[wv][a][bi], cnt);
+        }
+    }
+    __syncthreads();
+    int32_t axis = -1, split = -1;
+    if (active) sah_choose_wave(s_bin[wv], s_cnt[wv], ext, &axis, &split);
+    int mid = (b + e) / 2;
+    if (axis >= 0) {
+        const float scale = (float)kBins / ext[axis], l0 = lo[axis];
+        int nleft = 0;
+        for (int start = b; start < e; start += 64) {
+            const int i = start + lane;
+            const bool left = i < e && bin_of(item_c(it, axis, i), l0, scale) <= split;
+            nleft += __popcll(__ballot(left));
+        }
+        int lrun = b, rrun = b + nleft;
+        for (int start = b; start < e; start += 64) {
+            const int i = start + lane;
+            const bool has = i < e;
+            const bool left = has && bin_of(item_c(it, axis, i), l0, scale) <= split;
+            const unsigned long long ml = __ballot(left), mr = __ballot(has && !left);
+            if (has) copy_item(it, i, tmp, left ? lrun + __popcll(ml & below) : rrun + __popcll(mr & below));
+            lrun += __popcll(ml);
+            rrun += __popcll(mr);
+        }
+        mid = b + nleft;
+    }
+    __syncthreads();      // the partition's global writes are visible to the wave's other lanes
+    uint32_t mn[2][3], mx[2][3];
+    for (int sd = 0; sd < 2; sd++)
+        for (int q = 0; q < 3; q++) { mn[sd][q] = kinf; mx[sd][q] = kninf; }
+    for (int i = b + lane; i < e; i += 64) {
+        DBox ib;
+        if (axis >= 0) {
+            ib = ldbox(tmp.box, i);
+            copy_item(tmp, i, it, i);
+        } else {
+            ib = ldbox(it.box, i);
+        }
+        const int side = (i < mid) ? 0 : 1;
+        for (int q = 0; q < 3; q++) {
+            mn[side][q] = min(mn[side][q], fkey(ib.lo[q]));
+            mx[side][q] = max(mx[side][q], fkey(ib.hi[q]));
+        }
+    }
+    float bx[12];
+    for (int sd = 0; sd < 2; sd++)
+        for (int q = 0; q < 3; q++) {
+            bx[sd * 6 + q] = fromkey(wave_min_u32(mn[sd][q]));
+            bx[sd * 6 + 3 + q] = fromkey(wave_max_u32(mx[sd][q]));
+        }
+    if (active && lane < 12) {
+        float v = bx[0];
+        for (int j = 1; j < 12; j++) v = (lane == j) ? bx[j] : v;
+        out[t].box[lane / 6][lane % 6] = v;
+    }
+    if (active && lane == 0) out[t].mid = mid;
+}
+
+// Numbering of a level's inner children as an exclusive scan over the nodes:
+// k_link_count gives each node its number of inner children (child ranges of
+// more than one item), the scan their first number, k_link_write the children
+// (breadth-first numbering, left before right, as k_top_link), k_link_finish
+// the next level's size and the next free node id.
+__global__ void __launch_bounds__(256) k_link_count(int ntask, const int32_t* task_b, const int32_t* task_e,
+                                                    const TopSplit* sp, int32_t* inner) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntask) return;
+    const int b = task_b[t], e = task_e[t], mid = sp[t].mid;
+    inner[t] = (mid - b > 1 ? 1 : 0) + (e - mid > 1 ? 1 : 0);
+}
+__global__ void __launch_bounds__(256) k_link_write(int ntask, const int32_t* task_b, const int32_t* task_e,
+                                                    const int32_t* task_node, const TopSplit* sp, const int32_t* item_ref,
+                                                    const int32_t* first, BvhNode* nodes, int32_t* next_b, int32_t* next_e,
+                                                    int32_t* next_node, const int32_t* next_id) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntask) return;
+    const int b = task_b[t], e = task_e[t], mid = sp[t].mid;
+    const int rb[2] = {b, mid}, re[2] = {mid, e};
+    const int32_t id0 = *next_id;
+    int q = first[t];
+    BvhNode* nd = nodes + task_node[t];
+    for (int side = 0; side < 2; side++) {
+        DBox bx;
+        for (int k = 0; k < 3; k++) { bx.lo[k] = sp[t].box[side][k]; bx.hi[k] = sp[t].box[side][3 + k]; }
+        int32_t ref;
+        if (re[side] - rb[side] == 1) {
+            ref = item_ref[rb[side]];
+        } else {
+            ref = id0 + q;
+            next_b[q] = rb[side]; next_e[q] = re[side]; next_node[q] = ref;
+            q++;
+        }
+        set_child(nd, side, bx, ref);
+    }
+}
+__global__ void k_link_finish(int ntask, const int32_t* inner, const int32_t* first, int32_t* next_id, int32_t* next_count) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int32_t n = first[ntask - 1] + inner[ntask - 1];
+    *next_count = n;
+    *next_id += n;
 }
 
 // Numbers the inner children of one level (one workgroup; `next_id` holds the
@@ -1058,8 +1532,15 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
         int32_t* le[2] = {tmp.get<int32_t>(m), tmp.get<int32_t>(m)};
         int32_t* ln[2] = {tmp.get<int32_t>(m), tmp.get<int32_t>(m)};
         TopSplit* sp = tmp.get<TopSplit>(m);
+        WideAcc* wacc = tmp.get<WideAcc>(1);
+        int32_t* link_inner = tmp.get<int32_t>(m);
+        int32_t* link_first = tmp.get<int32_t>(m);
+        size_t link_bytes = 0;
+        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, link_bytes, link_inner, link_first, m, st);
+        void* link_tmp = tmp.get<char>(link_bytes);
+        int32_t* chunk_left = tmp.get<int32_t>((size_t)m / kWideChunk + kWideTasks + 1);
         int32_t* ctr = tmp.get<int32_t>(2);     // next free node id, next level size
-        if (tmp.failed || !ctr) return set_err(c, SRT_E_OOM, "top-level scratch");
+        if (tmp.failed || !ctr || !link_tmp) return set_err(c, SRT_E_OOM, "top-level scratch");
         k_items<<<nblk(m), 256, 0, st>>>(m, clusters, order, tb, K, kfirst, top, it);
         // level 0: the root (node 0) over all items; ids 1.. follow breadth-first
         k_fill_i32<<<1, 64, 0, st>>>(lb[0], 1, 0);
@@ -1068,9 +1549,35 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
         k_fill_i32<<<1, 64, 0, st>>>(ctr, 1, 1);
         int ntask = 1, cur = 0;
         while (ntask > 0) {
-            k_top_split<<<ntask, kSplitThreads, 0, st>>>(it, it2, lb[cur], le[cur], sp);
-            k_top_link<<<1, kTopThreads, 0, st>>>(ntask, lb[cur], le[cur], ln[cur], sp, it.ref, nodes, lb[cur ^ 1],
-                                                  le[cur ^ 1], ln[cur ^ 1], ctr, ctr + 1);
+            if (ntask <= kWideTasks) {
+                // chunks of the level: at most m / kWideChunk + one partial chunk per node
+                const int grid = std::min(kWideMaxGrid, m / kWideChunk + ntask);
+                k_wide_init<<<8, 256, 0, st>>>(ntask, wacc);
+                k_wide_bounds<<<grid, 256, 0, st>>>(ntask, lb[cur], le[cur], it, wacc);
+                k_wide_bins<<<grid, 256, 0, st>>>(ntask, lb[cur], le[cur], it, wacc);
+                k_wide_sah<<<ntask, 64, 0, st>>>(wacc);
+                k_wide_count<<<grid, 256, 0, st>>>(ntask, lb[cur], le[cur], it, wacc, chunk_left);
+                k_wide_scatter<<<grid, 256, 0, st>>>(ntask, lb[cur], le[cur], it, it2, wacc, chunk_left);
+                k_wide_boxes<<<grid, 256, 0, st>>>(ntask, lb[cur], le[cur], it, it2, wacc);
+                k_wide_final<<<ntask, 64, 0, st>>>(ntask, lb[cur], le[cur], wacc, sp);
+            } else if (ntask <= kWaveSplitMin) {
+                k_top_split<<<ntask, kSplitThreads, 0, st>>>(it, it2, lb[cur], le[cur], sp);
+            } else {
+                k_top_split_wave<<<(ntask + kSplitWaves - 1) / kSplitWaves, 256, 0, st>>>(ntask, it, it2, lb[cur], le[cur],
+                                                                                           sp);
+            }
+            if (ntask <= kTopThreads) {
+                k_top_link<<<1, kTopThreads, 0, st>>>(ntask, lb[cur], le[cur], ln[cur], sp, it.ref, nodes, lb[cur ^ 1],
+                                                      le[cur ^ 1], ln[cur ^ 1], ctr, ctr + 1);
+            } else {
+                k_link_count<<<nblk(ntask), 256, 0, st>>>(ntask, lb[cur], le[cur], sp, link_inner);
+                if (int r = hip_check(c, hipcub::DeviceScan::ExclusiveSum(link_tmp, link_bytes, link_inner, link_first, ntask, st),
+                                      "link scan"))
+                    return r;
+                k_link_write<<<nblk(ntask), 256, 0, st>>>(ntask, lb[cur], le[cur], ln[cur], sp, it.ref, link_first, nodes,
+                                                          lb[cur ^ 1], le[cur ^ 1], ln[cur ^ 1], ctr);
+                k_link_finish<<<1, 64, 0, st>>>(ntask, link_inner, link_first, ctr, ctr + 1);
+            }
             int32_t next = 0;
             if (int r = hip_check(c, hipMemcpyAsync(&next, ctr + 1, 4, hipMemcpyDeviceToHost, st), "top level")) return r;
             if (int r = hip_check(c, hipStreamSynchronize(st), "top level")) return r;
