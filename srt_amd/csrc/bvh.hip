@@ -1208,12 +1208,20 @@ __global__ void __launch_bounds__(256) k_refit_level(const uint32_t* ids, int co
 }
 
 // ---------------------------------------------------------------------------
-// 4-wide tree collapsed from the binary tree, top-down by levels: a 4-wide node
-// made from binary node x starts with x's two children and repeatedly replaces
-// the inner child of largest surface area by its two children until it has 4
-// children or only leaves.  Nodes are numbered level by level in order
-// (deterministic); boxes are copies of binary-tree boxes, so both trees give
-// every ray the same box tests.
+// 4-wide tree collapsed from the binary tree.  The binary leaves stay leaves;
+// which binary inner nodes become 4-wide nodes is chosen by a bottom-up
+// dynamic program that minimises the summed surface area of the 4-wide nodes
+// (the surface-area estimate of the node steps of a traversal; the leaves'
+// triangle tests do not depend on the choice).  Per binary inner node x:
+//   W(x)    = area(x) + G(x, 4)    x as a 4-wide node
+//   G(x, k) = min over j in 1..k-1 of H(l, j) + H(r, k - j)
+//   H(c, j) = 0 for a leaf, W(c) for j = 1, min(W(c), G(c, j)) for j > 1
+// (H(c, j): subtree c in at most j child slots, c kept as one slot or opened).
+// The 4-wide nodes are then emitted top-down by levels following the stored
+// choices (ties keep a child whole, and give the left child the fewest
+// slots).  Nodes are numbered level by level in order (deterministic); boxes
+// are copies of binary-tree boxes, so both trees give every ray the same box
+// tests.
 // ---------------------------------------------------------------------------
 SRT_D DBox child_box(const BvhNode& nd, int side) {
     DBox b;
@@ -1228,32 +1236,80 @@ struct Kids4 {
     int32_t ref[4], src[4];
     int n;
 };
-SRT_D Kids4 collapse_children(const BvhNode* nodes, int32_t x) {
-    Kids4 k;
-    float area[4];
+// dynamic-program tables per binary inner node: W, G(k) for k = 2, 3, 4, and
+// the slots given to the left child for each k (2 bits each)
+struct C4Dp {
+    float* W;
+    float* G;            // 3 per node
+    uint32_t* J;         // bits 2(k-2) .. 2(k-2)+1: left slots for k
+};
+SRT_D float c4_H(const C4Dp& dp, int32_t c, int j) {
+    if (c < 0) return 0.0f;
+    const float w = dp.W[c];
+    if (j == 1) return w;
+    const float g = dp.G[3 * (size_t)c + (j - 2)];
+    return g < w ? g : w;
+}
+__global__ void __launch_bounds__(256) k_c4_dp(const uint32_t* ids, int count, const BvhNode* nodes, C4Dp dp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int32_t x = (int32_t)ids[i];
     const BvhNode nd = nodes[x];
-    k.ref[0] = nd.d.x; k.src[0] = x << 1; area[0] = box_area(child_box(nd, 0));
-    k.ref[1] = nd.d.y; k.src[1] = (x << 1) | 1; area[1] = box_area(child_box(nd, 1));
-    k.ref[2] = k.ref[3] = 0; k.src[2] = k.src[3] = -1; area[2] = area[3] = 0.0f;
-    k.n = 2;
-    while (k.n < 4) {
-        int best = -1;
-        float ba = -1.0f;
-        for (int j = 0; j < k.n; j++)
-            if (k.ref[j] >= 0 && area[j] > ba) { ba = area[j]; best = j; }
-        if (best < 0) break;
-        const int32_t y = k.ref[best];
-        const BvhNode ny = nodes[y];
-        k.ref[best] = ny.d.x; k.src[best] = y << 1; area[best] = box_area(child_box(ny, 0));
-        k.ref[k.n] = ny.d.y; k.src[k.n] = (y << 1) | 1; area[k.n] = box_area(child_box(ny, 1));
-        k.n++;
+    DBox b = child_box(nd, 0);
+    grow(b, child_box(nd, 1));
+    const int32_t l = nd.d.x, r = nd.d.y;
+    uint32_t jbits = 0;
+    float g4 = 0.0f;
+    for (int k = 2; k <= 4; k++) {
+        float best = __builtin_huge_valf();
+        int bj = 1;
+        for (int j = 1; j < k; j++) {
+            const float v = c4_H(dp, l, j) + c4_H(dp, r, k - j);
+            if (v < best) { best = v; bj = j; }
+        }
+        dp.G[3 * (size_t)x + (k - 2)] = best;
+        jbits |= (uint32_t)bj << (2 * (k - 2));
+        if (k == 4) g4 = best;
+    }
+    dp.J[x] = jbits;
+    dp.W[x] = box_area(b) + g4;
+}
+// children of the 4-wide node made from binary node x: subtree (c, j) is kept
+// as one slot when c is a leaf, j == 1 or opening it is not cheaper, else
+// opened into its children with the stored split of j slots
+SRT_D Kids4 collapse_children(const BvhNode* nodes, int32_t x, const C4Dp& dp) {
+    Kids4 k;
+    k.n = 0;
+    for (int j = 0; j < 4; j++) { k.ref[j] = 0; k.src[j] = -1; }
+    int32_t st_ref[4], st_src[4], st_j[4];
+    int sp = 0;
+    const BvhNode nd = nodes[x];
+    const int jl = min(max((int)(dp.J[x] >> 4) & 3, 1), 3);      // k = 4; every split gives each side a slot
+    st_ref[sp] = nd.d.y; st_src[sp] = (x << 1) | 1; st_j[sp] = 4 - jl; sp++;
+    st_ref[sp] = nd.d.x; st_src[sp] = x << 1; st_j[sp] = jl; sp++;
+    while (sp > 0) {
+        sp--;
+        const int32_t c = st_ref[sp], src = st_src[sp];
+        const int j = st_j[sp];
+        const bool open = c >= 0 && j > 1 && dp.G[3 * (size_t)c + (j - 2)] < dp.W[c];
+        if (!open) {
+            k.ref[k.n] = c;
+            k.src[k.n] = src;
+            k.n++;
+            continue;
+        }
+        const BvhNode nc = nodes[c];
+        const int cl = min(max((int)(dp.J[c] >> (2 * (j - 2))) & 3, 1), j - 1);
+        st_ref[sp] = nc.d.y; st_src[sp] = (c << 1) | 1; st_j[sp] = j - cl; sp++;
+        st_ref[sp] = nc.d.x; st_src[sp] = c << 1; st_j[sp] = cl; sp++;
     }
     return k;
 }
-__global__ void __launch_bounds__(256) k_c4_count(int nf, const int32_t* level, const BvhNode* nodes, uint32_t* ninner) {
+__global__ void __launch_bounds__(256) k_c4_count(int nf, const int32_t* level, const BvhNode* nodes, C4Dp dp,
+                                                  uint32_t* ninner) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nf) return;
-    const Kids4 k = collapse_children(nodes, level[i]);
+    const Kids4 k = collapse_children(nodes, level[i], dp);
     uint32_t m = 0;
     for (int j = 0; j < k.n; j++) m += (k.ref[j] >= 0) ? 1u : 0u;
     ninner[i] = m;
@@ -1263,11 +1319,12 @@ SRT_D void set4(Bvh4Node& o, int j, const DBox& b) {
     float* lz = &o.loz.x; float* hz = &o.hiz.x;
     lx[j] = b.lo[0]; hx[j] = b.hi[0]; ly[j] = b.lo[1]; hy[j] = b.hi[1]; lz[j] = b.lo[2]; hz[j] = b.hi[2];
 }
-__global__ void __launch_bounds__(256) k_c4_emit(int nf, const int32_t* level, const BvhNode* nodes, const uint32_t* off,
-                                                 int base_this, int base_next, Bvh4Node* out, int32_t* next_level) {
+__global__ void __launch_bounds__(256) k_c4_emit(int nf, const int32_t* level, const BvhNode* nodes, C4Dp dp,
+                                                 const uint32_t* off, int base_this, int base_next, Bvh4Node* out,
+                                                 int32_t* next_level) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nf) return;
-    const Kids4 k = collapse_children(nodes, level[i]);
+    const Kids4 k = collapse_children(nodes, level[i], dp);
     Bvh4Node o;
     int32_t* ref = &o.ref.x;
     int32_t* src = &o.src.x;
@@ -1674,7 +1731,19 @@ int bvh4_build(srt_ctx* c, SceneRec* s) {
     size_t scan_bytes = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, ninner, off, n2, st);
     void* scan_tmp = tmp.get<uint8_t>(scan_bytes);
-    if (!scan_tmp) return SRT_E_OOM;
+    C4Dp dp;
+    dp.W = tmp.get<float>(n2);
+    dp.G = tmp.get<float>(3 * (size_t)n2);
+    dp.J = tmp.get<uint32_t>(n2);
+    if (!scan_tmp || !dp.W || !dp.G || !dp.J) return SRT_E_OOM;
+    // the dynamic program, deepest level first (levels 0 .. depth-1 hold the inner nodes)
+    const int nlev = (int)s->level_first.size() - 1;
+    if (nlev < 1) return set_err(c, SRT_E_STATE, "4-wide collapse without the binary tree's levels");
+    for (int L = nlev - 1; L >= 0; L--) {
+        const int first = s->level_first[L], count = s->level_first[L + 1] - first;
+        k_c4_dp<<<nblk(count), 256, 0, st>>>((const uint32_t*)s->level_nodes.p + first, count, (const BvhNode*)s->nodes.p,
+                                             dp);
+    }
     k_fill_i32<<<1, 64, 0, st>>>(lev[0], 1, s->root);
     Bvh4Node* out = (Bvh4Node*)s->nodes4.p;
     int base = 0, nf = 1, cur = 0, levels = 0;
@@ -1688,10 +1757,10 @@ int bvh4_build(srt_ctx* c, SceneRec* s) {
             s->depth4 = 0;
             return SRT_OK;
         }
-        k_c4_count<<<nblk(nf), 256, 0, st>>>(nf, lev[cur], (const BvhNode*)s->nodes.p, ninner);
+        k_c4_count<<<nblk(nf), 256, 0, st>>>(nf, lev[cur], (const BvhNode*)s->nodes.p, dp, ninner);
         if (int r = hip_check(c, hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, ninner, off, nf, st), "scan"))
             return r;
-        k_c4_emit<<<nblk(nf), 256, 0, st>>>(nf, lev[cur], (const BvhNode*)s->nodes.p, off, base, base + nf, out,
+        k_c4_emit<<<nblk(nf), 256, 0, st>>>(nf, lev[cur], (const BvhNode*)s->nodes.p, dp, off, base, base + nf, out,
                                             lev[cur ^ 1]);
         uint32_t last[2] = {0, 0};
         if (int r = hip_check(c, hipMemcpyAsync(&last[0], off + nf - 1, 4, hipMemcpyDeviceToHost, st), "count")) return r;
