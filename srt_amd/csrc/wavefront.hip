@@ -372,7 +372,11 @@ SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uin
             const float4 nx = ld4(rsrc, nb + st.onx), fx = ld4(rsrc, nb + (st.onx ^ 16u));
             const float4 ny = ld4(rsrc, nb + st.ony), fy = ld4(rsrc, nb + (st.ony ^ 16u));
             const float4 nz = ld4(rsrc, nb + st.onz), fz = ld4(rsrc, nb + (st.onz ^ 16u));
-            const int4 rf = ld4i(rsrc, nb + 96u);
+            int4 rf = ld4i(rsrc, nb + 96u);
+            // the child refs lie in the same 128-byte line as the planes: have them
+            // arrive with the planes, instead of a second dependent load after the
+            // box tests (where the compiler would otherwise move it)
+            __asm__ volatile("" : "+v"(rf.x), "+v"(rf.y), "+v"(rf.z), "+v"(rf.w));
             float k0, k1, k2, k3;
             const bool h0 = slab(st, nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, &k0);
             const bool h1 = slab(st, nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, &k1);
