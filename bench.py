@@ -303,8 +303,8 @@ def kstats_summary(d, mode="frame"):
                  "    python3 bench.py --kstats-summary gpurun_out/prof_inst --kstats-mode instanced", "",
                  "16 icosphere meshes (320 triangles each) under 3125 instances plus 4 light instances; the run",
                  "renders 6 frames (the counting pass, 1 warmup, 2 timed, 2 copied to host memory) with the",
-                 "instanced traversal kernels (k_trace<.., .., true>, k_shadow<.., .., true>); the per-mesh trees",
-                 "are built once by the same kernels as the flat build.", "",
+                 "instanced traversal kernels (k_trace<.., .., true>, k_shadow<.., .., true>); the two-level tree",
+                 "is built twice (cold, then a rebuild), its per-mesh trees by the same kernels as the flat build.", "",
                  "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
         out_file = INST_KSTATS_FILE
     else:
