@@ -133,6 +133,7 @@ struct TravState {
     int32_t cur;       // next node (>= 0) or leaf (< 0) to visit; valid while cur_ok
     int32_t pend;      // postponed leaf; valid while has_pend
     int32_t sp;        // stack depth
+    int32_t top;       // the stack's top entry (valid while sp > 0), kept in a register
     bool sx, sy, sz;   // direction signs select the near/far slab planes
     bool cur_ok, has_pend;
     uint32_t onx, ony, onz;   // byte offsets of the near planes in a Bvh4Node (far plane: offset ^ 16)
@@ -201,6 +202,7 @@ SRT_D void trav_begin(TravState& st, f3 o, f3 d, float tmin, float tmax, int32_t
     st.cur = root;
     st.pend = 0;
     st.sp = 0;
+    st.top = 0;
     st.cur_ok = true;
     st.has_pend = false;
 }
@@ -229,13 +231,22 @@ SRT_D int32_t stk_get(const TravStack& k, int32_t sp) {
     __asm__ volatile("" : "+v"(v));
     return v;
 }
+// A pop takes the top from its register, so the next node's fetch does not
+// wait for a stack read; the read of the entry below (for the next pop) then
+// overlaps that fetch.  Every push also sets the register (push_top).
 SRT_D void stk_pop(const TravStack& k, TravState& st) {
     if (st.sp == 0) {
         st.cur_ok = false;
     } else {
         st.sp--;
-        st.cur = stk_get(k, st.sp);
+        st.cur = st.top;
+        if (st.sp > 0) st.top = stk_get(k, st.sp - 1);
     }
+}
+SRT_D void push_top(const TravStack& k, TravState& st, int32_t v) {
+    stk_push(k, st.sp, v);
+    st.sp++;
+    st.top = v;
 }
 
 // Inner nodes, with speculative leaf postponement (Aila & Laine 2009): a lane
@@ -274,8 +285,7 @@ SRT_D void trav_inner(const DevScene& S, TravState& st, const TravStack& k, uint
             const bool swap = hr && (!hl || trn < tln);        // right child first
             const int32_t first = swap ? nd.y : nd.x, second = swap ? nd.x : nd.y;
             if (hl && hr) {
-                stk_push(k, st.sp, second);
-                st.sp++;
+                push_top(k, st, second);
             }
             if (hl || hr) st.cur = first;
             else stk_pop(k, st);
@@ -396,7 +406,7 @@ SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uin
             } else {
                 if (nh > 3) { stk_push(k, st.sp, r3); st.sp++; }
                 if (nh > 2) { stk_push(k, st.sp, r2); st.sp++; }
-                if (nh > 1) { stk_push(k, st.sp, r1); st.sp++; }
+                if (nh > 1) push_top(k, st, r1);
                 st.cur = r0;
             }
         }
@@ -451,8 +461,7 @@ SRT_D bool trav_pend(const DevScene& S, TravState& st, const TravStack& k, uint3
         set_ray(st, xf_point(r0, r1, r2, o), xf_vector(r0, r1, r2, d));
         st.inst = i;
         if (st.cur_ok) { stk_push(k, st.sp, st.cur); st.sp++; }
-        stk_push(k, st.sp, kInstReturn);
-        st.sp++;
+        push_top(k, st, kInstReturn);
         st.cur = S.inst_root[i];
         st.cur_ok = true;
         return false;
