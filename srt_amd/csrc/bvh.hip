@@ -1303,6 +1303,16 @@ SRT_D Kids4 collapse_children(const BvhNode* nodes, int32_t x, const C4Dp& dp) {
         st_ref[sp] = nc.d.y; st_src[sp] = (c << 1) | 1; st_j[sp] = j - cl; sp++;
         st_ref[sp] = nc.d.x; st_src[sp] = c << 1; st_j[sp] = cl; sp++;
     }
+    // slots by decreasing box area (stable): any-hit traversals visit the hit
+    // children in slot order, the likeliest to hold an occluder first
+    float area[4];
+    for (int j = 0; j < k.n; j++) area[j] = box_area(child_box(nodes[k.src[j] >> 1], k.src[j] & 1));
+    for (int a = 1; a < k.n; a++)
+        for (int b = a; b > 0 && area[b] > area[b - 1]; b--) {
+            const float ta = area[b]; area[b] = area[b - 1]; area[b - 1] = ta;
+            const int32_t tr = k.ref[b]; k.ref[b] = k.ref[b - 1]; k.ref[b - 1] = tr;
+            const int32_t ts = k.src[b]; k.src[b] = k.src[b - 1]; k.src[b - 1] = ts;
+        }
     return k;
 }
 __global__ void __launch_bounds__(256) k_c4_count(int nf, const int32_t* level, const BvhNode* nodes, C4Dp dp,

@@ -354,7 +354,7 @@ SRT_D f3 xf_vector(float4 r0, float4 r1, float4 r2, f3 v) {
     return mk(r0.x * v.x + r0.y * v.y + r0.z * v.z, r1.x * v.x + r1.y * v.y + r1.z * v.z,
               r2.x * v.x + r2.y * v.y + r2.z * v.z);
 }
-template <bool COUNT, bool INST = false>
+template <bool COUNT, bool INST = false, bool ANY = false>
 SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uint32_t& nv, int inner_exit) {
     const __amdgpu_buffer_rsrc_t rsrc = node4_rsrc(S);
     while (st.cur_ok) {
@@ -392,6 +392,19 @@ SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uin
             const bool h1 = slab(st, nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, &k1);
             const bool h2 = slab(st, nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, &k2);
             const bool h3 = slab(st, nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, &k3);
+            if (ANY) {
+                // any hit: the hit children in slot order (largest box first), no sort
+                if (!(h0 || h1 || h2 || h3)) {
+                    stk_pop(k, st);
+                } else {
+                    if (h3 && (h0 || h1 || h2)) { stk_push(k, st.sp, rf.w); st.sp++; st.top = rf.w; }
+                    if (h2 && (h0 || h1)) { stk_push(k, st.sp, rf.z); st.sp++; st.top = rf.z; }
+                    if (h1 && h0) push_top(k, st, rf.y);
+                    st.cur = h0 ? rf.x : (h1 ? rf.y : (h2 ? rf.z : rf.w));
+                }
+                if (__popcll(__ballot(!st.has_pend)) <= (unsigned)inner_exit) break;
+                continue;
+            }
             const float inf = __builtin_huge_valf();
             k0 = h0 ? k0 : inf; k1 = h1 ? k1 : inf; k2 = h2 ? k2 : inf; k3 = h3 ? k3 : inf;
             int32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
@@ -523,7 +536,7 @@ SRT_D void trace_persistent(const DevScene& S, const View& view, uint32_t lo, ui
         }
         if (act == 0ull) break;
         if (has) {
-            if (WIDE) trav_inner4<COUNT, INST>(S, st, k, nv, k.inner_exit);
+            if (WIDE) trav_inner4<COUNT, INST, ANY>(S, st, k, nv, k.inner_exit);
             else trav_inner<COUNT>(S, st, k, nv);
             if (st.has_pend) occl = trav_pend<ANY, COUNT, INST>(S, st, k, nt);
             if (occl || (!st.cur_ok && !st.has_pend)) {
