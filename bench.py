@@ -44,7 +44,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TLANE_OPS = 256 * 4 * 16 * 2.4e9 / 1e12   # 256 CUs x 4 SIMDs x 16 lanes per clock x 2.4 GHz
+# MI355X's published FP32 vector peak, 157.3 TFLOP/s, as lane operations (an FMA
+# is 2 FLOP): 256 CUs x 4 SIMDs x 32 lanes per clock x 2.4 GHz = 78.6 T/s
+VALU_PEAK_TLANE_OPS = 256 * 4 * 32 * 2.4e9 / 1e12
 MFMA_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA, MI355X
 MLP_TRAIN_FLOP = 99072           # forward + backward + weight gradients per training sample (16,512 MACs x 6)
 MLP_FWD_FLOP = 33024             # forward per query (16,512 MACs x 2)
@@ -849,8 +851,11 @@ def frame_roofline(kern, st, steps, bytes_step, segments):
     node and triangle fetches mostly hit in L2/MALL.  So `achieved` is the
     kernel's active VALU lane-operations per second (wave64 VALU instructions
     per launch x 64 x active-lane fraction, from the committed SQ counter
-    summary, over the live HIP-event launch time) against the peak of 256 CUs x
-    4 SIMDs x 16 lanes per clock at 2.4 GHz; `frac` = achieved / peak.
+    summary, over the live HIP-event launch time) against the published FP32
+    vector peak as lane operations (157.3 TFLOP/s = 78.6 T lane-FMA/s: 256 CUs
+    x 4 SIMDs x 32 lanes per clock at 2.4 GHz); `frac` = achieved / peak.
+    Compares, selects and min / max issue at about half that rate on this GPU,
+    and they are about half of a node step's VALU instructions.
     `traffic` is the kernel's calibrated HBM bytes per launch (FETCH_SIZE +
     WRITE_SIZE counters), and `hbm` gives, for each frame kernel, those bytes
     over its live launch time against the 8 TB/s HBM peak.  `algorithmic`

@@ -40,7 +40,7 @@ def test_roofline_is_against_the_binding_resource():
     r = bench.frame_roofline("trace", st, 1, bytes_step, 208_297_870)
     assert r["bound"] == "valu" and r["unit"] == "Tlane-op/s"
     assert 0.0 < r["frac"] <= 1.0 and abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-3
-    assert r["peak"] == 256 * 4 * 16 * 2.4e9 / 1e12
+    assert abs(r["peak"] - 157.3 / 2) < 0.1      # the published FP32 vector peak in lane-FMA/s
     assert r["traffic"] == bench.measured_traffic("trace")
     for k, v in r["hbm"].items():
         assert 0.0 < v["frac"] <= 1.0, k
