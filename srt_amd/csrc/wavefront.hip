@@ -392,8 +392,10 @@ SRT_D void trav_inner4(const DevScene& S, TravState& st, const TravStack& k, uin
             const bool h1 = slab(st, nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, &k1);
             const bool h2 = slab(st, nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, &k2);
             const bool h3 = slab(st, nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, &k3);
-            if (ANY) {
-                // any hit: the hit children in slot order (largest box first), no sort
+            if (ANY && !INST) {
+                // any hit: the hit children in slot order (largest box first), no sort;
+                // instanced scenes keep the distance order (their top-level slots are
+                // not ordered, and entering an instance costs a ray transform)
                 if (!(h0 || h1 || h2 || h3)) {
                     stk_pop(k, st);
                 } else {
