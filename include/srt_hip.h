@@ -108,12 +108,12 @@ const char* srt_last_error(const srt_ctx* ctx);
  * "count_visits" (0/1 count BVH node visits and triangle tests), "hbm_budget"
  * (bytes; at least srt_hbm_used), "trace_stack_lds" (traversal stack entries per
  * lane in LDS, 4..64, default 16), "trace_refill" / "shadow_refill" (finished
- * lanes that make a traversal wave fetch new rays, 1..64, defaults 44 / 32),
+ * lanes that make a traversal wave fetch new rays, 1..64, defaults 40 / 32),
  * "trace_blocks_per_cu"
  * (0: as many as fit), "trace_bvh_width" (4: traverse the 4-wide tree collapsed
  * from the binary one, default; 2: the binary tree; results are identical),
  * "trace_inner_exit" (a traversal wave leaves its inner-node loop once at most
- * this many lanes still search for a leaf, 0..63, default 10), "max_wide_nodes"
+ * this many lanes still search for a leaf, 0..63, default 18), "max_wide_nodes"
  * (a BVH built while this is set keeps its 4-wide form only up to this many
  * 4-wide nodes, else it is traversed in binary form with the same results;
  * 0..16777215, the default: 4-wide nodes are addressed with 32-bit byte
