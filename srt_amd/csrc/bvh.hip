@@ -1416,8 +1416,12 @@ __global__ void __launch_bounds__(256) k_skin(int nv, const float* op, const flo
 // ---------------------------------------------------------------------------
 namespace {
 // Build scratch: 256-byte aligned slices bumped out of 16 MiB (or larger)
-// device chunks, all freed when the build ends.  A hipMalloc/hipFree per
-// scratch array cost more than the build's kernels.
+// device chunks.  The chunks go back to the context when the build ends and
+// the next build takes them again (the smallest kept chunk that fits), so a
+// rebuild of the same scene calls neither hipMalloc nor hipFree: each hipFree
+// of a chunk cost about 0.25 ms of host time, a third of a 1M-triangle
+// rebuild.  Kernels of a build still queued when its chunks go back are
+// ordered before the next build's by the context's stream.
 struct Tmp {
     static constexpr size_t kChunk = (size_t)16 << 20;
     srt_ctx* c;
@@ -1425,19 +1429,37 @@ struct Tmp {
     size_t used = 0, cap = 0;      // of the last chunk
     bool failed = false;           // some get() returned nullptr
     explicit Tmp(srt_ctx* cc) : c(cc) {}
-    ~Tmp() { for (auto& b : chunks) dev_free(c, b); }
+    ~Tmp() {
+        if (!c->keep_build_scratch) {
+            for (auto& b : chunks) dev_free(c, b);
+            return;
+        }
+        std::lock_guard<std::mutex> g(c->scratch_mu);
+        for (auto& b : chunks) c->build_scratch.push_back(b);
+    }
+    bool take_kept(size_t size, DevBuf& out) {
+        std::lock_guard<std::mutex> g(c->scratch_mu);
+        auto& kept = c->build_scratch;
+        int best = -1;
+        for (int i = 0; i < (int)kept.size(); i++)
+            if (kept[i].bytes >= size && (best < 0 || kept[i].bytes < kept[best].bytes)) best = i;
+        if (best < 0) return false;
+        out = kept[best];
+        kept.erase(kept.begin() + best);
+        return true;
+    }
     template <class T> T* get(size_t count) {
         const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
         if (chunks.empty() || used + bytes > cap) {
-            chunks.emplace_back();
+            DevBuf b;
             const size_t size = std::max(bytes, kChunk);
-            if (dev_alloc(c, chunks.back(), size)) {
-                chunks.pop_back();
+            if (!take_kept(size, b) && dev_alloc(c, b, size)) {
                 failed = true;
                 return nullptr;
             }
+            chunks.push_back(b);
             used = 0;
-            cap = size;
+            cap = b.bytes;
         }
         T* p = (T*)((char*)chunks.back().p + used);
         used += bytes;
@@ -1485,9 +1507,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     if (tmp.failed || !cen || !K.collapse) return set_err(c, SRT_E_OOM, "BVH build scratch exceeds the HBM budget");
 
     // final outputs
-    dev_free(c, s->order);
-    dev_free(c, s->tris);
-    if (dev_alloc(c, s->order, (size_t)n * 4) || dev_alloc(c, s->tris, (size_t)n * 48)) return SRT_E_OOM;
+    if (dev_realloc(c, s->order, (size_t)n * 4) || dev_realloc(c, s->tris, (size_t)n * 48)) return SRT_E_OOM;
     uint32_t* order = (uint32_t*)s->order.p;
 
     k_tri_boxes<<<nb, 256, 0, st>>>((const float*)s->pos.p, (const uint32_t*)s->idx.p, n, tb, cen, partial);
@@ -1572,12 +1592,9 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     }
     const int top = m - 1;
     const int num_nodes = std::max(top + std::max(n - 1, 0), 1);
-    dev_free(c, s->nodes);
-    dev_free(c, s->node_parent);
-    dev_free(c, s->level_nodes);
-    dev_free(c, s->node_valid);
-    if (dev_alloc(c, s->nodes, (size_t)num_nodes * sizeof(BvhNode)) || dev_alloc(c, s->node_parent, (size_t)num_nodes * 4) ||
-        dev_alloc(c, s->level_nodes, (size_t)num_nodes * 4) || dev_alloc(c, s->node_valid, (size_t)num_nodes * 4))
+    if (dev_realloc(c, s->nodes, (size_t)num_nodes * sizeof(BvhNode)) ||
+        dev_realloc(c, s->node_parent, (size_t)num_nodes * 4) || dev_realloc(c, s->level_nodes, (size_t)num_nodes * 4) ||
+        dev_realloc(c, s->node_valid, (size_t)num_nodes * 4))
         return SRT_E_OOM;
     BvhNode* nodes = (BvhNode*)s->nodes.p;
     int32_t* valid = (int32_t*)s->node_valid.p;
@@ -1661,8 +1678,7 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
     }
     if (n > 1)
         k_emit<<<nblk(n - 1), 256, 0, st>>>(n, T, top, max_leaf, kl, kr, kfirst, par_int, order, tb, K, nodes, valid);
-    dev_free(c, s->tri_rec);
-    if (dev_alloc(c, s->tri_rec, (size_t)n * 8 * sizeof(float4))) return SRT_E_OOM;
+    if (dev_realloc(c, s->tri_rec, (size_t)n * 8 * sizeof(float4))) return SRT_E_OOM;
     launch_tris(s, st);
     int32_t* parent = (int32_t*)s->node_parent.p;
     k_fill_i32<<<nblk(num_nodes), 256, 0, st>>>(parent, num_nodes, -1);
@@ -1722,16 +1738,16 @@ int bvh_build(srt_ctx* c, SceneRec* s, int max_leaf, int cluster_size) {
 }
 
 int bvh4_build(srt_ctx* c, SceneRec* s) {
-    dev_free(c, s->nodes4);
     s->num_nodes4 = 0;
     s->depth4 = 0;
     if (s->root < 0) {            // a single leaf
+        dev_free(c, s->nodes4);
         s->root4 = s->root;
         return SRT_OK;
     }
     hipStream_t st = c->stream;
     const int n2 = s->num_nodes;
-    if (dev_alloc(c, s->nodes4, (size_t)n2 * sizeof(Bvh4Node))) return SRT_E_OOM;
+    if (dev_realloc(c, s->nodes4, (size_t)n2 * sizeof(Bvh4Node))) return SRT_E_OOM;
     Tmp tmp(c);
     int32_t* lev[2] = {tmp.get<int32_t>(n2), tmp.get<int32_t>(n2)};
     uint32_t* ninner = tmp.get<uint32_t>(n2);

@@ -92,6 +92,7 @@ static void dispatcher(srt_ctx* c) {
                 Request* q = S->pending.front();
                 size_t need = request_bytes(q->req);
                 if (c->used.load() + need > c->budget) {
+                    if (S->active.empty() && release_build_scratch(c)) continue;   // kept BVH-build chunks first
                     if (S->active.empty()) {          // nothing will ever free memory for it
                         S->pending.pop_front();
                         set_err(c, SRT_E_OOM, "request %llu needs %zu bytes beyond the HBM budget",

@@ -28,12 +28,27 @@ int hip_check(srt_ctx* c, hipError_t e, const char* what) {
     return set_err(c, SRT_E_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+bool release_build_scratch(srt_ctx* c) {
+    std::vector<DevBuf> drop;
+    {
+        std::lock_guard<std::mutex> g(c->scratch_mu);
+        drop.swap(c->build_scratch);
+    }
+    for (auto& b : drop) dev_free(c, b);
+    return !drop.empty();
+}
+
 int dev_alloc(srt_ctx* c, DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
+    if (c->used + bytes > c->budget) release_build_scratch(c);
     if (c->used + bytes > c->budget)
         return set_err(c, SRT_E_OOM, "allocation of %zu bytes exceeds the HBM budget (%zu of %zu used)", bytes, c->used.load(), c->budget);
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess && release_build_scratch(c)) {
+        (void)hipGetLastError();          // the failed attempt's error is not this call's result
+        e = hipMalloc(&p, bytes);
+    }
     if (e != hipSuccess) return hip_check(c, e, "hipMalloc");
     b.p = p;
     b.bytes = bytes;
@@ -48,6 +63,13 @@ void dev_free(srt_ctx* c, DevBuf& b) {
     }
     b.p = nullptr;
     b.bytes = 0;
+}
+
+int dev_realloc(srt_ctx* c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.p && b.bytes == bytes) return SRT_OK;
+    dev_free(c, b);
+    return dev_alloc(c, b, bytes);
 }
 
 void prof_begin(srt_ctx* c, const char* name, hipEvent_t* ev) {
@@ -184,6 +206,7 @@ void srt_destroy(srt_ctx* c) {
     c->user_bufs.clear();
     for (auto& b : c->mlpbuf) dev_free(c, b);
     dev_free(c, c->scan_tmp); dev_free(c, c->train_in9); dev_free(c, c->train_t); dev_free(c, c->segbuf);
+    release_build_scratch(c);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -242,6 +265,12 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
     if (!strcmp(key, "trace_blocks_per_cu")) {
         if (value < 0 || value > 64) return set_err(c, SRT_E_INVALID, "trace_blocks_per_cu must be in [0, 64]");
         c->trace_blocks_per_cu = (int32_t)value;
+        return SRT_OK;
+    }
+    if (!strcmp(key, "keep_build_scratch")) {
+        if (value != 0 && value != 1) return set_err(c, SRT_E_INVALID, "keep_build_scratch must be 0 or 1");
+        c->keep_build_scratch = value == 1;
+        if (!c->keep_build_scratch) release_build_scratch(c);
         return SRT_OK;
     }
     if (!strcmp(key, "hbm_budget")) {

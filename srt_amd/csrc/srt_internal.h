@@ -309,6 +309,12 @@ struct srt_ctx {
     int64_t pool_cap = 0;                 // path slots allocated
     int64_t pool_request = 1ll << 27;      // ~184 B per slot: 24.7 GB covers a 1080p x 64 spp frame in one chunk
     srt::DevBuf q[2][14], hits[5], shq[11], Lbuf[3], counters, film_acc, film_out, gstack;
+    // device chunks of BVH-build scratch kept between builds (a rebuild then
+    // allocates and frees nothing); released when an allocation would exceed
+    // the budget or fail, and at srt_destroy
+    std::vector<srt::DevBuf> build_scratch;
+    std::mutex scratch_mu;
+    bool keep_build_scratch = true;
     int64_t film_pixels = 0;
     size_t gstack_entries = 0;            // int32 entries allocated in gstack
     int32_t trace_lds_depth = 16, trace_refill = 40, shadow_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
@@ -350,6 +356,10 @@ namespace srt {
 // allocation helpers (budget-checked)
 int dev_alloc(srt_ctx* c, DevBuf& b, size_t bytes);
 void dev_free(srt_ctx* c, DevBuf& b);
+// keeps b when it already holds exactly `bytes`, else frees and allocates
+int dev_realloc(srt_ctx* c, DevBuf& b, size_t bytes);
+// frees the kept build-scratch chunks; returns whether there were any
+bool release_build_scratch(srt_ctx* c);
 int set_err(srt_ctx* c, int code, const char* fmt, ...);
 int hip_check(srt_ctx* c, hipError_t e, const char* what);
 // kernel timing

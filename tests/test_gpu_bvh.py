@@ -63,6 +63,36 @@ def test_cornell(ctx):
     _compare(ctx, scenes.cornell())
 
 
+def test_rebuilds_with_kept_scratch(ctx):
+    """Builds take the device scratch chunks kept by earlier builds (larger and
+    smaller scenes, the same scene again): every tree equals the oracle's, and
+    with keep_build_scratch 0 the chunks are freed."""
+    _compare(ctx, _soup(40000, seed=21))
+    _compare(ctx, _soup(3000, seed=22))
+    s = _soup(20000, seed=23)
+    h = ctx.upload(s)
+    ctx.build_bvh(h)
+    first = ctx.bvh_download(h)
+    for _ in range(2):
+        ctx.build_bvh(h)
+        again = ctx.bvh_download(h)
+        for a, b in zip(first[:3], again[:3]):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    kept = ctx.lib.srt_hbm_used(ctx.h)
+    ctx.set_option("keep_build_scratch", 0)
+    try:
+        assert ctx.lib.srt_hbm_used(ctx.h) < kept
+        ctx.build_bvh(h)
+        assert ctx.lib.srt_hbm_used(ctx.h) < kept
+        again = ctx.bvh_download(h)
+        for a, b in zip(first[:3], again[:3]):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    finally:
+        ctx.set_option("keep_build_scratch", 1)
+        ctx.release(h)
+    _compare(ctx, _soup(5000, seed=24), cluster_size=16)
+
+
 def test_spheres_1m(ctx):
     info = _compare(ctx, scenes.spheres(), max_leaf=2)     # the library's defaults
     assert info["triangles"] == 1000008

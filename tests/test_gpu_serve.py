@@ -86,6 +86,7 @@ def test_admission_by_budget(served):
     ctx, reqs, alone = served
     h, cam, W, H, spp, d, rr, nee, mis, seed, m = reqs[0]
     film = W * H * 28
+    ctx.set_option("keep_build_scratch", 0)     # memory in use stays fixed while serving
     budget0 = ctx.lib.srt_hbm_used(ctx.h)
     ctx.set_option("hbm_budget", budget0 + film + film // 2)   # one request's film at a time
     ctx.serve_start(0)
@@ -97,6 +98,7 @@ def test_admission_by_budget(served):
     finally:
         ctx.serve_stop()
         ctx.set_option("hbm_budget", 1 << 62)
+        ctx.set_option("keep_build_scratch", 1)
     for (t, img), r in zip(tickets, res):
         assert r["status"] == 2
         assert np.array_equal(img, alone[0])
