@@ -118,7 +118,11 @@ const char* srt_last_error(const srt_ctx* ctx);
  * 4-wide nodes, else it is traversed in binary form with the same results;
  * 0..16777215, the default: 4-wide nodes are addressed with 32-bit byte
  * offsets), "serve_age" (serving order, below: the weight of the time a request
- * has waited, per mille, 0..1000, default 250), "keep_build_scratch" (1, the
+ * has waited, per mille, 0..1000, default 250), "trace_packet" (camera rays of a
+ * flat scene with a 4-wide tree are traversed as 64-ray packets, one node
+ * sequence per wave; the value is the number of lanes holding postponed leaves
+ * that makes the wave test them, 1..64, default 16; 0: per-ray traversal; the
+ * results are identical), "keep_build_scratch" (1, the
  * default: the device scratch of a BVH build stays allocated for the next build
  * and counts in srt_hbm_used; it is freed when an allocation would exceed the
  * budget or fail; 0: freed now and at the end of every build).  Limits: 2^28 - 1 triangles per scene; "path_pool" and the pool

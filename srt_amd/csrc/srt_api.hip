@@ -267,6 +267,11 @@ int srt_set_option(srt_ctx* c, const char* key, int64_t value) {
         c->trace_blocks_per_cu = (int32_t)value;
         return SRT_OK;
     }
+    if (!strcmp(key, "trace_packet")) {
+        if (value < 0 || value > 64) return set_err(c, SRT_E_INVALID, "trace_packet must be in [0, 64]");
+        c->trace_packet = (int32_t)value;
+        return SRT_OK;
+    }
     if (!strcmp(key, "keep_build_scratch")) {
         if (value != 0 && value != 1) return set_err(c, SRT_E_INVALID, "keep_build_scratch must be 0 or 1");
         c->keep_build_scratch = value == 1;

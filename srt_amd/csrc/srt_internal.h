@@ -61,6 +61,8 @@ struct TraceCfg {
     int32_t refill;      // finished lanes that make a wave fetch new rays, 1..64
     int32_t wide;        // 1: traverse the 4-wide tree, 0: the binary tree
     int32_t inner_exit;  // leave the inner-node loop once at most this many lanes still search a leaf
+    int32_t packet;      // > 0: each wave traverses its 64 rays as one packet (coherent camera rays),
+                         // testing postponed leaves once this many lanes hold one
 };
 
 // ---------------------------------------------------------------------------
@@ -320,6 +322,7 @@ struct srt_ctx {
     int32_t trace_lds_depth = 16, trace_refill = 40, shadow_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
     int32_t trace_bvh_width = 4;          // traversal over the 4-wide (4) or binary (2) tree
     int32_t trace_inner_exit = 18;        // stragglers resume after the leaf step
+    int32_t trace_packet = 16;            // camera rays traversed as 64-ray packets (0: per ray)
     int64_t max_wide_nodes = srt::kMaxNodes4;   // 4-wide trees above this many nodes are traversed in binary form
     // serving order: remaining paths at 2 Gpaths/s minus serve_age/1000 of the
     // time waited (0: shortest remaining first; 1000: earliest due time)

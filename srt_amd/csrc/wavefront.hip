@@ -306,6 +306,15 @@ SRT_D bool slab(const TravState& st, float bnx, float bfx, float bny, float bfy,
     *tn = tnear;
     return tnear <= tfar;
 }
+SRT_D void ucswap(uint32_t& ka, int32_t& ra, uint32_t& kb, int32_t& rb) {
+    const bool s = kb < ka;
+    const uint32_t k = s ? kb : ka;
+    const int32_t r = s ? rb : ra;
+    kb = s ? ka : kb;
+    rb = s ? ra : rb;
+    ka = k;
+    ra = r;
+}
 SRT_D void cswap(float& ka, int32_t& ra, float& kb, int32_t& rb) {
     const bool s = kb < ka;
     const float k = s ? kb : ka;
@@ -549,6 +558,130 @@ SRT_D void trace_persistent(const DevScene& S, const View& view, uint32_t lo, ui
     }
 }
 
+// Packet traversal of coherent rays (camera rays: the 64 paths of a wave are
+// one 8x8 pixel tile): the wave walks one node sequence for its 64 rays.  At
+// an inner node every lane tests its ray against the four child boxes (the
+// same slab test as trav_inner4).  An inner child is visited when any lane's
+// ray hits it, nearest first in the order of the first lane that hits one;
+// the wave's stack, that order and the loop control are wave-uniform (SGPRs,
+// a scalar sort of the entry distances' bit patterns, which order as unsigned
+// integers since tnear >= tmin > 0), so a node step costs the slab tests and a
+// few cross-lane reads instead of a per-lane sort and stack.  A leaf child is
+// not visited by the wave: each lane whose ray hits it appends it to its own
+// list of postponed leaves, and the lanes test their lists together once
+// `flush` lanes hold one (or a list could overflow, or the traversal ends), so
+// the triangle tests run with most lanes active and each lane tests only the
+// leaves its own box tests admit.  The closest hit is the same as with
+// per-ray traversal: each lane tests every leaf whose box its ray enters
+// before its current closest hit, and the (t, primitive) minimum does not
+// depend on the order.  The wave's stack holds at most 3 entries per tree
+// level; it lives in the first rows of the wave's LDS stack area (entry e at
+// row e / 64, lane e mod 64), the leaf lists in its last kPacketLeafRows rows
+// (the host enables packets only when both fit).
+// The wave takes 64 consecutive entries of [lo, hi) at a time from `fetch`.
+constexpr int kPacketLeafRows = 8;       // postponed leaves per lane
+template <class IO, class View>
+SRT_D void trace_packet(const DevScene& S, const View& view, uint32_t lo, uint32_t hi, uint32_t* fetch, int lds_depth,
+                        int flush, IO& io) {
+    const __amdgpu_buffer_rsrc_t rsrc = node4_rsrc(S);
+    const int lane = __lane_id();
+    int32_t* wst = trav_lds + (threadIdx.x & ~63u);
+    int32_t* lf = trav_lds + (lds_depth - kPacketLeafRows) * kTraceBlock + threadIdx.x;
+    const uint32_t n = hi - lo;
+    const float inf = __builtin_huge_valf();
+    uint32_t nt = 0;
+    for (;;) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(fetch, 64u);
+        b = __builtin_amdgcn_readfirstlane(b);
+        if (b >= n) break;
+        const uint32_t j = b + (uint32_t)lane;
+        const bool valid = j < n;
+        const uint32_t ri = view.entry(lo + (valid ? j : n - 1u));   // lanes past the end repeat the last ray
+        TravState st;
+        io.template load<false>(ri, st, S.root4);
+        int cnt = 0;                      // this lane's postponed leaves, at lf[i * kTraceBlock]
+        auto test_leaves = [&]() {
+            for (int i = 0; __ballot(i < cnt) != 0ull; i++) {
+                if (i < cnt) {
+                    st.pend = lf[i * kTraceBlock];
+                    (void)trav_leaf<false, false, false>(S, st, nt);
+                }
+            }
+            cnt = 0;
+        };
+        int32_t cur = S.root4;
+        int32_t sp = 0;
+        if (cur < 0) {                    // the whole tree is one leaf
+            st.pend = cur;
+            (void)trav_leaf<false, false, false>(S, st, nt);
+        }
+        while (cur >= 0) {
+            const uint32_t nb = (uint32_t)cur * (uint32_t)sizeof(Bvh4Node);
+            const float4 nx = ld4(rsrc, nb + st.onx), fx = ld4(rsrc, nb + (st.onx ^ 16u));
+            const float4 ny = ld4(rsrc, nb + st.ony), fy = ld4(rsrc, nb + (st.ony ^ 16u));
+            const float4 nz = ld4(rsrc, nb + st.onz), fz = ld4(rsrc, nb + (st.onz ^ 16u));
+            const int4 rf = ld4i(rsrc, nb + 96u);
+            float k0, k1, k2, k3;
+            const bool h0 = slab(st, nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, &k0);
+            const bool h1 = slab(st, nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, &k1);
+            const bool h2 = slab(st, nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, &k2);
+            const bool h3 = slab(st, nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, &k3);
+            int32_t r0 = __builtin_amdgcn_readfirstlane(rf.x), r1 = __builtin_amdgcn_readfirstlane(rf.y);
+            int32_t r2 = __builtin_amdgcn_readfirstlane(rf.z), r3 = __builtin_amdgcn_readfirstlane(rf.w);
+            // leaf children go to the lists of the lanes that hit them
+            if (r0 < 0 && h0) { lf[cnt * kTraceBlock] = r0; cnt++; }
+            if (r1 < 0 && h1) { lf[cnt * kTraceBlock] = r1; cnt++; }
+            if (r2 < 0 && h2) { lf[cnt * kTraceBlock] = r2; cnt++; }
+            if (r3 < 0 && h3) { lf[cnt * kTraceBlock] = r3; cnt++; }
+            const unsigned long long m0 = r0 >= 0 ? __ballot(h0) : 0ull, m1 = r1 >= 0 ? __ballot(h1) : 0ull;
+            const unsigned long long m2 = r2 >= 0 ? __ballot(h2) : 0ull, m3 = r3 >= 0 ? __ballot(h3) : 0ull;
+            const unsigned long long any = m0 | m1 | m2 | m3;
+            if (any != 0ull) {
+                // keys: the first such lane's entry distances (+inf for a child only
+                // other lanes hit), absent children last
+                const int ld = __builtin_ctzll(any);
+                uint32_t q0 = __builtin_amdgcn_readlane(__float_as_uint(h0 ? k0 : inf), ld);
+                uint32_t q1 = __builtin_amdgcn_readlane(__float_as_uint(h1 ? k1 : inf), ld);
+                uint32_t q2 = __builtin_amdgcn_readlane(__float_as_uint(h2 ? k2 : inf), ld);
+                uint32_t q3 = __builtin_amdgcn_readlane(__float_as_uint(h3 ? k3 : inf), ld);
+                q0 = m0 ? q0 : 0xffffffffu; q1 = m1 ? q1 : 0xffffffffu;
+                q2 = m2 ? q2 : 0xffffffffu; q3 = m3 ? q3 : 0xffffffffu;
+                const int nin = (m0 ? 1 : 0) + (m1 ? 1 : 0) + (m2 ? 1 : 0) + (m3 ? 1 : 0);
+                ucswap(q0, r0, q1, r1);
+                ucswap(q2, r2, q3, r3);
+                ucswap(q0, r0, q2, r2);
+                ucswap(q1, r1, q3, r3);
+                ucswap(q1, r1, q2, r2);
+                // push the others farthest first: entry sp + i holds sorted child nin - 1 - i
+                const int np = nin - 1;
+                if (lane < np) {
+                    const int c = np - lane;        // 1 .. 3
+                    const int32_t v = c == 1 ? r1 : (c == 2 ? r2 : r3);
+                    const int e = sp + lane;
+                    wst[(e >> 6) * kTraceBlock + (e & 63)] = v;
+                }
+                __builtin_amdgcn_wave_barrier();
+                sp += np;
+                cur = r0;
+            } else if (sp > 0) {
+                sp--;
+                cur = __builtin_amdgcn_readfirstlane(wst[(sp >> 6) * kTraceBlock + (sp & 63)]);
+            } else {
+                cur = -1;                     // traversal done
+            }
+            // test the postponed leaves when enough lanes hold one, when a list
+            // could not take another node's four, or at the end
+            const unsigned long long pend = __ballot(cnt > 0);
+            if (pend != 0ull &&
+                (cur < 0 || __popcll(pend) >= (unsigned)flush || __ballot(cnt > kPacketLeafRows - 4) != 0ull))
+                test_leaves();
+        }
+        if (valid) io.store(ri, st, false);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // A partitioned queue read as the concatenation of its partitions: prefix[p]
 // entries come before partition p, prefix[kParts] is the total (in LDS).
 struct QueueView {
@@ -766,6 +899,26 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, 
         atomicAdd(&totals[2], (unsigned long long)nv);
         atomicAdd(&totals[3], (unsigned long long)nt);
     }
+}
+
+// Closest hit of the camera rays of queue `qi` as 64-ray packets (trace_packet);
+// the same contract as k_trace<false, true>.
+__global__ void __launch_bounds__(kTraceBlock) k_trace_packet(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
+                                                             uint32_t cap, unsigned long long* totals, TraceCfg cfg) {
+    __shared__ uint32_t prefix[kParts + 1];
+    uint32_t lo, hi;
+    const QueueView view = queue_piece(counts, cnt_q(qi), cap, prefix, lo, hi);
+    uint32_t* fetch = counts + CNT_FETCH_T + my_part() * kCntLine;
+    if (blockIdx.x == 0) {
+        clear_parts(counts + CNT_FETCH_S);
+        clear_parts(counts + cnt_q(qi ^ 1));
+        clear_parts(counts + CNT_SHADOW);
+        clear_parts(counts + CNT_MLPQ_PARTS);
+        if (threadIdx.x == 0) counts[CNT_MLPQ] = 0u;
+    }
+    if (blockIdx.x < (unsigned)kParts && threadIdx.x == 0) atomicAdd(&totals[0], (unsigned long long)(hi - lo));
+    QueueIO io{q, h};
+    trace_packet<QueueIO, QueueView>(S, view, lo, hi, fetch, cfg.lds_depth, cfg.packet, io);
 }
 
 struct LightSample { f3 pos, normal, emission; float pdf; };
@@ -1674,7 +1827,17 @@ static TraceCfg trace_cfg(srt_ctx* c, const SceneRec* s) {
     cfg.refill = c->trace_refill;
     cfg.wide = use_wide(c, s) ? 1 : 0;
     cfg.inner_exit = c->trace_inner_exit;
+    cfg.packet = 0;
     return cfg;
+}
+// Packet traversal of a launch's camera rays (TraceCfg.packet: the number of
+// lanes holding postponed leaves that makes a wave test them; 0: per-ray
+// traversal): a 4-wide flat scene whose wave stack (3 entries per level) fits
+// the rows of a wave's LDS stack area that the leaf lists leave.
+static int trace_packet_ok(const srt_ctx* c, const SceneRec* s, const TraceCfg& cfg) {
+    if (c->trace_packet <= 0 || !cfg.wide || s->ninst > 0 || s->num_nodes4 <= 0) return 0;
+    const int64_t rows = cfg.lds_depth - kPacketLeafRows;
+    return 3 * ((int64_t)s->depth4 + 1) <= 64 * rows ? c->trace_packet : 0;
 }
 
 static size_t trace_lds_bytes(const TraceCfg& cfg, bool inst = false) {
@@ -1683,13 +1846,14 @@ static size_t trace_lds_bytes(const TraceCfg& cfg, bool inst = false) {
 
 // Persistent traversal grid: as many workgroups as are resident at once
 // (occupancy of the traversal kernel with its LDS stack), capped by the work.
-static int trace_grid(srt_ctx* c, int64_t n) {
+static int trace_grid(srt_ctx* c, int64_t n, const void* kernel = nullptr) {
     const TraceCfg cfg = trace_cfg(c, nullptr);
     int per_cu = c->trace_blocks_per_cu;
     if (per_cu <= 0) {
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, true>, kTraceBlock, trace_lds_bytes(cfg)) !=
-                hipSuccess || nb <= 0)
+        if (!kernel) kernel = reinterpret_cast<const void*>(k_trace<false, true>);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kTraceBlock, trace_lds_bytes(cfg)) != hipSuccess ||
+            nb <= 0)
             nb = 8;
         per_cu = nb;
     }
@@ -1826,6 +1990,7 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     const MlpBind bind(c, s->handle);       // mode 1 queries the scene's MLP (else the context's own)
     if (int r = ensure_pool(c, n)) return r;
     const int tgrid = part_grid(trace_grid(c, n));
+    const int pgrid = part_grid(trace_grid(c, n, reinterpret_cast<const void*>(k_trace_packet)));
     const uint32_t cap = part_cap((uint32_t)n);
     if (int r = ensure_gstack(c, s, (int64_t)tgrid * kTraceBlock)) return r;
     const TraceCfg tcfg = trace_cfg(c, s);
@@ -1888,8 +2053,12 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
                 k_trace<true, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
             else k_trace<false, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
         } else if (tcfg.wide) {
+            TraceCfg pcfg = tcfg;
+            pcfg.packet = depth == 0 && !c->count_visits ? trace_packet_ok(c, s, tcfg) : 0;
             if (c->count_visits)
                 k_trace<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+            else if (pcfg.packet)
+                k_trace_packet<<<pgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, totals, pcfg);
             else k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
         } else {
             if (c->count_visits)

@@ -84,6 +84,27 @@ def test_tiny_images(ctx, W, H, spp):
     assert np.array_equal(img, ref)
 
 
+@pytest.mark.parametrize("packet", [1, 16, 64])
+def test_packet_traversal_equals_per_ray(ctx, packet):
+    """Camera rays traversed as 64-ray packets (option trace_packet: lanes with
+    postponed leaves that make a wave test them) give the per-ray traversal's
+    image bit for bit, also for partial last waves and scene edges."""
+    for s, W, H, spp, depth in ((scenes.spheres(), 96, 54, 2, 3), (scenes.cornell(), 37, 23, 3, 4)):
+        h = ctx.upload(s)
+        ctx.build_bvh(h)
+        try:
+            ctx.set_option("trace_packet", 0)
+            ref = ctx.render(h, s.camera, W, H, spp, depth, 4, 1, 1, 5)
+            ctx.set_option("trace_packet", packet)
+            img = ctx.render(h, s.camera, W, H, spp, depth, 4, 1, 1, 5)
+        finally:
+            ctx.set_option("trace_packet", 16)
+            ctx.release(h)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    with pytest.raises(Exception):
+        ctx.set_option("trace_packet", 65)
+
+
 def test_spheres_small(ctx):
     s = scenes.spheres()
     img, ref = _render_pair(ctx, s, 192, 108, 2, 8)
