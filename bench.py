@@ -69,8 +69,8 @@ VALU_FILE = os.path.join(ROOT, "profiles", "trace_spheres1m_valu_issue.md")
 MI355X_SIMDS, MI355X_XCDS = 256 * 4, 8
 COPY_ELEMS = 1 << 28          # calibration copy: 1 GiB read + 1 GiB written (float32)
 # kernels of the frame, as rocprofv3 names them
-# the timed (not counting) 4-wide kernels of a flat scene; the frame's closest-hit
-# launches are k_trace_packet (camera rays, depth 0) and k_trace (depths 1..)
+# the timed (not counting) 4-wide kernels of a flat scene; with the option
+# trace_packet set, the closest hits of camera rays run in k_trace_packet
 KERNEL_KEYS = {"trace": ("k_trace<false, true, false>", "k_trace_packet("), "shadow": ("k_shadow<false, true, false>",),
                "shade": ("k_shade(",)}
 
@@ -153,8 +153,7 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
              "gpurun_out/pmc_copy_FETCH_SIZE gpurun_out/pmc_copy_WRITE_SIZE", "",
              "Each rocprofv3 run: cd /tmp && export TMPDIR=/tmp, bench.py by its path in the repository.",
              "The frame runs hold one counting pass (k_trace<true, true>/k_shadow<true, true>, not counted here; its k_shade",
-             "dispatches are) and one frame; FETCH_SIZE/WRITE_SIZE are the L2-to-fabric bytes (MALL or HBM); trace is",
-             "k_trace_packet (camera rays) and k_trace<false, true, false> (later bounces).", "",
+             "dispatches are) and one frame; FETCH_SIZE/WRITE_SIZE are the L2-to-fabric bytes (MALL or HBM).", "",
              f"Calibration: copy kernel of {COPY_ELEMS * 4} B read + {COPY_ELEMS * 4} B written: "
              f"FETCH_SIZE {cf:.1f} KiB, WRITE_SIZE {cw:.1f} KiB per dispatch -> factors fetch {ff:.4f}, "
              f"write {fw:.4f}", "",
@@ -196,8 +195,7 @@ def valu_summary(d):
              "--steps 1 --warmup 0 --no-cpu --no-serve",
              "    python3 bench.py --valu-summary gpurun_out/pmc_valu", "",
              "The rocprofv3 run: cd /tmp && export TMPDIR=/tmp, bench.py by its path in the repository; the counting",
-             "pass's k_trace<true, true>/k_shadow<true, true> are not counted here (its k_shade dispatches are); trace",
-             "is k_trace_packet (camera rays) and k_trace<false, true, false> (later bounces).",
+             "pass's k_trace<true, true>/k_shadow<true, true> are not counted here (its k_shade dispatches are).",
              f"busy = 4 SQ_ACTIVE_INST_VALU / ({MI355X_SIMDS} SIMDs x GRBM_GUI_ACTIVE / {MI355X_XCDS}), summed over the "
              "kernel's dispatches; lanes = SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU); wait = SQ_WAIT_ANY / "
              "SQ_WAVE_CYCLES.", "",
@@ -327,8 +325,7 @@ def kstats_summary(d, mode="frame"):
                  "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_ks -o run -- python3 bench.py --steps 2 --warmup 1 --no-cpu --no-serve",
                  "    python3 bench.py --kstats-summary gpurun_out/prof_ks", "",
                  "The run renders 6 frames: the untimed counting pass (k_trace<true, true>, k_shadow<true, true>), 1 warmup,",
-                 "2 timed frames and 2 frames copied to host memory; closest hits of camera rays run in k_trace_packet, of",
-                 "later bounces in k_trace<false, true, false>; the BVH build runs twice (cold, then a rebuild): the",
+                 "2 timed frames and 2 frames copied to host memory; the BVH build runs twice (cold, then a rebuild): the",
                  "top-level SAH levels (k_wide_* for levels of <= 64 nodes, k_top_split, k_top_split_wave for levels of",
                  "> 1024 nodes, k_top_link / k_link_*), k_bottom_up_level, sorts, 4-wide collapse.", "",
                  "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]

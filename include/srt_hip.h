@@ -121,8 +121,10 @@ const char* srt_last_error(const srt_ctx* ctx);
  * has waited, per mille, 0..1000, default 250), "trace_packet" (camera rays of a
  * flat scene with a 4-wide tree are traversed as 64-ray packets, one node
  * sequence per wave; the value is the number of lanes holding postponed leaves
- * that makes the wave test them, 1..64, default 16; 0: per-ray traversal; the
- * results are identical), "keep_build_scratch" (1, the
+ * that makes the wave test them, 1..64; 0, the default: per-ray traversal; the
+ * results are identical; packets gain 1 % of the closest-hit time on a scene
+ * that fills the frame with deep traversals and lose 10 % where camera rays
+ * visit few nodes), "keep_build_scratch" (1, the
  * default: the device scratch of a BVH build stays allocated for the next build
  * and counts in srt_hbm_used; it is freed when an allocation would exceed the
  * budget or fail; 0: freed now and at the end of every build).  Limits: 2^28 - 1 triangles per scene; "path_pool" and the pool

@@ -322,7 +322,7 @@ struct srt_ctx {
     int32_t trace_lds_depth = 16, trace_refill = 40, shadow_refill = 32, trace_blocks_per_cu = 0;   // 0: occupancy limit
     int32_t trace_bvh_width = 4;          // traversal over the 4-wide (4) or binary (2) tree
     int32_t trace_inner_exit = 18;        // stragglers resume after the leaf step
-    int32_t trace_packet = 16;            // camera rays traversed as 64-ray packets (0: per ray)
+    int32_t trace_packet = 0;             // > 0: camera rays traversed as 64-ray packets (0: per ray)
     int64_t max_wide_nodes = srt::kMaxNodes4;   // 4-wide trees above this many nodes are traversed in binary form
     // serving order: remaining paths at 2 Gpaths/s minus serve_age/1000 of the
     // time waited (0: shortest remaining first; 1000: earliest due time)

@@ -578,7 +578,8 @@ SRT_D void trace_persistent(const DevScene& S, const View& view, uint32_t lo, ui
 // level; it lives in the first rows of the wave's LDS stack area (entry e at
 // row e / 64, lane e mod 64), the leaf lists in its last kPacketLeafRows rows
 // (the host enables packets only when both fit).
-// The wave takes 64 consecutive entries of [lo, hi) at a time from `fetch`.
+// The wave takes 64 consecutive entries of [lo, hi) at a time from `fetch`,
+// asking for the next 64 when it starts on the current ones.
 constexpr int kPacketLeafRows = 8;       // postponed leaves per lane
 template <class IO, class View>
 SRT_D void trace_packet(const DevScene& S, const View& view, uint32_t lo, uint32_t hi, uint32_t* fetch, int lds_depth,
@@ -590,11 +591,12 @@ SRT_D void trace_packet(const DevScene& S, const View& view, uint32_t lo, uint32
     const uint32_t n = hi - lo;
     const float inf = __builtin_huge_valf();
     uint32_t nt = 0;
+    uint32_t bn = 0;                      // the next batch: fetched while the current one is traversed
+    if (lane == 0) bn = atomicAdd(fetch, 64u);
     for (;;) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(fetch, 64u);
-        b = __builtin_amdgcn_readfirstlane(b);
+        const uint32_t b = __builtin_amdgcn_readfirstlane(bn);
         if (b >= n) break;
+        if (lane == 0) bn = atomicAdd(fetch, 64u);
         const uint32_t j = b + (uint32_t)lane;
         const bool valid = j < n;
         const uint32_t ri = view.entry(lo + (valid ? j : n - 1u));   // lanes past the end repeat the last ray

@@ -98,7 +98,7 @@ def test_packet_traversal_equals_per_ray(ctx, packet):
             ctx.set_option("trace_packet", packet)
             img = ctx.render(h, s.camera, W, H, spp, depth, 4, 1, 1, 5)
         finally:
-            ctx.set_option("trace_packet", 16)
+            ctx.set_option("trace_packet", 0)
             ctx.release(h)
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
     with pytest.raises(Exception):
