@@ -789,35 +789,37 @@ SRT_D uint32_t path_sample(int64_t pid, int32_t wh) {
     return (uint32_t)(pid / wh);
 }
 
-// One camera ray per slot, at queue entry = slot.  Slot i belongs to
-// segment k: path id pid = path_begin + (i - slot_offset), sample = pid / (W H),
-// pixel = tile_pixel(pid mod (W H)).
-__global__ void __launch_bounds__(256) k_raygen(PathQueue q, float* Lr, float* Lg, float* Lb, const Segment* segs, int nseg,
-                                                int n, uint32_t cap, uint32_t* counts) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0) {
-        clear_parts(counts + CNT_FETCH_T);
-        if (threadIdx.x < (unsigned)kParts)
-            counts[CNT_Q0 + threadIdx.x * kCntLine] = part_size((uint32_t)n, cap, threadIdx.x);
-    }
-    if (i >= n) return;
-    const int k = find_segment(segs, nseg, (uint32_t)i);
+// The state of path `slot` at its first bounce (the camera ray of its pixel
+// and sample, path_tracer.cu:66-68).  Slot i belongs to segment k: path id
+// pid = path_begin + (i - slot_offset), sample = pid / (W H), pixel =
+// tile_pixel(pid mod (W H)).  Throughput 1, scatter pdf 1, flags: specular
+// (bit 0) and the segment (bits 8..); rng: the stream after the two jitter
+// draws.  The first bounce's queue is the slots themselves (entry = slot, in
+// slot order over the partitions): the depth-0 trace and shade stages compute
+// this state where they need it instead of reading it from a queue.
+struct PathStart {
+    f3 o, d;
+    uint32_t flags;
+    uint64_t rng;
+};
+SRT_D PathStart path_start(const Segment* segs, int nseg, uint32_t slot) {
+    const int k = find_segment(segs, nseg, slot);
     const Segment& sg = segs[k];
-    int64_t pid = sg.path_begin + (i - sg.slot_offset);
-    uint32_t sample = path_sample(pid, sg.wh);
-    uint32_t pixel = tile_pixel((uint32_t)(pid - (int64_t)sample * sg.wh), (uint32_t)sg.width, (uint32_t)sg.height);
-    f3 ro, rd;
+    const int64_t pid = sg.path_begin + (slot - sg.slot_offset);
+    const uint32_t sample = path_sample(pid, sg.wh);
+    const uint32_t pixel = tile_pixel((uint32_t)(pid - (int64_t)sample * sg.wh), (uint32_t)sg.width, (uint32_t)sg.height);
+    PathStart p;
     Pcg32 rng;
-    camera_ray(sg.cam, sg.width, sg.height, pixel, sg.seed, sample, ro, rd, rng);
-    const uint32_t e = (uint32_t)i;           // partition i / cap, in slot order
-    q.ox[e] = ro.x; q.oy[e] = ro.y; q.oz[e] = ro.z;
-    q.dx[e] = rd.x; q.dy[e] = rd.y; q.dz[e] = rd.z;
-    q.br[e] = 1.0f; q.bg[e] = 1.0f; q.bb[e] = 1.0f;
-    q.pdf[e] = 1.0f;
-    q.flags[e] = 1u | ((uint32_t)k << 8);        // bit 0: specular, bits 8..: segment
-    q.rng[e] = rng.state;
-    q.slot[e] = (uint32_t)i;
-    Lr[i] = 0.0f; Lg[i] = 0.0f; Lb[i] = 0.0f;
+    camera_ray(sg.cam, sg.width, sg.height, pixel, sg.seed, sample, p.o, p.d, rng);
+    p.flags = 1u | ((uint32_t)k << 8);
+    p.rng = rng.state;
+    return p;
+}
+// Sizes of the first bounce's queue partitions (n slots, partition p holds
+// slots [p cap, p cap + size)); clears the traversal fetch counters.
+__global__ void k_path_counts(uint32_t n, uint32_t cap, uint32_t* counts) {
+    clear_parts(counts + CNT_FETCH_T);
+    if (threadIdx.x < (unsigned)kParts) counts[CNT_Q0 + threadIdx.x * kCntLine] = part_size(n, cap, threadIdx.x);
 }
 
 // Ray sources/sinks of the persistent traversal loop.
@@ -835,6 +837,20 @@ struct QueueIO {       // closest hit of queue entries -> hit buffer (same index
         h.v[i] = hit ? st.bv : 0.0f;
         h.tri[i] = hit ? st.best_pos : -1;
         if (h.inst) h.inst[i] = hit ? st.best_inst : -1;
+    }
+};
+struct CamIO {         // closest hit of the first bounce's camera rays (entry = slot) -> hit buffer
+    const Segment* segs;
+    int nseg;
+    HitBuf h;
+    template <bool INST>
+    SRT_D void load(uint32_t i, TravState& st, int32_t root) const {
+        const PathStart p = path_start(segs, nseg, i);
+        trav_begin<INST>(st, p.o, p.d, 1e-3f, 1e16f, root);
+    }
+    SRT_D void store(uint32_t i, const TravState& st, bool occluded) const {
+        const QueueIO q{PathQueue{}, h};
+        q.store(i, st, occluded);
     }
 };
 // L[slot] += c for the three channels; the three loads go out before the stores
@@ -877,10 +893,9 @@ SRT_D TravStack make_stack(int32_t* gstack, TraceCfg cfg) {
 // following shade and shadow stages append to or fetch from.  COUNT: also
 // total node visits and triangle tests (an untimed pass used for the
 // algorithmic byte count).
-template <bool COUNT, bool WIDE, bool INST = false>
-__global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
-                                                      uint32_t cap, int32_t* gstack, unsigned long long* totals,
-                                                      TraceCfg cfg) {
+template <bool COUNT, bool WIDE, bool INST, class IO>
+SRT_D void trace_stage(const DevScene& S, IO& io, uint32_t* counts, int qi, uint32_t cap, int32_t* gstack,
+                       unsigned long long* totals, TraceCfg cfg) {
     __shared__ uint32_t prefix[kParts + 1];
     uint32_t lo, hi;
     const QueueView view = queue_piece(counts, cnt_q(qi), cap, prefix, lo, hi);
@@ -894,19 +909,36 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, 
     }
     if (blockIdx.x < (unsigned)kParts && threadIdx.x == 0) atomicAdd(&totals[0], (unsigned long long)(hi - lo));
     const TravStack k = make_stack<INST>(gstack, cfg);
-    QueueIO io{q, h};
     uint32_t nv = 0, nt = 0;
-    trace_persistent<false, COUNT, WIDE, QueueIO, QueueView, INST>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
+    trace_persistent<false, COUNT, WIDE, IO, QueueView, INST>(S, view, lo, hi, fetch, cfg.refill, k, io, nv, nt);
     if (COUNT) {
         atomicAdd(&totals[2], (unsigned long long)nv);
         atomicAdd(&totals[3], (unsigned long long)nt);
     }
 }
+template <bool COUNT, bool WIDE, bool INST = false>
+__global__ void __launch_bounds__(kTraceBlock) k_trace(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
+                                                      uint32_t cap, int32_t* gstack, unsigned long long* totals,
+                                                      TraceCfg cfg) {
+    QueueIO io{q, h};
+    trace_stage<COUNT, WIDE, INST, QueueIO>(S, io, counts, qi, cap, gstack, totals, cfg);
+}
+// The first bounce (queue 0): the same, with each camera ray computed from
+// its slot (path_start) instead of read from the queue.
+template <bool COUNT, bool WIDE, bool INST = false>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_cam(DevScene S, const Segment* segs, int nseg, HitBuf h,
+                                                          uint32_t* counts, uint32_t cap, int32_t* gstack,
+                                                          unsigned long long* totals, TraceCfg cfg) {
+    CamIO io{segs, nseg, h};
+    trace_stage<COUNT, WIDE, INST, CamIO>(S, io, counts, 0, cap, gstack, totals, cfg);
+}
 
-// Closest hit of the camera rays of queue `qi` as 64-ray packets (trace_packet);
-// the same contract as k_trace<false, true>.
-__global__ void __launch_bounds__(kTraceBlock) k_trace_packet(DevScene S, PathQueue q, HitBuf h, uint32_t* counts, int qi,
-                                                             uint32_t cap, unsigned long long* totals, TraceCfg cfg) {
+// Closest hit of the first bounce's camera rays as 64-ray packets
+// (trace_packet); the same contract as k_trace_cam<false, true>.
+__global__ void __launch_bounds__(kTraceBlock) k_trace_packet(DevScene S, const Segment* segs, int nseg, HitBuf h,
+                                                             uint32_t* counts, uint32_t cap, unsigned long long* totals,
+                                                             TraceCfg cfg) {
+    constexpr int qi = 0;
     __shared__ uint32_t prefix[kParts + 1];
     uint32_t lo, hi;
     const QueueView view = queue_piece(counts, cnt_q(qi), cap, prefix, lo, hi);
@@ -919,8 +951,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_packet(DevScene S, PathQu
         if (threadIdx.x == 0) counts[CNT_MLPQ] = 0u;
     }
     if (blockIdx.x < (unsigned)kParts && threadIdx.x == 0) atomicAdd(&totals[0], (unsigned long long)(hi - lo));
-    QueueIO io{q, h};
-    trace_packet<QueueIO, QueueView>(S, view, lo, hi, fetch, cfg.lds_depth, cfg.packet, io);
+    CamIO io{segs, nseg, h};
+    trace_packet<CamIO, QueueView>(S, view, lo, hi, fetch, cfg.lds_depth, cfg.packet, io);
 }
 
 struct LightSample { f3 pos, normal, emission; float pdf; };
@@ -1060,14 +1092,22 @@ SRT_D MatRec load_mat(const MatRec* mats, int32_t id) {
     return m;
 }
 
-SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const PathQueue& qin, const HitBuf& h,
+// CAM: the first bounce, whose entries are the slots and whose path state is
+// path_start's; it also sets the path's radiance (no earlier stage does).
+template <bool CAM>
+SRT_D void shade_hit(const DevScene& S, const Segment* segs, int nseg, int depth, const PathQueue& qin, const HitBuf& h,
                      const PathQueue& qout, const ShadowQueue& sq, float* Lr, float* Lg, float* Lb, uint32_t* counts,
                      int qi, uint32_t part, uint32_t cap, const MlpIO& mio, uint32_t i) {
-    const uint32_t slot = qin.slot[i];
-    const f3 rd = mk(qin.dx[i], qin.dy[i], qin.dz[i]);
-    f3 beta = mk(qin.br[i], qin.bg[i], qin.bb[i]);
+    PathStart ps{};
+    if constexpr (CAM) ps = path_start(segs, nseg, i);
+    const uint32_t slot = CAM ? i : qin.slot[i];
+    const f3 rd = CAM ? ps.d : mk(qin.dx[i], qin.dy[i], qin.dz[i]);
+    f3 beta = CAM ? mk(1.0f) : mk(qin.br[i], qin.bg[i], qin.bb[i]);
+    if constexpr (CAM) {
+        Lr[slot] = 0.0f; Lg[slot] = 0.0f; Lb[slot] = 0.0f;
+    }
     const int32_t tri = h.tri[i];
-    const uint32_t flags = qin.flags[i];
+    const uint32_t flags = CAM ? ps.flags : qin.flags[i];
     const bool specular = (flags & 1u) != 0u;
     const uint32_t segk = flags >> 8;
     const SegHead P = load_seg_head(segs, segk);
@@ -1088,10 +1128,10 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
             if (P.use_nee && !specular) {
                 mis = 0.0f;
                 if (P.use_mis) {
-                    const f3 ro = mk(qin.ox[i], qin.oy[i], qin.oz[i]);
+                    const f3 ro = CAM ? ps.o : mk(qin.ox[i], qin.oy[i], qin.oz[i]);
                     const float t2 = dot(pos - ro, pos - ro);
                     const float lpdf = (m.intensity / S.light_wsum) * t2 / cos_i;
-                    const float scatter_pdf = qin.pdf[i];
+                    const float scatter_pdf = CAM ? 1.0f : qin.pdf[i];
                     mis = scatter_pdf / (lpdf + scatter_pdf);
                 }
             }
@@ -1109,7 +1149,7 @@ SRT_D void shade_hit(const DevScene& S, const Segment* segs, int depth, const Pa
     const bool last = (depth + 1 >= P.max_depth);
     const int64_t pid = P.path_begin + (int64_t)(slot - (uint32_t)P.slot_offset);
     const uint32_t sample = path_sample(pid, P.wh);
-    Pcg32 rng = rng_resume(qin.rng[i], sample);
+    Pcg32 rng = rng_resume(CAM ? ps.rng : qin.rng[i], sample);
     // next-event estimation (path_tracer.cu:113-142); the shadow ray and its
     // would-be contribution go to the shadow queue
     if (P.use_nee && !mat_is_specular(m)) {
@@ -1214,10 +1254,11 @@ SRT_D void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
-__global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segment* segs, int depth, PathQueue qin,
-                                                         HitBuf h, PathQueue qout, ShadowQueue sq, float* Lr,
-                                                         float* Lg, float* Lb, uint32_t* counts, int qi, uint32_t cap,
-                                                         MlpIO mio) {
+template <bool CAM>
+__global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segment* segs, int nseg, int depth,
+                                                         PathQueue qin, HitBuf h, PathQueue qout, ShadowQueue sq,
+                                                         float* Lr, float* Lg, float* Lb, uint32_t* counts, int qi,
+                                                         uint32_t cap, MlpIO mio) {
     __shared__ uint32_t gather[kShadeBlock / 64][128];
     const int lane = threadIdx.x & 63;
     uint32_t* buf = gather[threadIdx.x >> 6];
@@ -1243,9 +1284,14 @@ __global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segm
             const uint32_t i = valid ? view.entry(lo + r) : 0u;
             const int32_t tri = valid ? h.tri[i] : -1;
             if (valid && tri < 0) {
-                const uint32_t slot = qin.slot[i];
-                const f3 add = mk(qin.br[i], qin.bg[i], qin.bb[i]) * env;
-                add_rgb(Lr, Lg, Lb, slot, add.x, add.y, add.z);
+                if constexpr (CAM) {
+                    // a camera ray's miss: its radiance is beta (1) times the environment
+                    Lr[i] = 0.0f + env.x; Lg[i] = 0.0f + env.y; Lb[i] = 0.0f + env.z;
+                } else {
+                    const uint32_t slot = qin.slot[i];
+                    const f3 add = mk(qin.br[i], qin.bg[i], qin.bb[i]) * env;
+                    add_rgb(Lr, Lg, Lb, slot, add.x, add.y, add.z);
+                }
             }
             const bool hit = valid && tri >= 0;
             const unsigned long long m = __ballot(hit);
@@ -1261,7 +1307,8 @@ __global__ void __launch_bounds__(kShadeBlock, 4) k_shade(DevScene S, const Segm
         wave_sync_lds();
         if (lane + 64u < nbuf) buf[lane] = rest;
         nbuf -= take;
-        if ((uint32_t)lane < take) shade_hit(S, segs, depth, qin, h, qout, sq, Lr, Lg, Lb, counts, qi, part, cap, mio, e);
+        if ((uint32_t)lane < take)
+            shade_hit<CAM>(S, segs, nseg, depth, qin, h, qout, sq, Lr, Lg, Lb, counts, qi, part, cap, mio, e);
         wave_sync_lds();
     }
 }
@@ -2043,33 +2090,42 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     const int shade_grid = part_grid(std::min<int64_t>((n + 255) / 256, (int64_t)num_cus(c) * 8));
     const int nblk = (int)((n + 255) / 256);
     hipEvent_t ev;
-    prof_begin(c, "raygen", &ev);
-    k_raygen<<<nblk, 256, 0, st>>>(Q[0], Lr, Lg, Lb, segs, nseg, (int)n, cap, counts);
-    prof_end(c, "raygen", ev, n);
+    // the first bounce's queue is the slots (its trace and shade stages compute
+    // each path's camera ray and state); only its partition sizes are set here
+    k_path_counts<<<1, 64, 0, st>>>((uint32_t)n, cap, counts);
+    if (max_depth == 0)                   // no shade stage sets the radiance
+        for (float* L : {Lr, Lg, Lb}) k_zero<<<nblk, 256, 0, st>>>(L, (size_t)n);
     if (mlp_mode == 2) k_zero<<<nblk, 256, 0, st>>>((float*)mio.rec_valid, (size_t)n);
     for (int depth = 0; depth < max_depth; depth++) {
         int qi = depth & 1;
+        const bool cam = depth == 0;
+        const bool cnt = c->count_visits;
         prof_begin(c, "trace", &ev);
         if (inst) {
-            if (c->count_visits)
-                k_trace<true, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+            if (cam && cnt) k_trace_cam<true, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, segs, nseg, H, counts, cap, gst, totals, tcfg);
+            else if (cam) k_trace_cam<false, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, segs, nseg, H, counts, cap, gst, totals, tcfg);
+            else if (cnt) k_trace<true, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
             else k_trace<false, true, true><<<tgrid, kTraceBlock, ilds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
         } else if (tcfg.wide) {
             TraceCfg pcfg = tcfg;
-            pcfg.packet = depth == 0 && !c->count_visits ? trace_packet_ok(c, s, tcfg) : 0;
-            if (c->count_visits)
-                k_trace<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
-            else if (pcfg.packet)
-                k_trace_packet<<<pgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, totals, pcfg);
+            pcfg.packet = cam && !cnt ? trace_packet_ok(c, s, tcfg) : 0;
+            if (cam && cnt) k_trace_cam<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, segs, nseg, H, counts, cap, gst, totals, tcfg);
+            else if (pcfg.packet) k_trace_packet<<<pgrid, kTraceBlock, tlds, st>>>(S, segs, nseg, H, counts, cap, totals, pcfg);
+            else if (cam) k_trace_cam<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, segs, nseg, H, counts, cap, gst, totals, tcfg);
+            else if (cnt) k_trace<true, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
             else k_trace<false, true><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
         } else {
-            if (c->count_visits)
-                k_trace<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
+            if (cam && cnt) k_trace_cam<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, segs, nseg, H, counts, cap, gst, totals, tcfg);
+            else if (cam) k_trace_cam<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, segs, nseg, H, counts, cap, gst, totals, tcfg);
+            else if (cnt) k_trace<true, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
             else k_trace<false, false><<<tgrid, kTraceBlock, tlds, st>>>(S, Q[qi], H, counts, qi, cap, gst, totals, tcfg);
         }
         prof_end(c, "trace", ev, 0);
         prof_begin(c, "shade", &ev);
-        k_shade<<<shade_grid, 256, 0, st>>>(S, segs, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, cap, mio);
+        if (cam)
+            k_shade<true><<<shade_grid, 256, 0, st>>>(S, segs, nseg, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, cap, mio);
+        else
+            k_shade<false><<<shade_grid, 256, 0, st>>>(S, segs, nseg, depth, Q[qi], H, Q[qi ^ 1], SQ, Lr, Lg, Lb, counts, qi, cap, mio);
         prof_end(c, "shade", ev, 0);
         if (any_nee) {
             prof_begin(c, "shadow", &ev);
