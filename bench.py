@@ -7,8 +7,8 @@ materials, 1920x1080, 64 spp, max depth 8, NEE + MIS on - rendered through the
 C ABI (srt_render) into a device-resident film.  Inputs (scene, BVH) are
 resident in HBM before the timed region; the BVH build is timed separately.
 
-Output: one JSON line with Mpaths/s (whole frame: raygen -> traverse -> shade
--> shadow -> finalize), per-request latency percentiles over the timed steps,
+Output: one JSON line with Mpaths/s (whole frame: traverse -> shade -> shadow
+per bounce, the first bounce computing its camera rays, -> finalize), per-request latency percentiles over the timed steps,
 the roofline of the dominant kernel (algorithmic bytes from the node-visit and
 triangle-test counts of an untimed counting pass over the same frame, divided
 by that kernel's mean launch time from HIP events on its stream), and the
@@ -53,7 +53,8 @@ MLP_FWD_FLOP = 33024             # forward per query (16,512 MACs x 2)
 METRIC = "Mpaths/s at equal RMSE vs FP64 oracle; p50/p99 render-request latency"
 
 # algorithmic bytes per unit (SoA layout of srt_amd/csrc/srt_internal.h)
-RAY_IN = 24          # origin + direction read by the traversal kernel
+RAY_IN = 24          # origin + direction read by the traversal kernel (bounces after the first, whose
+                     # camera rays are computed from the slot)
 HIT_OUT = 16         # t, u, v, triangle written by the traversal kernel
 NODE = 112           # one 4-wide node visit: 6 plane vectors (near/far x, y, z of 4 children) + 4 refs
 TRI = 36             # three vertices of a tested triangle
@@ -69,9 +70,10 @@ VALU_FILE = os.path.join(ROOT, "profiles", "trace_spheres1m_valu_issue.md")
 MI355X_SIMDS, MI355X_XCDS = 256 * 4, 8
 COPY_ELEMS = 1 << 28          # calibration copy: 1 GiB read + 1 GiB written (float32)
 # kernels of the frame, as rocprofv3 names them
-# the timed (not counting) 4-wide kernels of a flat scene; with the option
-# trace_packet set, the closest hits of camera rays run in k_trace_packet
-KERNEL_KEYS = {"trace": ("k_trace<false, true, false>", "k_trace_packet("), "shadow": ("k_shadow<false, true, false>",),
+# the timed (not counting) 4-wide kernels of a flat scene; the first bounce's
+# closest hits run in k_trace_cam (k_trace_packet with the option trace_packet)
+KERNEL_KEYS = {"trace": ("k_trace<false, true, false>", "k_trace_cam<false, true, false>", "k_trace_packet("),
+               "shadow": ("k_shadow<false, true, false>",),
                "shade": ("k_shade(",)}
 
 
@@ -153,7 +155,8 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
              "gpurun_out/pmc_copy_FETCH_SIZE gpurun_out/pmc_copy_WRITE_SIZE", "",
              "Each rocprofv3 run: cd /tmp && export TMPDIR=/tmp, bench.py by its path in the repository.",
              "The frame runs hold one counting pass (k_trace<true, true>/k_shadow<true, true>, not counted here; its k_shade",
-             "dispatches are) and one frame; FETCH_SIZE/WRITE_SIZE are the L2-to-fabric bytes (MALL or HBM).", "",
+             "dispatches are) and one frame; trace is k_trace_cam (first bounce) and k_trace (later bounces);",
+             "FETCH_SIZE/WRITE_SIZE are the L2-to-fabric bytes (MALL or HBM).", "",
              f"Calibration: copy kernel of {COPY_ELEMS * 4} B read + {COPY_ELEMS * 4} B written: "
              f"FETCH_SIZE {cf:.1f} KiB, WRITE_SIZE {cw:.1f} KiB per dispatch -> factors fetch {ff:.4f}, "
              f"write {fw:.4f}", "",
@@ -195,7 +198,8 @@ def valu_summary(d):
              "--steps 1 --warmup 0 --no-cpu --no-serve",
              "    python3 bench.py --valu-summary gpurun_out/pmc_valu", "",
              "The rocprofv3 run: cd /tmp && export TMPDIR=/tmp, bench.py by its path in the repository; the counting",
-             "pass's k_trace<true, true>/k_shadow<true, true> are not counted here (its k_shade dispatches are).",
+             "pass's k_trace<true, true>/k_shadow<true, true> are not counted here (its k_shade dispatches are); trace",
+             "is k_trace_cam (first bounce) and k_trace (later bounces).",
              f"busy = 4 SQ_ACTIVE_INST_VALU / ({MI355X_SIMDS} SIMDs x GRBM_GUI_ACTIVE / {MI355X_XCDS}), summed over the "
              "kernel's dispatches; lanes = SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU); wait = SQ_WAIT_ANY / "
              "SQ_WAVE_CYCLES.", "",
@@ -293,7 +297,7 @@ def kstats_summary(d, mode="frame"):
                  "    python3 bench.py --kstats-summary gpurun_out/prof_skin --kstats-mode skin", "",
                  "The run builds the BVH once, then renders 1 warmup frame and 60 timed frames; every frame",
                  "skins the tube (k_skin), refits the binary tree level by level (k_refit_level) and the 4-wide",
-                 "tree (k_refit4), then renders (k_raygen, k_trace, k_shade, k_shadow, k_finalize).", "",
+                 "tree (k_refit4), then renders (k_trace_cam / k_trace, k_shade, k_shadow, k_finalize).", "",
                  "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
         out_file = SKIN_KSTATS_FILE
     elif mode == "passes":
@@ -324,8 +328,10 @@ def kstats_summary(d, mode="frame"):
                  "Command (from the repository root; TMPDIR=/tmp):", "",
                  "    rocprofv3 --kernel-trace --stats -d gpurun_out/prof_ks -o run -- python3 bench.py --steps 2 --warmup 1 --no-cpu --no-serve",
                  "    python3 bench.py --kstats-summary gpurun_out/prof_ks", "",
-                 "The run renders 6 frames: the untimed counting pass (k_trace<true, true>, k_shadow<true, true>), 1 warmup,",
-                 "2 timed frames and 2 frames copied to host memory; the BVH build runs twice (cold, then a rebuild): the",
+                 "The run renders 6 frames: the untimed counting pass (k_trace_cam<true, true>, k_trace<true, true>,",
+                 "k_shadow<true, true>), 1 warmup, 2 timed frames and 2 frames copied to host memory; the first bounce's",
+                 "closest hits run in k_trace_cam (camera rays computed from the slot), later ones in k_trace; the BVH",
+                 "build runs twice (cold, then a rebuild): the",
                  "top-level SAH levels (k_wide_* for levels of <= 64 nodes, k_top_split, k_top_split_wave for levels of",
                  "> 1024 nodes, k_top_link / k_link_*), k_bottom_up_level, sorts, 4-wide collapse.", "",
                  "| kernel | calls | total ms | average us | share |", "|---|---|---|---|---|"]
@@ -1064,13 +1070,13 @@ def main():
                     "after every frame"}
     log(f"host-output rate {pcie['value']} Mpaths/s ({pcie['ms_per_frame']} ms/frame)")
 
-    # algorithmic bytes per frame per kernel
-    emitted = segments - paths     # queue entries written by shade
+    # algorithmic bytes per frame per kernel; the first bounce's rays and path
+    # state are computed from the slot, so only later segments read a queue entry
+    emitted = segments - paths     # queue entries written by shade (= entries read after the first bounce)
     bytes_step = {
-        "trace": RAY_IN * segments + HIT_OUT * segments + NODE * visits["closest_nodes"] + TRI * visits["closest_tris"],
+        "trace": RAY_IN * emitted + HIT_OUT * segments + NODE * visits["closest_nodes"] + TRI * visits["closest_tris"],
         "shadow": (SHADOW_IN + L_RMW) * shadow_rays + NODE * visits["shadow_nodes"] + TRI * visits["shadow_tris"],
-        "shade": (QUEUE_ENTRY + HIT_OUT + TRI_FETCH + MATERIAL) * segments + QUEUE_ENTRY * emitted
-                 + SHADOW_IN * shadow_rays,
+        "shade": (HIT_OUT + TRI_FETCH + MATERIAL) * segments + 2 * QUEUE_ENTRY * emitted + SHADOW_IN * shadow_rays,
     }
     kern = max((k for k in st if k in bytes_step), key=lambda k: st[k]["ms"])
     # the VALU and traffic counters behind the roofline were collected on the flat field
