@@ -74,7 +74,7 @@ COPY_ELEMS = 1 << 28          # calibration copy: 1 GiB read + 1 GiB written (fl
 # closest hits run in k_trace_cam (k_trace_packet with the option trace_packet)
 KERNEL_KEYS = {"trace": ("k_trace<false, true, false>", "k_trace_cam<false, true, false>", "k_trace_packet("),
                "shadow": ("k_shadow<false, true, false>",),
-               "shade": ("k_shade(",)}
+               "shade": ("k_shade<", "k_shade(")}
 
 
 def kernel_of(name):
