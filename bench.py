@@ -55,7 +55,7 @@ METRIC = "Mpaths/s at equal RMSE vs FP64 oracle; p50/p99 render-request latency"
 # algorithmic bytes per unit (SoA layout of srt_amd/csrc/srt_internal.h)
 RAY_IN = 24          # origin + direction read by the traversal kernel (bounces after the first, whose
                      # camera rays are computed from the slot)
-HIT_OUT = 16         # t, u, v, triangle written by the traversal kernel
+HIT_OUT = 12         # hit record: barycentrics u, v and the sorted triangle
 NODE = 112           # one 4-wide node visit: 6 plane vectors (near/far x, y, z of 4 children) + 4 refs
 TRI = 36             # three vertices of a tested triangle
 SHADOW_IN = 44       # shadow item: origin, direction, tmax, contribution, slot

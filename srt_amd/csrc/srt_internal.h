@@ -182,8 +182,8 @@ struct PathQueue {
     uint64_t* rng;                        // PCG32 state (stream implied by the sample)
     uint32_t* slot;                       // path slot in the current chunk
 };
-struct HitBuf {
-    float *t, *u, *v;
+struct HitBuf {                           // (shading rebuilds the hit point from u, v)
+    float *u, *v;
     int32_t* tri;                         // sorted triangle position or -1
     int32_t* inst = nullptr;              // instanced scenes: the instance hit
 };

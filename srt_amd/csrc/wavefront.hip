@@ -832,7 +832,6 @@ struct QueueIO {       // closest hit of queue entries -> hit buffer (same index
     }
     SRT_D void store(uint32_t i, const TravState& st, bool) const {
         const bool hit = st.best_pos >= 0;
-        h.t[i] = hit ? st.best_t : 0.0f;
         h.u[i] = hit ? st.bu : 0.0f;
         h.v[i] = hit ? st.bv : 0.0f;
         h.tri[i] = hit ? st.best_pos : -1;
@@ -1482,7 +1481,6 @@ struct SimpleIO {
     }
     SRT_D void store(uint32_t i, const TravState& st, bool) const {
         const bool hit = st.best_pos >= 0;
-        h.t[i] = hit ? st.best_t : 0.0f;
         h.u[i] = hit ? st.bu : 0.0f;
         h.v[i] = hit ? st.bv : 0.0f;
         h.tri[i] = hit ? st.best_pos : -1;
@@ -1952,7 +1950,7 @@ static int ensure_pool(srt_ctx* c, int64_t slots) {
             if (bytes && dev_alloc(c, c->q[k][j], bytes)) return SRT_E_OOM;
         }
     }
-    for (int j = 0; j < 5; j++) if (dev_alloc(c, c->hits[j], 4 * qn)) return SRT_E_OOM;   // 4: instance hit
+    for (int j = 1; j < 5; j++) if (dev_alloc(c, c->hits[j], 4 * qn)) return SRT_E_OOM;   // 1, 2: u, v; 3: triangle; 4: instance
     for (int j = 0; j < 11; j++) if (dev_alloc(c, c->shq[j], 4 * qn)) return SRT_E_OOM;
     for (int j = 0; j < 3; j++) if (dev_alloc(c, c->Lbuf[j], 4 * n)) return SRT_E_OOM;
     c->pool_cap = slots;
@@ -2076,7 +2074,7 @@ int run_chunk(srt_ctx* c, SceneRec* s, const Segment* segs_host, int nseg, Train
     PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
     const bool inst = s->ninst > 0;
     HitBuf H;
-    H.t = (float*)c->hits[0].p; H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
+    H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
     H.inst = inst ? (int32_t*)c->hits[4].p : nullptr;
     const size_t ilds = trace_lds_bytes(tcfg, true);
     ShadowQueue SQ;
@@ -2256,7 +2254,7 @@ int render_simple(srt_ctx* c, SceneRec* s, const DevCamera& cam, int type, int s
     io.width = width;
     io.height = height;
     io.seed = seed;
-    io.h.t = (float*)c->hits[0].p; io.h.u = (float*)c->hits[1].p; io.h.v = (float*)c->hits[2].p;
+    io.h.u = (float*)c->hits[1].p; io.h.v = (float*)c->hits[2].p;
     io.h.tri = (int32_t*)c->hits[3].p;
     const bool inst = s->ninst > 0;
     io.h.inst = inst ? (int32_t*)c->hits[4].p : nullptr;
@@ -2304,7 +2302,7 @@ int render_light(srt_ctx* c, SceneRec* s, const DevCamera& cam, int spp, int max
     unsigned long long* totals = (unsigned long long*)((char*)c->counters.p + kTotalsOffset);
     PathQueue Q[2] = {make_queue(c, 0), make_queue(c, 1)};
     HitBuf H;
-    H.t = (float*)c->hits[0].p; H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
+    H.u = (float*)c->hits[1].p; H.v = (float*)c->hits[2].p; H.tri = (int32_t*)c->hits[3].p;
     H.inst = inst ? (int32_t*)c->hits[4].p : nullptr;
     ShadowQueue SQ;
     {
