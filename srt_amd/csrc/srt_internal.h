@@ -309,7 +309,7 @@ struct srt_ctx {
     std::map<uint64_t, srt::SceneRec*> scenes;
     // render pool
     int64_t pool_cap = 0;                 // path slots allocated
-    int64_t pool_request = 1ll << 27;      // ~184 B per slot: 24.7 GB covers a 1080p x 64 spp frame in one chunk
+    int64_t pool_request = 1ll << 27;      // ~180 B per slot: 24.2 GB covers a 1080p x 64 spp frame in one chunk
     srt::DevBuf q[2][14], hits[5], shq[11], Lbuf[3], counters, film_acc, film_out, gstack;
     // device chunks of BVH-build scratch kept between builds (a rebuild then
     // allocates and frees nothing); released when an allocation would exceed
