@@ -84,6 +84,19 @@ def test_tiny_images(ctx, W, H, spp):
     assert np.array_equal(img, ref)
 
 
+@pytest.mark.parametrize("depth", [0, 1])
+def test_shallow_paths(ctx, depth):
+    """max_depth 0 (no bounce: the radiance is cleared, not shaded) and 1 (the
+    first bounce alone, whose stages compute the camera rays) equal the oracle,
+    also after a deeper render has left radiance in the same path slots."""
+    s = scenes.cornell()
+    _render_pair(ctx, s, 24, 16, 3, 5, seed=4)
+    img, ref = _render_pair(ctx, s, 24, 16, 3, depth, seed=4)
+    assert np.array_equal(img, ref)
+    if depth == 0:
+        assert np.all(img[:, :3] == 0.0)
+
+
 @pytest.mark.parametrize("packet", [1, 16, 64])
 def test_packet_traversal_equals_per_ray(ctx, packet):
     """Camera rays traversed as 64-ray packets (option trace_packet: lanes with
