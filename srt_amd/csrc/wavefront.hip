@@ -752,6 +752,21 @@ SRT_D int find_segment(const Segment* segs, int nseg, uint32_t i) {
 // cover a compact tile.  tile_pixel is a bijection of [0, W H).  Results do
 // not depend on the order: each path's random stream is
 // keyed by (pixel, sample) and each pixel sums its samples in sample order.
+// The same bijection as column x and row y; the full 8x8 tiles of the image
+// (all but the last band and block) divide by shifts.
+SRT_D void tile_xy(uint32_t rank, uint32_t W, uint32_t H, uint32_t& x, uint32_t& y) {
+    const uint32_t band = rank / (8u * W);
+    const uint32_t r = rank - band * 8u * W;
+    const bool full_band = H - 8u * band >= 8u;
+    const uint32_t b = full_band ? r >> 6 : r / (8u * (H - 8u * band));
+    const uint32_t bh = full_band ? 8u : H - 8u * band;
+    const uint32_t r2 = r - b * 8u * bh;
+    const bool full_block = W - 8u * b >= 8u;
+    const uint32_t ly = full_block ? r2 >> 3 : r2 / (W - 8u * b);
+    const uint32_t lx = r2 - ly * (full_block ? 8u : W - 8u * b);
+    x = b * 8u + lx;
+    y = band * 8u + ly;
+}
 SRT_D uint32_t tile_pixel(uint32_t rank, uint32_t W, uint32_t H) {
     const uint32_t band = rank / (8u * W);
     const uint32_t bh = (H - 8u * band < 8u) ? H - 8u * band : 8u;
@@ -766,9 +781,8 @@ SRT_D uint32_t tile_pixel(uint32_t rank, uint32_t W, uint32_t H) {
 // Camera ray of one sample of one pixel (path_tracer.cu:66-68,
 // camera.h:107-117, :27-32): two jitter draws from the (pixel, sample)
 // stream; rng is left after them.
-SRT_D void camera_ray(const DevCamera& cam, int width, int height, uint32_t pixel, uint32_t seed, uint32_t sample, f3& ro,
-                      f3& rd, Pcg32& rng) {
-    const int ix = (int)(pixel % (uint32_t)width), iy = (int)(pixel / (uint32_t)width);
+SRT_D void camera_ray_at(const DevCamera& cam, int width, int height, int ix, int iy, uint32_t pixel, uint32_t seed,
+                         uint32_t sample, f3& ro, f3& rd, Pcg32& rng) {
     rng = path_rng(pixel, seed, sample);
     float r0 = rng.next_float();
     float xx = ((float)ix + r0) / (float)width;
@@ -780,6 +794,11 @@ SRT_D void camera_ray(const DevCamera& cam, int width, int height, uint32_t pixe
     f3 zero = mk(0.0f);
     ro = zero.x * X + zero.y * Y + zero.z * Z + CP;
     rd = dir.x * X + dir.y * Y + dir.z * Z;
+}
+SRT_D void camera_ray(const DevCamera& cam, int width, int height, uint32_t pixel, uint32_t seed, uint32_t sample, f3& ro,
+                      f3& rd, Pcg32& rng) {
+    camera_ray_at(cam, width, height, (int)(pixel % (uint32_t)width), (int)(pixel / (uint32_t)width), pixel, seed, sample,
+                  ro, rd, rng);
 }
 
 // Sample index of path id pid (>= 0) in planes of wh > 0 pixels: a 32-bit
@@ -807,10 +826,11 @@ SRT_D PathStart path_start(const Segment* segs, int nseg, uint32_t slot) {
     const Segment& sg = segs[k];
     const int64_t pid = sg.path_begin + (slot - sg.slot_offset);
     const uint32_t sample = path_sample(pid, sg.wh);
-    const uint32_t pixel = tile_pixel((uint32_t)(pid - (int64_t)sample * sg.wh), (uint32_t)sg.width, (uint32_t)sg.height);
+    uint32_t x, y;
+    tile_xy((uint32_t)(pid - (int64_t)sample * sg.wh), (uint32_t)sg.width, (uint32_t)sg.height, x, y);
     PathStart p;
     Pcg32 rng;
-    camera_ray(sg.cam, sg.width, sg.height, pixel, sg.seed, sample, p.o, p.d, rng);
+    camera_ray_at(sg.cam, sg.width, sg.height, (int)x, (int)y, y * (uint32_t)sg.width + x, sg.seed, sample, p.o, p.d, rng);
     p.flags = 1u | ((uint32_t)k << 8);
     p.rng = rng.state;
     return p;
